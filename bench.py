@@ -1,0 +1,133 @@
+"""Headline benchmark: spin updates/s and effective samples/s of E of the device
+Metropolis sweep on bcc 32^3 (65,536 spins; body-2 l<=2 to the 3rd shell, l=1
+triangles and tetrahedra, isotropic SALC tensors).
+
+One "step" is one Metropolis sweep of the chain (every active site attempted
+once), with the energy recorded on the device every sweep.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "scemontecarlo.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_update(H) -> float:
+    """Compulsory HBM bytes of one site update on the reference table layout:
+    site_ptr pair + colour list (12), site_prog + site_col per pair entry (8),
+    + site_col2 per triplet entry (12), site_prog + site_col + site_inst + the
+    member share of the 20-byte instance record per general entry (8 + 9), own
+    rows (8 nlm) + spin (24), dE/acc staging (12), accepted writes at p = 0.5."""
+    s = 0 if H.n_active == 0 else int(H.color_sites[0])
+    lo, hi = int(H.site_ptr[s - 1]), int(H.site_ptr[s])
+    col = H.site_col[lo - 1:hi - 1]
+    a_pair = int(np.sum(col > 0))
+    a_trip = int(np.sum(col < 0))
+    a_gen = int(np.sum(col == 0))
+    nlm = H.nlm
+    return (12 + 8 * a_pair + 12 * a_trip + 8 * a_gen + 9 * a_gen + 8 * nlm + 24 + 12
+            + 0.5 * (8 * nlm + 36))
+
+
+def cpu_baseline(na, terms, dims, cfg, kT, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    T = O.Tiled(na, terms, dims)
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    ch = O.CpuChain(T, cfg, 12345, 0.6)
+    ch.metropolis_sweep(1.0 / kT, nthreads)     # warm
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        ch.metropolis_sweep(1.0 / kT, nthreads)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 200:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": T.n_active * n / dt, "unit": "spin updates/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{n} Metropolis sweeps of the reference CPU path (site_coeffs! + "
+                      f"delta_energy, colour classes over {nthreads} threads) on the same "
+                      f"{T.n_sites}-spin model, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--workload", default="bcc32")
+    ap.add_argument("--lanes", type=int, default=64)
+    ap.add_argument("--kT", type=float, default=2.0e-3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box has "
+              "one MI355X; replicas shard across GPUs by design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import sce_amd as S
+    from sce_amd import models as M
+
+    na, terms, dims = M.config_model(args.workload, seed=0)
+    H = S.TiledHamiltonian(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    cfg = S.random_config(H.n_sites, np.random.default_rng(0))
+    st = S.ChainState(H, cfg, 0.6)
+    gst = S.GPUChainState(gH, st, seed=20261203)
+    beta = 1.0 / args.kT
+    S.gpu_run_sweeps(gst, gH, None, beta, args.warmup, renorm_interval=1000, lanes=args.lanes)
+    gst.reset_counters()
+    gst.set_timing(True)
+    t0 = time.perf_counter()
+    obs = S.gpu_run_sweeps(gst, gH, None, beta, args.steps, renorm_interval=1000,
+                           lanes=args.lanes, measure_interval=1)
+    wall = time.perf_counter() - t0
+    tot_ms, walk_ms_avg, walk_n = gst.timing()
+    inf = gst.info()
+    ups = H.n_active * args.steps / wall
+    ess = S.effective_samples(obs[:, 0])
+    b_site = algorithmic_bytes_per_update(H)
+    walk_s = walk_ms_avg * walk_n / 1e3
+    achieved = b_site * H.n_active * args.steps / walk_s / 1e9 if walk_s > 0 else float("nan")
+    line = {
+        "metric": "spin updates/s + effective samples/s of E at 32^3 (plain device sweep)",
+        "value": ups,
+        "unit": "spin updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random spins, seeded N(0, 1 meV) isotropic SALC couplings)",
+        "config": {"workload": f"{args.workload}: bcc {dims[0]}^3 x 2 atoms = {H.n_sites} spins, "
+                               "body-2 l<=2 to 3rd shell + l=1 triangles + l=1 tetrahedra",
+                   "kT_eV": args.kT, "lanes_per_site": args.lanes, "colours": H.n_colors},
+        "effective_samples_per_s_E": ess / wall if math.isfinite(ess) else None,
+        "tau_int_E": (args.steps / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
+        "acceptance": inf.acc_metro / max(inf.att_metro, 1),
+        "device_ms_per_step": tot_ms / args.steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_metro (colour walk)", "bytes_per_update": b_site,
+                     "walk_launch_ms_avg": walk_ms_avg, "walk_launches": walk_n},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(na, terms, dims, cfg, args.kT)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

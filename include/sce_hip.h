@@ -1,0 +1,182 @@
+/*
+ * sce_hip.h — C ABI of libsce_hip.so, the MI355X (gfx950) device Metropolis
+ * sweep of SCEMonteCarlo.jl.
+ *
+ * Every entry point returns SCE_OK (0) or a negative error code; the message of
+ * the last failure on the calling thread is read with sce_last_error().  No C++
+ * exception crosses this boundary.  Handles are not thread-safe; distinct handles
+ * are.  Arrays follow the reference's Julia conventions unchanged: site ids,
+ * CSR pointers and tesseral rows are 1-based Int32, `config` is 3 x n_sites and
+ * `zrows` is nlm x n_sites, both column-major (column s contiguous).  The caller
+ * owns every host buffer; the library copies in and out and owns device memory.
+ *
+ * Reference interface (Julia, src/SCEMonteCarlo.jl:86-88):
+ *   GPUTiledHamiltonian(backend, H)          src/gpu/gpu_hamiltonian.jl:61-88
+ *   GPUChainState(gH, st; seed)              src/gpu/gpu_state.jl:49-61
+ *   gpu_metropolis_sweep!(gst, gH, β; ws)    src/gpu/gpu_sweep.jl:158-190
+ *   gpu_run_sweeps!(gst, gH, st, β, n; ...)  src/gpu/gpu_sweep.jl:204-218
+ *   to_host!(st, gst) / _from_host!(gst, st) src/gpu/gpu_state.jl:71-86
+ *   _renormalize!(st, H)                     src/state.jl:168-185
+ *   _swap_payload!(a, b)                     src/gpu/gpu_pt.jl:46-51
+ *   gpu_run_pt(gH; kT, exchange_interval...) src/gpu/gpu_pt.jl:261-319
+ */
+#ifndef SCE_HIP_H
+#define SCE_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SCE_OK = 0,
+  SCE_EINVAL = -1,        /* ArgumentError in the reference */
+  SCE_EDIM = -2,          /* DimensionMismatch */
+  SCE_ERANGE_COUNTER = -3,/* 32-bit sweep counter exhausted (gpu_sweep.jl:163-164) */
+  SCE_EHIP = -4,          /* HIP runtime failure */
+  SCE_ENOMEM = -5,
+  SCE_ELAYOUT = -6,       /* tables are not a translation-invariant tiling */
+  SCE_ENODEV = -7         /* no HIP device */
+};
+
+typedef struct sce_tiled sce_tiled;   /* host TiledHamiltonian (tables owner) */
+typedef struct sce_tables sce_tables; /* device tables (GPUTiledHamiltonian) */
+typedef struct sce_chain sce_chain;   /* device chain (GPUChainState) */
+typedef struct sce_batch sce_batch;   /* many chains in one launch (replica batch) */
+
+/* The flat tables of one TiledHamiltonian (src/hamiltonian.jl:234-266,
+ * _ContractionPrograms :62-91), as a Julia caller holds them. */
+typedef struct sce_tables_desc {
+  int32_t n_cell_atoms;
+  int32_t dims[3];
+  int32_t n_sites, lmax, nlm, n_terms, n_inst, n_active, n_colors, n_prog;
+  int64_t n_inst_sites; /* = length(site_inst) = length(site_prog) = ... */
+  int64_t n_sent, n_sfac, n_eent, n_efac;
+  const int32_t *inst_term, *inst_ptr, *inst_sites;
+  const int32_t *site_ptr, *site_inst;
+  const int8_t *site_slot;
+  const uint8_t *site_has_l1, *site_active;
+  const int32_t *site_prog, *sprog_ptr;
+  const double *sent_w;
+  const int32_t *sent_tgt, *sfac_ptr, *sfac_row;
+  const int8_t *sfac_slot;
+  const int32_t *site_col, *site_col2, *pent_row, *pent_row2;
+  const int32_t *eprog_ptr;
+  const double *eent_w;
+  const int32_t *efac_ptr, *efac_row;
+  const double *term_coef;
+  const int32_t *color_ptr, *color_sites;
+} sce_tables_desc;
+
+typedef struct sce_chain_info {
+  double energy;        /* incremental energy, model units, intercept excluded */
+  double step;          /* Metropolis rotation scale (radians) */
+  double max_drift;     /* worst |E_inc - E_full| seen at renormalisation */
+  double last_drift;
+  int64_t sweep_index;  /* completed Metropolis sweeps: the keyed-RNG counter */
+  int64_t acc_metro, att_metro, acc_or, att_or;
+  uint64_t seed;
+} sce_chain_info;
+
+int32_t sce_version(void);
+int32_t sce_last_error(char* buf, size_t len);
+int32_t sce_device_count(int32_t* n);
+
+/* ---- host TiledHamiltonian -------------------------------------------------
+ * TiledHamiltonian(n_cell_atoms, terms; dims, magmoms, field)
+ * (src/hamiltonian.jl:267-393).  Terms are flattened: term k has body[k] members;
+ * atoms/ls are concatenated (1-based atoms), shifts are 3 ints per member,
+ * folded is the column-major tensor of each term concatenated.  magmoms
+ * (n_cell_atoms) and field (3) may be NULL. */
+int32_t sce_tiled_build(int32_t n_cell_atoms, int32_t n_terms, const int32_t* body,
+                        const double* coef, const int32_t* atoms, const int32_t* shifts,
+                        const int32_t* ls, const double* folded, const int32_t dims[3],
+                        const double* magmoms, const double* field, sce_tiled** out);
+int32_t sce_tiled_desc(const sce_tiled* t, sce_tables_desc* out);
+int32_t sce_tiled_free(sce_tiled* t);
+/* Host-side rows and total energy: _zrows / _total_energy (src/energy.jl:41-75). */
+int32_t sce_host_zrows(int32_t lmax, int64_t n, const double* config, double* zrows);
+int32_t sce_host_total_energy(const sce_tables_desc* d, const double* zrows, double* E);
+
+/* ---- device tables: GPUTiledHamiltonian(backend, H) -------------------------
+ * Uploads the tables and derives the per-sublattice walk layout; rejects
+ * lmax > 6 (gpu_hamiltonian.jl:67-68) and bodies > 4 with SCE_EINVAL. */
+int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out);
+int32_t sce_tables_destroy(sce_tables* t);
+/* Derive and verify the walk layout on the host only (no device needed);
+ * writes the same facts as sce_tables_layout_info. */
+int32_t sce_layout_check(const sce_tables_desc* d, int64_t out[4]);
+/* Layout facts: [n_sublattices, max_neighbours, max_walk_entries, total_walk_entries] */
+int32_t sce_tables_layout_info(const sce_tables* t, int64_t out[4]);
+
+/* ---- device chain: GPUChainState(gH, st; seed) ------------------------------ */
+int32_t sce_chain_create(sce_tables* t, const double* config, const double* zrows,
+                         double energy, uint64_t seed, double step, sce_chain** out);
+int32_t sce_chain_destroy(sce_chain* c);
+int32_t sce_chain_info_get(sce_chain* c, sce_chain_info* out);
+int32_t sce_chain_set_step(sce_chain* c, double step);
+int32_t sce_chain_reset_counters(sce_chain* c);
+/* to_host!(st, gst): config (3 x n), zrows (nlm x n), energy; any may be NULL. */
+int32_t sce_download(sce_chain* c, double* config, double* zrows, double* energy);
+/* _from_host!(gst, st) */
+int32_t sce_upload(sce_chain* c, const double* config, const double* zrows, double energy);
+/* _swap_payload!(a, b): exchanges config / zrows / energy (O(1)). */
+int32_t sce_swap_payload(sce_chain* a, sce_chain* b);
+
+/* gpu_metropolis_sweep!(gst, gH, β; workgroupsize): one Metropolis sweep;
+ * `lanes` (16, 32 or 64) is the number of lanes that walk one site — part of
+ * the determinism scope, as the reference's workgroupsize.  Writes the number
+ * of accepted moves. */
+int32_t sce_metropolis_sweep(sce_chain* c, double beta, int32_t lanes, int64_t* nacc);
+/* One overrelaxation sweep (updates.jl:66-87,231-290 on the device); `or_pass`
+ * selects the keyed accept stream of the pass within a compound sweep. */
+int32_t sce_overrelaxation_sweep(sce_chain* c, double beta, int32_t or_pass,
+                                 int32_t lanes, int64_t* nacc);
+/* gpu_run_sweeps!(gst, gH, st, β, n; renorm_interval, workgroupsize) without the
+ * final to_host!: n compound sweeps (1 Metropolis + or_per_metropolis OR each),
+ * renormalising on the device every renorm_interval sweeps (<= 0: never).
+ * When measure_interval > 0, every measure_interval-th sweep appends
+ * (E, m_x, m_y, m_z) to `obs` (capacity obs_cap rows), m = mean active spin. */
+int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps,
+                       int32_t or_per_metropolis, int64_t renorm_interval,
+                       int64_t measure_interval, int32_t lanes, double* obs,
+                       int64_t obs_cap, int64_t* n_obs);
+/* _renormalize!(st, H) on the device: normalise active spins, rebuild rows,
+ * recompute the FP64 total energy, re-anchor.  Writes the drift. */
+int32_t sce_renormalize(sce_chain* c, double* drift);
+/* Device total energy of the chain's current rows (no re-anchoring). */
+int32_t sce_chain_total_energy(sce_chain* c, double* E);
+/* Enable (on != 0) HIP-event timing of sce_run_sweeps: the wall time of the
+ * device work and the per-launch durations of the Metropolis walk kernel. */
+int32_t sce_chain_set_timing(sce_chain* c, int32_t on);
+/* Wall time of the device work enqueued by the last timed sce_run_sweeps call, and
+ * the average duration of its Metropolis walk launches (both from HIP events). */
+int32_t sce_chain_timing(sce_chain* c, double* total_ms, double* walk_ms_avg,
+                         int64_t* walk_launches);
+
+/* ---- diagnostics (no reference counterpart): the device arithmetic exposed
+ * for bit-exact parity tests ------------------------------------------------- */
+int32_t sce_test_philox(int64_t n, const uint64_t* seeds, const uint32_t* ctr4,
+                        uint32_t* out4);
+/* op: 0 exp, 1 log, 2 sin, 3 cos */
+int32_t sce_test_detmath(int64_t n, int32_t op, const double* x, double* out);
+int32_t sce_test_zlm(int64_t n, int32_t lmax, const double* dirs3, double* out);
+int32_t sce_test_proposal(int64_t n, uint64_t seed, const int32_t* sites,
+                          const int32_t* sweeps, const double* e3, double step,
+                          double* e2_out, double* u_out);
+/* ΔE of the kernel walk for arbitrary proposals: sites (1-based, active),
+ * znew (nlm x n), against rows `zrows` (nlm x n_sites). */
+int32_t sce_test_walk_delta(sce_tables* t, const double* zrows, int64_t n,
+                            const int32_t* sites, const double* znew, int32_t lanes,
+                            double* out);
+/* Leave-one-out coefficients c (nlm x n) of the overrelaxation walk. */
+int32_t sce_test_walk_coeffs(sce_tables* t, const double* zrows, int64_t n,
+                             const int32_t* sites, int32_t lanes, double* out);
+/* Device deterministic tree sum of x (the sweep/energy reduction). */
+int32_t sce_test_reduce(int64_t n, const double* x, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+// layout.cpp — per-sublattice walk layout of a tiled Hamiltonian (see
+// sce_internal.h).  The factor products it encodes are those of site_coeffs!
+// (src/energy.jl:113-136): pair p = z, triplet p = z1*z2, general
+// p = ((1*z1)*z2)*z3 — the device evaluates (Z[f0]*Z[f1])*Z[f2] with unused
+// factors reading 1.0, which is bit-identical for every body <= 4.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "sce_internal.h"
+
+namespace {
+
+struct Geo {
+  int na, d[3];
+  void cell_of(int s0, int c[3]) const {  // s0: 0-based site
+    int cell = s0 / na;
+    c[0] = cell % d[0];
+    c[1] = (cell / d[0]) % d[1];
+    c[2] = cell / (d[0] * d[1]);
+  }
+  int site(int atom0, const int c[3]) const {
+    return atom0 + na * (c[0] + d[0] * (c[1] + d[1] * c[2]));
+  }
+};
+
+// Factor sites of adjacency entry j (1-based), in the product order of the
+// entry's program: pair [col]; triplet [-col, col2]; general the instance's
+// members at the factor slots (ascending slot, all but the site's own).
+int factor_sites(const sce_tables_desc* d, int j, int sites[8]) {
+  const int col = d->site_col[j - 1];
+  if (col > 0) {
+    sites[0] = col;
+    return 1;
+  }
+  if (col < 0) {
+    sites[0] = -col;
+    sites[1] = d->site_col2[j - 1];
+    return 2;
+  }
+  const int inst = d->site_inst[j - 1];
+  const int off = d->inst_ptr[inst - 1] - 1;
+  const int body = d->inst_ptr[inst] - d->inst_ptr[inst - 1];
+  const int own = d->site_slot[j - 1];
+  int n = 0;
+  for (int k = 1; k <= body; ++k)
+    if (k != own) sites[n++] = d->inst_sites[off + k - 1];
+  return n;
+}
+
+}  // namespace
+
+int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
+  const Geo g{d->n_cell_atoms, {d->dims[0], d->dims[1], d->dims[2]}};
+  const int na = g.na, nlm = d->nlm, N = d->n_sites;
+  if (na * g.d[0] * g.d[1] * g.d[2] != N)
+    return sce_fail(SCE_ELAYOUT, "n_sites %d != n_cell_atoms * prod(dims)", N);
+  L->na = na;
+  L->walk_ptr.assign(na + 1, 0);
+  L->nb_ptr.assign(na + 1, 0);
+  L->walk.clear();
+  L->nb.clear();
+  L->nb_max = 0;
+  L->w_max = 0;
+  // representative site of sublattice a: atom a in cell 0 (0-based site id a)
+  std::vector<std::vector<int>> rep_js(na);      // pid-sorted adjacency of the rep
+  std::vector<std::vector<int>> rep_nbidx(na);   // per rep adjacency entry: nb index per factor
+  auto sorted_adj = [&](int s0) {
+    std::vector<int> js;
+    for (int j = d->site_ptr[s0]; j < d->site_ptr[s0 + 1]; ++j) js.push_back(j);
+    std::stable_sort(js.begin(), js.end(),
+                     [&](int a, int b) { return d->site_prog[a - 1] < d->site_prog[b - 1]; });
+    return js;
+  };
+  for (int a = 0; a < na; ++a) {
+    L->walk_ptr[a] = (int32_t)L->walk.size();
+    L->nb_ptr[a] = (int32_t)(L->nb.size() / 4);
+    if (!d->site_active[a]) continue;
+    std::map<std::tuple<int, int, int, int>, int> nbmap;
+    std::vector<std::array<int, 4>> nbl;
+    auto nb_index = [&](int site1) {
+      int c[3];
+      g.cell_of(site1 - 1, c);
+      const auto key = std::make_tuple((site1 - 1) % na, c[0], c[1], c[2]);
+      auto it = nbmap.find(key);
+      if (it != nbmap.end()) return it->second;
+      const int k = (int)nbl.size();
+      nbmap.emplace(key, k);
+      nbl.push_back({std::get<0>(key), c[0], c[1], c[2]});
+      return k;
+    };
+    rep_js[a] = sorted_adj(a);
+    for (int j : rep_js[a]) {
+      int fs[8];
+      const int nf = factor_sites(d, j, fs);
+      if (nf > 3)
+        return sce_fail(SCE_EINVAL, "cluster body %d > 4 is not supported by the device walk", nf + 1);
+      for (int f = 0; f < nf; ++f) rep_nbidx[a].push_back(nb_index(fs[f]));
+      rep_nbidx[a].push_back(-1);  // separator
+    }
+    const int NB = (int)nbl.size();
+    if ((NB + 1) * nlm + 1 > 65535)
+      return sce_fail(SCE_EINVAL, "neighbour stencil of %d sites too large for the device walk", NB);
+    const uint16_t ONE = (uint16_t)(NB * nlm);
+    size_t pos = 0;
+    for (int j : rep_js[a]) {
+      int nbi[3];
+      int nf = 0;
+      while (rep_nbidx[a][pos] >= 0) nbi[nf++] = rep_nbidx[a][pos++];
+      ++pos;
+      const int pid = d->site_prog[j - 1];
+      const int col = d->site_col[j - 1];
+      for (int e = d->sprog_ptr[pid - 1]; e < d->sprog_ptr[pid]; ++e) {
+        WalkEntry we;
+        we.w = d->sent_w[e - 1];
+        we.tgt = (uint16_t)(d->sent_tgt[e - 1] - 1);
+        int rows[3];
+        int nr = 0;
+        if (col > 0) {
+          rows[nr++] = d->pent_row[e - 1];
+        } else if (col < 0) {
+          rows[nr++] = d->pent_row[e - 1];
+          rows[nr++] = d->pent_row2[e - 1];
+        } else {
+          for (int f = d->sfac_ptr[e - 1]; f < d->sfac_ptr[e]; ++f) rows[nr++] = d->sfac_row[f - 1];
+        }
+        if (nr != nf) return sce_fail(SCE_ELAYOUT, "program %d: factor count mismatch", pid);
+        uint16_t fi[3] = {ONE, ONE, ONE};
+        for (int f = 0; f < nr; ++f) fi[f] = (uint16_t)(nbi[f] * nlm + rows[f] - 1);
+        we.f0 = fi[0];
+        we.f1 = fi[1];
+        we.f2 = fi[2];
+        L->walk.push_back(we);
+      }
+    }
+    for (auto& v : nbl) L->nb.insert(L->nb.end(), v.begin(), v.end());
+    L->nb_max = std::max(L->nb_max, NB);
+  }
+  L->walk_ptr[na] = (int32_t)L->walk.size();
+  L->nb_ptr[na] = (int32_t)(L->nb.size() / 4);
+  for (int a = 0; a < na; ++a) L->w_max = std::max(L->w_max, L->walk_ptr[a + 1] - L->walk_ptr[a]);
+
+  // verification: every active site is its sublattice stencil, translated
+  for (int s0 = 0; s0 < N; ++s0) {
+    const int a = s0 % na;
+    const bool act = d->site_active[s0] != 0;
+    if (act != (d->site_active[a] != 0))
+      return sce_fail(SCE_ELAYOUT, "site %d: activity differs from its sublattice", s0 + 1);
+    if (!act) continue;
+    const std::vector<int> js = sorted_adj(s0);
+    if (js.size() != rep_js[a].size())
+      return sce_fail(SCE_ELAYOUT, "site %d: adjacency length differs from its sublattice", s0 + 1);
+    int c[3];
+    g.cell_of(s0, c);
+    size_t pos = 0;
+    for (size_t q = 0; q < js.size(); ++q) {
+      const int j = js[q], jr = rep_js[a][q];
+      if (d->site_prog[j - 1] != d->site_prog[jr - 1] ||
+          ((d->site_col[j - 1] > 0) != (d->site_col[jr - 1] > 0)) ||
+          ((d->site_col[j - 1] < 0) != (d->site_col[jr - 1] < 0)))
+        return sce_fail(SCE_ELAYOUT, "site %d: program sequence differs from its sublattice", s0 + 1);
+      int fs[8];
+      const int nf = factor_sites(d, j, fs);
+      for (int f = 0; f < nf; ++f) {
+        const int k = rep_nbidx[a][pos++];
+        const int32_t* o = &L->nb[4 * (L->nb_ptr[a] + k)];
+        int cc[3];
+        for (int x = 0; x < 3; ++x) {
+          cc[x] = c[x] + o[1 + x];
+          if (cc[x] >= g.d[x]) cc[x] -= g.d[x];
+        }
+        if (g.site(o[0], cc) + 1 != fs[f])
+          return sce_fail(SCE_ELAYOUT, "site %d: neighbour %d is not the translated stencil", s0 + 1, f + 1);
+      }
+      if (rep_nbidx[a][pos] != -1)
+        return sce_fail(SCE_ELAYOUT, "site %d: factor count differs from its sublattice", s0 + 1);
+      ++pos;
+    }
+  }
+  return SCE_OK;
+}

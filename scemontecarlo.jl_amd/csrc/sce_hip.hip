@@ -1,0 +1,1362 @@
+// sce_hip.hip — gfx950 kernels and the C ABI of the device Metropolis sweep.
+//
+// Reference path: src/gpu/gpu_sweep.jl (the fused colour kernel, :92-145, and
+// its drivers, :158-218), src/gpu/gpu_state.jl, src/gpu/gpu_hamiltonian.jl,
+// src/updates.jl (overrelaxation, :66-87), src/state.jl (_renormalize!,
+// :168-185), src/energy.jl (_total_energy, :55-75).
+//
+// One Metropolis sweep is
+//   k_propose            one lane per active site: keyed proposal + Z row
+//   k_metro   x colours  `lanes` lanes per site: neighbour rows staged in LDS,
+//                        the site's sublattice walk dealt round-robin over the
+//                        lanes, fma-accumulated ΔE, shuffle-tree fold, FP64
+//                        accept; per-site writes only (no atomics on state)
+//   k_tree               deterministic tree sum of the staged ΔE + counters,
+//                        folded into the device-resident chain state
+// all on the chain's stream with no host synchronisation between sweeps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "sce_internal.h"
+
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[1024] = "";
+
+int32_t sce_fail(int32_t code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      return sce_fail(SCE_EHIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                      __LINE__);                                                     \
+  } while (0)
+
+#define CHK(x)                  \
+  do {                          \
+    int32_t r_ = (x);           \
+    if (r_ != SCE_OK) return r_; \
+  } while (0)
+
+extern "C" int32_t sce_version(void) { return 10000; }
+
+extern "C" int32_t sce_last_error(char* buf, size_t len) {
+  if (buf && len) {
+    std::strncpy(buf, g_err, len - 1);
+    buf[len - 1] = 0;
+  }
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_device_count(int32_t* n) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  if (n) *n = c;
+  return c > 0 ? SCE_OK : sce_fail(SCE_ENODEV, "no HIP device available");
+}
+
+// ---------------------------------------------------------------------------
+// device-side structures
+// ---------------------------------------------------------------------------
+struct ChainDev {
+  double energy, step, beta, max_drift, last_drift, e_full;
+  long long sweep_index, acc_metro, att_metro, acc_or, att_or, nacc_last;
+  unsigned long long seed;
+  unsigned long long iacc, iatt;
+  unsigned int ticket;
+  int n_obs;
+};
+
+struct DevTables {
+  int n_sites, n_active, nlm, lmax, na, d0, d1, d2, nb_max, n_inst;
+  const int32_t* color_sites;  // 0-based site ids, colour order
+  const uint8_t* has_l1;
+  const uint8_t* active;
+  const int32_t* walk_ptr;
+  const WalkEntry* walk;
+  const int32_t* nb_ptr;
+  const int4* nb;
+  const int32_t* inst_term;  // 0-based
+  const int32_t* inst_ptr;   // 0-based offsets
+  const int32_t* inst_sites; // 0-based
+  const int32_t* eprog_ptr;  // 0-based offsets
+  const double* eent_w;
+  const int32_t* efac_ptr;   // 0-based offsets
+  const int32_t* efac_row;   // 0-based rows
+  const double* term_coef;
+};
+
+constexpr int TPB = 256;      // threads per block of the walk kernels
+constexpr int TREE_B = 512;   // elements per tree chunk
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks a contiguous range of the colour class (neighbour rows then stay in
+// that XCD's L2).
+__device__ __forceinline__ int xcd_swizzle(int b, int nb) {
+  const int x = b & 7, l = b >> 3, per = nb >> 3, rem = nb & 7;
+  return x * per + (x < rem ? x : rem) + l;
+}
+
+// ---------------------------------------------------------------------------
+// proposals: one lane per active site, at sweep start (a site's spin and row
+// change only at its own colour step, so this equals proposing there)
+// ---------------------------------------------------------------------------
+template <int LMAX>
+__global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __restrict__ config,
+                                                 double* __restrict__ prop,
+                                                 const ChainDev* __restrict__ st) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= T.n_active) return;
+  const int s = T.color_sites[q];
+  const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
+  double e2[3], u, z[NLM];
+  sce::keyed_proposal(st->seed, s + 1, (int32_t)(st->sweep_index + 1), e, st->step, e2, &u);
+  sce::zlm_row<LMAX>(e2[0], e2[1], e2[2], z);
+  double* o = prop + (size_t)q * PS;
+  o[0] = e2[0];
+  o[1] = e2[1];
+  o[2] = e2[2];
+  o[3] = u;
+#pragma unroll
+  for (int k = 0; k < NLM; ++k) o[4 + k] = z[k];
+}
+
+// Stage the neighbour rows of site s into Z (nb*NLM + row) and 1.0 at NB*NLM.
+template <int L, int NLM>
+__device__ __forceinline__ void stage_neighbours(const DevTables& T, const double* __restrict__ zrows,
+                                                 int s, int lane, double* Z, int* atom_out) {
+  const int atom = s % T.na;
+  const int cell = s / T.na;
+  const int c1 = cell % T.d0;
+  const int r = cell / T.d0;
+  const int c2 = r % T.d1;
+  const int c3 = r / T.d1;
+  const int nb0 = T.nb_ptr[atom];
+  const int NB = T.nb_ptr[atom + 1] - nb0;
+  for (int idx = lane; idx < NB * NLM; idx += L) {
+    const int nbi = idx / NLM;
+    const int row = idx - nbi * NLM;
+    const int4 o = T.nb[nb0 + nbi];
+    int x = c1 + o.y, y = c2 + o.z, z = c3 + o.w;
+    if (x >= T.d0) x -= T.d0;
+    if (y >= T.d1) y -= T.d1;
+    if (z >= T.d2) z -= T.d2;
+    const int n = o.x + T.na * (x + T.d0 * (y + T.d1 * z));
+    Z[idx] = zrows[(size_t)n * NLM + row];
+  }
+  if (lane == 0) Z[NB * NLM] = 1.0;
+  *atom_out = atom;
+}
+
+// ---------------------------------------------------------------------------
+// the colour walk: `L` lanes per site, 256/L sites per block
+// TEST: ΔE only, for caller-chosen sites and rows (parity hook)
+// ---------------------------------------------------------------------------
+template <int L, int LMAX, bool TEST>
+__global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__ config,
+                                               double* __restrict__ zrows,
+                                               const double* __restrict__ prop,
+                                               double* __restrict__ dE, int* __restrict__ acc,
+                                               const ChainDev* __restrict__ st, int q_lo,
+                                               int q_n, const int* __restrict__ test_sites) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4, G = TPB / L;
+  extern __shared__ double lds[];
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
+  const bool valid = qi < q_n;
+  const int q = q_lo + qi;
+  const int stride = T.nb_max * NLM + 1 + NLM;
+  double* Z = lds + g * stride;
+  double* dz = Z + T.nb_max * NLM + 1;
+  int s = 0, atom = 0;
+  if (valid) {
+    s = TEST ? test_sites[q] : T.color_sites[q];
+    stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+    const double* pz = prop + (size_t)q * PS + 4;
+    for (int k = lane; k < NLM; k += L) dz[k] = pz[k] - zrows[(size_t)s * NLM + k];
+  }
+  __syncthreads();
+  double a = 0.0;
+  if (valid) {
+    const int w0 = T.walk_ptr[atom];
+    const int W = T.walk_ptr[atom + 1] - w0;
+    const WalkEntry* __restrict__ wl = T.walk + w0;
+    for (int i = lane; i < W; i += L) {
+      const WalkEntry en = wl[i];
+      const double p = (Z[en.f0] * Z[en.f1]) * Z[en.f2];
+      if (p != 0.0) a = __builtin_fma(en.w * p, dz[en.tgt], a);
+    }
+  }
+#pragma unroll
+  for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
+  if (!valid) return;
+  if (TEST) {
+    if (lane == 0) dE[q] = a;
+    return;
+  }
+  int accept = 0;
+  const double* pq = prop + (size_t)q * PS;
+  if (lane == 0) accept = (a <= 0.0 || pq[3] < sce::det_exp(-st->beta * a)) ? 1 : 0;
+  accept = __shfl(accept, 0, L);
+  if (accept) {
+    for (int k = lane; k < NLM; k += L) zrows[(size_t)s * NLM + k] = pq[4 + k];
+    if (lane == 0) {
+      config[3 * (size_t)s] = pq[0];
+      config[3 * (size_t)s + 1] = pq[1];
+      config[3 * (size_t)s + 2] = pq[2];
+    }
+  }
+  if (lane == 0) {
+    dE[q] = accept ? a : 0.0;
+    acc[q] = accept;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// overrelaxation (updates.jl:66-87): leave-one-out coefficients c by the same
+// walk (lane partials per target row, shuffle-tree fold), reflection about the
+// l = 1 axis h = (c[3], c[1], c[2]), exact ΔE = Σ_k c_k (znew_k - zold_k)
+// TEST: writes c only.
+// ---------------------------------------------------------------------------
+template <int L, int LMAX, bool TEST>
+__global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ config,
+                                            double* __restrict__ zrows, double* __restrict__ dE,
+                                            int* __restrict__ acc, int* __restrict__ att,
+                                            const ChainDev* __restrict__ st, int or_pass,
+                                            int q_lo, int q_n, const int* __restrict__ test_sites,
+                                            double* __restrict__ c_out) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), G = TPB / L;
+  extern __shared__ double lds[];
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
+  const int q = q_lo + qi;
+  int s = 0, atom = 0;
+  bool valid = qi < q_n;
+  if (valid) {
+    s = TEST ? test_sites[q] : T.color_sites[q];
+    if (!TEST && !T.has_l1[s]) {
+      if (lane == 0) {
+        dE[q] = 0.0;
+        acc[q] = 0;
+        att[q] = 0;
+      }
+      valid = false;
+    }
+  }
+  const int stride = T.nb_max * NLM + 1 + NLM;
+  double* Z = lds + g * stride;
+  if (valid) stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+  __syncthreads();
+  double c[NLM];
+#pragma unroll
+  for (int k = 0; k < NLM; ++k) c[k] = 0.0;
+  if (valid) {
+    const int w0 = T.walk_ptr[atom];
+    const int W = T.walk_ptr[atom + 1] - w0;
+    const WalkEntry* __restrict__ wl = T.walk + w0;
+    for (int i = lane; i < W; i += L) {
+      const WalkEntry en = wl[i];
+      const double p = (Z[en.f0] * Z[en.f1]) * Z[en.f2];
+      if (p != 0.0) {
+        const double v = en.w * p;
+#pragma unroll
+        for (int k = 0; k < NLM; ++k)
+          if (k == en.tgt) c[k] = c[k] + v;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = L / 2; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < NLM; ++k) c[k] = c[k] + __shfl_down(c[k], off, L);
+  }
+  if (!valid || lane != 0) return;
+  if (TEST) {
+#pragma unroll
+    for (int k = 0; k < NLM; ++k) c_out[(size_t)q * NLM + k] = c[k];
+    return;
+  }
+  int at = 0, ac = 0;
+  double d_e = 0.0;
+  if constexpr (LMAX >= 1) {
+    const double h0 = c[3], h1 = c[1], h2 = c[2];
+    const double hn = __builtin_sqrt((h0 * h0 + h1 * h1) + h2 * h2);
+    if (!(hn < 1e-12)) {
+      at = 1;
+      const double ax[3] = {h0 / hn, h1 / hn, h2 / hn};
+      double* e = config + 3 * (size_t)s;
+      const double dd = (e[0] * ax[0] + e[1] * ax[1]) + e[2] * ax[2];
+      const double tt = 2.0 * dd;
+      const double e2[3] = {tt * ax[0] - e[0], tt * ax[1] - e[1], tt * ax[2] - e[2]};
+      double zn[NLM];
+      sce::zlm_row<LMAX>(e2[0], e2[1], e2[2], zn);
+      double* zo = zrows + (size_t)s * NLM;
+      double dEv = 0.0;
+#pragma unroll
+      for (int k = 0; k < NLM; ++k) {
+        if (c[k] == 0.0) continue;
+        dEv += c[k] * (zn[k] - zo[k]);
+      }
+      const sce::u32x4 b = sce::philox_block(st->seed, s + 1, (int32_t)st->sweep_index,
+                                             sce::SLOT_OR0 + (uint32_t)or_pass, 0u);
+      const double u = sce::uniform01(b.x, b.y);
+      if (dEv <= 0.0 || u < sce::det_exp(-st->beta * dEv)) {
+        ac = 1;
+        d_e = dEv;
+        e[0] = e2[0];
+        e[1] = e2[1];
+        e[2] = e2[2];
+#pragma unroll
+        for (int k = 0; k < NLM; ++k) zo[k] = zn[k];
+      }
+    }
+  }
+  dE[q] = d_e;
+  acc[q] = ac;
+  att[q] = at;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic tree reduction: chunks of 512 (zero padded) folded by offsets
+// 256..1; a multi-chunk input recurses over the chunk sums in the last block.
+// ---------------------------------------------------------------------------
+enum { MODE_METRO = 0, MODE_OR = 1, MODE_RENORM = 2, MODE_TOTAL = 3, MODE_MEASURE = 4, MODE_TEST = 5 };
+
+__device__ double block_tree(double x0, double x1, double* sh) {
+  // 256 threads, v[t] = x0 + x1 is the offset-256 step
+  const int t = threadIdx.x;
+  double v = x0 + x1;
+  sh[t] = v;
+  __syncthreads();
+  if (t < 128) sh[t] = v = v + sh[t + 128];
+  __syncthreads();
+  if (t < 64) v = v + sh[t + 64];
+  if (t < 64) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_down(v, off, 64);
+  }
+  __syncthreads();
+  return v;  // valid in thread 0
+}
+
+template <int MODE, int NC>
+__global__ __launch_bounds__(256) void k_tree(const double* __restrict__ x, long long n,
+                                              const int* __restrict__ iacc,
+                                              const int* __restrict__ iatt,
+                                              const uint8_t* __restrict__ active,
+                                              double* __restrict__ partial, ChainDev* st,
+                                              double* __restrict__ out, long long n_att_fixed,
+                                              double n_div) {
+  __shared__ double sh[256];
+  __shared__ double res[NC];
+  __shared__ unsigned int ticket_sh;
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * TREE_B;
+  const long long i0 = base + t, i1 = base + t + 256;
+  long long ci = 0, ca = 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    double x0 = 0.0, x1 = 0.0;
+    if (MODE == MODE_MEASURE) {
+      if (i0 < n && active[i0]) x0 = x[3 * i0 + c];
+      if (i1 < n && active[i1]) x1 = x[3 * i1 + c];
+    } else {
+      if (i0 < n) x0 = x[i0];
+      if (i1 < n) x1 = x[i1];
+    }
+    const double v = block_tree(x0, x1, sh);
+    if (t == 0) res[c] = v;
+  }
+  if (MODE == MODE_METRO || MODE == MODE_OR) {
+    if (i0 < n) ci += iacc[i0];
+    if (i1 < n) ci += iacc[i1];
+    if (MODE == MODE_OR) {
+      if (i0 < n) ca += iatt[i0];
+      if (i1 < n) ca += iatt[i1];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      ci += __shfl_down(ci, off, 64);
+      ca += __shfl_down(ca, off, 64);
+    }
+    if ((t & 63) == 0) {
+      if (ci) atomicAdd(&st->iacc, (unsigned long long)ci);
+      if (ca) atomicAdd(&st->iatt, (unsigned long long)ca);
+    }
+  }
+  __syncthreads();
+  double fin[NC];
+  if (gridDim.x == 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) fin[c] = res[c];
+  } else {
+    if (t == 0) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) partial[(size_t)c * gridDim.x + blockIdx.x] = res[c];
+    }
+    __threadfence();
+    if (t == 0) ticket_sh = atomicAdd(&st->ticket, 1u);
+    __syncthreads();
+    if (ticket_sh != gridDim.x - 1) return;
+    __threadfence();
+    // last block: reduce the chunk sums level by level
+    volatile double* vp = partial;
+    long long R = 2;  // per-component room for the higher levels
+    for (long long m = (gridDim.x + TREE_B - 1) / TREE_B;; m = (m + TREE_B - 1) / TREE_B) {
+      R += m + 2;
+      if (m <= 1) break;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      long long m = gridDim.x;
+      size_t src = (size_t)c * gridDim.x;
+      size_t dst = (size_t)NC * gridDim.x + (size_t)c * (size_t)R;
+      while (true) {
+        if (m <= TREE_B) {
+          const double a0 = t < m ? vp[src + t] : 0.0;
+          const double a1 = t + 256 < m ? vp[src + t + 256] : 0.0;
+          const double v = block_tree(a0, a1, sh);
+          if (t == 0) res[c] = v;
+          __syncthreads();
+          break;
+        }
+        const long long nb = (m + TREE_B - 1) / TREE_B;
+        for (long long b = 0; b < nb; ++b) {
+          const long long j0 = b * TREE_B + t, j1 = j0 + 256;
+          const double a0 = j0 < m ? vp[src + j0] : 0.0;
+          const double a1 = j1 < m ? vp[src + j1] : 0.0;
+          const double v = block_tree(a0, a1, sh);
+          if (t == 0) vp[dst + b] = v;
+          __syncthreads();
+        }
+        __threadfence_block();
+        src = dst;
+        dst = dst + (size_t)nb;
+        m = nb;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) fin[c] = res[c];
+  }
+  if (t != 0) return;
+  if (MODE == MODE_METRO) {
+    const long long nacc = (long long)atomicExch(&st->iacc, 0ull);
+    st->energy = st->energy + fin[0];
+    st->acc_metro += nacc;
+    st->att_metro += n_att_fixed;
+    st->nacc_last = nacc;
+    st->sweep_index += 1;
+  } else if (MODE == MODE_OR) {
+    const long long nacc = (long long)atomicExch(&st->iacc, 0ull);
+    const long long natt = (long long)atomicExch(&st->iatt, 0ull);
+    st->energy = st->energy + fin[0];
+    st->acc_or += nacc;
+    st->att_or += natt;
+    st->nacc_last = nacc;
+  } else if (MODE == MODE_RENORM) {
+    const double E = fin[0];
+    const double drift = fabs(st->energy - E);
+    st->last_drift = drift;
+    if (drift > st->max_drift) st->max_drift = drift;
+    st->energy = E;
+  } else if (MODE == MODE_TOTAL) {
+    st->e_full = fin[0];
+  } else if (MODE == MODE_MEASURE) {
+    const int row = st->n_obs;
+    out[4 * (size_t)row] = st->energy;
+    out[4 * (size_t)row + 1] = fin[0] / n_div;
+    out[4 * (size_t)row + 2] = fin[1] / n_div;
+    out[4 * (size_t)row + 3] = fin[2] / n_div;
+    st->n_obs = row + 1;
+  } else {
+    out[0] = fin[0];
+  }
+  st->ticket = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// renormalisation + full energy (state.jl:168-185, energy.jl:55-75)
+// ---------------------------------------------------------------------------
+template <int LMAX>
+__global__ void k_renorm(DevTables T, double* __restrict__ config, double* __restrict__ zrows) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= T.n_sites || !T.active[s]) return;
+  double* e = config + 3 * (size_t)s;
+  const double n = __builtin_sqrt((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]);
+  const double u0 = e[0] / n, u1 = e[1] / n, u2 = e[2] / n;
+  e[0] = u0;
+  e[1] = u1;
+  e[2] = u2;
+  double z[NLM];
+  sce::zlm_row<LMAX>(u0, u1, u2, z);
+#pragma unroll
+  for (int k = 0; k < NLM; ++k) zrows[(size_t)s * NLM + k] = z[k];
+}
+
+__global__ void k_inst_energy(DevTables T, const double* __restrict__ zrows,
+                              double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.n_inst) return;
+  const int k = T.inst_term[i];
+  const int off = T.inst_ptr[i];
+  double Ei = 0.0;
+  for (int e = T.eprog_ptr[k]; e < T.eprog_ptr[k + 1]; ++e) {
+    double p = 1.0;
+    const int f0 = T.efac_ptr[e];
+    const int nf = T.efac_ptr[e + 1] - f0;
+    for (int m = 0; m < nf; ++m)
+      p *= zrows[(size_t)T.inst_sites[off + m] * T.nlm + T.efac_row[f0 + m]];
+    Ei += T.eent_w[e] * p;
+  }
+  out[i] = T.term_coef[k] * Ei;
+}
+
+__global__ void k_set_params(ChainDev* st, double beta, double step, int set_step) {
+  st->beta = beta;
+  if (set_step) st->step = step;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics kernels
+// ---------------------------------------------------------------------------
+__global__ void k_test_philox(long long n, const uint64_t* seeds, const uint32_t* ctr,
+                              uint32_t* out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sce::u32x4 c{ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]};
+  const sce::u32x4 r = sce::philox4x32(c, (uint32_t)seeds[i], (uint32_t)(seeds[i] >> 32));
+  out[4 * i] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+__global__ void k_test_detmath(long long n, int op, const double* x, double* out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s, c;
+  switch (op) {
+    case 0: out[i] = sce::det_exp(x[i]); break;
+    case 1: out[i] = sce::det_log(x[i]); break;
+    case 2: sce::det_sincos(x[i], &s, &c); out[i] = s; break;
+    default: sce::det_sincos(x[i], &s, &c); out[i] = c; break;
+  }
+}
+
+template <int LMAX>
+__global__ void k_test_zlm(long long n, const double* dirs, double* out) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double z[NLM];
+  sce::zlm_row<LMAX>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2], z);
+  for (int k = 0; k < NLM; ++k) out[i * NLM + k] = z[k];
+}
+
+__global__ void k_test_proposal(long long n, uint64_t seed, const int32_t* sites,
+                                const int32_t* sweeps, const double* e3, double step,
+                                double* e2o, double* uo) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double e[3] = {e3[3 * i], e3[3 * i + 1], e3[3 * i + 2]};
+  double e2[3], u;
+  sce::keyed_proposal(seed, sites[i], sweeps[i], e, step, e2, &u);
+  e2o[3 * i] = e2[0];
+  e2o[3 * i + 1] = e2[1];
+  e2o[3 * i + 2] = e2[2];
+  uo[i] = u;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  std::vector<void*> ptrs;
+  ~DevBuf() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+template <class T>
+static int32_t dalloc(DevBuf& b, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return sce_fail(e == hipErrorOutOfMemory ? SCE_ENOMEM : SCE_EHIP, "hipMalloc(%zu): %s",
+                    n * sizeof(T), hipGetErrorString(e));
+  b.ptrs.push_back(*p);
+  return SCE_OK;
+}
+
+template <class T>
+static int32_t dupload(DevBuf& b, T** p, const T* src, size_t n) {
+  CHK(dalloc(b, p, n));
+  if (n) HIPCHK(hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+struct sce_tables {
+  DevBuf mem;
+  DevTables dev{};
+  std::vector<int32_t> color_ptr;  // 0-based offsets, n_colors + 1
+  int n_colors = 0;
+  int64_t total_walk = 0;
+  int w_max = 0;
+};
+
+struct sce_chain {
+  sce_tables* t = nullptr;
+  DevBuf mem;
+  double *config = nullptr, *zrows = nullptr, *prop = nullptr, *dE = nullptr, *partial = nullptr,
+         *inst_e = nullptr, *obs = nullptr;
+  int *acc = nullptr, *att = nullptr;
+  ChainDev* st = nullptr;
+  size_t obs_cap = 0;
+  hipStream_t stream = nullptr;
+  long long sweeps_done = 0;  // host mirror of sweep_index
+  double beta_dev = std::nan("");
+  // timing of the last sce_run_sweeps
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  bool timing = false;
+  bool timing_enabled = false;
+  double total_ms = 0.0, walk_ms = 0.0;
+  long long walk_launches = 0;
+  ~sce_chain() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static size_t partial_capacity(long long n) {
+  size_t cap = 16;
+  long long m = (n + TREE_B - 1) / TREE_B;
+  while (m > 1) {
+    cap += (size_t)m + 2;
+    m = (m + TREE_B - 1) / TREE_B;
+  }
+  return 3 * cap + 16;
+}
+
+extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out) {
+  if (!d || !out) return sce_fail(SCE_EINVAL, "sce_tables_create: NULL argument");
+  *out = nullptr;
+  if (d->lmax < 0 || d->lmax > 6)
+    return sce_fail(SCE_EINVAL, "lmax = %d unsupported on the device path (<= 6)", d->lmax);
+  if (d->nlm != (d->lmax + 1) * (d->lmax + 1)) return sce_fail(SCE_EDIM, "nlm != (lmax+1)^2");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return sce_fail(SCE_ENODEV, "no HIP device available");
+  HostLayout L;
+  CHK(sce_derive_layout(d, &L));
+  auto* t = new sce_tables();
+  auto fail = [&](int32_t r) {
+    delete t;
+    return r;
+  };
+  DevTables& D = t->dev;
+  D.n_sites = d->n_sites;
+  D.n_active = d->n_active;
+  D.nlm = d->nlm;
+  D.lmax = d->lmax;
+  D.na = d->n_cell_atoms;
+  D.d0 = d->dims[0];
+  D.d1 = d->dims[1];
+  D.d2 = d->dims[2];
+  D.nb_max = L.nb_max;
+  D.n_inst = d->n_inst;
+  std::vector<int32_t> cs(d->n_active);
+  for (int q = 0; q < d->n_active; ++q) cs[q] = d->color_sites[q] - 1;
+  int32_t r;
+  int32_t* p32;
+  if ((r = dupload(t->mem, &p32, cs.data(), cs.size())) != SCE_OK) return fail(r);
+  D.color_sites = p32;
+  uint8_t* p8;
+  if ((r = dupload(t->mem, &p8, d->site_has_l1, (size_t)d->n_sites)) != SCE_OK) return fail(r);
+  D.has_l1 = p8;
+  if ((r = dupload(t->mem, &p8, d->site_active, (size_t)d->n_sites)) != SCE_OK) return fail(r);
+  D.active = p8;
+  if ((r = dupload(t->mem, &p32, L.walk_ptr.data(), L.walk_ptr.size())) != SCE_OK) return fail(r);
+  D.walk_ptr = p32;
+  WalkEntry* pw;
+  if ((r = dupload(t->mem, &pw, L.walk.data(), L.walk.size())) != SCE_OK) return fail(r);
+  D.walk = pw;
+  if ((r = dupload(t->mem, &p32, L.nb_ptr.data(), L.nb_ptr.size())) != SCE_OK) return fail(r);
+  D.nb_ptr = p32;
+  int4* pn;
+  if ((r = dupload(t->mem, &pn, reinterpret_cast<const int4*>(L.nb.data()), L.nb.size() / 4)) != SCE_OK)
+    return fail(r);
+  D.nb = pn;
+  // energy tables, 0-based
+  std::vector<int32_t> v;
+  v.resize(d->n_inst);
+  for (int i = 0; i < d->n_inst; ++i) v[i] = d->inst_term[i] - 1;
+  if ((r = dupload(t->mem, &p32, v.data(), v.size())) != SCE_OK) return fail(r);
+  D.inst_term = p32;
+  v.resize((size_t)d->n_inst + 1);
+  for (int i = 0; i <= d->n_inst; ++i) v[i] = d->inst_ptr[i] - 1;
+  if ((r = dupload(t->mem, &p32, v.data(), v.size())) != SCE_OK) return fail(r);
+  D.inst_ptr = p32;
+  v.resize((size_t)d->n_inst_sites);
+  for (int64_t i = 0; i < d->n_inst_sites; ++i) v[i] = d->inst_sites[i] - 1;
+  if ((r = dupload(t->mem, &p32, v.data(), v.size())) != SCE_OK) return fail(r);
+  D.inst_sites = p32;
+  v.resize((size_t)d->n_terms + 1);
+  for (int i = 0; i <= d->n_terms; ++i) v[i] = d->eprog_ptr[i] - 1;
+  if ((r = dupload(t->mem, &p32, v.data(), v.size())) != SCE_OK) return fail(r);
+  D.eprog_ptr = p32;
+  double* pd;
+  if ((r = dupload(t->mem, &pd, d->eent_w, (size_t)d->n_eent)) != SCE_OK) return fail(r);
+  D.eent_w = pd;
+  v.resize((size_t)d->n_eent + 1);
+  for (int64_t i = 0; i <= d->n_eent; ++i) v[i] = d->efac_ptr[i] - 1;
+  if ((r = dupload(t->mem, &p32, v.data(), v.size())) != SCE_OK) return fail(r);
+  D.efac_ptr = p32;
+  v.resize((size_t)d->n_efac);
+  for (int64_t i = 0; i < d->n_efac; ++i) v[i] = d->efac_row[i] - 1;
+  if ((r = dupload(t->mem, &p32, v.data(), v.size())) != SCE_OK) return fail(r);
+  D.efac_row = p32;
+  if ((r = dupload(t->mem, &pd, d->term_coef, (size_t)d->n_terms)) != SCE_OK) return fail(r);
+  D.term_coef = pd;
+  t->n_colors = d->n_colors;
+  t->color_ptr.resize((size_t)d->n_colors + 1);
+  for (int c = 0; c <= d->n_colors; ++c) t->color_ptr[c] = d->color_ptr[c] - 1;
+  t->total_walk = (int64_t)L.walk.size();
+  t->w_max = L.w_max;
+  *out = t;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_layout_check(const sce_tables_desc* d, int64_t o[4]) {
+  if (!d || !o) return sce_fail(SCE_EINVAL, "sce_layout_check: NULL argument");
+  HostLayout L;
+  CHK(sce_derive_layout(d, &L));
+  o[0] = L.na;
+  o[1] = L.nb_max;
+  o[2] = L.w_max;
+  o[3] = (int64_t)L.walk.size();
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_tables_destroy(sce_tables* t) {
+  delete t;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_tables_layout_info(const sce_tables* t, int64_t o[4]) {
+  if (!t || !o) return sce_fail(SCE_EINVAL, "NULL argument");
+  o[0] = t->dev.na;
+  o[1] = t->dev.nb_max;
+  o[2] = t->w_max;
+  o[3] = t->total_walk;
+  return SCE_OK;
+}
+
+// ---- launch helpers --------------------------------------------------------
+static size_t walk_lds_bytes(const DevTables& D, int L) {
+  return (size_t)(TPB / L) * (size_t)(D.nb_max * D.nlm + 1 + D.nlm) * sizeof(double);
+}
+
+static int32_t check_lanes(int L) {
+  if (L != 16 && L != 32 && L != 64)
+    return sce_fail(SCE_EINVAL, "lanes must be 16, 32 or 64 (got %d)", L);
+  return SCE_OK;
+}
+
+static int32_t check_lds(const DevTables& D, int L) {
+  if (walk_lds_bytes(D, L) > 65536)
+    return sce_fail(SCE_EINVAL, "neighbour stencil of %d sites needs %zu B of LDS per block at "
+                    "lanes = %d; use more lanes", D.nb_max, walk_lds_bytes(D, L), L);
+  return SCE_OK;
+}
+
+#define SCE_DISPATCH_LMAX(lmax, F, ...)                  \
+  switch (lmax) {                                        \
+    case 0: F<0>(__VA_ARGS__); break;                    \
+    case 1: F<1>(__VA_ARGS__); break;                    \
+    case 2: F<2>(__VA_ARGS__); break;                    \
+    case 3: F<3>(__VA_ARGS__); break;                    \
+    case 4: F<4>(__VA_ARGS__); break;                    \
+    case 5: F<5>(__VA_ARGS__); break;                    \
+    default: F<6>(__VA_ARGS__); break;                   \
+  }
+
+template <int LMAX>
+static void launch_propose(const DevTables& D, const double* config, double* prop,
+                           const ChainDev* st, hipStream_t s) {
+  const int nb = (D.n_active + 255) / 256;
+  if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st);
+}
+
+template <int L, int LMAX, bool TEST>
+static void launch_metro_l(const DevTables& D, double* config, double* zrows, const double* prop,
+                           double* dE, int* acc, const ChainDev* st, int q_lo, int q_n,
+                           const int* ts, hipStream_t s) {
+  constexpr int G = TPB / L;
+  const int nb = (q_n + G - 1) / G;
+  if (nb) k_metro<L, LMAX, TEST><<<nb, TPB, walk_lds_bytes(D, L), s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts);
+}
+
+template <int LMAX, bool TEST>
+static void launch_metro(int L, const DevTables& D, double* config, double* zrows,
+                         const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo,
+                         int q_n, const int* ts, hipStream_t s) {
+  if (L == 16) launch_metro_l<16, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s);
+  else if (L == 32) launch_metro_l<32, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s);
+  else launch_metro_l<64, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s);
+}
+template <int LMAX> static void launch_metro_run(int L, const DevTables& D, double* config, double* zrows, const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo, int q_n, hipStream_t s) {
+  launch_metro<LMAX, false>(L, D, config, zrows, prop, dE, acc, st, q_lo, q_n, nullptr, s);
+}
+template <int LMAX> static void launch_metro_test(int L, const DevTables& D, double* zrows, const double* prop, double* dE, int q_n, const int* ts, hipStream_t s) {
+  launch_metro<LMAX, true>(L, D, nullptr, zrows, prop, dE, nullptr, nullptr, 0, q_n, ts, s);
+}
+
+template <int L, int LMAX, bool TEST>
+static void launch_or_l(const DevTables& D, double* config, double* zrows, double* dE, int* acc,
+                        int* att, const ChainDev* st, int pass, int q_lo, int q_n, const int* ts,
+                        double* cout, hipStream_t s) {
+  constexpr int G = TPB / L;
+  const int nb = (q_n + G - 1) / G;
+  if (nb) k_or<L, LMAX, TEST><<<nb, TPB, walk_lds_bytes(D, L), s>>>(D, config, zrows, dE, acc, att, st, pass, q_lo, q_n, ts, cout);
+}
+template <int LMAX, bool TEST>
+static void launch_or(int L, const DevTables& D, double* config, double* zrows, double* dE,
+                      int* acc, int* att, const ChainDev* st, int pass, int q_lo, int q_n,
+                      const int* ts, double* cout, hipStream_t s) {
+  if (L == 16) launch_or_l<16, LMAX, TEST>(D, config, zrows, dE, acc, att, st, pass, q_lo, q_n, ts, cout, s);
+  else if (L == 32) launch_or_l<32, LMAX, TEST>(D, config, zrows, dE, acc, att, st, pass, q_lo, q_n, ts, cout, s);
+  else launch_or_l<64, LMAX, TEST>(D, config, zrows, dE, acc, att, st, pass, q_lo, q_n, ts, cout, s);
+}
+template <int LMAX> static void launch_or_run(int L, const DevTables& D, double* config, double* zrows, double* dE, int* acc, int* att, const ChainDev* st, int pass, int q_lo, int q_n, hipStream_t s) {
+  launch_or<LMAX, false>(L, D, config, zrows, dE, acc, att, st, pass, q_lo, q_n, nullptr, nullptr, s);
+}
+template <int LMAX> static void launch_or_test(int L, const DevTables& D, double* zrows, int q_n, const int* ts, double* cout, hipStream_t s) {
+  launch_or<LMAX, true>(L, D, nullptr, zrows, nullptr, nullptr, nullptr, nullptr, 0, 0, q_n, ts, cout, s);
+}
+
+template <int LMAX>
+static void launch_renorm(const DevTables& D, double* config, double* zrows, hipStream_t s) {
+  const int nb = (D.n_sites + 255) / 256;
+  if (nb) k_renorm<LMAX><<<nb, 256, 0, s>>>(D, config, zrows);
+}
+
+template <int MODE, int NC>
+static void launch_tree(const double* x, long long n, const int* iacc, const int* iatt,
+                        const uint8_t* active, double* partial, ChainDev* st, double* out,
+                        long long n_att, double n_div, hipStream_t s) {
+  long long nb = (n + TREE_B - 1) / TREE_B;
+  if (nb < 1) nb = 1;
+  k_tree<MODE, NC><<<(unsigned)nb, 256, 0, s>>>(x, n, iacc, iatt, active, partial, st, out, n_att, n_div);
+}
+
+static int32_t chain_check(sce_chain* c) {
+  if (!c || !c->t) return sce_fail(SCE_EINVAL, "NULL chain");
+  return SCE_OK;
+}
+
+static hipEvent_t next_event(sce_chain* c) {
+  if (c->ev_used == c->ev.size()) {
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    c->ev.push_back(e);
+  }
+  return c->ev[c->ev_used++];
+}
+
+static int32_t set_beta(sce_chain* c, double beta) {
+  if (!(std::isfinite(beta) && beta >= 0.0)) return sce_fail(SCE_EINVAL, "beta must be finite and >= 0");
+  if (!(c->beta_dev == beta)) {
+    k_set_params<<<1, 1, 0, c->stream>>>(c->st, beta, 0.0, 0);
+    c->beta_dev = beta;
+  }
+  return SCE_OK;
+}
+
+// Enqueue one Metropolis sweep (no synchronisation).
+static int32_t enqueue_metro(sce_chain* c, int L) {
+  CHK(check_lds(c->t->dev, L));
+  if (c->sweeps_done >= 2147483647LL - 1)
+    return sce_fail(SCE_ERANGE_COUNTER, "sweep_index exhausted the 32-bit RNG counter word");
+  const DevTables& D = c->t->dev;
+  hipStream_t s = c->stream;
+  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
+  for (int col = 0; col < c->t->n_colors; ++col) {
+    const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
+    if (n == 0) continue;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+      e0 = next_event(c);
+      (void)hipEventRecord(e0, s);
+    }
+    SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n, s);
+    if (c->timing) {
+      e1 = next_event(c);
+      (void)hipEventRecord(e1, s);
+    }
+  }
+  launch_tree<MODE_METRO, 1>(c->dE, D.n_active, c->acc, nullptr, nullptr, c->partial, c->st, nullptr,
+                             D.n_active, 0.0, s);
+  c->sweeps_done += 1;
+  return SCE_OK;
+}
+
+static int32_t enqueue_or(sce_chain* c, int L, int pass) {
+  const DevTables& D = c->t->dev;
+  CHK(check_lds(D, L));
+  hipStream_t s = c->stream;
+  for (int col = 0; col < c->t->n_colors; ++col) {
+    const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
+    if (n == 0) continue;
+    SCE_DISPATCH_LMAX(D.lmax, launch_or_run, L, D, c->config, c->zrows, c->dE, c->acc, c->att, c->st, pass, lo, n, s);
+  }
+  launch_tree<MODE_OR, 1>(c->dE, D.n_active, c->acc, c->att, nullptr, c->partial, c->st, nullptr, 0,
+                          0.0, s);
+  return SCE_OK;
+}
+
+static void enqueue_energy(sce_chain* c, int mode) {
+  const DevTables& D = c->t->dev;
+  const int nb = (D.n_inst + 255) / 256;
+  if (nb) k_inst_energy<<<nb, 256, 0, c->stream>>>(D, c->zrows, c->inst_e);
+  if (mode == MODE_RENORM)
+    launch_tree<MODE_RENORM, 1>(c->inst_e, D.n_inst, nullptr, nullptr, nullptr, c->partial, c->st, nullptr, 0, 0.0, c->stream);
+  else
+    launch_tree<MODE_TOTAL, 1>(c->inst_e, D.n_inst, nullptr, nullptr, nullptr, c->partial, c->st, nullptr, 0, 0.0, c->stream);
+}
+
+static void enqueue_renorm(sce_chain* c) {
+  const DevTables& D = c->t->dev;
+  SCE_DISPATCH_LMAX(D.lmax, launch_renorm, D, c->config, c->zrows, c->stream);
+  enqueue_energy(c, MODE_RENORM);
+}
+
+static int32_t read_state(sce_chain* c, ChainDev* h) {
+  HIPCHK(hipMemcpyAsync(h, c->st, sizeof(ChainDev), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_create(sce_tables* t, const double* config, const double* zrows,
+                                    double energy, uint64_t seed, double step, sce_chain** out) {
+  if (!t || !config || !zrows || !out) return sce_fail(SCE_EINVAL, "sce_chain_create: NULL argument");
+  *out = nullptr;
+  if (!(step > 0.0)) return sce_fail(SCE_EINVAL, "step must be > 0; got %g", step);
+  const DevTables& D = t->dev;
+  auto* c = new sce_chain();
+  c->t = t;
+  auto fail = [&](int32_t r) {
+    delete c;
+    return r;
+  };
+  int32_t r;
+  const size_t N = (size_t)D.n_sites, nlm = (size_t)D.nlm, na = (size_t)std::max(D.n_active, 1);
+  if ((r = dupload(c->mem, &c->config, config, 3 * N)) != SCE_OK) return fail(r);
+  if ((r = dupload(c->mem, &c->zrows, zrows, nlm * N)) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->prop, na * (nlm + 4))) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->dE, na)) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->acc, na)) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->att, na)) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->inst_e, (size_t)std::max(D.n_inst, 1))) != SCE_OK) return fail(r);
+  const long long nmax = std::max<long long>({(long long)D.n_active, (long long)D.n_inst, (long long)D.n_sites});
+  if ((r = dalloc(c->mem, &c->partial, partial_capacity(nmax))) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->st, 1)) != SCE_OK) return fail(r);
+  ChainDev h{};
+  h.energy = energy;
+  h.step = step;
+  h.beta = 0.0;
+  h.seed = seed;
+  if (hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(sce_fail(SCE_EHIP, "upload of chain state failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(sce_fail(SCE_EHIP, "hipStreamCreate failed"));
+  c->beta_dev = 0.0;
+  *out = c;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_destroy(sce_chain* c) {
+  if (c && c->stream) (void)hipStreamSynchronize(c->stream);
+  delete c;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_info_get(sce_chain* c, sce_chain_info* o) {
+  CHK(chain_check(c));
+  if (!o) return sce_fail(SCE_EINVAL, "NULL info");
+  ChainDev h;
+  CHK(read_state(c, &h));
+  o->energy = h.energy;
+  o->step = h.step;
+  o->max_drift = h.max_drift;
+  o->last_drift = h.last_drift;
+  o->sweep_index = h.sweep_index;
+  o->acc_metro = h.acc_metro;
+  o->att_metro = h.att_metro;
+  o->acc_or = h.acc_or;
+  o->att_or = h.att_or;
+  o->seed = h.seed;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_set_step(sce_chain* c, double step) {
+  CHK(chain_check(c));
+  if (!(step > 0.0)) return sce_fail(SCE_EINVAL, "step must be > 0");
+  const double b = std::isnan(c->beta_dev) ? 0.0 : c->beta_dev;
+  k_set_params<<<1, 1, 0, c->stream>>>(c->st, b, step, 1);
+  HIPCHK(hipGetLastError());
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_reset_counters(sce_chain* c) {
+  CHK(chain_check(c));
+  ChainDev h;
+  CHK(read_state(c, &h));
+  h.acc_metro = h.att_metro = h.acc_or = h.att_or = 0;
+  h.max_drift = 0.0;
+  HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_download(sce_chain* c, double* config, double* zrows, double* energy) {
+  CHK(chain_check(c));
+  const DevTables& D = c->t->dev;
+  if (config)
+    HIPCHK(hipMemcpyAsync(config, c->config, 3 * (size_t)D.n_sites * 8, hipMemcpyDeviceToHost, c->stream));
+  if (zrows)
+    HIPCHK(hipMemcpyAsync(zrows, c->zrows, (size_t)D.nlm * D.n_sites * 8, hipMemcpyDeviceToHost, c->stream));
+  ChainDev h;
+  CHK(read_state(c, &h));
+  if (energy) *energy = h.energy;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_upload(sce_chain* c, const double* config, const double* zrows, double energy) {
+  CHK(chain_check(c));
+  const DevTables& D = c->t->dev;
+  ChainDev h;
+  CHK(read_state(c, &h));
+  if (config) HIPCHK(hipMemcpy(c->config, config, 3 * (size_t)D.n_sites * 8, hipMemcpyHostToDevice));
+  if (zrows) HIPCHK(hipMemcpy(c->zrows, zrows, (size_t)D.nlm * D.n_sites * 8, hipMemcpyHostToDevice));
+  h.energy = energy;
+  HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_swap_payload(sce_chain* a, sce_chain* b) {
+  CHK(chain_check(a));
+  CHK(chain_check(b));
+  if (a->t != b->t) return sce_fail(SCE_EINVAL, "swap_payload: chains on different tables");
+  HIPCHK(hipStreamSynchronize(a->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  ChainDev ha, hb;
+  CHK(read_state(a, &ha));
+  CHK(read_state(b, &hb));
+  std::swap(a->config, b->config);
+  std::swap(a->zrows, b->zrows);
+  std::swap(ha.energy, hb.energy);
+  HIPCHK(hipMemcpy(a->st, &ha, sizeof ha, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->st, &hb, sizeof hb, hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_metropolis_sweep(sce_chain* c, double beta, int32_t lanes, int64_t* nacc) {
+  CHK(chain_check(c));
+  CHK(check_lanes(lanes));
+  CHK(set_beta(c, beta));
+  c->timing = false;
+  CHK(enqueue_metro(c, lanes));
+  HIPCHK(hipGetLastError());
+  ChainDev h;
+  CHK(read_state(c, &h));
+  if (nacc) *nacc = h.nacc_last;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_overrelaxation_sweep(sce_chain* c, double beta, int32_t or_pass,
+                                            int32_t lanes, int64_t* nacc) {
+  CHK(chain_check(c));
+  CHK(check_lanes(lanes));
+  if (or_pass < 0 || or_pass > 1000) return sce_fail(SCE_EINVAL, "or_pass out of range");
+  CHK(set_beta(c, beta));
+  CHK(enqueue_or(c, lanes, or_pass));
+  HIPCHK(hipGetLastError());
+  ChainDev h;
+  CHK(read_state(c, &h));
+  if (nacc) *nacc = h.nacc_last;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, int32_t or_per_metropolis,
+                                  int64_t renorm_interval, int64_t measure_interval, int32_t lanes,
+                                  double* obs, int64_t obs_cap, int64_t* n_obs) {
+  CHK(chain_check(c));
+  CHK(check_lanes(lanes));
+  if (nsweeps < 0) return sce_fail(SCE_EINVAL, "nsweeps must be >= 0");
+  if (or_per_metropolis < 0) return sce_fail(SCE_EINVAL, "or_per_metropolis must be >= 0");
+  const int64_t want_obs = measure_interval > 0 ? nsweeps / measure_interval : 0;
+  if (want_obs > 0 && (!obs || obs_cap < want_obs))
+    return sce_fail(SCE_EDIM, "obs buffer holds %lld rows; %lld needed", (long long)obs_cap, (long long)want_obs);
+  if (c->sweeps_done + nsweeps >= 2147483647LL - 1)
+    return sce_fail(SCE_ERANGE_COUNTER, "sweep_index would exhaust the 32-bit RNG counter word");
+  if (want_obs > 0 && (size_t)want_obs > c->obs_cap) {
+    double* p;
+    CHK(dalloc(c->mem, &p, 4 * (size_t)want_obs));
+    c->obs = p;
+    c->obs_cap = (size_t)want_obs;
+  }
+  CHK(set_beta(c, beta));
+  // reset the observation counter
+  {
+    ChainDev h;
+    CHK(read_state(c, &h));
+    h.n_obs = 0;
+    HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
+  }
+  c->timing = c->timing_enabled;
+  c->ev_used = 0;
+  hipEvent_t t0 = next_event(c);
+  HIPCHK(hipEventRecord(t0, c->stream));
+  const DevTables& D = c->t->dev;
+  for (int64_t i = 1; i <= nsweeps; ++i) {
+    CHK(enqueue_metro(c, lanes));
+    for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
+    if (renorm_interval > 0 && i % renorm_interval == 0) enqueue_renorm(c);
+    if (measure_interval > 0 && i % measure_interval == 0)
+      launch_tree<MODE_MEASURE, 3>(c->config, D.n_sites, nullptr, nullptr, D.active, c->partial, c->st,
+                                   c->obs, 0, (double)D.n_active, c->stream);
+  }
+  hipEvent_t t1 = next_event(c);
+  HIPCHK(hipEventRecord(t1, c->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->timing = false;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+  c->total_ms = ms;
+  c->walk_ms = 0.0;
+  c->walk_launches = 0;
+  for (size_t k = 1; k + 1 < c->ev_used; k += 2) {
+    float m = 0.f;
+    HIPCHK(hipEventElapsedTime(&m, c->ev[k], c->ev[k + 1]));
+    c->walk_ms += m;
+    c->walk_launches += 1;
+  }
+  if (want_obs > 0) {
+    HIPCHK(hipMemcpy(obs, c->obs, 4 * (size_t)want_obs * 8, hipMemcpyDeviceToHost));
+  }
+  if (n_obs) *n_obs = want_obs;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_set_timing(sce_chain* c, int32_t on) {
+  CHK(chain_check(c));
+  c->timing_enabled = on != 0;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_timing(sce_chain* c, double* total_ms, double* walk_ms_avg,
+                                    int64_t* walk_launches) {
+  CHK(chain_check(c));
+  if (total_ms) *total_ms = c->total_ms;
+  if (walk_ms_avg) *walk_ms_avg = c->walk_launches ? c->walk_ms / (double)c->walk_launches : 0.0;
+  if (walk_launches) *walk_launches = c->walk_launches;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_renormalize(sce_chain* c, double* drift) {
+  CHK(chain_check(c));
+  enqueue_renorm(c);
+  HIPCHK(hipGetLastError());
+  ChainDev h;
+  CHK(read_state(c, &h));
+  if (drift) *drift = h.last_drift;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_total_energy(sce_chain* c, double* E) {
+  CHK(chain_check(c));
+  enqueue_energy(c, MODE_TOTAL);
+  HIPCHK(hipGetLastError());
+  ChainDev h;
+  CHK(read_state(c, &h));
+  if (E) *E = h.e_full;
+  return SCE_OK;
+}
+
+// ---- diagnostics -------------------------------------------------------------
+template <class T>
+struct Scratch {
+  T* p = nullptr;
+  ~Scratch() {
+    if (p) (void)hipFree(p);
+  }
+  int32_t alloc(size_t n) {
+    hipError_t e = hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T));
+    return e == hipSuccess ? SCE_OK : sce_fail(SCE_EHIP, "hipMalloc: %s", hipGetErrorString(e));
+  }
+  int32_t up(const T* src, size_t n) {
+    CHK(alloc(n));
+    if (n) HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return SCE_OK;
+  }
+  int32_t down(T* dst, size_t n) {
+    if (n) HIPCHK(hipMemcpy(dst, p, n * sizeof(T), hipMemcpyDeviceToHost));
+    return SCE_OK;
+  }
+};
+
+static int32_t need_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return sce_fail(SCE_ENODEV, "no HIP device available");
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_test_philox(int64_t n, const uint64_t* seeds, const uint32_t* ctr4, uint32_t* out4) {
+  CHK(need_device());
+  Scratch<uint64_t> s;
+  Scratch<uint32_t> c, o;
+  CHK(s.up(seeds, n));
+  CHK(c.up(ctr4, 4 * n));
+  CHK(o.alloc(4 * n));
+  if (n) k_test_philox<<<(unsigned)((n + 255) / 256), 256>>>(n, s.p, c.p, o.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return o.down(out4, 4 * n);
+}
+
+extern "C" int32_t sce_test_detmath(int64_t n, int32_t op, const double* x, double* out) {
+  CHK(need_device());
+  Scratch<double> xi, o;
+  CHK(xi.up(x, n));
+  CHK(o.alloc(n));
+  if (n) k_test_detmath<<<(unsigned)((n + 255) / 256), 256>>>(n, op, xi.p, o.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return o.down(out, n);
+}
+
+template <int LMAX>
+static void launch_test_zlm(long long n, const double* d, double* o) {
+  if (n) k_test_zlm<LMAX><<<(unsigned)((n + 255) / 256), 256>>>(n, d, o);
+}
+
+extern "C" int32_t sce_test_zlm(int64_t n, int32_t lmax, const double* dirs3, double* out) {
+  CHK(need_device());
+  if (lmax < 0 || lmax > 6) return sce_fail(SCE_EINVAL, "lmax out of range");
+  const int nlm = (lmax + 1) * (lmax + 1);
+  Scratch<double> d, o;
+  CHK(d.up(dirs3, 3 * n));
+  CHK(o.alloc((size_t)nlm * n));
+  SCE_DISPATCH_LMAX(lmax, launch_test_zlm, n, d.p, o.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return o.down(out, (size_t)nlm * n);
+}
+
+extern "C" int32_t sce_test_proposal(int64_t n, uint64_t seed, const int32_t* sites, const int32_t* sweeps,
+                                     const double* e3, double step, double* e2_out, double* u_out) {
+  CHK(need_device());
+  Scratch<int32_t> si, sw;
+  Scratch<double> e, e2, u;
+  CHK(si.up(sites, n));
+  CHK(sw.up(sweeps, n));
+  CHK(e.up(e3, 3 * n));
+  CHK(e2.alloc(3 * n));
+  CHK(u.alloc(n));
+  if (n) k_test_proposal<<<(unsigned)((n + 255) / 256), 256>>>(n, seed, si.p, sw.p, e.p, step, e2.p, u.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  CHK(e2.down(e2_out, 3 * n));
+  return u.down(u_out, n);
+}
+
+static int32_t check_sites(const sce_tables* t, int64_t n, const int32_t* sites, std::vector<int32_t>& z0) {
+  z0.resize(n);
+  std::vector<uint8_t> act(t->dev.n_sites);
+  HIPCHK(hipMemcpy(act.data(), t->dev.active, act.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; ++i) {
+    if (sites[i] < 1 || sites[i] > t->dev.n_sites) return sce_fail(SCE_EINVAL, "site %d out of range", sites[i]);
+    if (!act[sites[i] - 1]) return sce_fail(SCE_EINVAL, "site %d is inactive", sites[i]);
+    z0[i] = sites[i] - 1;
+  }
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_test_walk_delta(sce_tables* t, const double* zrows, int64_t n, const int32_t* sites,
+                                       const double* znew, int32_t lanes, double* out) {
+  if (!t) return sce_fail(SCE_EINVAL, "NULL tables");
+  CHK(check_lanes(lanes));
+  const DevTables& D = t->dev;
+  CHK(check_lds(D, lanes));
+  std::vector<int32_t> s0;
+  CHK(check_sites(t, n, sites, s0));
+  const int nlm = D.nlm;
+  std::vector<double> prop((size_t)n * (nlm + 4), 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < nlm; ++k) prop[(size_t)i * (nlm + 4) + 4 + k] = znew[(size_t)i * nlm + k];
+  Scratch<double> z, p, o;
+  Scratch<int32_t> si;
+  CHK(z.up(zrows, (size_t)nlm * D.n_sites));
+  CHK(p.up(prop.data(), prop.size()));
+  CHK(o.alloc(n));
+  CHK(si.up(s0.data(), n));
+  SCE_DISPATCH_LMAX(D.lmax, launch_metro_test, lanes, D, z.p, p.p, o.p, (int)n, si.p, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return o.down(out, n);
+}
+
+extern "C" int32_t sce_test_walk_coeffs(sce_tables* t, const double* zrows, int64_t n, const int32_t* sites,
+                                        int32_t lanes, double* out) {
+  if (!t) return sce_fail(SCE_EINVAL, "NULL tables");
+  CHK(check_lanes(lanes));
+  const DevTables& D = t->dev;
+  CHK(check_lds(D, lanes));
+  std::vector<int32_t> s0;
+  CHK(check_sites(t, n, sites, s0));
+  const int nlm = D.nlm;
+  Scratch<double> z, o;
+  Scratch<int32_t> si;
+  CHK(z.up(zrows, (size_t)nlm * D.n_sites));
+  CHK(o.alloc((size_t)n * nlm));
+  CHK(si.up(s0.data(), n));
+  SCE_DISPATCH_LMAX(D.lmax, launch_or_test, lanes, D, z.p, (int)n, si.p, o.p, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return o.down(out, (size_t)n * nlm);
+}
+
+extern "C" int32_t sce_test_reduce(int64_t n, const double* x, double* out) {
+  CHK(need_device());
+  Scratch<double> xi, part, o;
+  Scratch<ChainDev> st;
+  CHK(xi.up(x, n));
+  CHK(part.alloc(partial_capacity(n)));
+  CHK(o.alloc(1));
+  ChainDev h{};
+  CHK(st.up(&h, 1));
+  launch_tree<MODE_TEST, 1>(xi.p, n, nullptr, nullptr, nullptr, part.p, st.p, o.p, 0, 0.0, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return o.down(out, 1);
+}

@@ -1,0 +1,46 @@
+// sce_internal.h — shared internals of libsce_hip.so (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/sce_hip.h"
+#include "sce_math.h"
+
+// Records the thread-local error message and returns `code`.
+int32_t sce_fail(int32_t code, const char* fmt, ...);
+
+// One flattened walk entry of a sublattice program: the weight, the three factor
+// slots of the staged neighbour block (index nb*nlm + row; unused factors point
+// at a slot holding 1.0) and the target row of the site's own tesseral row.
+struct alignas(16) WalkEntry {
+  double w;
+  uint16_t f0, f1, f2, tgt;
+};
+static_assert(sizeof(WalkEntry) == 16, "WalkEntry must be 16 bytes");
+
+// Per-sublattice device layout of a tiled Hamiltonian.
+//
+// For a TiledHamiltonian every site of training-cell atom a touches exactly the
+// same site programs (one per (template, member slot) whose slot sits on atom
+// a), with neighbour sites displaced by fixed cell offsets.  The walk of a site
+// of sublattice a is therefore one shared list: adjacency entries in ascending
+// program id, each expanded into its program entries, with factor columns
+// addressed through a per-sublattice neighbour stencil (atom, cell offset).
+struct HostLayout {
+  int na = 0;
+  int nb_max = 0;   // largest neighbour stencil
+  int w_max = 0;    // largest walk (entries)
+  std::vector<int32_t> walk_ptr;  // na + 1
+  std::vector<WalkEntry> walk;
+  std::vector<int32_t> nb_ptr;    // na + 1
+  std::vector<int32_t> nb;        // 4 ints per neighbour: atom (0-based), dc1, dc2, dc3
+  std::vector<int32_t> walk_len_l1; // unused placeholder for future layouts
+};
+
+// Derives the layout from the reference tables and verifies, for every active
+// site, that its adjacency is the sublattice stencil translated to its cell.
+// Returns SCE_ELAYOUT (with a message) for tables that are not such a tiling.
+int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* out);

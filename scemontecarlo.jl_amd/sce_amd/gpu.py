@@ -1,0 +1,260 @@
+"""Device tables and chains — the drop-in names of the reference's GPU path.
+
+=========================  ==============================================
+reference (Julia)          here
+=========================  ==============================================
+GPUTiledHamiltonian(b, H)  GPUTiledHamiltonian(H)       gpu_hamiltonian.jl:61-88
+GPUChainState(gH, st)      GPUChainState(gH, st, seed)  gpu_state.jl:49-61
+gpu_metropolis_sweep!      gpu_metropolis_sweep         gpu_sweep.jl:158-190
+gpu_run_sweeps!            gpu_run_sweeps               gpu_sweep.jl:204-218
+to_host!                   to_host                      gpu_state.jl:71-76
+_from_host!                from_host                    gpu_state.jl:81-86
+_swap_payload!             swap_payload                 gpu_pt.jl:46-51
+=========================  ==============================================
+
+``workgroupsize`` of the reference is ``lanes`` here: the number of lanes that
+walk one site (16, 32 or 64; default 64 = one wavefront), part of the
+determinism scope exactly like the reference's pinned workgroup size.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .hamiltonian import ChainState, TiledHamiltonian
+
+DEFAULT_LANES = 64
+
+
+class GPUTiledHamiltonian:
+    """Device copy of the tables of ``H`` plus its per-sublattice walk layout."""
+
+    def __init__(self, H: TiledHamiltonian):
+        self.host = H
+        h = C.c_void_p()
+        L.check(L.lib().sce_tables_create(C.byref(H.desc), C.byref(h)))
+        self._h = h
+        info = np.zeros(4, np.int64)
+        L.check(L.lib().sce_tables_layout_info(h, L.ptr(info)))
+        self.n_sublattices, self.nb_max, self.walk_max, self.walk_total = (int(x) for x in info)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n_sites(self) -> int:
+        return self.host.n_sites
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L._lib is not None:
+            L._lib.sce_tables_destroy(h)
+            self._h = None
+
+
+class GPUChainState:
+    """Device-resident chain: config, rows, staged ΔE, and the chain scalars
+    (incremental energy, keyed-RNG seed and sweep index, step, counters) —
+    all kept on the device between calls."""
+
+    def __init__(self, gH: GPUTiledHamiltonian, st: ChainState, seed: Optional[int] = None):
+        if seed is None:
+            seed = int(np.random.default_rng().integers(0, 2**63))
+        self.gH = gH
+        self.seed = int(seed) & (2**64 - 1)
+        cfg = np.ascontiguousarray(st.config, np.float64)
+        z = np.ascontiguousarray(st.zrows, np.float64)
+        h = C.c_void_p()
+        L.check(L.lib().sce_chain_create(gH.handle, L.ptr(cfg), L.ptr(z), float(st.energy),
+                                         C.c_uint64(self.seed), float(st.step), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> L.ChainInfo:
+        inf = L.ChainInfo()
+        L.check(L.lib().sce_chain_info_get(self._h, C.byref(inf)))
+        return inf
+
+    @property
+    def energy(self) -> float:
+        return self.info().energy
+
+    @property
+    def sweep_index(self) -> int:
+        return self.info().sweep_index
+
+    @property
+    def step(self) -> float:
+        return self.info().step
+
+    @step.setter
+    def step(self, v: float) -> None:
+        L.check(L.lib().sce_chain_set_step(self._h, float(v)))
+
+    @property
+    def acc_metro(self) -> int:
+        return self.info().acc_metro
+
+    @property
+    def att_metro(self) -> int:
+        return self.info().att_metro
+
+    def reset_counters(self) -> None:
+        L.check(L.lib().sce_chain_reset_counters(self._h))
+
+    def download(self):
+        H = self.gH.host
+        cfg = np.empty((H.n_sites, 3))
+        z = np.empty((H.n_sites, H.nlm))
+        e = C.c_double()
+        L.check(L.lib().sce_download(self._h, L.ptr(cfg), L.ptr(z), C.byref(e)))
+        return cfg, z, e.value
+
+    def set_timing(self, on: bool) -> None:
+        L.check(L.lib().sce_chain_set_timing(self._h, 1 if on else 0))
+
+    def timing(self):
+        tot, walk, n = C.c_double(), C.c_double(), C.c_int64()
+        L.check(L.lib().sce_chain_timing(self._h, C.byref(tot), C.byref(walk), C.byref(n)))
+        return tot.value, walk.value, n.value
+
+    def total_energy(self) -> float:
+        e = C.c_double()
+        L.check(L.lib().sce_chain_total_energy(self._h, C.byref(e)))
+        return e.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L._lib is not None:
+            L._lib.sce_chain_destroy(h)
+            self._h = None
+
+
+def gpu_metropolis_sweep(gst: GPUChainState, gH: GPUTiledHamiltonian, beta: float,
+                         lanes: int = DEFAULT_LANES) -> int:
+    """One device Metropolis sweep; returns the number of accepted moves."""
+    if gst.gH is not gH:
+        raise ValueError("chain was created on other tables")
+    n = C.c_int64()
+    L.check(L.lib().sce_metropolis_sweep(gst.handle, float(beta), int(lanes), C.byref(n)))
+    return n.value
+
+
+def gpu_overrelaxation_sweep(gst: GPUChainState, gH: GPUTiledHamiltonian, beta: float,
+                             or_pass: int = 0, lanes: int = DEFAULT_LANES) -> int:
+    n = C.c_int64()
+    L.check(L.lib().sce_overrelaxation_sweep(gst.handle, float(beta), int(or_pass), int(lanes),
+                                             C.byref(n)))
+    return n.value
+
+
+def gpu_run_sweeps(gst: GPUChainState, gH: GPUTiledHamiltonian, st: Optional[ChainState],
+                   beta: float, nsweeps: int, renorm_interval: int = 1000,
+                   lanes: int = DEFAULT_LANES, or_per_metropolis: int = 0,
+                   measure_interval: int = 0):
+    """``gpu_run_sweeps!``: ``nsweeps`` sweeps with device renormalisation every
+    ``renorm_interval`` sweeps (<= 0 disables); downloads the final state into
+    ``st`` when given.  Returns the (E, m_x, m_y, m_z) rows of every
+    ``measure_interval``-th sweep (empty when measure_interval <= 0)."""
+    nobs = nsweeps // measure_interval if measure_interval > 0 else 0
+    obs = np.zeros((max(nobs, 1), 4))
+    n = C.c_int64()
+    L.check(L.lib().sce_run_sweeps(gst.handle, float(beta), int(nsweeps), int(or_per_metropolis),
+                                   int(renorm_interval), int(measure_interval), int(lanes),
+                                   L.ptr(obs), int(max(nobs, 1)), C.byref(n)))
+    if st is not None:
+        to_host(st, gst)
+    return obs[: n.value]
+
+
+def gpu_renormalize(gst: GPUChainState) -> float:
+    d = C.c_double()
+    L.check(L.lib().sce_renormalize(gst.handle, C.byref(d)))
+    return d.value
+
+
+def to_host(st: ChainState, gst: GPUChainState) -> ChainState:
+    cfg, z, e = gst.download()
+    st.config[...] = cfg
+    st.zrows[...] = z
+    st.energy = e
+    return st
+
+
+def from_host(gst: GPUChainState, st: ChainState) -> GPUChainState:
+    cfg = np.ascontiguousarray(st.config, np.float64)
+    z = np.ascontiguousarray(st.zrows, np.float64)
+    L.check(L.lib().sce_upload(gst.handle, L.ptr(cfg), L.ptr(z), float(st.energy)))
+    return gst
+
+
+def swap_payload(a: GPUChainState, b: GPUChainState) -> None:
+    L.check(L.lib().sce_swap_payload(a.handle, b.handle))
+
+
+# ---- device arithmetic exposed for parity tests -------------------------------
+
+def dev_philox(seeds, ctrs):
+    seeds = np.ascontiguousarray(seeds, np.uint64)
+    ctrs = np.ascontiguousarray(ctrs, np.uint32).reshape(-1, 4)
+    out = np.zeros_like(ctrs)
+    L.check(L.lib().sce_test_philox(len(seeds), L.ptr(seeds), L.ptr(ctrs), L.ptr(out)))
+    return out
+
+
+def dev_detmath(op: int, x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    L.check(L.lib().sce_test_detmath(len(x), op, L.ptr(x), L.ptr(out)))
+    return out
+
+
+def dev_zlm(dirs, lmax):
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    out = np.zeros((len(d), (lmax + 1) ** 2))
+    L.check(L.lib().sce_test_zlm(len(d), lmax, L.ptr(d), L.ptr(out)))
+    return out
+
+
+def dev_proposal(seed, sites, sweeps, e, step):
+    sites = np.ascontiguousarray(sites, np.int32)
+    sweeps = np.ascontiguousarray(sweeps, np.int32)
+    e = np.ascontiguousarray(e, np.float64).reshape(-1, 3)
+    e2 = np.zeros_like(e)
+    u = np.zeros(len(sites))
+    L.check(L.lib().sce_test_proposal(len(sites), C.c_uint64(seed), L.ptr(sites), L.ptr(sweeps),
+                                      L.ptr(e), float(step), L.ptr(e2), L.ptr(u)))
+    return e2, u
+
+
+def dev_walk_delta(gH: GPUTiledHamiltonian, zrows, sites, znew, lanes):
+    z = np.ascontiguousarray(zrows, np.float64)
+    sites = np.ascontiguousarray(sites, np.int32)
+    zn = np.ascontiguousarray(znew, np.float64)
+    out = np.zeros(len(sites))
+    L.check(L.lib().sce_test_walk_delta(gH.handle, L.ptr(z), len(sites), L.ptr(sites), L.ptr(zn),
+                                        int(lanes), L.ptr(out)))
+    return out
+
+
+def dev_walk_coeffs(gH: GPUTiledHamiltonian, zrows, sites, lanes):
+    z = np.ascontiguousarray(zrows, np.float64)
+    sites = np.ascontiguousarray(sites, np.int32)
+    out = np.zeros((len(sites), gH.host.nlm))
+    L.check(L.lib().sce_test_walk_coeffs(gH.handle, L.ptr(z), len(sites), L.ptr(sites), int(lanes),
+                                         L.ptr(out)))
+    return out
+
+
+def dev_reduce(x):
+    x = np.ascontiguousarray(x, np.float64)
+    o = C.c_double()
+    L.check(L.lib().sce_test_reduce(len(x), L.ptr(x), C.byref(o)))
+    return o.value

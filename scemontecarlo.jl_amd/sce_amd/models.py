@@ -1,0 +1,325 @@
+"""Synthetic SCE model generator — the stand-in for SCEFitting's ``multipole_terms``.
+
+The reference tiles a fitted training-cell model (``MultipoleTerm`` lists from
+SCEFitting, src/hamiltonian.jl:267-400).  SCEFitting is not part of the reference
+tree, so this module writes seeded ``MultipoleTerm`` lists for cubic lattices:
+
+* lattices: simple cubic (1 atom), bcc (2-atom cubic cell), fcc (4-atom cubic cell);
+* one term per physical cluster (SCEFitting's canonical convention), clusters
+  enumerated up to lattice translation with the first member in the home cell
+  (``shifts[0] == 0``, the reference's anchoring rule, hamiltonian.jl:306);
+* body-2 bonds on the first k neighbour shells, one template per (l1, l2) block;
+* bcc body-3 triangles (1NN, 1NN, 2NN) and body-4 tetrahedra (four 1NN and two
+  2NN edges) — the bcc 1NN graph is bipartite, so it has no all-1NN triangle;
+* coefficients: raw ``coef ~ N(0, scale)`` per template (scale = 1 meV by default),
+  and either dense random ``folded`` tensors (entries N(0,1)/sqrt(size)) or the
+  isotropic SALC tensors (identity for (l,l) pairs, Levi-Civita for l=1 triangles,
+  the three pairings for l=1 tetrahedra).
+
+Hand-built fixtures of the reference's unit suite (test/unit/fixtures.jl:94-131)
+are reproduced value for value in :func:`chain_terms` etc.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import dataclass, field
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+
+@dataclass
+class MultipoleTerm:
+    """One raw (unscaled) SCE term: ``coef``, member ``atoms`` (1-based training-cell
+    atoms), per-member integer lattice ``shifts`` (``shifts[0] == (0,0,0)``), angular
+    momenta ``ls`` and the rank-``body`` tensor ``folded`` of shape ``(2l+1, ...)``."""
+
+    coef: float
+    atoms: List[int]
+    shifts: List[Tuple[int, int, int]]
+    ls: List[int]
+    folded: np.ndarray = field(repr=False)
+
+    @property
+    def body(self) -> int:
+        return len(self.atoms)
+
+
+BASIS = {
+    "sc": np.array([[0.0, 0.0, 0.0]]),
+    "bcc": np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5]]),
+    "fcc": np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]]),
+}
+
+FOURPI = 4.0 * math.pi
+N1 = math.sqrt(3.0 / FOURPI)          # Z_1m = N1 * (y, z, x)[m + 2]
+
+
+# ---------------------------------------------------------------------------
+# reference unit-suite fixtures (test/unit/fixtures.jl:94-131, test_hamiltonian.jl)
+# ---------------------------------------------------------------------------
+
+def chain_terms(J: float) -> List[MultipoleTerm]:
+    """Self-image l=1 chain along x (fixtures.jl:94-103): both directed members."""
+    f = np.zeros((3, 3))
+    f[0, 0] = f[1, 1] = f[2, 2] = 1.0
+    raw = J / (2 * N1 ** 2 * FOURPI)
+    z, x, mx = (0, 0, 0), (1, 0, 0), (-1, 0, 0)
+    return [MultipoleTerm(raw, [1, 1], [z, x], [1, 1], f.copy()),
+            MultipoleTerm(raw, [1, 1], [z, mx], [1, 1], f.copy())]
+
+
+def threebody_terms(J: float) -> List[MultipoleTerm]:
+    """3-body chain (0, +x, +2x) — triplet fast path (fixtures.jl:109-117)."""
+    f = np.zeros((3, 3, 3))
+    f[1, 1, 1] = 1.0
+    f[0, 2, 0] = 0.7
+    f[2, 0, 1] = -0.4
+    return [MultipoleTerm(J, [1, 1, 1], [(0, 0, 0), (1, 0, 0), (2, 0, 0)], [1, 1, 1], f)]
+
+
+def fourbody_terms(J: float) -> List[MultipoleTerm]:
+    """4-body chain (0, +x, +2x, +3x) — general branch (fixtures.jl:122-131)."""
+    f = np.zeros((3, 3, 3, 3))
+    f[1, 1, 1, 1] = 1.0
+    f[0, 2, 1, 0] = 0.6
+    f[2, 1, 0, 2] = -0.3
+    return [MultipoleTerm(J, [1, 1, 1, 1], [(0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0)],
+                          [1, 1, 1, 1], f)]
+
+
+def field_term(c0: float) -> List[MultipoleTerm]:
+    """Single-site l=1 term V(e) = c0 N1 e_z (test_hamiltonian.jl:110-122)."""
+    f = np.zeros(3)
+    f[1] = 1.0
+    return [MultipoleTerm(c0 / math.sqrt(FOURPI), [1], [(0, 0, 0)], [1], f)]
+
+
+def dimer_terms(J: float) -> List[MultipoleTerm]:
+    """An isotropic l=1 dimer on atoms 1-2 of a 4-atom cell (atoms 3-4 inactive):
+    E = J e1.e2 — the hand-built analogue of the reference's SCEFitting dimer
+    (fixtures.jl:20-31)."""
+    f = np.eye(3)
+    raw = J / (N1 ** 2 * FOURPI)
+    return [MultipoleTerm(raw, [1, 2], [(0, 0, 0), (0, 0, 0)], [1, 1], f)]
+
+
+# ---------------------------------------------------------------------------
+# lattice clusters
+# ---------------------------------------------------------------------------
+
+def _sites_within(lattice: str, rmax: float):
+    basis = BASIS[lattice]
+    R = int(math.ceil(rmax)) + 1
+    out = []
+    for c in itertools.product(range(-R, R + 1), repeat=3):
+        for a, b in enumerate(basis):
+            out.append((a + 1, tuple(int(x) for x in c), np.asarray(c, float) + b))
+    return out
+
+
+def shell_distances(lattice: str, nshells: int) -> List[float]:
+    basis = BASIS[lattice]
+    ds = set()
+    for a, b in enumerate(basis):
+        for (_, _, p) in _sites_within(lattice, 3.0):
+            d = float(np.linalg.norm(p - b))
+            if d > 1e-9:
+                ds.add(round(d, 9))
+    return sorted(ds)[:nshells]
+
+
+def _canon(members):
+    """Translation-canonical form of a cluster given as (atom, cell) members:
+    the lexicographically smallest sorted tuple over translations that put one
+    member in the home cell."""
+    best = None
+    for (_, c0) in members:
+        tr = tuple(sorted((a, (c[0] - c0[0], c[1] - c0[1], c[2] - c0[2])) for (a, c) in members))
+        # reorder so that a member in the home cell comes first
+        home = [m for m in tr if m[1] == (0, 0, 0)]
+        rest = [m for m in tr if m[1] != (0, 0, 0)]
+        key = tuple(home + rest)
+        if best is None or key < best:
+            best = key
+    return best
+
+
+def _pos(lattice, a, c):
+    return np.asarray(c, float) + BASIS[lattice][a - 1]
+
+
+def bonds(lattice: str, shell: int) -> List[Tuple[Tuple[int, Tuple[int, int, int]], ...]]:
+    """Unique bonds (up to translation) of neighbour shell ``shell`` (1-based)."""
+    d = shell_distances(lattice, shell)[shell - 1]
+    basis = BASIS[lattice]
+    seen = set()
+    out = []
+    for a0 in range(1, len(basis) + 1):
+        p0 = _pos(lattice, a0, (0, 0, 0))
+        for (a, c, p) in _sites_within(lattice, d + 1):
+            if abs(np.linalg.norm(p - p0) - d) < 1e-9:
+                key = _canon([(a0, (0, 0, 0)), (a, c)])
+                if key not in seen:
+                    seen.add(key)
+                    out.append(key)
+    return sorted(out)
+
+
+def _clusters_by_edges(lattice: str, body: int, want: Sequence[float]):
+    """Clusters of ``body`` sites whose sorted pair distances equal ``want``."""
+    dmax = max(want)
+    basis = BASIS[lattice]
+    near = [(a, c, p) for (a, c, p) in _sites_within(lattice, dmax + 1)]
+    want = sorted(round(x, 9) for x in want)
+    seen = set()
+    out = []
+    for a0 in range(1, len(basis) + 1):
+        p0 = _pos(lattice, a0, (0, 0, 0))
+        cand = [(a, c, p) for (a, c, p) in near
+                if 1e-9 < np.linalg.norm(p - p0) < dmax + 1e-9]
+        for combo in itertools.combinations(cand, body - 1):
+            pts = [p0] + [x[2] for x in combo]
+            ds = sorted(round(float(np.linalg.norm(pts[i] - pts[j])), 9)
+                        for i in range(body) for j in range(i + 1, body))
+            if ds != want:
+                continue
+            key = _canon([(a0, (0, 0, 0))] + [(x[0], x[1]) for x in combo])
+            if key not in seen:
+                seen.add(key)
+                out.append(key)
+    return sorted(out)
+
+
+def bcc_triangles():
+    d1, d2 = shell_distances("bcc", 2)
+    return _clusters_by_edges("bcc", 3, [d1, d1, d2])
+
+
+def bcc_tetrahedra():
+    d1, d2 = shell_distances("bcc", 2)
+    return _clusters_by_edges("bcc", 4, [d1, d1, d1, d1, d2, d2])
+
+
+# ---------------------------------------------------------------------------
+# tensors
+# ---------------------------------------------------------------------------
+
+def _levi_civita():
+    f = np.zeros((3, 3, 3))
+    for (i, j, k), s in (((0, 1, 2), 1), ((1, 2, 0), 1), ((2, 0, 1), 1),
+                         ((0, 2, 1), -1), ((2, 1, 0), -1), ((1, 0, 2), -1)):
+        f[i, j, k] = s
+    return f
+
+
+def _three_pairings():
+    f = np.zeros((3, 3, 3, 3))
+    I = np.eye(3)
+    for a, b, c, d in itertools.product(range(3), repeat=4):
+        f[a, b, c, d] = I[a, b] * I[c, d] + I[a, c] * I[b, d] + I[a, d] * I[b, c]
+    return f
+
+
+def _tensor(rng, ls, isotropic):
+    shape = tuple(2 * l + 1 for l in ls)
+    if not isotropic:
+        t = rng.standard_normal(shape)
+        return t / math.sqrt(t.size)
+    if len(ls) == 2:
+        return np.eye(shape[0]) if ls[0] == ls[1] else None
+    if len(ls) == 3 and all(l == 1 for l in ls):
+        return _levi_civita()
+    if len(ls) == 4 and all(l == 1 for l in ls):
+        return _three_pairings()
+    if len(ls) == 1:
+        return None
+    raise ValueError(f"no isotropic tensor for ls={ls}")
+
+
+def _term(members, coef, ls, folded):
+    atoms = [a for (a, _) in members]
+    shifts = [c for (_, c) in members]
+    return MultipoleTerm(float(coef), atoms, shifts, list(ls), folded)
+
+
+def lattice_model(lattice: str, *, pair_shells: int = 1, pair_lmax: int = 1,
+                  triangles: bool = False, tetrahedra: bool = False,
+                  onsite_l2: bool = False, isotropic: bool = False,
+                  seed: int = 0, scale: float = 1e-3) -> Tuple[int, List[MultipoleTerm]]:
+    """Seeded synthetic model on a cubic lattice: returns ``(n_cell_atoms, terms)``.
+
+    Term order: on-site body-1 l=2 anisotropy, then bonds shell by shell with the
+    (l1, l2) templates (1,1), (1,2), (2,1), (2,2) as allowed by ``pair_lmax``,
+    then bcc triangles, then bcc tetrahedra.
+    """
+    rng = np.random.default_rng(seed)
+    na = len(BASIS[lattice])
+    terms: List[MultipoleTerm] = []
+    if onsite_l2:
+        for a in range(1, na + 1):
+            f = rng.standard_normal(5) / math.sqrt(5)
+            terms.append(MultipoleTerm(float(rng.normal(0, scale)), [a], [(0, 0, 0)], [2], f))
+    lpairs = [(l1, l2) for l1 in range(1, pair_lmax + 1) for l2 in range(1, pair_lmax + 1)]
+    for sh in range(1, pair_shells + 1):
+        for b in bonds(lattice, sh):
+            for (l1, l2) in lpairs:
+                f = _tensor(rng, (l1, l2), isotropic)
+                c = rng.normal(0, scale)
+                if f is None:
+                    continue
+                terms.append(_term(b, c, (l1, l2), f))
+    if triangles:
+        if lattice != "bcc":
+            raise ValueError("triangles are defined for bcc only")
+        for tri in bcc_triangles():
+            f = _tensor(rng, (1, 1, 1), isotropic)
+            terms.append(_term(tri, rng.normal(0, scale), (1, 1, 1), f))
+    if tetrahedra:
+        if lattice != "bcc":
+            raise ValueError("tetrahedra are defined for bcc only")
+        for tet in bcc_tetrahedra():
+            f = _tensor(rng, (1, 1, 1, 1), isotropic)
+            terms.append(_term(tet, rng.normal(0, scale), (1, 1, 1, 1), f))
+    return na, terms
+
+
+# ---------------------------------------------------------------------------
+# the workload configurations
+# ---------------------------------------------------------------------------
+
+def config_model(name: str, seed: int = 0):
+    """``(n_cell_atoms, terms, dims)`` of a named workload.
+
+    * ``sc8``     simple cubic 8^3 (512 spins): body-1 l=2 + body-2 l=1 on 2 shells
+    * ``bcc16``   bcc 16^3 (8192 spins): body-2 l<=2 to 3rd shell + l=1 triangles
+    * ``fcc12``   fcc 12^3 (6912 spins): body-2 l<=2 to 4th shell
+    * ``bcc32``   bcc 32^3 (65536 spins): body-2 l<=2 to 3rd shell + l=1 triangles
+                  + l=1 tetrahedra, isotropic SALC tensors
+    * ``bcc32d``  the same with dense random tensors
+    """
+    if name == "sc8":
+        na, t = lattice_model("sc", pair_shells=2, pair_lmax=1, onsite_l2=True, seed=seed)
+        return na, t, (8, 8, 8)
+    if name == "bcc16":
+        na, t = lattice_model("bcc", pair_shells=3, pair_lmax=2, triangles=True, seed=seed)
+        return na, t, (16, 16, 16)
+    if name == "fcc12":
+        na, t = lattice_model("fcc", pair_shells=4, pair_lmax=2, seed=seed)
+        return na, t, (12, 12, 12)
+    if name == "bcc32":
+        na, t = lattice_model("bcc", pair_shells=3, pair_lmax=2, triangles=True,
+                              tetrahedra=True, isotropic=True, seed=seed)
+        return na, t, (32, 32, 32)
+    if name == "bcc32d":
+        na, t = lattice_model("bcc", pair_shells=3, pair_lmax=2, triangles=True,
+                              tetrahedra=True, isotropic=False, seed=seed)
+        return na, t, (32, 32, 32)
+    raise KeyError(name)
+
+
+def random_config(n_sites: int, rng) -> np.ndarray:
+    """Uniform unit spins (normalised Gaussian triples, state.jl:80-86), n x 3."""
+    v = rng.standard_normal((n_sites, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)

@@ -1,0 +1,214 @@
+"""Device parity: every piece of device arithmetic against the CPU oracle.
+
+Bit-exact: Philox blocks, deterministic exp/log/sincos, Z rows, keyed
+proposals, the walk ΔE / coefficients, the tree reduction, whole sweeps
+(config, rows, energy, counters), overrelaxation sweeps, renormalisation.
+Tolerance: walk ΔE vs the reference's site_coeffs! + delta_energy (1e-12
+scaled, test_gpu.jl:135-178).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import sce_amd as S
+from sce_amd import gpu as G
+from fixtures import build_pair, fixture_cases, rand_config, bcc_small
+
+pytestmark = pytest.mark.gpu
+
+LANES = (16, 32, 64)
+
+
+def _dirs(n, seed):
+    rng = np.random.default_rng(seed)
+    d = [(0, 0, 1.0), (0, 0, -1.0), (1, 0, 0.0), (-1, 0, 0.0), (0, 1, 0.0), (0, -1, 0.0)]
+    d += [(np.cos(k * np.pi / 6), np.sin(k * np.pi / 6), 0.0) for k in range(12)]
+    d = np.array(d, float)
+    v = rng.standard_normal((n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return np.vstack([d, v])
+
+
+def test_philox_kat_and_streams_on_device():
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    seeds = [k[0] | (k[1] << 32) for _, k, _ in kat]
+    out = G.dev_philox(seeds, [c for c, _, _ in kat])
+    for row, (_, _, want) in zip(out, kat):
+        assert tuple(int(x) for x in row) == want
+    rng = np.random.default_rng(1)
+    seeds = rng.integers(0, 2**63, 500, dtype=np.uint64)
+    ctrs = rng.integers(0, 2**32, (500, 4), dtype=np.uint64).astype(np.uint32)
+    dev = G.dev_philox(seeds, ctrs)
+    for s, c, d in zip(seeds, ctrs, dev):
+        assert O.philox4x32(c, (int(s) & 0xffffffff, int(s) >> 32)) == tuple(int(x) for x in d)
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+def test_detmath_bitwise(op):
+    rng = np.random.default_rng(op)
+    if op == 0:
+        x = np.concatenate([-rng.exponential(5, 5000), rng.uniform(-700, 700, 500), [0.0, -745.0, 709.0]])
+        ref = [O.exp(v) for v in x]
+    elif op == 1:
+        x = np.concatenate([rng.uniform(0, 1, 5000), [2.0**-53, 1 - 2.0**-53, 5e-324, 1.0, 3.5e7]])
+        ref = [O.log(v) for v in x]
+    else:
+        x = np.concatenate([rng.uniform(0, 2 * np.pi, 3000), rng.normal(0, 5, 3000), [0.0, np.pi / 4]])
+        ref = [O.sincos(v)[op - 2] for v in x]
+    dev = G.dev_detmath(op, x)
+    assert np.array_equal(dev.view(np.uint64), np.asarray(ref).view(np.uint64))
+
+
+@pytest.mark.parametrize("lmax", range(7))
+def test_zlm_row_bitwise(lmax):
+    d = _dirs(2000, 2026 + lmax)
+    dev = G.dev_zlm(d, lmax)
+    ref = np.array([O.zlm_row(u, lmax) for u in d])
+    assert np.array_equal(dev.view(np.uint64), ref.view(np.uint64))
+
+
+def test_keyed_proposal_bitwise():
+    rng = np.random.default_rng(5)
+    n = 3000
+    sites = rng.integers(1, 70000, n).astype(np.int32)
+    sweeps = rng.integers(1, 10000, n).astype(np.int32)
+    e = _dirs(n - 18, 3)
+    seed = 0xbeef1234cafe
+    e2, u = G.dev_proposal(seed, sites, sweeps, e, 0.6)
+    nflip = 0
+    for i in range(n):
+        r2, ru = O.keyed_proposal(seed, int(sites[i]), int(sweeps[i]), e[i], 0.6)
+        assert np.array_equal(r2.view(np.uint64), e2[i].view(np.uint64)) and ru == u[i]
+        nflip += int(np.array_equal(r2, -e[i]))
+    assert 0.1 * n < nflip < 0.3 * n
+
+
+@pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
+def test_walk_delta_and_coeffs(case):
+    name, na, terms, dims, mm, fl = case
+    H, T = build_pair(na, terms, dims, mm, fl)
+    gH = S.GPUTiledHamiltonian(H)
+    cfg = rand_config(H.n_sites, 31)
+    z = H.zrows(cfg)
+    rng = np.random.default_rng(7)
+    sites = np.array(T.color_sites[: min(64, T.n_active)], np.int32)
+    znew = np.array([O.zlm_row(v, H.lmax) for v in S.random_config(len(sites), rng)])
+    for lanes in LANES:
+        dev = G.dev_walk_delta(gH, z, sites, znew, lanes)
+        cdev = G.dev_walk_coeffs(gH, z, sites, lanes)
+        for i, s in enumerate(sites):
+            ref = T.walk_delta(int(s), z, znew[i], lanes)
+            assert dev[i] == ref, (name, lanes, s)
+            cref = T.walk_coeffs(int(s), z, lanes)
+            assert np.array_equal(cdev[i], cref)
+            c = T.site_coeffs(int(s), z)
+            dE_cpu = O.delta_energy(c, z[s - 1], znew[i])
+            scale = np.abs(c).sum() * (np.abs(znew[i]).sum() + np.abs(z[s - 1]).sum())
+            assert abs(dev[i] - dE_cpu) <= 1e-12 * max(scale, 1e-30)
+            assert np.all(np.abs(cdev[i] - c) <= 1e-12 * max(np.abs(c).sum(), 1e-30))
+
+
+@pytest.mark.parametrize("n", [1, 7, 511, 512, 513, 100_000, 300_000])
+def test_reduce_tree_bitwise(n):
+    x = np.random.default_rng(n).standard_normal(n) * 10.0 ** np.random.default_rng(n + 1).integers(-8, 3, n)
+    assert G.dev_reduce(x) == O.reduce_tree(x)
+
+
+def _sweeps_case(case, lanes, nsweeps=5, beta=1 / 0.05, seed=0xc0ffee, step=0.6):
+    name, na, terms, dims, mm, fl = case
+    H, T = build_pair(na, terms, dims, mm, fl)
+    gH = S.GPUTiledHamiltonian(H)
+    cfg = rand_config(H.n_sites, 7)
+    st = S.ChainState(H, cfg, step)
+    gst = S.GPUChainState(gH, st, seed=seed)
+    c2, z2, E = st.config.copy(), st.zrows.copy(), st.energy
+    naccs, naccs_ref = [], []
+    for sw in range(1, nsweeps + 1):
+        naccs.append(S.gpu_metropolis_sweep(gst, gH, beta, lanes=lanes))
+        n, E, _, _ = T.keyed_sweep(c2, z2, E, beta, step, seed, sw, lanes)
+        naccs_ref.append(n)
+    cfg_d, z_d, E_d = gst.download()
+    return (H, T, gH, gst, st), (cfg_d, z_d, E_d, naccs), (c2, z2, E, naccs_ref)
+
+
+@pytest.mark.parametrize("lanes", LANES)
+@pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
+def test_full_sweep_bitwise(case, lanes):
+    (H, T, gH, gst, st), dev, ref = _sweeps_case(case, lanes)
+    cfg_d, z_d, E_d, naccs = dev
+    c2, z2, E, naccs_ref = ref
+    assert np.array_equal(cfg_d.view(np.uint64), c2.view(np.uint64))
+    assert np.array_equal(z_d.view(np.uint64), z2.view(np.uint64))
+    assert E_d == E
+    assert naccs == naccs_ref
+    inf = gst.info()
+    assert inf.acc_metro == sum(naccs_ref) and inf.att_metro == 5 * H.n_active
+    assert inf.sweep_index == 5
+
+
+@pytest.mark.parametrize("case", [c for c in fixture_cases() if c[0] in ("dimer", "3body", "bcc-l2-tri", "bcc-tet", "dimer+field")],
+                         ids=lambda c: c[0])
+def test_overrelaxation_bitwise(case):
+    name, na, terms, dims, mm, fl = case
+    H, T = build_pair(na, terms, dims, mm, fl)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 9), 0.6)
+    seed, beta, lanes = 77, 1 / 0.05, 32
+    gst = S.GPUChainState(gH, st, seed=seed)
+    c2, z2, E = st.config.copy(), st.zrows.copy(), st.energy
+    acc_or = att_or = 0
+    for sw in range(1, 4):
+        S.gpu_metropolis_sweep(gst, gH, beta, lanes=lanes)
+        _, E, _, _ = T.keyed_sweep(c2, z2, E, beta, 0.6, seed, sw, lanes)
+        for p in range(2):
+            S.gpu_overrelaxation_sweep(gst, gH, beta, or_pass=p, lanes=lanes)
+            a, n, E = T.keyed_or_sweep(c2, z2, E, beta, seed, sw, p, lanes)
+            acc_or += a
+            att_or += n
+    cfg_d, z_d, E_d = gst.download()
+    assert np.array_equal(cfg_d.view(np.uint64), c2.view(np.uint64))
+    assert np.array_equal(z_d.view(np.uint64), z2.view(np.uint64))
+    assert E_d == E
+    inf = gst.info()
+    assert (inf.acc_or, inf.att_or) == (acc_or, att_or)
+
+
+def test_renormalize_and_total_energy_bitwise():
+    na, terms, dims = bcc_small(3)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 4), 0.6)
+    gst = S.GPUChainState(gH, st, seed=5)
+    S.gpu_run_sweeps(gst, gH, None, 1 / 0.05, 30, renorm_interval=0)
+    cfg, z, E = gst.download()
+    assert gst.total_energy() == T.total_energy_tree(z)
+    drift = S.gpu_renormalize(gst)
+    c2, z2 = cfg.copy(), z.copy()
+    d_ref, E_ref = T.renormalize(c2, z2, E)
+    cfg_d, z_d, E_d = gst.download()
+    assert np.array_equal(cfg_d, c2) and np.array_equal(z_d, z2)
+    assert E_d == E_ref and drift == d_ref
+    assert abs(E_ref - T.total_energy(z2)) <= 1e-12 * max(1.0, abs(E_ref))
+
+
+def test_run_sweeps_matches_single_sweeps():
+    na, terms, dims = bcc_small(4)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 8), 0.6)
+    a = S.GPUChainState(gH, st, seed=11)
+    b = S.GPUChainState(gH, st, seed=11)
+    obs = S.gpu_run_sweeps(a, gH, None, 20.0, 12, renorm_interval=5, measure_interval=3)
+    assert obs.shape == (4, 4)
+    for i in range(1, 13):
+        S.gpu_metropolis_sweep(b, gH, 20.0)
+        if i % 5 == 0:
+            S.gpu_renormalize(b)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    m = np.mean(ca, axis=0)
+    assert np.allclose(obs[-1, 1:], m, atol=1e-14) and obs[-1, 0] == Ea
