@@ -142,6 +142,10 @@ int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps,
                        int32_t or_per_metropolis, int64_t renorm_interval,
                        int64_t measure_interval, int32_t lanes, double* obs,
                        int64_t obs_cap, int64_t* n_obs);
+/* Run the Metropolis-only sweeps of sce_run_sweeps as one persistent
+ * cooperative launch per renormalisation segment (on != 0, the default) or as
+ * per-colour launches (0).  Both give bit-identical results. */
+int32_t sce_chain_set_persistent(sce_chain* c, int32_t on);
 /* _renormalize!(st, H) on the device: normalise active spins, rebuild rows,
  * recompute the FP64 total energy, re-anchor.  Writes the drift. */
 int32_t sce_renormalize(sce_chain* c, double* drift);

@@ -150,6 +150,7 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
   const int c3 = r / T.d1;
   const int nb0 = T.nb_ptr[atom];
   const int NB = T.nb_ptr[atom + 1] - nb0;
+#pragma unroll 4
   for (int idx = lane; idx < NB * NLM; idx += L) {
     const int nbi = idx / NLM;
     const int row = idx - nbi * NLM;
@@ -166,22 +167,48 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// the colour walk: `L` lanes per site, 256/L sites per block
-// TEST: ΔE only, for caller-chosen sites and rows (parity hook)
+// the colour walk: `L` lanes per site, 256/L sites per block.
+// The walk entries of a lane are loaded UNROLL at a time ahead of their use
+// (independent loads in flight), then consumed in ascending entry order, so
+// the per-lane accumulation order is unchanged.
 // ---------------------------------------------------------------------------
+constexpr int WALK_UNROLL = 8;
+
+template <int L, int NLM>
+__device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, int W, int lane,
+                                             const double* Z, const double* dz) {
+  double a = 0.0;
+  for (int base = 0; base < W; base += WALK_UNROLL * L) {
+    WalkEntry en[WALK_UNROLL];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) {
+      const int i = base + u * L + lane;
+      if (i < W) en[u] = wl[i];
+    }
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) {
+      const int i = base + u * L + lane;
+      if (i < W) {
+        const double p = (Z[en[u].f0] * Z[en[u].f1]) * Z[en[u].f2];
+        if (p != 0.0) a = __builtin_fma(en[u].w * p, dz[en[u].tgt], a);
+      }
+    }
+  }
+  return a;
+}
+
+// One group of L lanes updates one site of the colour (colour-order index q).
+// Called by every thread of the block (contains a block barrier).
+// TEST: ΔE only, for caller-chosen sites and rows (parity hook).
 template <int L, int LMAX, bool TEST>
-__global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__ config,
-                                               double* __restrict__ zrows,
-                                               const double* __restrict__ prop,
-                                               double* __restrict__ dE, int* __restrict__ acc,
-                                               const ChainDev* __restrict__ st, int q_lo,
-                                               int q_n, const int* __restrict__ test_sites) {
-  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4, G = TPB / L;
-  extern __shared__ double lds[];
+__device__ __forceinline__ void metro_group(const DevTables& T, double* __restrict__ config,
+                                            double* __restrict__ zrows,
+                                            const double* __restrict__ prop,
+                                            double* __restrict__ dE, int* __restrict__ acc,
+                                            double beta, int q, bool valid,
+                                            const int* __restrict__ test_sites, double* lds) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
-  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
-  const bool valid = qi < q_n;
-  const int q = q_lo + qi;
   const int stride = T.nb_max * NLM + 1 + NLM;
   double* Z = lds + g * stride;
   double* dz = Z + T.nb_max * NLM + 1;
@@ -196,16 +223,11 @@ __global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__
   double a = 0.0;
   if (valid) {
     const int w0 = T.walk_ptr[atom];
-    const int W = T.walk_ptr[atom + 1] - w0;
-    const WalkEntry* __restrict__ wl = T.walk + w0;
-    for (int i = lane; i < W; i += L) {
-      const WalkEntry en = wl[i];
-      const double p = (Z[en.f0] * Z[en.f1]) * Z[en.f2];
-      if (p != 0.0) a = __builtin_fma(en.w * p, dz[en.tgt], a);
-    }
+    a = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
   }
 #pragma unroll
   for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
+  __syncthreads();  // the block's LDS is reused by the next site group
   if (!valid) return;
   if (TEST) {
     if (lane == 0) dE[q] = a;
@@ -213,7 +235,7 @@ __global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__
   }
   int accept = 0;
   const double* pq = prop + (size_t)q * PS;
-  if (lane == 0) accept = (a <= 0.0 || pq[3] < sce::det_exp(-st->beta * a)) ? 1 : 0;
+  if (lane == 0) accept = (a <= 0.0 || pq[3] < sce::det_exp(-beta * a)) ? 1 : 0;
   accept = __shfl(accept, 0, L);
   if (accept) {
     for (int k = lane; k < NLM; k += L) zrows[(size_t)s * NLM + k] = pq[4 + k];
@@ -227,6 +249,21 @@ __global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__
     dE[q] = accept ? a : 0.0;
     acc[q] = accept;
   }
+}
+
+template <int L, int LMAX, bool TEST>
+__global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__ config,
+                                               double* __restrict__ zrows,
+                                               const double* __restrict__ prop,
+                                               double* __restrict__ dE, int* __restrict__ acc,
+                                               const ChainDev* __restrict__ st, int q_lo,
+                                               int q_n, const int* __restrict__ test_sites) {
+  constexpr int G = TPB / L;
+  extern __shared__ double lds[];
+  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / L;
+  const double beta = TEST ? 0.0 : st->beta;
+  metro_group<L, LMAX, TEST>(T, config, zrows, prop, dE, acc, beta, q_lo + qi, qi < q_n,
+                             test_sites, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,14 +308,24 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
     const int w0 = T.walk_ptr[atom];
     const int W = T.walk_ptr[atom + 1] - w0;
     const WalkEntry* __restrict__ wl = T.walk + w0;
-    for (int i = lane; i < W; i += L) {
-      const WalkEntry en = wl[i];
-      const double p = (Z[en.f0] * Z[en.f1]) * Z[en.f2];
-      if (p != 0.0) {
-        const double v = en.w * p;
+    for (int base = 0; base < W; base += WALK_UNROLL * L) {
+      WalkEntry en[WALK_UNROLL];
 #pragma unroll
-        for (int k = 0; k < NLM; ++k)
-          if (k == en.tgt) c[k] = c[k] + v;
+      for (int u = 0; u < WALK_UNROLL; ++u) {
+        const int i = base + u * L + lane;
+        if (i < W) en[u] = wl[i];
+      }
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL; ++u) {
+        const int i = base + u * L + lane;
+        if (i >= W) continue;
+        const double p = (Z[en[u].f0] * Z[en[u].f1]) * Z[en[u].f2];
+        if (p != 0.0) {
+          const double v = en[u].w * p;
+#pragma unroll
+          for (int k = 0; k < NLM; ++k)
+            if (k == en[u].tgt) c[k] = c[k] + v;
+        }
       }
     }
   }
@@ -492,6 +539,225 @@ __global__ __launch_bounds__(256) void k_tree(const double* __restrict__ x, long
 }
 
 // ---------------------------------------------------------------------------
+// persistent sweeps: one cooperative launch runs many Metropolis sweeps —
+// proposals, the colour steps and the deterministic reductions — separated by
+// grid barriers instead of kernel boundaries.  The arithmetic is exactly that
+// of k_propose / k_metro / k_tree.
+// ---------------------------------------------------------------------------
+constexpr int BAR_GROUP = 64;  // blocks per first-level barrier counter
+constexpr int BAR_PAD = 32;    // u32 words between counters (one 128-byte line)
+constexpr unsigned BAR_SPIN_LIMIT = 1u << 26;
+
+// Two-level grid barrier with a bounded wait: a block that waits longer than
+// the spin limit (or sees another block's error) sets *err and returns false,
+// so every wave leaves the kernel even if co-residency were ever violated.
+// bar[0] = generation, bar[1] = top-level count, bar[2 + BAR_PAD*g] = group g.
+__device__ bool grid_sync(unsigned* bar, int* err, unsigned* gen_sh, int* ok_sh) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned g = *gen_sh;
+    const unsigned nb = gridDim.x;
+    const unsigned grp = blockIdx.x / BAR_GROUP;
+    const unsigned ngrp = (nb + BAR_GROUP - 1) / BAR_GROUP;
+    const unsigned gsize = min((unsigned)BAR_GROUP, nb - grp * BAR_GROUP);
+    __threadfence();
+    if (atomicAdd(&bar[2 + BAR_PAD * grp], 1u) == gsize - 1) {
+      atomicExch(&bar[2 + BAR_PAD * grp], 0u);
+      if (atomicAdd(&bar[1], 1u) == ngrp - 1) {
+        atomicExch(&bar[1], 0u);
+        __threadfence();
+        atomicAdd(&bar[0], 1u);
+      }
+    }
+    unsigned spins = 0;
+    while (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > BAR_SPIN_LIMIT) {
+        atomicExch(err, 1);
+        ok = 0;
+        break;
+      }
+      if ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        ok = 0;
+        break;
+      }
+    }
+    __threadfence();
+    *gen_sh = g + 1;
+    *ok_sh = ok;
+  }
+  __syncthreads();
+  return *ok_sh != 0;
+}
+
+// Block-level recursive tree over m partial sums (the levels above the first
+// of ora-style tree: m <= 512 -> one chunk tree, else chunk sums recursively).
+// Valid in thread 0.  `scratch` receives intermediate levels.
+__device__ double final_tree(const double* buf, long long m, double* scratch, double* sh) {
+  volatile const double* vb = buf;
+  if (m == 1) return vb[0];
+  while (true) {
+    const int t = threadIdx.x;
+    if (m <= TREE_B) {
+      const double a0 = t < m ? vb[t] : 0.0;
+      const double a1 = t + 256 < m ? vb[t + 256] : 0.0;
+      return block_tree(a0, a1, sh);
+    }
+    const long long nb = (m + TREE_B - 1) / TREE_B;
+    volatile double* vs = scratch;
+    for (long long b = 0; b < nb; ++b) {
+      const long long j0 = b * TREE_B + t, j1 = j0 + 256;
+      const double a0 = j0 < m ? vb[j0] : 0.0;
+      const double a1 = j1 < m ? vb[j1] : 0.0;
+      const double v = block_tree(a0, a1, sh);
+      if (t == 0) vs[b] = v;
+      __syncthreads();
+    }
+    vb = scratch;
+    scratch += nb;
+    m = nb;
+  }
+}
+
+struct SweepArgs {
+  DevTables T;
+  double *config, *zrows, *prop, *dE, *partial, *obs;
+  int *acc, *ipart;
+  ChainDev* st;
+  const int* cptr;
+  int n_colors;
+  long long nsweeps, sweep0, i0;  // sweeps before this launch: chain total, this call
+  long long obs_row0;
+  int measure_interval;
+  unsigned* bar;
+  int* err;
+  double beta, step;
+  unsigned long long seed;
+};
+
+template <int LMAX>
+__device__ __forceinline__ void propose_one(const DevTables& T, const double* __restrict__ config,
+                                            double* __restrict__ prop, int q, uint64_t seed,
+                                            int32_t sweep, double step) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  const int s = T.color_sites[q];
+  const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
+  double e2[3], u, z[NLM];
+  sce::keyed_proposal(seed, s + 1, sweep, e, step, e2, &u);
+  sce::zlm_row<LMAX>(e2[0], e2[1], e2[2], z);
+  double* o = prop + (size_t)q * PS;
+  o[0] = e2[0];
+  o[1] = e2[1];
+  o[2] = e2[2];
+  o[3] = u;
+#pragma unroll
+  for (int k = 0; k < NLM; ++k) o[4 + k] = z[k];
+}
+
+template <int L, int LMAX>
+__global__ __launch_bounds__(TPB) void k_sweeps(SweepArgs a) {
+  constexpr int G = TPB / L;
+  extern __shared__ double lds[];
+  __shared__ double sh[256];
+  __shared__ unsigned gen_sh;
+  __shared__ int ok_sh;
+  const DevTables& T = a.T;
+  const int t = threadIdx.x;
+  if (t == 0) gen_sh = __hip_atomic_load(&a.bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int nthreads = gridDim.x * TPB;
+  const int gtid = blockIdx.x * TPB + t;
+  const int vblk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const long long n = T.n_active;
+  const long long nch = (n + TREE_B - 1) / TREE_B;
+  const long long nchm = ((long long)T.n_sites + TREE_B - 1) / TREE_B;
+  double* mpart = a.partial + nch;              // 3 x nchm
+  double* scratch = mpart + 3 * nchm;
+  long long row = a.obs_row0;
+  for (long long i = 1; i <= a.nsweeps; ++i) {
+    const int32_t sweep = (int32_t)(a.sweep0 + i);
+    for (int q = gtid; q < (int)n; q += nthreads)
+      propose_one<LMAX>(T, a.config, a.prop, q, a.seed, sweep, a.step);
+    if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
+    for (int c = 0; c < a.n_colors; ++c) {
+      const int lo = a.cptr[c], nc = a.cptr[c + 1] - lo;
+      if (nc == 0) continue;
+      for (int base = vblk * G; base < nc; base += gridDim.x * G) {
+        const int qi = base + t / L;
+        metro_group<L, LMAX, false>(T, a.config, a.zrows, a.prop, a.dE, a.acc, a.beta, lo + qi,
+                                    qi < nc, nullptr, lds);
+      }
+      if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
+    }
+    // first level of the deterministic reductions: 512-element chunk trees
+    const bool meas = a.measure_interval > 0 && ((a.i0 + i) % a.measure_interval) == 0;
+    for (long long ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+      const long long i0 = ch * TREE_B + t, i1 = i0 + 256;
+      const double v = block_tree(i0 < n ? a.dE[i0] : 0.0, i1 < n ? a.dE[i1] : 0.0, sh);
+      long long cnt = (i0 < n ? a.acc[i0] : 0) + (i1 < n ? a.acc[i1] : 0);
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+      __shared__ long long csum[4];
+      if ((t & 63) == 0) csum[t >> 6] = cnt;
+      __syncthreads();
+      if (t == 0) {
+        a.partial[ch] = v;
+        a.ipart[ch] = (int)(csum[0] + csum[1] + csum[2] + csum[3]);
+      }
+      __syncthreads();
+    }
+    if (meas) {
+      const long long N = T.n_sites;
+      for (long long ch = blockIdx.x; ch < nchm; ch += gridDim.x) {
+        const long long i0 = ch * TREE_B + t, i1 = i0 + 256;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double x0 = (i0 < N && T.active[i0]) ? a.config[3 * i0 + c] : 0.0;
+          const double x1 = (i1 < N && T.active[i1]) ? a.config[3 * i1 + c] : 0.0;
+          const double v = block_tree(x0, x1, sh);
+          if (t == 0) mpart[c * nchm + ch] = v;
+        }
+      }
+    }
+    if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
+    if (blockIdx.x == 0) {
+      const double dsum = final_tree(a.partial, nch, scratch, sh);
+      double m3[3] = {0.0, 0.0, 0.0};
+      if (meas) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double v = final_tree(mpart + c * nchm, nchm, scratch, sh);
+          __syncthreads();
+          m3[c] = v;
+        }
+      }
+      if (t == 0) {
+        long long nacc = 0;
+        volatile const int* vi = a.ipart;
+        for (long long ch = 0; ch < nch; ++ch) nacc += vi[ch];
+        ChainDev* st = a.st;
+        st->energy = st->energy + dsum;
+        st->acc_metro += nacc;
+        st->att_metro += n;
+        st->nacc_last = nacc;
+        st->sweep_index += 1;
+        if (meas) {
+          const double nd = (double)n;
+          a.obs[4 * row] = st->energy;
+          a.obs[4 * row + 1] = m3[0] / nd;
+          a.obs[4 * row + 2] = m3[1] / nd;
+          a.obs[4 * row + 3] = m3[2] / nd;
+          st->n_obs = (int)(row + 1);
+        }
+      }
+      __syncthreads();
+    }
+    if (meas) ++row;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // renormalisation + full energy (state.jl:168-185, energy.jl:55-75)
 // ---------------------------------------------------------------------------
 template <int LMAX>
@@ -618,6 +884,8 @@ struct sce_tables {
   DevBuf mem;
   DevTables dev{};
   std::vector<int32_t> color_ptr;  // 0-based offsets, n_colors + 1
+  int32_t* d_color_ptr = nullptr;
+  int max_class = 0;
   int n_colors = 0;
   int64_t total_walk = 0;
   int w_max = 0;
@@ -628,8 +896,11 @@ struct sce_chain {
   DevBuf mem;
   double *config = nullptr, *zrows = nullptr, *prop = nullptr, *dE = nullptr, *partial = nullptr,
          *inst_e = nullptr, *obs = nullptr;
-  int *acc = nullptr, *att = nullptr;
+  int *acc = nullptr, *att = nullptr, *ipart = nullptr, *err = nullptr;
+  unsigned* bar = nullptr;
+  size_t bar_words = 0;
   ChainDev* st = nullptr;
+  bool persistent = true;   // run sweeps in one cooperative launch when possible
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
   long long sweeps_done = 0;  // host mirror of sweep_index
@@ -740,6 +1011,10 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   t->n_colors = d->n_colors;
   t->color_ptr.resize((size_t)d->n_colors + 1);
   for (int c = 0; c <= d->n_colors; ++c) t->color_ptr[c] = d->color_ptr[c] - 1;
+  for (int c = 0; c < d->n_colors; ++c)
+    t->max_class = std::max(t->max_class, t->color_ptr[c + 1] - t->color_ptr[c]);
+  if ((r = dupload(t->mem, &t->d_color_ptr, t->color_ptr.data(), t->color_ptr.size())) != SCE_OK)
+    return fail(r);
   t->total_walk = (int64_t)L.walk.size();
   t->w_max = L.w_max;
   *out = t;
@@ -978,7 +1253,12 @@ extern "C" int32_t sce_chain_create(sce_tables* t, const double* config, const d
   if ((r = dalloc(c->mem, &c->att, na)) != SCE_OK) return fail(r);
   if ((r = dalloc(c->mem, &c->inst_e, (size_t)std::max(D.n_inst, 1))) != SCE_OK) return fail(r);
   const long long nmax = std::max<long long>({(long long)D.n_active, (long long)D.n_inst, (long long)D.n_sites});
-  if ((r = dalloc(c->mem, &c->partial, partial_capacity(nmax))) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->partial, partial_capacity(nmax) + 4 * ((size_t)nmax / TREE_B + 4))) != SCE_OK)
+    return fail(r);
+  if ((r = dalloc(c->mem, &c->ipart, (size_t)nmax / TREE_B + 4)) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->err, 4)) != SCE_OK) return fail(r);
+  c->bar_words = 2 + (size_t)BAR_PAD * (65536 / BAR_GROUP + 2);
+  if ((r = dalloc(c->mem, &c->bar, c->bar_words)) != SCE_OK) return fail(r);
   if ((r = dalloc(c->mem, &c->st, 1)) != SCE_OK) return fail(r);
   ChainDev h{};
   h.energy = energy;
@@ -1106,6 +1386,84 @@ extern "C" int32_t sce_overrelaxation_sweep(sce_chain* c, double beta, int32_t o
   return SCE_OK;
 }
 
+template <int L, int LMAX>
+static int32_t launch_sweeps_l(sce_chain* c, SweepArgs& sa, hipStream_t s) {
+  const DevTables& D = c->t->dev;
+  const size_t shmem = walk_lds_bytes(D, L);
+  const void* f = reinterpret_cast<const void*>(&k_sweeps<L, LMAX>);
+  int per_cu = 0, cus = 0, dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, TPB, shmem));
+  constexpr int G = TPB / L;
+  long long want = std::max<long long>((c->t->max_class + G - 1) / G, 1);
+  long long grid = std::min<long long>((long long)per_cu * cus, want);
+  grid = std::min<long long>(grid, 65536);
+  if (grid < 1) return sce_fail(SCE_EHIP, "persistent sweep kernel cannot be resident");
+  HIPCHK(hipMemsetAsync(c->bar, 0, c->bar_words * sizeof(unsigned), s));
+  HIPCHK(hipMemsetAsync(c->err, 0, sizeof(int), s));
+  void* args[] = {&sa};
+  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)grid), dim3(TPB), args, (unsigned)shmem, s));
+  return SCE_OK;
+}
+
+template <int LMAX>
+static int32_t launch_sweeps(int L, sce_chain* c, SweepArgs& sa, hipStream_t s) {
+  if (L == 16) return launch_sweeps_l<16, LMAX>(c, sa, s);
+  if (L == 32) return launch_sweeps_l<32, LMAX>(c, sa, s);
+  return launch_sweeps_l<64, LMAX>(c, sa, s);
+}
+
+// Enqueue `n` Metropolis sweeps as one persistent launch.
+static int32_t enqueue_persistent(sce_chain* c, int L, double beta, long long n, long long i0,
+                                  long long row0, int64_t measure_interval) {
+  CHK(check_lds(c->t->dev, L));
+  ChainDev h;
+  CHK(read_state(c, &h));
+  SweepArgs sa{};
+  sa.T = c->t->dev;
+  sa.config = c->config;
+  sa.zrows = c->zrows;
+  sa.prop = c->prop;
+  sa.dE = c->dE;
+  sa.partial = c->partial;
+  sa.obs = c->obs;
+  sa.acc = c->acc;
+  sa.ipart = c->ipart;
+  sa.st = c->st;
+  sa.cptr = c->t->d_color_ptr;
+  sa.n_colors = c->t->n_colors;
+  sa.nsweeps = n;
+  sa.sweep0 = c->sweeps_done;
+  sa.i0 = i0;
+  sa.obs_row0 = row0;
+  sa.measure_interval = (int)measure_interval;
+  sa.bar = c->bar;
+  sa.err = c->err;
+  sa.beta = beta;
+  sa.step = h.step;
+  sa.seed = h.seed;
+  int32_t r = SCE_OK;
+  switch (c->t->dev.lmax) {
+    case 0: r = launch_sweeps<0>(L, c, sa, c->stream); break;
+    case 1: r = launch_sweeps<1>(L, c, sa, c->stream); break;
+    case 2: r = launch_sweeps<2>(L, c, sa, c->stream); break;
+    case 3: r = launch_sweeps<3>(L, c, sa, c->stream); break;
+    case 4: r = launch_sweeps<4>(L, c, sa, c->stream); break;
+    case 5: r = launch_sweeps<5>(L, c, sa, c->stream); break;
+    default: r = launch_sweeps<6>(L, c, sa, c->stream); break;
+  }
+  if (r != SCE_OK) return r;
+  c->sweeps_done += n;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_set_persistent(sce_chain* c, int32_t on) {
+  CHK(chain_check(c));
+  c->persistent = on != 0;
+  return SCE_OK;
+}
+
 extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, int32_t or_per_metropolis,
                                   int64_t renorm_interval, int64_t measure_interval, int32_t lanes,
                                   double* obs, int64_t obs_cap, int64_t* n_obs) {
@@ -1137,19 +1495,43 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   hipEvent_t t0 = next_event(c);
   HIPCHK(hipEventRecord(t0, c->stream));
   const DevTables& D = c->t->dev;
-  for (int64_t i = 1; i <= nsweeps; ++i) {
-    CHK(enqueue_metro(c, lanes));
-    for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
-    if (renorm_interval > 0 && i % renorm_interval == 0) enqueue_renorm(c);
-    if (measure_interval > 0 && i % measure_interval == 0)
-      launch_tree<MODE_MEASURE, 3>(c->config, D.n_sites, nullptr, nullptr, D.active, c->partial, c->st,
-                                   c->obs, 0, (double)D.n_active, c->stream);
+  if (c->persistent && or_per_metropolis == 0) {
+    // segments end at renormalisation points; each is one cooperative launch
+    int64_t i = 0, row = 0;
+    while (i < nsweeps) {
+      int64_t seg = nsweeps - i;
+      if (renorm_interval > 0) seg = std::min<int64_t>(seg, renorm_interval - (i % renorm_interval));
+      hipEvent_t e0 = nullptr;
+      if (c->timing) {
+        e0 = next_event(c);
+        HIPCHK(hipEventRecord(e0, c->stream));
+      }
+      CHK(enqueue_persistent(c, lanes, beta, seg, i, row, measure_interval));
+      if (c->timing) HIPCHK(hipEventRecord(next_event(c), c->stream));
+      if (measure_interval > 0) row += (i + seg) / measure_interval - i / measure_interval;
+      i += seg;
+      if (renorm_interval > 0 && i % renorm_interval == 0) enqueue_renorm(c);
+    }
+  } else {
+    for (int64_t i = 1; i <= nsweeps; ++i) {
+      CHK(enqueue_metro(c, lanes));
+      for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
+      if (renorm_interval > 0 && i % renorm_interval == 0) enqueue_renorm(c);
+      if (measure_interval > 0 && i % measure_interval == 0)
+        launch_tree<MODE_MEASURE, 3>(c->config, D.n_sites, nullptr, nullptr, D.active, c->partial, c->st,
+                                     c->obs, 0, (double)D.n_active, c->stream);
+    }
   }
   hipEvent_t t1 = next_event(c);
   HIPCHK(hipEventRecord(t1, c->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->timing = false;
+  {
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, c->err, sizeof e, hipMemcpyDeviceToHost));
+    if (e) return sce_fail(SCE_EHIP, "persistent sweep: grid barrier wait exceeded its bound");
+  }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, t0, t1));
   c->total_ms = ms;

@@ -89,7 +89,7 @@ EXPORTS = [
     "sce_tables_destroy", "sce_tables_layout_info", "sce_layout_check", "sce_chain_create", "sce_chain_destroy",
     "sce_chain_info_get", "sce_chain_set_step", "sce_chain_reset_counters", "sce_download",
     "sce_upload", "sce_swap_payload", "sce_metropolis_sweep", "sce_overrelaxation_sweep",
-    "sce_run_sweeps", "sce_renormalize", "sce_chain_total_energy", "sce_chain_set_timing",
+    "sce_run_sweeps", "sce_renormalize", "sce_chain_total_energy", "sce_chain_set_timing", "sce_chain_set_persistent",
     "sce_chain_timing", "sce_test_philox", "sce_test_detmath", "sce_test_zlm",
     "sce_test_proposal", "sce_test_walk_delta", "sce_test_walk_coeffs", "sce_test_reduce",
 ]
@@ -120,7 +120,7 @@ def lib():
             "sce_overrelaxation_sweep": [P, d, i32, i32, P],
             "sce_run_sweeps": [P, d, i64, i32, i64, i64, i32, P, i64, P],
             "sce_renormalize": [P, P], "sce_chain_total_energy": [P, P],
-            "sce_chain_set_timing": [P, i32], "sce_chain_timing": [P, P, P, P],
+            "sce_chain_set_timing": [P, i32], "sce_chain_set_persistent": [P, i32], "sce_chain_timing": [P, P, P, P],
             "sce_test_philox": [i64, P, P, P], "sce_test_detmath": [i64, i32, P, P],
             "sce_test_zlm": [i64, i32, P, P],
             "sce_test_proposal": [i64, u64, P, P, P, d, P, P],

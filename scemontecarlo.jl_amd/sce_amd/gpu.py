@@ -117,6 +117,10 @@ class GPUChainState:
         L.check(L.lib().sce_download(self._h, L.ptr(cfg), L.ptr(z), C.byref(e)))
         return cfg, z, e.value
 
+    def set_persistent(self, on: bool) -> None:
+        """Persistent cooperative launches (default) or per-colour launches."""
+        L.check(L.lib().sce_chain_set_persistent(self._h, 1 if on else 0))
+
     def set_timing(self, on: bool) -> None:
         L.check(L.lib().sce_chain_set_timing(self._h, 1 if on else 0))
 

@@ -212,3 +212,32 @@ def test_run_sweeps_matches_single_sweeps():
     assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
     m = np.mean(ca, axis=0)
     assert np.allclose(obs[-1, 1:], m, atol=1e-14) and obs[-1, 0] == Ea
+
+
+@pytest.mark.parametrize("lanes", LANES)
+@pytest.mark.parametrize("case", [c for c in fixture_cases() if c[0] in ("dimer", "dimer+field", "bcc-tet", "fcc-l2", "sc-onsite")],
+                         ids=lambda c: c[0])
+def test_persistent_equals_per_colour_and_oracle(case, lanes):
+    name, na, terms, dims, mm, fl = case
+    H, T = build_pair(na, terms, dims, mm, fl)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 12), 0.6)
+    a = S.GPUChainState(gH, st, seed=99)
+    b = S.GPUChainState(gH, st, seed=99)
+    b.set_persistent(False)
+    beta = 1 / 0.05
+    oa = S.gpu_run_sweeps(a, gH, None, beta, 23, renorm_interval=10, lanes=lanes, measure_interval=2)
+    ob = S.gpu_run_sweeps(b, gH, None, beta, 23, renorm_interval=10, lanes=lanes, measure_interval=2)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    assert np.array_equal(oa[:, 0], ob[:, 0]) and np.allclose(oa, ob, rtol=0, atol=1e-15)
+    # oracle: the same sweeps with device-contract renormalisations
+    c2, z2, E = st.config.copy(), st.zrows.copy(), st.energy
+    for sw in range(1, 24):
+        _, E, _, _ = T.keyed_sweep(c2, z2, E, beta, 0.6, 99, sw, lanes)
+        if sw % 10 == 0:
+            _, E = T.renormalize(c2, z2, E)
+    assert np.array_equal(ca, c2) and np.array_equal(za, z2) and Ea == E
+    ia, ib = a.info(), b.info()
+    assert (ia.acc_metro, ia.att_metro, ia.sweep_index) == (ib.acc_metro, ib.att_metro, ib.sweep_index)
