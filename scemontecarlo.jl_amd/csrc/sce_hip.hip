@@ -1259,6 +1259,8 @@ extern "C" int32_t sce_chain_create(sce_tables* t, const double* config, const d
   if ((r = dalloc(c->mem, &c->err, 4)) != SCE_OK) return fail(r);
   c->bar_words = 2 + (size_t)BAR_PAD * (65536 / BAR_GROUP + 2);
   if ((r = dalloc(c->mem, &c->bar, c->bar_words)) != SCE_OK) return fail(r);
+  if (hipMemset(c->err, 0, 4 * sizeof(int)) != hipSuccess || hipMemset(c->bar, 0, c->bar_words * 4) != hipSuccess)
+    return fail(sce_fail(SCE_EHIP, "hipMemset of the barrier state failed"));
   if ((r = dalloc(c->mem, &c->st, 1)) != SCE_OK) return fail(r);
   ChainDev h{};
   h.energy = energy;
@@ -1527,7 +1529,7 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->timing = false;
-  {
+  if (c->persistent && or_per_metropolis == 0 && nsweeps > 0) {
     int e = 0;
     HIPCHK(hipMemcpy(&e, c->err, sizeof e, hipMemcpyDeviceToHost));
     if (e) return sce_fail(SCE_EHIP, "persistent sweep: grid barrier wait exceeded its bound");
