@@ -630,6 +630,7 @@ struct SweepArgs {
   long long nsweeps, sweep0, i0;  // sweeps before this launch: chain total, this call
   long long obs_row0;
   int measure_interval;
+  int defer_last;  // the last sweep's measurement follows a renormalisation
   unsigned* bar;
   int* err;
   double beta, step;
@@ -691,7 +692,8 @@ __global__ __launch_bounds__(TPB) void k_sweeps(SweepArgs a) {
       if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
     }
     // first level of the deterministic reductions: 512-element chunk trees
-    const bool meas = a.measure_interval > 0 && ((a.i0 + i) % a.measure_interval) == 0;
+    const bool meas = a.measure_interval > 0 && ((a.i0 + i) % a.measure_interval) == 0 &&
+                      !(a.defer_last && i == a.nsweeps);
     for (long long ch = blockIdx.x; ch < nch; ch += gridDim.x) {
       const long long i0 = ch * TREE_B + t, i1 = i0 + 256;
       const double v = block_tree(i0 < n ? a.dE[i0] : 0.0, i1 < n ? a.dE[i1] : 0.0, sh);
@@ -1418,7 +1420,7 @@ static int32_t launch_sweeps(int L, sce_chain* c, SweepArgs& sa, hipStream_t s) 
 
 // Enqueue `n` Metropolis sweeps as one persistent launch.
 static int32_t enqueue_persistent(sce_chain* c, int L, double beta, long long n, long long i0,
-                                  long long row0, int64_t measure_interval) {
+                                  long long row0, int64_t measure_interval, int defer_last) {
   CHK(check_lds(c->t->dev, L));
   ChainDev h;
   CHK(read_state(c, &h));
@@ -1440,6 +1442,7 @@ static int32_t enqueue_persistent(sce_chain* c, int L, double beta, long long n,
   sa.i0 = i0;
   sa.obs_row0 = row0;
   sa.measure_interval = (int)measure_interval;
+  sa.defer_last = defer_last;
   sa.bar = c->bar;
   sa.err = c->err;
   sa.beta = beta;
@@ -1508,11 +1511,18 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
         e0 = next_event(c);
         HIPCHK(hipEventRecord(e0, c->stream));
       }
-      CHK(enqueue_persistent(c, lanes, beta, seg, i, row, measure_interval));
+      const bool renorm_after = renorm_interval > 0 && (i + seg) % renorm_interval == 0;
+      const bool meas_end = measure_interval > 0 && (i + seg) % measure_interval == 0;
+      CHK(enqueue_persistent(c, lanes, beta, seg, i, row, measure_interval, renorm_after && meas_end));
       if (c->timing) HIPCHK(hipEventRecord(next_event(c), c->stream));
       if (measure_interval > 0) row += (i + seg) / measure_interval - i / measure_interval;
       i += seg;
-      if (renorm_interval > 0 && i % renorm_interval == 0) enqueue_renorm(c);
+      if (renorm_after) {
+        enqueue_renorm(c);
+        if (meas_end)  // the reference measures after renormalising (run.jl:191-200)
+          launch_tree<MODE_MEASURE, 3>(c->config, D.n_sites, nullptr, nullptr, D.active, c->partial,
+                                       c->st, c->obs, 0, (double)D.n_active, c->stream);
+      }
     }
   } else {
     for (int64_t i = 1; i <= nsweeps; ++i) {
