@@ -11,9 +11,11 @@
 //                        the site's sublattice walk dealt round-robin over the
 //                        lanes, fma-accumulated ΔE, shuffle-tree fold, FP64
 //                        accept; per-site writes only (no atomics on state)
-//   k_tree               deterministic tree sum of the staged ΔE + counters,
-//                        folded into the device-resident chain state
-// all on the chain's stream with no host synchronisation between sweeps.
+//   k_chunks + k_final   deterministic tree sum of the staged ΔE + counters
+//                        (+ the mean spin when measuring), folded into the
+//                        device-resident chain state
+// all on the chain's stream with no host synchronisation between sweeps; a
+// sweep is captured once as a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
 
 // ---------------------------------------------------------------------------
 // deterministic tree reduction: chunks of 512 (zero padded) folded by offsets
-// 256..1; a multi-chunk input recurses over the chunk sums in the last block.
+// 256..1; a multi-chunk input recurses over the chunk sums.
 // ---------------------------------------------------------------------------
 enum { MODE_METRO = 0, MODE_OR = 1, MODE_RENORM = 2, MODE_TOTAL = 3, MODE_MEASURE = 4, MODE_TEST = 5 };
 
@@ -403,146 +405,126 @@ __device__ double block_tree(double x0, double x1, double* sh) {
   return v;  // valid in thread 0
 }
 
-template <int MODE, int NC>
-__global__ __launch_bounds__(256) void k_tree(const double* __restrict__ x, long long n,
-                                              const int* __restrict__ iacc,
-                                              const int* __restrict__ iatt,
-                                              const uint8_t* __restrict__ active,
-                                              double* __restrict__ partial, ChainDev* st,
-                                              double* __restrict__ out, long long n_att_fixed,
-                                              double n_div) {
+// ---------------------------------------------------------------------------
+// fence-free two-kernel reduction: k_chunks writes the 512-element chunk trees
+// (and integer counts, and optionally the three mean-spin components); one
+// block of k_final folds them (the kernel boundary makes the partials visible
+// across XCDs, so no device-scope fence is needed) and updates the chain.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_chunks(const double* __restrict__ x, long long n,
+                                                const int* __restrict__ iacc,
+                                                const int* __restrict__ iatt,
+                                                double* __restrict__ part,
+                                                long long* __restrict__ ipart,
+                                                const double* __restrict__ config,
+                                                const uint8_t* __restrict__ active,
+                                                long long n_sites, int meas) {
   __shared__ double sh[256];
-  __shared__ double res[NC];
-  __shared__ unsigned int ticket_sh;
+  __shared__ long long csum[8];
   const int t = threadIdx.x;
-  const long long base = (long long)blockIdx.x * TREE_B;
-  const long long i0 = base + t, i1 = base + t + 256;
-  long long ci = 0, ca = 0;
+  const long long b = blockIdx.x;
+  const long long nbx = (n + TREE_B - 1) / TREE_B > 0 ? (n + TREE_B - 1) / TREE_B : 1;
+  const long long nchm = (n_sites + TREE_B - 1) / TREE_B;
+  if (b < nbx) {
+    const long long i0 = b * TREE_B + t, i1 = i0 + 256;
+    const double v = block_tree(i0 < n ? x[i0] : 0.0, i1 < n ? x[i1] : 0.0, sh);
+    if (t == 0) part[b] = v;
+    if (MODE == MODE_METRO || MODE == MODE_OR) {
+      long long ca = (i0 < n ? iacc[i0] : 0) + (i1 < n ? iacc[i1] : 0);
+      long long ct = 0;
+      if (MODE == MODE_OR) ct = (i0 < n ? iatt[i0] : 0) + (i1 < n ? iatt[i1] : 0);
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    double x0 = 0.0, x1 = 0.0;
-    if (MODE == MODE_MEASURE) {
-      if (i0 < n && active[i0]) x0 = x[3 * i0 + c];
-      if (i1 < n && active[i1]) x1 = x[3 * i1 + c];
-    } else {
-      if (i0 < n) x0 = x[i0];
-      if (i1 < n) x1 = x[i1];
-    }
-    const double v = block_tree(x0, x1, sh);
-    if (t == 0) res[c] = v;
-  }
-  if (MODE == MODE_METRO || MODE == MODE_OR) {
-    if (i0 < n) ci += iacc[i0];
-    if (i1 < n) ci += iacc[i1];
-    if (MODE == MODE_OR) {
-      if (i0 < n) ca += iatt[i0];
-      if (i1 < n) ca += iatt[i1];
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      ci += __shfl_down(ci, off, 64);
-      ca += __shfl_down(ca, off, 64);
-    }
-    if ((t & 63) == 0) {
-      if (ci) atomicAdd(&st->iacc, (unsigned long long)ci);
-      if (ca) atomicAdd(&st->iatt, (unsigned long long)ca);
-    }
-  }
-  __syncthreads();
-  double fin[NC];
-  if (gridDim.x == 1) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) fin[c] = res[c];
-  } else {
-    if (t == 0) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) partial[(size_t)c * gridDim.x + blockIdx.x] = res[c];
-    }
-    __threadfence();
-    if (t == 0) ticket_sh = atomicAdd(&st->ticket, 1u);
-    __syncthreads();
-    if (ticket_sh != gridDim.x - 1) return;
-    __threadfence();
-    // last block: reduce the chunk sums level by level
-    volatile double* vp = partial;
-    long long R = 2;  // per-component room for the higher levels
-    for (long long m = (gridDim.x + TREE_B - 1) / TREE_B;; m = (m + TREE_B - 1) / TREE_B) {
-      R += m + 2;
-      if (m <= 1) break;
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      long long m = gridDim.x;
-      size_t src = (size_t)c * gridDim.x;
-      size_t dst = (size_t)NC * gridDim.x + (size_t)c * (size_t)R;
-      while (true) {
-        if (m <= TREE_B) {
-          const double a0 = t < m ? vp[src + t] : 0.0;
-          const double a1 = t + 256 < m ? vp[src + t + 256] : 0.0;
-          const double v = block_tree(a0, a1, sh);
-          if (t == 0) res[c] = v;
-          __syncthreads();
-          break;
-        }
-        const long long nb = (m + TREE_B - 1) / TREE_B;
-        for (long long b = 0; b < nb; ++b) {
-          const long long j0 = b * TREE_B + t, j1 = j0 + 256;
-          const double a0 = j0 < m ? vp[src + j0] : 0.0;
-          const double a1 = j1 < m ? vp[src + j1] : 0.0;
-          const double v = block_tree(a0, a1, sh);
-          if (t == 0) vp[dst + b] = v;
-          __syncthreads();
-        }
-        __threadfence_block();
-        src = dst;
-        dst = dst + (size_t)nb;
-        m = nb;
+      for (int off = 32; off >= 1; off >>= 1) {
+        ca += __shfl_down(ca, off, 64);
+        ct += __shfl_down(ct, off, 64);
+      }
+      if ((t & 63) == 0) {
+        csum[t >> 6] = ca;
+        csum[4 + (t >> 6)] = ct;
+      }
+      __syncthreads();
+      if (t == 0) {
+        ipart[2 * b] = csum[0] + csum[1] + csum[2] + csum[3];
+        ipart[2 * b + 1] = csum[4] + csum[5] + csum[6] + csum[7];
       }
     }
+  }
+  if (meas && b < nchm) {
+    const long long i0 = b * TREE_B + t, i1 = i0 + 256;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) fin[c] = res[c];
+    for (int c = 0; c < 3; ++c) {
+      const double x0 = (i0 < n_sites && active[i0]) ? config[3 * i0 + c] : 0.0;
+      const double x1 = (i1 < n_sites && active[i1]) ? config[3 * i1 + c] : 0.0;
+      const double v = block_tree(x0, x1, sh);
+      if (t == 0) part[nbx + c * nchm + b] = v;
+    }
+  }
+}
+
+__device__ double final_tree(const double* buf, long long m, double* scratch, double* sh);
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_final(double* part, long long n, const long long* ipart,
+                                               long long n_sites, int meas, ChainDev* st,
+                                               double* __restrict__ out, long long n_att,
+                                               double n_div, double* scratch) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  const long long nbx = (n + TREE_B - 1) / TREE_B > 0 ? (n + TREE_B - 1) / TREE_B : 1;
+  const long long nchm = (n_sites + TREE_B - 1) / TREE_B;
+  const double sum = final_tree(part, nbx, scratch, sh);
+  double m3[3] = {0.0, 0.0, 0.0};
+  if (meas) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      __syncthreads();
+      m3[c] = final_tree(part + nbx + c * nchm, nchm, scratch, sh);
+    }
   }
   if (t != 0) return;
+  long long nacc = 0, natt = 0;
+  if (MODE == MODE_METRO || MODE == MODE_OR)
+    for (long long b = 0; b < nbx; ++b) {
+      nacc += ipart[2 * b];
+      natt += ipart[2 * b + 1];
+    }
   if (MODE == MODE_METRO) {
-    const long long nacc = (long long)atomicExch(&st->iacc, 0ull);
-    st->energy = st->energy + fin[0];
+    st->energy = st->energy + sum;
     st->acc_metro += nacc;
-    st->att_metro += n_att_fixed;
+    st->att_metro += n_att;
     st->nacc_last = nacc;
     st->sweep_index += 1;
   } else if (MODE == MODE_OR) {
-    const long long nacc = (long long)atomicExch(&st->iacc, 0ull);
-    const long long natt = (long long)atomicExch(&st->iatt, 0ull);
-    st->energy = st->energy + fin[0];
+    st->energy = st->energy + sum;
     st->acc_or += nacc;
     st->att_or += natt;
     st->nacc_last = nacc;
   } else if (MODE == MODE_RENORM) {
-    const double E = fin[0];
-    const double drift = fabs(st->energy - E);
+    const double drift = fabs(st->energy - sum);
     st->last_drift = drift;
     if (drift > st->max_drift) st->max_drift = drift;
-    st->energy = E;
+    st->energy = sum;
   } else if (MODE == MODE_TOTAL) {
-    st->e_full = fin[0];
-  } else if (MODE == MODE_MEASURE) {
+    st->e_full = sum;
+  } else if (MODE == MODE_TEST) {
+    out[0] = sum;
+  }
+  if (meas) {
     const int row = st->n_obs;
     out[4 * (size_t)row] = st->energy;
-    out[4 * (size_t)row + 1] = fin[0] / n_div;
-    out[4 * (size_t)row + 2] = fin[1] / n_div;
-    out[4 * (size_t)row + 3] = fin[2] / n_div;
+    out[4 * (size_t)row + 1] = m3[0] / n_div;
+    out[4 * (size_t)row + 2] = m3[1] / n_div;
+    out[4 * (size_t)row + 3] = m3[2] / n_div;
     st->n_obs = row + 1;
-  } else {
-    out[0] = fin[0];
   }
-  st->ticket = 0u;
 }
 
 // ---------------------------------------------------------------------------
 // persistent sweeps: one cooperative launch runs many Metropolis sweeps —
 // proposals, the colour steps and the deterministic reductions — separated by
 // grid barriers instead of kernel boundaries.  The arithmetic is exactly that
-// of k_propose / k_metro / k_tree.
+// of k_propose / k_metro / k_chunks + k_final.
 // ---------------------------------------------------------------------------
 constexpr int BAR_GROUP = 64;  // blocks per first-level barrier counter
 constexpr int BAR_PAD = 32;    // u32 words between counters (one 128-byte line)
@@ -902,7 +884,11 @@ struct sce_chain {
   unsigned* bar = nullptr;
   size_t bar_words = 0;
   ChainDev* st = nullptr;
-  bool persistent = true;   // run sweeps in one cooperative launch when possible
+  long long* ip64 = nullptr;  // integer partials of the reductions
+  bool persistent = false;  // run sweeps in one cooperative launch (opt-in)
+  bool use_graph = true;    // replay captured sweeps as hipGraphs
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // [plain, measured] sweep
+  int g_lanes = 0;
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
   long long sweeps_done = 0;  // host mirror of sweep_index
@@ -914,7 +900,16 @@ struct sce_chain {
   bool timing_enabled = false;
   double total_ms = 0.0, walk_ms = 0.0;
   long long walk_launches = 0;
+  void drop_graphs() {
+    for (auto& g : gexec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    g_lanes = 0;
+  }
   ~sce_chain() {
+    drop_graphs();
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -1137,13 +1132,19 @@ static void launch_renorm(const DevTables& D, double* config, double* zrows, hip
   if (nb) k_renorm<LMAX><<<nb, 256, 0, s>>>(D, config, zrows);
 }
 
-template <int MODE, int NC>
-static void launch_tree(const double* x, long long n, const int* iacc, const int* iatt,
-                        const uint8_t* active, double* partial, ChainDev* st, double* out,
-                        long long n_att, double n_div, hipStream_t s) {
-  long long nb = (n + TREE_B - 1) / TREE_B;
-  if (nb < 1) nb = 1;
-  k_tree<MODE, NC><<<(unsigned)nb, 256, 0, s>>>(x, n, iacc, iatt, active, partial, st, out, n_att, n_div);
+// Deterministic reduction of x[0..n) (+ the mean spin when meas) into the chain
+// state: k_chunks over max(#x chunks, #site chunks) blocks, then k_final.
+template <int MODE>
+static void launch_reduce(const double* x, long long n, const int* iacc, const int* iatt,
+                          double* part, long long* ipart, const double* config,
+                          const uint8_t* active, long long n_sites, int meas, ChainDev* st,
+                          double* out, long long n_att, double n_div, hipStream_t s) {
+  const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1);
+  const long long nchm = meas ? (n_sites + TREE_B - 1) / TREE_B : 0;
+  const long long grid = std::max(nbx, nchm);
+  double* scratch = part + nbx + 3 * std::max<long long>(nchm, 1);
+  k_chunks<MODE><<<(unsigned)grid, 256, 0, s>>>(x, n, iacc, iatt, part, ipart, config, active, n_sites, meas);
+  k_final<MODE><<<1, 256, 0, s>>>(part, n, ipart, n_sites, meas, st, out, n_att, n_div, scratch);
 }
 
 static int32_t chain_check(sce_chain* c) {
@@ -1170,7 +1171,7 @@ static int32_t set_beta(sce_chain* c, double beta) {
 }
 
 // Enqueue one Metropolis sweep (no synchronisation).
-static int32_t enqueue_metro(sce_chain* c, int L) {
+static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0) {
   CHK(check_lds(c->t->dev, L));
   if (c->sweeps_done >= 2147483647LL - 1)
     return sce_fail(SCE_ERANGE_COUNTER, "sweep_index exhausted the 32-bit RNG counter word");
@@ -1191,8 +1192,8 @@ static int32_t enqueue_metro(sce_chain* c, int L) {
       (void)hipEventRecord(e1, s);
     }
   }
-  launch_tree<MODE_METRO, 1>(c->dE, D.n_active, c->acc, nullptr, nullptr, c->partial, c->st, nullptr,
-                             D.n_active, 0.0, s);
+  launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config,
+                            D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s);
   c->sweeps_done += 1;
   return SCE_OK;
 }
@@ -1206,8 +1207,8 @@ static int32_t enqueue_or(sce_chain* c, int L, int pass) {
     if (n == 0) continue;
     SCE_DISPATCH_LMAX(D.lmax, launch_or_run, L, D, c->config, c->zrows, c->dE, c->acc, c->att, c->st, pass, lo, n, s);
   }
-  launch_tree<MODE_OR, 1>(c->dE, D.n_active, c->acc, c->att, nullptr, c->partial, c->st, nullptr, 0,
-                          0.0, s);
+  launch_reduce<MODE_OR>(c->dE, D.n_active, c->acc, c->att, c->partial, c->ip64, c->config, D.active,
+                         D.n_sites, 0, c->st, nullptr, 0, 1.0, s);
   return SCE_OK;
 }
 
@@ -1216,15 +1217,49 @@ static void enqueue_energy(sce_chain* c, int mode) {
   const int nb = (D.n_inst + 255) / 256;
   if (nb) k_inst_energy<<<nb, 256, 0, c->stream>>>(D, c->zrows, c->inst_e);
   if (mode == MODE_RENORM)
-    launch_tree<MODE_RENORM, 1>(c->inst_e, D.n_inst, nullptr, nullptr, nullptr, c->partial, c->st, nullptr, 0, 0.0, c->stream);
+    launch_reduce<MODE_RENORM>(c->inst_e, D.n_inst, nullptr, nullptr, c->partial, c->ip64, c->config,
+                               D.active, D.n_sites, 0, c->st, nullptr, 0, 1.0, c->stream);
   else
-    launch_tree<MODE_TOTAL, 1>(c->inst_e, D.n_inst, nullptr, nullptr, nullptr, c->partial, c->st, nullptr, 0, 0.0, c->stream);
+    launch_reduce<MODE_TOTAL>(c->inst_e, D.n_inst, nullptr, nullptr, c->partial, c->ip64, c->config,
+                              D.active, D.n_sites, 0, c->st, nullptr, 0, 1.0, c->stream);
 }
 
 static void enqueue_renorm(sce_chain* c) {
   const DevTables& D = c->t->dev;
   SCE_DISPATCH_LMAX(D.lmax, launch_renorm, D, c->config, c->zrows, c->stream);
   enqueue_energy(c, MODE_RENORM);
+}
+
+static void enqueue_measure(sce_chain* c) {
+  const DevTables& D = c->t->dev;
+  launch_reduce<MODE_MEASURE>(c->dE, 0, nullptr, nullptr, c->partial, c->ip64, c->config, D.active,
+                              D.n_sites, 1, c->st, c->obs, 0, (double)D.n_active, c->stream);
+}
+
+// Capture one Metropolis sweep (plain and with the fused measurement) as
+// hipGraphs.  Every kernel reads the sweep counter, β and step from the
+// device-resident chain state, so a captured sweep replays unchanged.
+static int32_t ensure_graphs(sce_chain* c, int L) {
+  if (c->gexec[0] && c->g_lanes == L) return SCE_OK;
+  c->drop_graphs();
+  for (int k = 0; k < 2; ++k) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const long long before = c->sweeps_done;
+    const int32_t r = enqueue_metro(c, L, k);
+    c->sweeps_done = before;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != SCE_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) return sce_fail(SCE_EHIP, "sweep capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  c->g_lanes = L;
+  return SCE_OK;
 }
 
 static int32_t read_state(sce_chain* c, ChainDev* h) {
@@ -1258,6 +1293,7 @@ extern "C" int32_t sce_chain_create(sce_tables* t, const double* config, const d
   if ((r = dalloc(c->mem, &c->partial, partial_capacity(nmax) + 4 * ((size_t)nmax / TREE_B + 4))) != SCE_OK)
     return fail(r);
   if ((r = dalloc(c->mem, &c->ipart, (size_t)nmax / TREE_B + 4)) != SCE_OK) return fail(r);
+  if ((r = dalloc(c->mem, &c->ip64, 2 * ((size_t)nmax / TREE_B + 4))) != SCE_OK) return fail(r);
   if ((r = dalloc(c->mem, &c->err, 4)) != SCE_OK) return fail(r);
   c->bar_words = 2 + (size_t)BAR_PAD * (65536 / BAR_GROUP + 2);
   if ((r = dalloc(c->mem, &c->bar, c->bar_words)) != SCE_OK) return fail(r);
@@ -1357,6 +1393,8 @@ extern "C" int32_t sce_swap_payload(sce_chain* a, sce_chain* b) {
   CHK(read_state(b, &hb));
   std::swap(a->config, b->config);
   std::swap(a->zrows, b->zrows);
+  a->drop_graphs();
+  b->drop_graphs();
   std::swap(ha.energy, hb.energy);
   HIPCHK(hipMemcpy(a->st, &ha, sizeof ha, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->st, &hb, sizeof hb, hipMemcpyHostToDevice));
@@ -1486,6 +1524,7 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
     CHK(dalloc(c->mem, &p, 4 * (size_t)want_obs));
     c->obs = p;
     c->obs_cap = (size_t)want_obs;
+    c->drop_graphs();
   }
   CHK(set_beta(c, beta));
   // reset the observation counter
@@ -1520,18 +1559,27 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
       if (renorm_after) {
         enqueue_renorm(c);
         if (meas_end)  // the reference measures after renormalising (run.jl:191-200)
-          launch_tree<MODE_MEASURE, 3>(c->config, D.n_sites, nullptr, nullptr, D.active, c->partial,
-                                       c->st, c->obs, 0, (double)D.n_active, c->stream);
+          enqueue_measure(c);
       }
     }
   } else {
+    const bool graphs = c->use_graph && !c->timing && or_per_metropolis == 0;
+    if (graphs) CHK(ensure_graphs(c, lanes));
     for (int64_t i = 1; i <= nsweeps; ++i) {
-      CHK(enqueue_metro(c, lanes));
+      const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
+      const bool meas = measure_interval > 0 && i % measure_interval == 0;
+      // a measurement rides on the sweep's own reduction unless something
+      // (overrelaxation, renormalisation) comes between them (run.jl:191-200)
+      const bool fused = meas && !renorm && or_per_metropolis == 0;
+      if (graphs) {
+        HIPCHK(hipGraphLaunch(c->gexec[fused ? 1 : 0], c->stream));
+        c->sweeps_done += 1;
+      } else {
+        CHK(enqueue_metro(c, lanes, fused ? 1 : 0));
+      }
       for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
-      if (renorm_interval > 0 && i % renorm_interval == 0) enqueue_renorm(c);
-      if (measure_interval > 0 && i % measure_interval == 0)
-        launch_tree<MODE_MEASURE, 3>(c->config, D.n_sites, nullptr, nullptr, D.active, c->partial, c->st,
-                                     c->obs, 0, (double)D.n_active, c->stream);
+      if (renorm) enqueue_renorm(c);
+      if (meas && !fused) enqueue_measure(c);
     }
   }
   hipEvent_t t1 = next_event(c);
@@ -1749,7 +1797,10 @@ extern "C" int32_t sce_test_reduce(int64_t n, const double* x, double* out) {
   CHK(o.alloc(1));
   ChainDev h{};
   CHK(st.up(&h, 1));
-  launch_tree<MODE_TEST, 1>(xi.p, n, nullptr, nullptr, nullptr, part.p, st.p, o.p, 0, 0.0, nullptr);
+  Scratch<long long> ip;
+  CHK(ip.alloc(2 * ((size_t)n / TREE_B + 4)));
+  launch_reduce<MODE_TEST>(xi.p, n, nullptr, nullptr, part.p, ip.p, nullptr, nullptr, 0, 0, st.p, o.p, 0,
+                           1.0, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   return o.down(out, 1);

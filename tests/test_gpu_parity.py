@@ -217,14 +217,14 @@ def test_run_sweeps_matches_single_sweeps():
 @pytest.mark.parametrize("lanes", LANES)
 @pytest.mark.parametrize("case", [c for c in fixture_cases() if c[0] in ("dimer", "dimer+field", "bcc-tet", "fcc-l2", "sc-onsite")],
                          ids=lambda c: c[0])
-def test_persistent_equals_per_colour_and_oracle(case, lanes):
+def test_persistent_equals_graph_path_and_oracle(case, lanes):
     name, na, terms, dims, mm, fl = case
     H, T = build_pair(na, terms, dims, mm, fl)
     gH = S.GPUTiledHamiltonian(H)
     st = S.ChainState(H, rand_config(H.n_sites, 12), 0.6)
     a = S.GPUChainState(gH, st, seed=99)
     b = S.GPUChainState(gH, st, seed=99)
-    b.set_persistent(False)
+    a.set_persistent(True)
     beta = 1 / 0.05
     oa = S.gpu_run_sweeps(a, gH, None, beta, 23, renorm_interval=10, lanes=lanes, measure_interval=2)
     ob = S.gpu_run_sweeps(b, gH, None, beta, 23, renorm_interval=10, lanes=lanes, measure_interval=2)
