@@ -122,10 +122,12 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
 template <int LMAX>
 __global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __restrict__ config,
                                                  double* __restrict__ prop,
-                                                 const ChainDev* __restrict__ st) {
+                                                 const ChainDev* __restrict__ st, int q_lo,
+                                                 int q_n) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= T.n_active) return;
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= q_n) return;
+  const int q = q_lo + qi;
   const int s = T.color_sites[q];
   const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
   double e2[3], u, z[NLM];
@@ -176,16 +178,30 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
 // ---------------------------------------------------------------------------
 constexpr int WALK_UNROLL = 8;
 
+// `pre` holds the lane's first WALK_UNROLL entries, loaded before the
+// neighbour staging so that their latency overlaps it.
+template <int L, int NLM>
+__device__ __forceinline__ void walk_prefetch(const WalkEntry* __restrict__ wl, int W, int lane,
+                                              WalkEntry* pre) {
+#pragma unroll
+  for (int u = 0; u < WALK_UNROLL; ++u) {
+    const int i = u * L + lane;
+    if (i < W) pre[u] = wl[i];
+  }
+}
+
 template <int L, int NLM>
 __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, int W, int lane,
-                                             const double* Z, const double* dz) {
+                                             const double* Z, const double* dz,
+                                             const WalkEntry* pre) {
   double a = 0.0;
   for (int base = 0; base < W; base += WALK_UNROLL * L) {
     WalkEntry en[WALK_UNROLL];
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
       const int i = base + u * L + lane;
-      if (i < W) en[u] = wl[i];
+      if (base == 0) en[u] = pre[u];
+      else if (i < W) en[u] = wl[i];
     }
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
@@ -214,19 +230,21 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   const int stride = T.nb_max * NLM + 1 + NLM;
   double* Z = lds + g * stride;
   double* dz = Z + T.nb_max * NLM + 1;
-  int s = 0, atom = 0;
+  int s = 0, atom = 0, w0 = 0, W = 0;
+  WalkEntry pre[WALK_UNROLL];
   if (valid) {
     s = TEST ? test_sites[q] : T.color_sites[q];
+    atom = s % T.na;
+    w0 = T.walk_ptr[atom];
+    W = T.walk_ptr[atom + 1] - w0;
+    walk_prefetch<L, NLM>(T.walk + w0, W, lane, pre);
     stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
     const double* pz = prop + (size_t)q * PS + 4;
     for (int k = lane; k < NLM; k += L) dz[k] = pz[k] - zrows[(size_t)s * NLM + k];
   }
   __syncthreads();
   double a = 0.0;
-  if (valid) {
-    const int w0 = T.walk_ptr[atom];
-    a = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
-  }
+  if (valid) a = walk_delta<L, NLM>(T.walk + w0, W, lane, Z, dz, pre);
 #pragma unroll
   for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
   __syncthreads();  // the block's LDS is reused by the next site group
@@ -473,22 +491,65 @@ __global__ __launch_bounds__(256) void k_final(double* part, long long n, const 
   const int t = threadIdx.x;
   const long long nbx = (n + TREE_B - 1) / TREE_B > 0 ? (n + TREE_B - 1) / TREE_B : 1;
   const long long nchm = (n_sites + TREE_B - 1) / TREE_B;
-  const double sum = final_tree(part, nbx, scratch, sh);
+  double sum;
   double m3[3] = {0.0, 0.0, 0.0};
-  if (meas) {
+  long long nacc = 0, natt = 0;
+  if (MODE == MODE_METRO || MODE == MODE_OR) {
+    // integer counts: order-free sum over the chunk partials
+    long long ca = 0, ct = 0;
+    for (long long b = t; b < nbx; b += 256) {
+      ca += ipart[2 * b];
+      ct += ipart[2 * b + 1];
+    }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      __syncthreads();
-      m3[c] = final_tree(part + nbx + c * nchm, nchm, scratch, sh);
+    for (int off = 32; off >= 1; off >>= 1) {
+      ca += __shfl_down(ca, off, 64);
+      ct += __shfl_down(ct, off, 64);
+    }
+    __shared__ long long cs[8];
+    if ((t & 63) == 0) {
+      cs[t >> 6] = ca;
+      cs[4 + (t >> 6)] = ct;
+    }
+    __syncthreads();
+    nacc = cs[0] + cs[1] + cs[2] + cs[3];
+    natt = cs[4] + cs[5] + cs[6] + cs[7];
+  }
+  if (nbx > 1 && nbx <= TREE_B && (!meas || nchm <= TREE_B)) {
+    // single-level case: every component's partials loaded in one pass
+    double a0[4], a1[4];
+    const long long len[4] = {nbx, nchm, nchm, nchm};
+    const double* src[4] = {part, part + nbx, part + nbx + nchm, part + nbx + 2 * nchm};
+    const int nc = meas ? 4 : 1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      a0[c] = (c < nc && t < len[c]) ? src[c][t] : 0.0;
+      a1[c] = (c < nc && t + 256 < len[c]) ? src[c][t + 256] : 0.0;
+    }
+    if (nchm == 1) {  // a single chunk is already the final value
+      a0[1] = t == 0 ? src[1][0] : 0.0;
+      a0[2] = t == 0 ? src[2][0] : 0.0;
+      a0[3] = t == 0 ? src[3][0] : 0.0;
+    }
+    sum = block_tree(a0[0], a1[0], sh);
+    if (meas) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double v = (nchm == 1) ? a0[c + 1] : block_tree(a0[c + 1], a1[c + 1], sh);
+        m3[c] = v;
+      }
+    }
+  } else {
+    sum = final_tree(part, nbx, scratch, sh);
+    if (meas) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        __syncthreads();
+        m3[c] = final_tree(part + nbx + c * nchm, nchm, scratch, sh);
+      }
     }
   }
   if (t != 0) return;
-  long long nacc = 0, natt = 0;
-  if (MODE == MODE_METRO || MODE == MODE_OR)
-    for (long long b = 0; b < nbx; ++b) {
-      nacc += ipart[2 * b];
-      natt += ipart[2 * b + 1];
-    }
   if (MODE == MODE_METRO) {
     st->energy = st->energy + sum;
     st->acc_metro += nacc;
@@ -889,6 +950,9 @@ struct sce_chain {
   bool use_graph = true;    // replay captured sweeps as hipGraphs
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // [plain, measured] sweep
   int g_lanes = 0;
+  hipStream_t side = nullptr;           // proposal branch of a captured sweep
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_col;       // one per colour
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
   long long sweeps_done = 0;  // host mirror of sweep_index
@@ -911,6 +975,9 @@ struct sce_chain {
   ~sce_chain() {
     drop_graphs();
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_col) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -1076,7 +1143,14 @@ template <int LMAX>
 static void launch_propose(const DevTables& D, const double* config, double* prop,
                            const ChainDev* st, hipStream_t s) {
   const int nb = (D.n_active + 255) / 256;
-  if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st);
+  if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st, 0, D.n_active);
+}
+
+template <int LMAX>
+static void launch_propose_range(const DevTables& D, const double* config, double* prop,
+                                 const ChainDev* st, int q_lo, int q_n, hipStream_t s) {
+  const int nb = (q_n + 127) / 128;
+  if (nb) k_propose<LMAX><<<nb, 128, 0, s>>>(D, config, prop, st, q_lo, q_n);
 }
 
 template <int L, int LMAX, bool TEST>
@@ -1171,15 +1245,29 @@ static int32_t set_beta(sce_chain* c, double beta) {
 }
 
 // Enqueue one Metropolis sweep (no synchronisation).
-static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0) {
+// With `side` (during graph capture) the proposals of colour c are a separate
+// kernel on a side stream that only colour c's walk waits for, so proposing
+// overlaps with the walks of the colours before it.
+static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, hipStream_t side = nullptr) {
   CHK(check_lds(c->t->dev, L));
   if (c->sweeps_done >= 2147483647LL - 1)
     return sce_fail(SCE_ERANGE_COUNTER, "sweep_index exhausted the 32-bit RNG counter word");
   const DevTables& D = c->t->dev;
   hipStream_t s = c->stream;
-  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
+  if (side) {
+    HIPCHK(hipEventRecord(c->ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(side, c->ev_fork, 0));
+    for (int col = 0; col < c->t->n_colors; ++col) {
+      const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
+      SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, lo, n, side);
+      HIPCHK(hipEventRecord(c->ev_col[col], side));
+    }
+  } else {
+    SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
+  }
   for (int col = 0; col < c->t->n_colors; ++col) {
     const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
+    if (side) HIPCHK(hipStreamWaitEvent(s, c->ev_col[col], 0));
     if (n == 0) continue;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
@@ -1246,7 +1334,7 @@ static int32_t ensure_graphs(sce_chain* c, int L) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     const long long before = c->sweeps_done;
-    const int32_t r = enqueue_metro(c, L, k);
+    const int32_t r = enqueue_metro(c, L, k, c->side);
     c->sweeps_done = before;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (r != SCE_OK) {
@@ -1307,8 +1395,15 @@ extern "C" int32_t sce_chain_create(sce_tables* t, const double* config, const d
   h.seed = seed;
   if (hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess)
     return fail(sce_fail(SCE_EHIP, "upload of chain state failed"));
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess)
     return fail(sce_fail(SCE_EHIP, "hipStreamCreate failed"));
+  if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+    return fail(sce_fail(SCE_EHIP, "hipEventCreate failed"));
+  c->ev_col.resize((size_t)t->n_colors);
+  for (auto& e : c->ev_col)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(sce_fail(SCE_EHIP, "hipEventCreate failed"));
   c->beta_dev = 0.0;
   *out = c;
   return SCE_OK;
