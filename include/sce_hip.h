@@ -159,6 +159,42 @@ int32_t sce_chain_set_timing(sce_chain* c, int32_t on);
 int32_t sce_chain_timing(sce_chain* c, double* total_ms, double* walk_ms_avg,
                          int64_t* walk_launches);
 
+/* ---- batches: many chains in one launch per colour --------------------------
+ * n_chains chains, each on its own tables (a replica ladder shares one tables
+ * handle; uneven batches mix models and sizes with equal lmax), with its own
+ * keyed stream seed, step and kT.  Every chain is bit-identical to the same
+ * chain run alone.  With exchange_interval > 0 (all chains on the same tables,
+ * kT ladder in chain order) adjacent chains exchange payloads on the device
+ * with the rule of _swap_accepts (src/pt.jl:92-96) and the alternating-parity
+ * schedule of _gpu_pt_phase! (src/gpu/gpu_pt.jl:115-140); the exchange uniform
+ * is keyed by (exchange_seed, pair, exchange count). */
+int32_t sce_batch_create(int32_t n_chains, sce_tables* const* tables,
+                         const double* const* configs, const double* const* zrows,
+                         const double* energies, const uint64_t* seeds, const double* steps,
+                         const double* kts, uint64_t exchange_seed, sce_batch** out);
+int32_t sce_batch_destroy(sce_batch* b);
+/* nsweeps sweeps of every chain.  Sweep j = phase_done + i (i = 1..nsweeps) of
+ * the current phase renormalises when j % renorm_interval == 0, measures
+ * (E, m) of every chain into obs (chain-major, obs_cap rows per chain) when
+ * j % measure_interval == 0, and exchanges when j % exchange_interval == 0 and
+ * j < phase_total (no exchange after a phase's last sweep, gpu_pt.jl:128). */
+int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, int64_t renorm_interval,
+                      int64_t measure_interval, int64_t exchange_interval, int64_t phase_done,
+                      int64_t phase_total, double* obs, int64_t obs_cap, int64_t* n_obs);
+int32_t sce_batch_download(sce_batch* b, int32_t k, double* config, double* zrows,
+                           double* energy);
+int32_t sce_batch_info(sce_batch* b, int32_t k, sce_chain_info* out);
+int32_t sce_batch_set_step(sce_batch* b, int32_t k, double step);
+/* Zeroes every chain's acceptance counters and drift record (exchange
+ * statistics accumulate over the whole run, as in the reference). */
+int32_t sce_batch_reset_counters(sce_batch* b);
+/* _renormalize! of every chain on the device (state.jl:168-185). */
+int32_t sce_batch_renormalize(sce_batch* b);
+/* Attempted / accepted exchanges of pair (k, k+1), k = 0..n_chains-2. */
+int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc);
+/* Replay captured sweeps as hipGraphs (on != 0, default) or launch directly. */
+int32_t sce_batch_set_graph(sce_batch* b, int32_t on);
+
 /* ---- diagnostics (no reference counterpart): the device arithmetic exposed
  * for bit-exact parity tests ------------------------------------------------- */
 int32_t sce_test_philox(int64_t n, const uint64_t* seeds, const uint32_t* ctr4,

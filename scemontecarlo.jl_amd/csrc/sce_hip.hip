@@ -178,30 +178,16 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
 // ---------------------------------------------------------------------------
 constexpr int WALK_UNROLL = 8;
 
-// `pre` holds the lane's first WALK_UNROLL entries, loaded before the
-// neighbour staging so that their latency overlaps it.
-template <int L, int NLM>
-__device__ __forceinline__ void walk_prefetch(const WalkEntry* __restrict__ wl, int W, int lane,
-                                              WalkEntry* pre) {
-#pragma unroll
-  for (int u = 0; u < WALK_UNROLL; ++u) {
-    const int i = u * L + lane;
-    if (i < W) pre[u] = wl[i];
-  }
-}
-
 template <int L, int NLM>
 __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, int W, int lane,
-                                             const double* Z, const double* dz,
-                                             const WalkEntry* pre) {
+                                             const double* Z, const double* dz) {
   double a = 0.0;
   for (int base = 0; base < W; base += WALK_UNROLL * L) {
     WalkEntry en[WALK_UNROLL];
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
       const int i = base + u * L + lane;
-      if (base == 0) en[u] = pre[u];
-      else if (i < W) en[u] = wl[i];
+      if (i < W) en[u] = wl[i];
     }
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
@@ -224,27 +210,25 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
                                             const double* __restrict__ prop,
                                             double* __restrict__ dE, int* __restrict__ acc,
                                             double beta, int q, bool valid,
-                                            const int* __restrict__ test_sites, double* lds) {
+                                            const int* __restrict__ test_sites, double* lds,
+                                            int stride) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
-  const int stride = T.nb_max * NLM + 1 + NLM;
   double* Z = lds + g * stride;
   double* dz = Z + T.nb_max * NLM + 1;
-  int s = 0, atom = 0, w0 = 0, W = 0;
-  WalkEntry pre[WALK_UNROLL];
+  int s = 0, atom = 0;
   if (valid) {
     s = TEST ? test_sites[q] : T.color_sites[q];
-    atom = s % T.na;
-    w0 = T.walk_ptr[atom];
-    W = T.walk_ptr[atom + 1] - w0;
-    walk_prefetch<L, NLM>(T.walk + w0, W, lane, pre);
     stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
     const double* pz = prop + (size_t)q * PS + 4;
     for (int k = lane; k < NLM; k += L) dz[k] = pz[k] - zrows[(size_t)s * NLM + k];
   }
   __syncthreads();
   double a = 0.0;
-  if (valid) a = walk_delta<L, NLM>(T.walk + w0, W, lane, Z, dz, pre);
+  if (valid) {
+    const int w0 = T.walk_ptr[atom];
+    a = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
+  }
 #pragma unroll
   for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
   __syncthreads();  // the block's LDS is reused by the next site group
@@ -283,7 +267,7 @@ __global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__
   const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / L;
   const double beta = TEST ? 0.0 : st->beta;
   metro_group<L, LMAX, TEST>(T, config, zrows, prop, dE, acc, beta, q_lo + qi, qi < q_n,
-                             test_sites, lds);
+                             test_sites, lds, T.nb_max * (LMAX + 1) * (LMAX + 1) + 1 + (LMAX + 1) * (LMAX + 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -429,19 +413,17 @@ __device__ double block_tree(double x0, double x1, double* sh) {
 // block of k_final folds them (the kernel boundary makes the partials visible
 // across XCDs, so no device-scope fence is needed) and updates the chain.
 // ---------------------------------------------------------------------------
+// One block's share of the first reduction level: chunk b of x (and of the
+// three spin components when meas).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_chunks(const double* __restrict__ x, long long n,
-                                                const int* __restrict__ iacc,
-                                                const int* __restrict__ iatt,
-                                                double* __restrict__ part,
-                                                long long* __restrict__ ipart,
-                                                const double* __restrict__ config,
-                                                const uint8_t* __restrict__ active,
-                                                long long n_sites, int meas) {
+__device__ void chunk_body(long long b, const double* __restrict__ x, long long n,
+                           const int* __restrict__ iacc, const int* __restrict__ iatt,
+                           double* __restrict__ part, long long* __restrict__ ipart,
+                           const double* __restrict__ config, const uint8_t* __restrict__ active,
+                           long long n_sites, int meas) {
   __shared__ double sh[256];
   __shared__ long long csum[8];
   const int t = threadIdx.x;
-  const long long b = blockIdx.x;
   const long long nbx = (n + TREE_B - 1) / TREE_B > 0 ? (n + TREE_B - 1) / TREE_B : 1;
   const long long nchm = (n_sites + TREE_B - 1) / TREE_B;
   if (b < nbx) {
@@ -480,13 +462,24 @@ __global__ __launch_bounds__(256) void k_chunks(const double* __restrict__ x, lo
   }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(256) void k_chunks(const double* __restrict__ x, long long n,
+                                                const int* __restrict__ iacc,
+                                                const int* __restrict__ iatt,
+                                                double* __restrict__ part,
+                                                long long* __restrict__ ipart,
+                                                const double* __restrict__ config,
+                                                const uint8_t* __restrict__ active,
+                                                long long n_sites, int meas) {
+  chunk_body<MODE>(blockIdx.x, x, n, iacc, iatt, part, ipart, config, active, n_sites, meas);
+}
+
 __device__ double final_tree(const double* buf, long long m, double* scratch, double* sh);
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_final(double* part, long long n, const long long* ipart,
-                                               long long n_sites, int meas, ChainDev* st,
-                                               double* __restrict__ out, long long n_att,
-                                               double n_div, double* scratch) {
+__device__ void final_body(double* part, long long n, const long long* ipart, long long n_sites,
+                           int meas, ChainDev* st, double* __restrict__ out, long long n_att,
+                           double n_div, double* scratch) {
   __shared__ double sh[256];
   const int t = threadIdx.x;
   const long long nbx = (n + TREE_B - 1) / TREE_B > 0 ? (n + TREE_B - 1) / TREE_B : 1;
@@ -579,6 +572,14 @@ __global__ __launch_bounds__(256) void k_final(double* part, long long n, const 
     out[4 * (size_t)row + 3] = m3[2] / n_div;
     st->n_obs = row + 1;
   }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_final(double* part, long long n, const long long* ipart,
+                                               long long n_sites, int meas, ChainDev* st,
+                                               double* __restrict__ out, long long n_att,
+                                               double n_div, double* scratch) {
+  final_body<MODE>(part, n, ipart, n_sites, meas, st, out, n_att, n_div, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -730,7 +731,8 @@ __global__ __launch_bounds__(TPB) void k_sweeps(SweepArgs a) {
       for (int base = vblk * G; base < nc; base += gridDim.x * G) {
         const int qi = base + t / L;
         metro_group<L, LMAX, false>(T, a.config, a.zrows, a.prop, a.dE, a.acc, a.beta, lo + qi,
-                                    qi < nc, nullptr, lds);
+                                    qi < nc, nullptr, lds,
+                                    T.nb_max * (LMAX + 1) * (LMAX + 1) + 1 + (LMAX + 1) * (LMAX + 1));
       }
       if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
     }
@@ -948,6 +950,7 @@ struct sce_chain {
   long long* ip64 = nullptr;  // integer partials of the reductions
   bool persistent = false;  // run sweeps in one cooperative launch (opt-in)
   bool use_graph = true;    // replay captured sweeps as hipGraphs
+  bool graph_fork = false;  // proposals as a parallel graph branch per colour
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // [plain, measured] sweep
   int g_lanes = 0;
   hipStream_t side = nullptr;           // proposal branch of a captured sweep
@@ -1334,7 +1337,7 @@ static int32_t ensure_graphs(sce_chain* c, int L) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     const long long before = c->sweeps_done;
-    const int32_t r = enqueue_metro(c, L, k, c->side);
+    const int32_t r = enqueue_metro(c, L, k, c->graph_fork ? c->side : nullptr);
     c->sweeps_done = before;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (r != SCE_OK) {
@@ -1899,4 +1902,518 @@ extern "C" int32_t sce_test_reduce(int64_t n, const double* x, double* out) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   return o.down(out, 1);
+}
+
+// ===========================================================================
+// Batches: many chains — replicas of one model (parallel tempering) or
+// independent chains of different models and sizes with the same lmax — in ONE
+// launch per colour index.  Each chain keeps its own keyed stream (seed, site,
+// sweep), its own device state and its own deterministic reductions, so every
+// chain of a batch is bit-identical to the same chain run alone.
+//
+// Replica exchange (src/gpu/gpu_pt.jl:46-63, src/pt.jl:92-96) runs on the
+// device: the payload (config, rows, energy) of adjacent rungs is swapped by
+// exchanging the payload pointers in the chain descriptors; seed, sweep index,
+// step and counters stay with the rung.  The exchange uniform is keyed:
+// block (exchange seed, pair i, exchange count, slot 4, 0), words 1-2.
+// ===========================================================================
+struct BChain {
+  DevTables T;
+  double *config, *zrows, *prop, *dE, *part, *obs, *inst_e;
+  int* acc;
+  long long* ip64;
+  ChainDev* st;
+};
+
+struct BSeg {
+  int chain, lo, n, start;  // start: first global work index of this chain's slice
+};
+
+struct BatchDev {
+  unsigned long long exch_seed;
+  long long exch_count;
+  int parity, pad;
+};
+
+constexpr uint32_t SLOT_EXCH = 4u;
+
+__device__ __forceinline__ int find_seg(const BSeg* __restrict__ seg, int nseg, int idx) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg[mid].start <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(256) void k_bpropose(const BChain* __restrict__ ch,
+                                                  const BSeg* __restrict__ seg, int nseg,
+                                                  int total) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const BSeg sg = seg[find_seg(seg, nseg, idx)];
+  const int i = idx - sg.start;
+  if (i >= sg.n) return;
+  const BChain* c = &ch[sg.chain];
+  const ChainDev* st = c->st;
+  propose_one<LMAX>(c->T, c->config, c->prop, sg.lo + i, st->seed,
+                    (int32_t)(st->sweep_index + 1), st->step);
+}
+
+template <int L, int LMAX>
+__global__ __launch_bounds__(TPB) void k_bmetro(const BChain* __restrict__ ch,
+                                                const BSeg* __restrict__ seg, int nseg,
+                                                int total_groups, int stride) {
+  constexpr int G = TPB / L;
+  extern __shared__ double lds[];
+  const int gidx = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / L;
+  bool valid = gidx < total_groups;
+  int k = 0, i = 0;
+  if (valid) {
+    k = find_seg(seg, nseg, gidx);
+    i = gidx - seg[k].start;
+    valid = i < seg[k].n;
+  }
+  const BSeg sg = seg[k];
+  const BChain* c = &ch[sg.chain];
+  metro_group<L, LMAX, false>(c->T, c->config, c->zrows, c->prop, c->dE, c->acc,
+                              valid ? c->st->beta : 0.0, sg.lo + i, valid, nullptr, lds, stride);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bchunks(const BChain* __restrict__ ch,
+                                                 const BSeg* __restrict__ seg, int nseg,
+                                                 int meas) {
+  const BSeg sg = seg[find_seg(seg, nseg, (int)blockIdx.x)];
+  const long long local = (long long)blockIdx.x - sg.start;
+  const BChain* c = &ch[sg.chain];
+  const long long n = MODE == MODE_MEASURE ? 0 : c->T.n_active;
+  chunk_body<MODE>(local, c->dE, n, c->acc, nullptr, c->part, c->ip64, c->config, c->T.active,
+                   c->T.n_sites, meas);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bfinal(const BChain* __restrict__ ch, int meas) {
+  const BChain* c = &ch[blockIdx.x];
+  const long long n = MODE == MODE_MEASURE ? 0 : c->T.n_active;
+  const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1);
+  const long long nchm = meas ? ((long long)c->T.n_sites + TREE_B - 1) / TREE_B : 0;
+  double* scratch = c->part + nbx + 3 * std::max<long long>(nchm, 1);
+  final_body<MODE>(c->part, n, c->ip64, c->T.n_sites, meas, c->st, c->obs, c->T.n_active,
+                   (double)c->T.n_active, scratch);
+}
+
+// Adjacent-pair exchange on the device (gpu_pt.jl:115-140 schedule: pairs
+// parity, parity+2, ... each with one keyed uniform, then the parity flips).
+__global__ void k_exchange(BChain* ch, int R, BatchDev* bd, const double* __restrict__ kts,
+                           long long* sw_att, long long* sw_acc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const long long ex = bd->exch_count + 1;
+  for (int i = bd->parity; i < R - 1; i += 2) {
+    const sce::u32x4 b = sce::philox_block(bd->exch_seed, i + 1, (int32_t)ex, SLOT_EXCH, 0u);
+    const double u = sce::uniform01(b.x, b.y);
+    ChainDev* sa = ch[i].st;
+    ChainDev* sb = ch[i + 1].st;
+    const double logw = (1.0 / kts[i] - 1.0 / kts[i + 1]) * (sa->energy - sb->energy);
+    const double m = logw < 0.0 ? logw : 0.0;
+    sw_att[i] += 1;
+    if (u < sce::det_exp(m)) {
+      double* t0 = ch[i].config;
+      ch[i].config = ch[i + 1].config;
+      ch[i + 1].config = t0;
+      double* t1 = ch[i].zrows;
+      ch[i].zrows = ch[i + 1].zrows;
+      ch[i + 1].zrows = t1;
+      const double e = sa->energy;
+      sa->energy = sb->energy;
+      sb->energy = e;
+      sw_acc[i] += 1;
+    }
+  }
+  bd->parity = 1 - bd->parity;
+  bd->exch_count = ex;
+}
+
+__global__ void k_breset_obs(BChain* ch, int R) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < R) ch[k].st->n_obs = 0;
+}
+
+struct sce_batch {
+  int R = 0, lmax = 0, max_colors = 0, stride = 0;
+  bool homogeneous = false;
+  std::vector<sce_tables*> tabs;
+  DevBuf mem;
+  BChain* d_ch = nullptr;
+  std::vector<BChain> h_ch;
+  BSeg* d_seg_prop = nullptr;
+  int nseg_prop = 0, total_prop = 0;
+  std::vector<BSeg*> d_seg_col;
+  std::vector<int> nseg_col, total_col;
+  BSeg* d_seg_chunk[2] = {nullptr, nullptr};  // [no measurement, measurement]
+  BSeg* d_seg_meas = nullptr;                 // measurement-only reduction
+  int nseg_chunk[2] = {0, 0}, total_chunk[2] = {0, 0}, nseg_meas = 0, total_meas = 0;
+  BatchDev* bd = nullptr;
+  double* kts = nullptr;
+  long long *sw_att = nullptr, *sw_acc = nullptr;
+  double* obs = nullptr;
+  size_t obs_cap = 0;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // [meas*2 + exch]
+  int g_lanes = 0;
+  long long sweeps_done = 0;
+  bool use_graph = true;
+  void drop_graphs() {
+    for (auto& g : gexec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    g_lanes = 0;
+  }
+  ~sce_batch() {
+    drop_graphs();
+    if (stream) (void)hipStreamSynchronize(stream), (void)hipStreamDestroy(stream);
+  }
+};
+
+template <class T>
+static int32_t upload_segs(DevBuf& mem, const std::vector<T>& v, T** out) {
+  return dupload(mem, out, v.data(), v.size());
+}
+
+extern "C" int32_t sce_batch_create(int32_t R, sce_tables* const* tables, const double* const* configs,
+                                    const double* const* zrows, const double* energies,
+                                    const uint64_t* seeds, const double* steps, const double* kts,
+                                    uint64_t exch_seed, sce_batch** out) {
+  if (!out || !tables || !configs || !zrows || !energies || !seeds || !steps || !kts)
+    return sce_fail(SCE_EINVAL, "sce_batch_create: NULL argument");
+  *out = nullptr;
+  if (R < 1) return sce_fail(SCE_EINVAL, "a batch needs at least one chain");
+  auto* b = new sce_batch();
+  auto fail = [&](int32_t r) {
+    delete b;
+    return r;
+  };
+  b->R = R;
+  b->lmax = tables[0]->dev.lmax;
+  b->homogeneous = true;
+  int32_t r;
+  for (int k = 0; k < R; ++k) {
+    if (!tables[k] || !configs[k] || !zrows[k]) return fail(sce_fail(SCE_EINVAL, "chain %d: NULL argument", k));
+    if (tables[k]->dev.lmax != b->lmax)
+      return fail(sce_fail(SCE_EINVAL, "all chains of a batch must share lmax (%d vs %d)", tables[k]->dev.lmax, b->lmax));
+    if (!(steps[k] > 0.0)) return fail(sce_fail(SCE_EINVAL, "chain %d: step must be > 0", k));
+    if (!(kts[k] > 0.0 && std::isfinite(kts[k]))) return fail(sce_fail(SCE_EINVAL, "chain %d: kT must be finite and > 0", k));
+    b->homogeneous = b->homogeneous && tables[k] == tables[0];
+    b->tabs.push_back(tables[k]);
+  }
+  const int nlm = (b->lmax + 1) * (b->lmax + 1);
+  b->h_ch.resize(R);
+  for (int k = 0; k < R; ++k) {
+    const DevTables& D = tables[k]->dev;
+    BChain& c = b->h_ch[k];
+    c.T = D;
+    const size_t N = (size_t)D.n_sites, na = (size_t)std::max(D.n_active, 1);
+    if ((r = dupload(b->mem, &c.config, configs[k], 3 * N)) != SCE_OK) return fail(r);
+    if ((r = dupload(b->mem, &c.zrows, zrows[k], (size_t)nlm * N)) != SCE_OK) return fail(r);
+    if ((r = dalloc(b->mem, &c.prop, na * (nlm + 4))) != SCE_OK) return fail(r);
+    if ((r = dalloc(b->mem, &c.dE, na)) != SCE_OK) return fail(r);
+    if ((r = dalloc(b->mem, &c.acc, na)) != SCE_OK) return fail(r);
+    const long long nmax = std::max<long long>({(long long)D.n_active, (long long)D.n_inst, (long long)D.n_sites});
+    if ((r = dalloc(b->mem, &c.part, partial_capacity(nmax) + 4 * ((size_t)nmax / TREE_B + 4))) != SCE_OK) return fail(r);
+    if ((r = dalloc(b->mem, &c.ip64, 2 * ((size_t)nmax / TREE_B + 4))) != SCE_OK) return fail(r);
+    if ((r = dalloc(b->mem, &c.inst_e, (size_t)std::max(D.n_inst, 1))) != SCE_OK) return fail(r);
+    if ((r = dalloc(b->mem, &c.st, 1)) != SCE_OK) return fail(r);
+    ChainDev h{};
+    h.energy = energies[k];
+    h.step = steps[k];
+    h.beta = 1.0 / kts[k];
+    h.seed = seeds[k];
+    if (hipMemcpy(c.st, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) return fail(sce_fail(SCE_EHIP, "state upload failed"));
+    c.obs = nullptr;
+    b->stride = std::max(b->stride, D.nb_max * nlm + 1 + nlm);
+    b->max_colors = std::max(b->max_colors, tables[k]->n_colors);
+  }
+  // work segments
+  std::vector<BSeg> sp;
+  int tot = 0;
+  for (int k = 0; k < R; ++k) {
+    sp.push_back({k, 0, tables[k]->dev.n_active, tot});
+    tot += tables[k]->dev.n_active;
+  }
+  b->nseg_prop = (int)sp.size();
+  b->total_prop = tot;
+  if ((r = upload_segs(b->mem, sp, &b->d_seg_prop)) != SCE_OK) return fail(r);
+  for (int col = 0; col < b->max_colors; ++col) {
+    std::vector<BSeg> sc;
+    int t = 0;
+    for (int k = 0; k < R; ++k) {
+      const sce_tables* tb = tables[k];
+      if (col >= tb->n_colors) continue;
+      const int lo = tb->color_ptr[col], n = tb->color_ptr[col + 1] - lo;
+      if (n == 0) continue;
+      sc.push_back({k, lo, n, t});
+      t += n;
+    }
+    if (sc.empty()) sc.push_back({0, 0, 0, 0});
+    BSeg* d;
+    if ((r = upload_segs(b->mem, sc, &d)) != SCE_OK) return fail(r);
+    b->d_seg_col.push_back(d);
+    b->nseg_col.push_back((int)sc.size());
+    b->total_col.push_back(t);
+  }
+  for (int m = 0; m < 3; ++m) {
+    std::vector<BSeg> sc;
+    int t = 0;
+    for (int k = 0; k < R; ++k) {
+      const DevTables& D = tables[k]->dev;
+      const long long nbx = std::max<long long>((D.n_active + TREE_B - 1) / TREE_B, 1);
+      const long long nchm = ((long long)D.n_sites + TREE_B - 1) / TREE_B;
+      const long long nb = m == 0 ? nbx : m == 1 ? std::max(nbx, nchm) : nchm;
+      sc.push_back({k, 0, (int)nb, t});
+      t += (int)nb;
+    }
+    BSeg* d;
+    if ((r = upload_segs(b->mem, sc, &d)) != SCE_OK) return fail(r);
+    if (m < 2) {
+      b->d_seg_chunk[m] = d;
+      b->nseg_chunk[m] = (int)sc.size();
+      b->total_chunk[m] = t;
+    } else {
+      b->d_seg_meas = d;
+      b->nseg_meas = (int)sc.size();
+      b->total_meas = t;
+    }
+  }
+  BatchDev hb{};
+  hb.exch_seed = exch_seed;
+  if ((r = dupload(b->mem, &b->bd, &hb, 1)) != SCE_OK) return fail(r);
+  if ((r = dupload(b->mem, &b->kts, kts, (size_t)R)) != SCE_OK) return fail(r);
+  std::vector<long long> z((size_t)R, 0);
+  if ((r = dupload(b->mem, &b->sw_att, z.data(), z.size())) != SCE_OK) return fail(r);
+  if ((r = dupload(b->mem, &b->sw_acc, z.data(), z.size())) != SCE_OK) return fail(r);
+  if ((r = dupload(b->mem, &b->d_ch, b->h_ch.data(), b->h_ch.size())) != SCE_OK) return fail(r);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(sce_fail(SCE_EHIP, "hipStreamCreate failed"));
+  *out = b;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_destroy(sce_batch* b) {
+  delete b;
+  return SCE_OK;
+}
+
+template <int LMAX>
+static void launch_bpropose(sce_batch* b, hipStream_t s) {
+  const int nb = (b->total_prop + 255) / 256;
+  if (nb) k_bpropose<LMAX><<<nb, 256, 0, s>>>(b->d_ch, b->d_seg_prop, b->nseg_prop, b->total_prop);
+}
+
+template <int L, int LMAX>
+static void launch_bmetro_l(sce_batch* b, int col, hipStream_t s) {
+  constexpr int G = TPB / L;
+  const int total = b->total_col[col];
+  const int nb = (total + G - 1) / G;
+  const size_t shmem = (size_t)G * (size_t)b->stride * sizeof(double);
+  if (nb) k_bmetro<L, LMAX><<<nb, TPB, shmem, s>>>(b->d_ch, b->d_seg_col[col], b->nseg_col[col], total, b->stride);
+}
+
+template <int LMAX>
+static void launch_bmetro(int L, sce_batch* b, int col, hipStream_t s) {
+  if (L == 16) launch_bmetro_l<16, LMAX>(b, col, s);
+  else if (L == 32) launch_bmetro_l<32, LMAX>(b, col, s);
+  else launch_bmetro_l<64, LMAX>(b, col, s);
+}
+
+static int32_t benqueue_sweep(sce_batch* b, int L, int meas, int exch) {
+  if ((size_t)(TPB / L) * (size_t)b->stride * sizeof(double) > 65536)
+    return sce_fail(SCE_EINVAL, "neighbour stencil too large for lanes = %d; use more lanes", L);
+  hipStream_t s = b->stream;
+  SCE_DISPATCH_LMAX(b->lmax, launch_bpropose, b, s);
+  for (int col = 0; col < b->max_colors; ++col) {
+    if (b->total_col[col] == 0) continue;
+    SCE_DISPATCH_LMAX(b->lmax, launch_bmetro, L, b, col, s);
+  }
+  k_bchunks<MODE_METRO><<<b->total_chunk[meas], 256, 0, s>>>(b->d_ch, b->d_seg_chunk[meas], b->nseg_chunk[meas], meas);
+  k_bfinal<MODE_METRO><<<b->R, 256, 0, s>>>(b->d_ch, meas);
+  if (exch) k_exchange<<<1, 1, 0, s>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
+  return SCE_OK;
+}
+
+static int32_t bensure_graph(sce_batch* b, int L, int meas, int exch) {
+  if (b->g_lanes != L) b->drop_graphs();
+  hipGraphExec_t& ge = b->gexec[meas * 2 + exch];
+  if (ge) return SCE_OK;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int32_t r = benqueue_sweep(b, L, meas, exch);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (r != SCE_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return sce_fail(SCE_EHIP, "batch sweep capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  b->g_lanes = L;
+  return SCE_OK;
+}
+
+static int32_t bsync_descriptors(sce_batch* b) {
+  HIPCHK(hipMemcpyAsync(b->h_ch.data(), b->d_ch, b->h_ch.size() * sizeof(BChain), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SCE_OK;
+}
+
+// Renormalise every chain (per-chain launches; every renorm_interval sweeps).
+static int32_t brenorm(sce_batch* b) {
+  CHK(bsync_descriptors(b));
+  for (int k = 0; k < b->R; ++k) {
+    BChain& c = b->h_ch[k];
+    const DevTables& D = c.T;
+    SCE_DISPATCH_LMAX(D.lmax, launch_renorm, D, c.config, c.zrows, b->stream);
+    const int nb = (D.n_inst + 255) / 256;
+    if (nb) k_inst_energy<<<nb, 256, 0, b->stream>>>(D, c.zrows, c.inst_e);
+    launch_reduce<MODE_RENORM>(c.inst_e, D.n_inst, nullptr, nullptr, c.part, c.ip64, c.config, D.active,
+                               D.n_sites, 0, c.st, nullptr, 0, 1.0, b->stream);
+  }
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, int64_t renorm_interval,
+                                 int64_t measure_interval, int64_t exchange_interval,
+                                 int64_t phase_done, int64_t phase_total, double* obs,
+                                 int64_t obs_cap, int64_t* n_obs) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  CHK(check_lanes(lanes));
+  if (nsweeps < 0) return sce_fail(SCE_EINVAL, "nsweeps must be >= 0");
+  if (exchange_interval > 0 && !b->homogeneous)
+    return sce_fail(SCE_EINVAL, "replica exchange needs every chain on the same tables");
+  if (b->sweeps_done + nsweeps >= 2147483647LL - 1)
+    return sce_fail(SCE_ERANGE_COUNTER, "sweep_index would exhaust the 32-bit RNG counter word");
+  if (phase_done < 0) return sce_fail(SCE_EINVAL, "phase_done must be >= 0");
+  const int64_t want =
+      measure_interval > 0 ? (phase_done + nsweeps) / measure_interval - phase_done / measure_interval : 0;
+  if (want > 0 && (!obs || obs_cap < want))
+    return sce_fail(SCE_EDIM, "obs holds %lld rows per chain; %lld needed", (long long)obs_cap, (long long)want);
+  if (want > 0 && (size_t)want > b->obs_cap) {
+    double* p;
+    CHK(dalloc(b->mem, &p, 4 * (size_t)want * (size_t)b->R));
+    b->obs = p;
+    b->obs_cap = (size_t)want;
+    CHK(bsync_descriptors(b));
+    for (int k = 0; k < b->R; ++k) b->h_ch[k].obs = p + 4 * (size_t)want * (size_t)k;
+    HIPCHK(hipMemcpy(b->d_ch, b->h_ch.data(), b->h_ch.size() * sizeof(BChain), hipMemcpyHostToDevice));
+    b->drop_graphs();
+  }
+  k_breset_obs<<<(b->R + 255) / 256, 256, 0, b->stream>>>(b->d_ch, b->R);
+  for (int64_t i = 1; i <= nsweeps; ++i) {
+    const int64_t j = phase_done + i;  // sweep number within the phase
+    const bool renorm = renorm_interval > 0 && j % renorm_interval == 0;
+    const bool meas = measure_interval > 0 && j % measure_interval == 0;
+    const bool exch = exchange_interval > 0 && j % exchange_interval == 0 && j < phase_total;
+    const bool fused_meas = meas && !renorm, fused_exch = exch && !renorm;
+    if (b->use_graph) {
+      CHK(bensure_graph(b, lanes, fused_meas, fused_exch));
+      HIPCHK(hipGraphLaunch(b->gexec[(fused_meas ? 2 : 0) + (fused_exch ? 1 : 0)], b->stream));
+    } else {
+      CHK(benqueue_sweep(b, lanes, fused_meas, fused_exch));
+    }
+    b->sweeps_done += 1;
+    if (renorm) {
+      CHK(brenorm(b));
+      if (meas) {
+        k_bchunks<MODE_MEASURE><<<b->total_meas, 256, 0, b->stream>>>(b->d_ch, b->d_seg_meas, b->nseg_meas, 1);
+        k_bfinal<MODE_MEASURE><<<b->R, 256, 0, b->stream>>>(b->d_ch, 1);
+      }
+      if (exch) k_exchange<<<1, 1, 0, b->stream>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (want > 0) HIPCHK(hipMemcpy(obs, b->obs, 4 * (size_t)want * (size_t)b->R * 8, hipMemcpyDeviceToHost));
+  if (n_obs) *n_obs = want;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_download(sce_batch* b, int32_t k, double* config, double* zrows, double* energy) {
+  if (!b || k < 0 || k >= b->R) return sce_fail(SCE_EINVAL, "bad batch or chain index");
+  CHK(bsync_descriptors(b));
+  const BChain& c = b->h_ch[k];
+  const int nlm = (b->lmax + 1) * (b->lmax + 1);
+  if (config) HIPCHK(hipMemcpy(config, c.config, 3 * (size_t)c.T.n_sites * 8, hipMemcpyDeviceToHost));
+  if (zrows) HIPCHK(hipMemcpy(zrows, c.zrows, (size_t)nlm * c.T.n_sites * 8, hipMemcpyDeviceToHost));
+  ChainDev h;
+  HIPCHK(hipMemcpy(&h, c.st, sizeof h, hipMemcpyDeviceToHost));
+  if (energy) *energy = h.energy;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_info(sce_batch* b, int32_t k, sce_chain_info* o) {
+  if (!b || !o || k < 0 || k >= b->R) return sce_fail(SCE_EINVAL, "bad batch or chain index");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  ChainDev h;
+  HIPCHK(hipMemcpy(&h, b->h_ch[k].st, sizeof h, hipMemcpyDeviceToHost));
+  o->energy = h.energy;
+  o->step = h.step;
+  o->max_drift = h.max_drift;
+  o->last_drift = h.last_drift;
+  o->sweep_index = h.sweep_index;
+  o->acc_metro = h.acc_metro;
+  o->att_metro = h.att_metro;
+  o->acc_or = h.acc_or;
+  o->att_or = h.att_or;
+  o->seed = h.seed;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_set_step(sce_batch* b, int32_t k, double step) {
+  if (!b || k < 0 || k >= b->R) return sce_fail(SCE_EINVAL, "bad batch or chain index");
+  if (!(step > 0.0)) return sce_fail(SCE_EINVAL, "step must be > 0");
+  ChainDev h;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(&h, b->h_ch[k].st, sizeof h, hipMemcpyDeviceToHost));
+  h.step = step;
+  HIPCHK(hipMemcpy(b->h_ch[k].st, &h, sizeof h, hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_reset_counters(sce_batch* b) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int k = 0; k < b->R; ++k) {
+    ChainDev h;
+    HIPCHK(hipMemcpy(&h, b->h_ch[k].st, sizeof h, hipMemcpyDeviceToHost));
+    h.acc_metro = h.att_metro = h.acc_or = h.att_or = 0;
+    h.max_drift = 0.0;
+    HIPCHK(hipMemcpy(b->h_ch[k].st, &h, sizeof h, hipMemcpyHostToDevice));
+  }
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_renormalize(sce_batch* b) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  CHK(brenorm(b));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (att) HIPCHK(hipMemcpy(att, b->sw_att, (size_t)b->R * 8, hipMemcpyDeviceToHost));
+  if (acc) HIPCHK(hipMemcpy(acc, b->sw_acc, (size_t)b->R * 8, hipMemcpyDeviceToHost));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_set_graph(sce_batch* b, int32_t on) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  b->use_graph = on != 0;
+  return SCE_OK;
 }

@@ -92,6 +92,9 @@ EXPORTS = [
     "sce_run_sweeps", "sce_renormalize", "sce_chain_total_energy", "sce_chain_set_timing", "sce_chain_set_persistent",
     "sce_chain_timing", "sce_test_philox", "sce_test_detmath", "sce_test_zlm",
     "sce_test_proposal", "sce_test_walk_delta", "sce_test_walk_coeffs", "sce_test_reduce",
+    "sce_batch_create", "sce_batch_destroy", "sce_batch_run", "sce_batch_download",
+    "sce_batch_info", "sce_batch_set_step", "sce_batch_reset_counters", "sce_batch_swap_stats",
+    "sce_batch_set_graph", "sce_batch_renormalize",
 ]
 
 
@@ -127,6 +130,13 @@ def lib():
             "sce_test_walk_delta": [P, P, i64, P, P, i32, P],
             "sce_test_walk_coeffs": [P, P, i64, P, i32, P],
             "sce_test_reduce": [i64, P, P],
+            "sce_batch_create": [i32, P, P, P, P, P, P, P, u64, P],
+            "sce_batch_destroy": [P],
+            "sce_batch_run": [P, i64, i32, i64, i64, i64, i64, i64, P, i64, P],
+            "sce_batch_download": [P, i32, P, P, P], "sce_batch_info": [P, i32, P],
+            "sce_batch_set_step": [P, i32, d], "sce_batch_reset_counters": [P],
+            "sce_batch_swap_stats": [P, P, P], "sce_batch_set_graph": [P, i32],
+            "sce_batch_renormalize": [P],
         }
         for name, args in sig.items():
             f = getattr(L, name)

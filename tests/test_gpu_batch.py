@@ -1,0 +1,124 @@
+"""Batched chains: every chain of a batch is bit-identical to the same chain run
+alone through the oracle, for a replica ladder on one model and for an uneven
+batch of different models and sizes; device replica exchange is bit-identical
+to the oracle's restatement of _swap_accepts (src/pt.jl:92-96) with the
+alternating-parity schedule of src/gpu/gpu_pt.jl:115-140."""
+import numpy as np
+import pytest
+
+import oracle as O
+import sce_amd as S
+from sce_amd import models as M
+from sce_amd.batch import GPUBatch
+from fixtures import bcc_tet_small, build_pair, rand_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_exchange(payload, kts, parity, ex, exch_seed):
+    R = len(payload)
+    att = np.zeros(R - 1, int)
+    acc = np.zeros(R - 1, int)
+    for i in range(parity, R - 1, 2):
+        b = O.philox_block(exch_seed, i + 1, ex, 4, 0)
+        u = O.uniform(int(b[0]), int(b[1]))
+        Ea, Eb = payload[i][2], payload[i + 1][2]
+        logw = (1.0 / kts[i] - 1.0 / kts[i + 1]) * (Ea - Eb)
+        att[i] += 1
+        if u < O.exp(min(0.0, logw) if logw < 0.0 else 0.0):
+            payload[i], payload[i + 1] = payload[i + 1], payload[i]
+            acc[i] += 1
+    return att, acc
+
+
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_ladder_with_device_exchange_bitwise(lanes):
+    na, terms, dims = bcc_tet_small(6)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    R = 5
+    kts = np.geomspace(0.001, 0.02, R)
+    seeds = [101 + 7 * r for r in range(R)]
+    states = [S.ChainState(H, rand_config(H.n_sites, 40 + r), 0.6) for r in range(R)]
+    exch_seed = 0xabcdef
+    b = GPUBatch([gH] * R, states, seeds, kts, exch_seed)
+    nsw = 12
+    b.run(nsw, lanes, renorm_interval=5, measure_interval=0, exchange_interval=1, phase_done=0,
+          phase_total=nsw)
+    payload = [[st.config.copy(), st.zrows.copy(), st.energy] for st in states]
+    att = np.zeros(R - 1, int)
+    acc = np.zeros(R - 1, int)
+    parity, ex = 0, 0
+    for j in range(1, nsw + 1):
+        for r in range(R):
+            c, z, E = payload[r]
+            _, E, _, _ = T.keyed_sweep(c, z, E, 1.0 / kts[r], 0.6, seeds[r], j, lanes)
+            payload[r][2] = E
+        if j % 5 == 0:
+            for r in range(R):
+                _, payload[r][2] = T.renormalize(payload[r][0], payload[r][1], payload[r][2])
+        if j < nsw:
+            ex += 1
+            a1, a2 = _oracle_exchange(payload, kts, parity, ex, exch_seed)
+            att += a1
+            acc += a2
+            parity = 1 - parity
+    for r in range(R):
+        cfg, z, E = b.download(r)
+        assert np.array_equal(cfg, payload[r][0]), r
+        assert np.array_equal(z, payload[r][1]), r
+        assert E == payload[r][2], r
+        assert b.info(r).sweep_index == nsw
+    da, dc = b.swap_stats()
+    assert np.array_equal(da, att) and np.array_equal(dc, acc)
+    assert acc.sum() > 0  # the ladder actually exchanged
+
+
+def test_uneven_batch_bitwise_and_equal_to_single_chains():
+    cases = [M.lattice_model("bcc", pair_shells=3, pair_lmax=2, triangles=True, seed=3) + ((3, 3, 3),),
+             M.lattice_model("fcc", pair_shells=2, pair_lmax=2, seed=5) + ((3, 3, 3),),
+             M.lattice_model("sc", pair_shells=2, pair_lmax=1, onsite_l2=True, seed=4) + ((5, 4, 3),),
+             M.lattice_model("bcc", pair_shells=2, pair_lmax=2, seed=9) + ((4, 4, 4),)]
+    gHs, states, Ts, kts, seeds = [], [], [], [], []
+    for k, (na, terms, dims) in enumerate(cases):
+        H, T = build_pair(na, terms, dims)
+        gHs.append(S.GPUTiledHamiltonian(H))
+        states.append(S.ChainState(H, rand_config(H.n_sites, 60 + k), 0.5 + 0.1 * k))
+        Ts.append(T)
+        kts.append(0.002 * (k + 1))
+        seeds.append(1000 + k)
+    b = GPUBatch(gHs, states, seeds, kts)
+    obs = b.run(9, 64, renorm_interval=4, measure_interval=3)
+    assert obs.shape == (4, 3, 4)
+    for k, T in enumerate(Ts):
+        c, z, E = states[k].config.copy(), states[k].zrows.copy(), states[k].energy
+        for j in range(1, 10):
+            _, E, _, _ = T.keyed_sweep(c, z, E, 1.0 / kts[k], 0.5 + 0.1 * k, seeds[k], j, 64)
+            if j % 4 == 0:
+                _, E = T.renormalize(c, z, E)
+        cfg, zd, Ed = b.download(k)
+        assert np.array_equal(cfg, c) and np.array_equal(zd, z) and Ed == E, k
+        # the batch chain equals the same chain run alone on the device
+        g = S.GPUChainState(gHs[k], states[k], seed=seeds[k])
+        o1 = S.gpu_run_sweeps(g, gHs[k], None, 1.0 / kts[k], 9, renorm_interval=4, measure_interval=3)
+        c1, z1, E1 = g.download()
+        assert np.array_equal(c1, cfg) and np.array_equal(z1, zd) and E1 == Ed
+        assert np.array_equal(o1[:, 0], obs[k, :, 0])
+
+
+def test_batch_direct_launch_equals_graph():
+    na, terms, dims = bcc_tet_small(2)
+    H = S.TiledHamiltonian(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    R = 3
+    states = [S.ChainState(H, rand_config(H.n_sites, r), 0.6) for r in range(R)]
+    kts = [0.002, 0.004, 0.008]
+    outs = []
+    for graph in (True, False):
+        b = GPUBatch([gH] * R, states, [5, 6, 7], kts, 3)
+        b.set_graph(graph)
+        b.run(8, 32, 0, 2, 2, 0, 8)
+        outs.append([b.download(r) for r in range(R)] + [b.swap_stats()])
+    for r in range(R):
+        assert np.array_equal(outs[0][r][0], outs[1][r][0]) and outs[0][r][2] == outs[1][r][2]
+    assert np.array_equal(outs[0][R][1], outs[1][R][1])
