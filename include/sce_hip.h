@@ -77,6 +77,7 @@ typedef struct sce_chain_info {
   int64_t sweep_index;  /* completed Metropolis sweeps: the keyed-RNG counter */
   int64_t acc_metro, att_metro, acc_or, att_or;
   uint64_t seed;
+  int64_t acc_stage1;   /* delayed acceptance: proposals that passed the surrogate stage */
 } sce_chain_info;
 
 int32_t sce_version(void);
@@ -106,9 +107,10 @@ int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out);
 int32_t sce_tables_destroy(sce_tables* t);
 /* Derive and verify the walk layout on the host only (no device needed);
  * writes the same facts as sce_tables_layout_info. */
-int32_t sce_layout_check(const sce_tables_desc* d, int64_t out[4]);
-/* Layout facts: [n_sublattices, max_neighbours, max_walk_entries, total_walk_entries] */
-int32_t sce_tables_layout_info(const sce_tables* t, int64_t out[4]);
+int32_t sce_layout_check(const sce_tables_desc* d, int64_t out[5]);
+/* Layout facts: [n_sublattices, max_neighbours, max_walk_entries,
+ * total_walk_entries, fmax = widest surrogate descriptor row] */
+int32_t sce_tables_layout_info(const sce_tables* t, int64_t out[5]);
 
 /* ---- device chain: GPUChainState(gH, st; seed) ------------------------------ */
 int32_t sce_chain_create(sce_tables* t, const double* config, const double* zrows,
@@ -194,6 +196,57 @@ int32_t sce_batch_renormalize(sce_batch* b);
 int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc);
 /* Replay captured sweeps as hipGraphs (on != 0, default) or launch directly. */
 int32_t sce_batch_set_graph(sce_batch* b, int32_t on);
+
+/* ---- learned delayed acceptance (no reference counterpart: an extension of the
+ * device sweep for callers that want more effective samples per second) ------
+ * A surrogate holds, per sublattice of the tables, a small MLP g(phi(e)) of a
+ * rotation-invariant descriptor phi of a site's spin e against its neighbour
+ * spins (features: e.e_n and (e.e_n)^2 per stencil neighbour, e.(e_x x e_y) per
+ * triplet, the three pairings of each body-4 cluster, the spin components when
+ * a body-1 term touches the site).  Network: W1 (64 x F) tanh, W2 (64 x 64)
+ * tanh, w3 (64) plus a linear path v (F); parameter vector
+ * [W1 row-major, b1, W2 row-major, b2, w3, v] (FP32), output scale (FP64).
+ * A delayed-acceptance sweep (Christen & Fox 2005) first accepts the keyed
+ * proposal with min(1, exp(-beta dE_s)), dE_s = scale (g(phi(e')) - g(phi(e))),
+ * and only then walks the exact FP64 dE and accepts with
+ * min(1, exp(-beta (dE - dE_s))) (keyed uniform: slot 3).  Detailed balance
+ * holds for any surrogate; the exact dE decides. */
+typedef struct sce_surrogate sce_surrogate;
+/* Glorot-initialised networks (splitmix64 stream from seed), v = 0, scale 1. */
+int32_t sce_sur_create(sce_tables* t, uint64_t seed, sce_surrogate** out);
+int32_t sce_sur_destroy(sce_surrogate* s);
+/* Features F and parameter count of sublattice `sub` (F = 0: inactive
+ * sublattice, no network), its output scale, and the harvested row count. */
+int32_t sce_sur_info(const sce_surrogate* s, int32_t sub, int32_t* F, int64_t* n_params,
+                     double* scale, int64_t* n_rows);
+int32_t sce_sur_get_params(sce_surrogate* s, int32_t sub, float* out);
+int32_t sce_sur_set_params(sce_surrogate* s, int32_t sub, const float* in, double scale);
+/* nsweeps plain Metropolis sweeps of chain c (the same chain evolution as
+ * sce_run_sweeps) that also store (phi(e), phi(e'), exact dE) of every attempt
+ * as training rows on the device. */
+int32_t sce_sur_harvest(sce_surrogate* s, sce_chain* c, double beta, int64_t nsweeps);
+/* Adam over sublattice sub's rows: `epochs` shuffled passes in minibatches of
+ * bsz pairs, loss mean((scale (g_B - g_A) - dE)^2) / scale^2 (scale fixed to
+ * the rms dE of the rows at the first call); per-epoch mean loss in loss_out. */
+int32_t sce_sur_train(sce_surrogate* s, int32_t sub, int32_t epochs, int32_t bsz, double lr,
+                      uint64_t seed, double* loss_out);
+/* Loss and its gradient (layout of the parameter vector) of n given pairs
+ * through the training kernels; A, B: n x fmax FP32 descriptor rows. */
+int32_t sce_sur_grad(sce_surrogate* s, int32_t sub, int32_t n, const float* A, const float* B,
+                     const double* y, float* grad_out, double* loss);
+/* dE_s of n given descriptor pairs through the sweep's inference path. */
+int32_t sce_sur_eval(sce_surrogate* s, int32_t sub, int32_t n, const float* A, const float* B,
+                     double* out);
+/* Descriptor rows phi(e_site), phi(e2) (n x fmax FP32, zero padded) of given
+ * active sites (1-based) and proposed spins e2 (3 x n) against config. */
+int32_t sce_sur_features(sce_tables* t, const double* config, int32_t n, const int32_t* sites,
+                         const double* e2, float* outA, float* outB);
+/* nsweeps delayed-acceptance sweeps of chain c with the same renormalisation
+ * and measurement schedule as sce_run_sweeps; acc_stage1 in sce_chain_info
+ * counts the proposals that passed the surrogate stage. */
+int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta, int64_t nsweeps,
+                          int64_t renorm_interval, int64_t measure_interval, double* obs,
+                          int64_t obs_cap, int64_t* n_obs);
 
 /* ---- diagnostics (no reference counterpart): the device arithmetic exposed
  * for bit-exact parity tests ------------------------------------------------- */

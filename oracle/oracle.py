@@ -81,6 +81,9 @@ def lib():
             "ora_cpu_set_step": (None, [P, d]),
             "ora_cpu_reset_counters": (None, [P]),
             "ora_cpu_counters": (None, [P, P]),
+            "ora_sur_features": (C.c_int, [P, P, C.c_int, P, P]),
+            "ora_sur_delta": (d, [C.c_int, P, d, P, P]),
+            "ora_da_sweep": (i64, [P, P, P, P, d, d, u64, i32, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -153,6 +156,13 @@ def keyed_proposal(seed, site, sweep, e, step):
     u = C.c_double()
     lib().ora_keyed_proposal(seed, site, sweep, _p(e), step, _p(e2), C.byref(u))
     return e2, u.value
+
+
+def sur_delta(F, params, scale, phiA, phiB):
+    p = np.ascontiguousarray(params, np.float32)
+    a = np.ascontiguousarray(phiA, np.float32)
+    b = np.ascontiguousarray(phiB, np.float32)
+    return lib().ora_sur_delta(F, _p(p), scale, _p(a), _p(b))
 
 
 def delta_energy(c, zold, znew):
@@ -254,6 +264,26 @@ class Tiled:
         n = lib().ora_keyed_or_sweep(self._h, _p(config), _p(zrows), C.byref(e), beta, seed,
                                      sweep, or_pass, lanes, _p(dE), _p(acc), C.byref(natt))
         return int(n), int(natt.value), e.value
+
+    def sur_features(self, config, s, e):
+        out = np.zeros(512, np.float32)
+        cfg = np.ascontiguousarray(config, np.float64)
+        ee = np.ascontiguousarray(e, np.float64)
+        F = lib().ora_sur_features(self._h, _p(cfg), s, _p(ee), _p(out))
+        return out[:F]
+
+    def da_sweep(self, config, zrows, energy, beta, step, seed, sweep, params, scales):
+        """params: list (per sublattice) of float32 parameter vectors."""
+        e = C.c_double(energy)
+        keep = [np.ascontiguousarray(p, np.float32) for p in params]
+        ptrs = (C.c_void_p * len(keep))(*[k.ctypes.data for k in keep])
+        sc = np.ascontiguousarray(scales, np.float64)
+        dE = np.zeros(max(self.n_active, 1))
+        acc = np.zeros(max(self.n_active, 1), np.int32)
+        n1 = C.c_int64()
+        n = lib().ora_da_sweep(self._h, _p(config), _p(zrows), C.byref(e), beta, step, seed, sweep,
+                               ptrs, _p(sc), _p(dE), _p(acc), C.byref(n1))
+        return int(n), int(n1.value), e.value
 
     def renormalize(self, config, zrows, energy):
         e = C.c_double(energy)

@@ -1150,3 +1150,162 @@ void ora_cpu_reset_counters(ora_cpu_chain* ch) {
 void ora_cpu_counters(const ora_cpu_chain* ch, int64_t c[4]) {
   c[0] = ch->acc_metro; c[1] = ch->att_metro; c[2] = ch->acc_or; c[3] = ch->att_or;
 }
+
+/* ======================================================================
+ * Learned delayed acceptance (this build's extension; no reference code).
+ * Descriptor of site s: its neighbour sites in order of first appearance in
+ * the pid-ordered walk (factor order as in site_coeffs!), features
+ * [e.e_n ..., (e.e_n)^2 ..., one per triplet: e.(e_x x e_y), one per body-4
+ * cluster: (e.e_x)(e_y.e_z) + (e.e_y)(e_x.e_z) + (e.e_z)(e_x.e_y), and the
+ * three spin components when a body-1 term touches s].  Features are formed in
+ * FP64 and rounded to FP32; the MLP (F -> 64 -> 64 -> 1, tanh, + linear path)
+ * runs in FP32 in the device's operation order.  Two-stage acceptance
+ * (Christen & Fox 2005): stage 1 on dE_s, stage 2 on dE - dE_s with the slot-3
+ * keyed uniform.
+ * ====================================================================== */
+#define SUR_H 64
+
+static double dot3d(const double* a, const double* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+static int site_factor_sites(const ora_tiled* t, int j, int* fs) {
+  int col = t->site_col.a[j - 1];
+  if (col > 0) { fs[0] = col; return 1; }
+  if (col < 0) { fs[0] = -col; fs[1] = t->site_col2.a[j - 1]; return 2; }
+  int inst = t->site_inst.a[j - 1];
+  int off = t->inst_ptr.a[inst - 1] - 1;
+  int body = t->inst_ptr.a[inst] - t->inst_ptr.a[inst - 1];
+  int own = t->site_slot.a[j - 1], n = 0;
+  for (int k = 1; k <= body; ++k) if (k != own) fs[n++] = t->inst_sites.a[off + k - 1];
+  return n;
+}
+
+/* Features of site s for spin e (3) against config; returns F. */
+int ora_sur_features(const ora_tiled* t, const double* config, int s, const double* e, float* out) {
+  int n, nb[256], nnb = 0, spec[512][4], nspec = 0, body1 = 0;
+  int* js = sorted_adjacency(t, s, &n);
+  for (int q = 0; q < n; ++q) {
+    int fs[8], idx[8];
+    int nf = site_factor_sites(t, js[q], fs);
+    for (int f = 0; f < nf; ++f) {
+      int k = 0;
+      while (k < nnb && nb[k] != fs[f]) ++k;
+      if (k == nnb) nb[nnb++] = fs[f];
+      idx[f] = k;
+    }
+    int col = t->site_col.a[js[q] - 1];
+    if (col < 0) { spec[nspec][0] = 1; spec[nspec][1] = idx[0]; spec[nspec][2] = idx[1]; ++nspec; }
+    else if (col == 0 && nf == 3) { spec[nspec][0] = 2; spec[nspec][1] = idx[0]; spec[nspec][2] = idx[1]; spec[nspec][3] = idx[2]; ++nspec; }
+    else if (col == 0 && nf == 0) body1 = 1;
+  }
+  free(js);
+  if (body1) for (int x = 0; x < 3; ++x) { spec[nspec][0] = 3; spec[nspec][1] = x; ++nspec; }
+  const double* E[256];
+  for (int k = 0; k < nnb; ++k) E[k] = config + 3 * (size_t)(nb[k] - 1);
+  int F = 0;
+  for (int k = 0; k < nnb; ++k) out[F++] = (float)dot3d(e, E[k]);
+  for (int k = 0; k < nnb; ++k) { double a = dot3d(e, E[k]); out[F++] = (float)(a * a); }
+  for (int q = 0; q < nspec; ++q) {
+    if (spec[q][0] == 1) {
+      const double *x = E[spec[q][1]], *y = E[spec[q][2]];
+      double cr[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2], x[0] * y[1] - x[1] * y[0]};
+      out[F++] = (float)dot3d(e, cr);
+    } else if (spec[q][0] == 2) {
+      const double *x = E[spec[q][1]], *y = E[spec[q][2]], *z = E[spec[q][3]];
+      out[F++] = (float)((dot3d(e, x) * dot3d(y, z) + dot3d(e, y) * dot3d(x, z)) + dot3d(e, z) * dot3d(x, y));
+    } else {
+      out[F++] = (float)e[spec[q][1]];
+    }
+  }
+  return F;
+}
+
+static float ora_tanhf(float z) {
+  double x = (double)z;
+  double t = ora_exp(-2.0 * fabs(x));
+  double r = (1.0 - t) / (1.0 + t);
+  return (float)(x < 0.0 ? -r : r);
+}
+
+/* params: [W1 (64 x F), b1 (64), W2 (64 x 64), b2 (64), w3 (64), v (F)] */
+double ora_sur_delta(int F, const float* params, double scale, const float* phiA, const float* phiB) {
+  const float* W1 = params;
+  const float* b1 = W1 + SUR_H * F;
+  const float* W2 = b1 + SUR_H;
+  const float* b2 = W2 + SUR_H * SUR_H;
+  const float* w3 = b2 + SUR_H;
+  const float* v = w3 + SUR_H;
+  float h1A[SUR_H], h1B[SUR_H], tA[SUR_H], tB[SUR_H];
+  for (int j = 0; j < SUR_H; ++j) {
+    float zA = b1[j], zB = b1[j];
+    for (int f = 0; f < F; ++f) {
+      float w = W1[j * F + f];
+      zA = zA + w * phiA[f];
+      zB = zB + w * phiB[f];
+    }
+    h1A[j] = ora_tanhf(zA);
+    h1B[j] = ora_tanhf(zB);
+  }
+  for (int j = 0; j < SUR_H; ++j) {
+    float zA = b2[j], zB = b2[j];
+    for (int k = 0; k < SUR_H; ++k) {
+      float w = W2[j * SUR_H + k];
+      zA = zA + w * h1A[k];
+      zB = zB + w * h1B[k];
+    }
+    float oA = w3[j] * ora_tanhf(zA), oB = w3[j] * ora_tanhf(zB);
+    float sA = 0.0f, sB = 0.0f;
+    for (int f = j; f < F; f += SUR_H) {
+      sA = sA + v[f] * phiA[f];
+      sB = sB + v[f] * phiB[f];
+    }
+    tA[j] = oA + sA;
+    tB[j] = oB + sB;
+  }
+  for (int off = SUR_H / 2; off >= 1; off >>= 1)
+    for (int l = 0; l < off; ++l) { tA[l] = tA[l] + tA[l + off]; tB[l] = tB[l] + tB[l + off]; }
+  return ((double)tB[0] - (double)tA[0]) * scale;
+}
+
+/* One keyed delayed-acceptance sweep (device contract, 64 lanes).  params[a],
+ * F[a], scale[a] per sublattice a = (site-1) % n_cell_atoms.  Returns nacc;
+ * *n_stage1 = proposals that passed the surrogate stage. */
+int64_t ora_da_sweep(const ora_tiled* t, double* config, double* zrows, double* energy, double beta,
+                     double step, uint64_t seed, int32_t sweep, const float* const* params,
+                     const double* scale, double* dE_q, int32_t* acc_q, int64_t* n_stage1) {
+  int nlm = t->nlm;
+  double znew[49], e2[3], u;
+  float phiA[512], phiB[512];
+  int64_t nacc = 0;
+  *n_stage1 = 0;
+  /* proposals of the whole sweep first (they depend only on the site's own spin) */
+  double* props = malloc((size_t)t->n_active * 4 * sizeof(double));
+  for (int q = 0; q < t->n_active; ++q) {
+    int s = t->color_sites.a[q];
+    ora_keyed_proposal(seed, s, sweep, config + 3 * (size_t)(s - 1), step, props + 4 * q, props + 4 * q + 3);
+  }
+  for (int q = 0; q < t->n_active; ++q) {
+    int s = t->color_sites.a[q];
+    double* e = config + 3 * (size_t)(s - 1);
+    e2[0] = props[4 * q]; e2[1] = props[4 * q + 1]; e2[2] = props[4 * q + 2]; u = props[4 * q + 3];
+    int a = (s - 1) % t->n_cell_atoms;
+    int F = ora_sur_features(t, config, s, e, phiA);
+    ora_sur_features(t, config, s, e2, phiB);
+    double dEs = ora_sur_delta(F, params[a], scale[a], phiA, phiB);
+    dE_q[q] = 0.0; acc_q[q] = 0;
+    if (!(dEs <= 0.0 || u < ora_exp(-beta * dEs))) continue;
+    ++*n_stage1;
+    ora_zlm_row(e2, t->lmax, znew);
+    double dE = ora_walk_delta(t, s, zrows, znew, 64);
+    double d2 = dE - dEs;
+    uint32_t b[4];
+    ora_philox_block(seed, s, sweep, 3u, 0u, b);
+    if (d2 <= 0.0 || ora_uniform(b[0], b[1]) < ora_exp(-beta * d2)) {
+      e[0] = e2[0]; e[1] = e2[1]; e[2] = e2[2];
+      memcpy(zrows + (size_t)(s - 1) * nlm, znew, (size_t)nlm * sizeof(double));
+      dE_q[q] = dE; acc_q[q] = 1; ++nacc;
+    }
+  }
+  free(props);
+  *energy = *energy + ora_reduce_tree(t->n_active, dE_q);
+  return nacc;
+}

@@ -81,6 +81,13 @@ int64_t ora_keyed_or_sweep(const ora_tiled* t, double* config, double* zrows,
                            int64_t* natt);
 double ora_renormalize(const ora_tiled* t, double* config, double* zrows, double* energy);
 
+/* ---- learned delayed acceptance (this build's extension) ---- */
+int ora_sur_features(const ora_tiled* t, const double* config, int s, const double* e, float* out);
+double ora_sur_delta(int F, const float* params, double scale, const float* phiA, const float* phiB);
+int64_t ora_da_sweep(const ora_tiled* t, double* config, double* zrows, double* energy, double beta,
+                     double step, uint64_t seed, int32_t sweep, const float* const* params,
+                     const double* scale, double* dE_q, int32_t* acc_q, int64_t* n_stage1);
+
 /* ---- reference CPU path (src/updates.jl, src/state.jl) ---- */
 typedef struct ora_cpu_chain ora_cpu_chain;
 ora_cpu_chain* ora_cpu_chain_create(const ora_tiled* t, const double* config,

@@ -61,6 +61,9 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
   L->nb_ptr.assign(na + 1, 0);
   L->walk.clear();
   L->nb.clear();
+  L->feat_ptr.assign(na + 1, 0);
+  L->feat.clear();
+  L->nfeat_max = 0;
   L->nb_max = 0;
   L->w_max = 0;
   // representative site of sublattice a: atom a in cell 0 (0-based site id a)
@@ -76,6 +79,7 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
   for (int a = 0; a < na; ++a) {
     L->walk_ptr[a] = (int32_t)L->walk.size();
     L->nb_ptr[a] = (int32_t)(L->nb.size() / 4);
+    L->feat_ptr[a] = (int32_t)(L->feat.size() / 4);
     if (!d->site_active[a]) continue;
     std::map<std::tuple<int, int, int, int>, int> nbmap;
     std::vector<std::array<int, 4>> nbl;
@@ -136,9 +140,31 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
     }
     for (auto& v : nbl) L->nb.insert(L->nb.end(), v.begin(), v.end());
     L->nb_max = std::max(L->nb_max, NB);
+    // descriptor cluster specs, in walk order
+    L->feat_ptr[a] = (int32_t)(L->feat.size() / 4);
+    bool body1 = false;
+    pos = 0;
+    for (int j : rep_js[a]) {
+      int nbi[3];
+      int nf = 0;
+      while (rep_nbidx[a][pos] >= 0) nbi[nf++] = rep_nbidx[a][pos++];
+      ++pos;
+      if (d->site_col[j - 1] < 0) {
+        L->feat.insert(L->feat.end(), {1, nbi[0], nbi[1], 0});
+      } else if (d->site_col[j - 1] == 0 && nf == 3) {
+        L->feat.insert(L->feat.end(), {2, nbi[0], nbi[1], nbi[2]});
+      } else if (d->site_col[j - 1] == 0 && nf == 0) {
+        body1 = true;
+      }
+    }
+    if (body1)
+      for (int x = 0; x < 3; ++x) L->feat.insert(L->feat.end(), {3, x, 0, 0});
+    const int nspec = (int)(L->feat.size() / 4) - L->feat_ptr[a];
+    L->nfeat_max = std::max(L->nfeat_max, 2 * NB + nspec);
   }
   L->walk_ptr[na] = (int32_t)L->walk.size();
   L->nb_ptr[na] = (int32_t)(L->nb.size() / 4);
+  L->feat_ptr[na] = (int32_t)(L->feat.size() / 4);
   for (int a = 0; a < na; ++a) L->w_max = std::max(L->w_max, L->walk_ptr[a + 1] - L->walk_ptr[a]);
 
   // verification: every active site is its sublattice stencil, translated

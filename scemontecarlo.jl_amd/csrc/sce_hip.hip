@@ -78,7 +78,7 @@ extern "C" int32_t sce_device_count(int32_t* n) {
 // ---------------------------------------------------------------------------
 struct ChainDev {
   double energy, step, beta, max_drift, last_drift, e_full;
-  long long sweep_index, acc_metro, att_metro, acc_or, att_or, nacc_last;
+  long long sweep_index, acc_metro, att_metro, acc_or, att_or, nacc_last, acc_stage1;
   unsigned long long seed;
   unsigned long long iacc, iatt;
   unsigned int ticket;
@@ -102,6 +102,9 @@ struct DevTables {
   const int32_t* efac_ptr;   // 0-based offsets
   const int32_t* efac_row;   // 0-based rows
   const double* term_coef;
+  const int32_t* feat_ptr;   // surrogate descriptor specs per sublattice
+  const int4* feat;
+  int nfeat_max;
 };
 
 constexpr int TPB = 256;      // threads per block of the walk kernels
@@ -211,7 +214,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
                                             double* __restrict__ dE, int* __restrict__ acc,
                                             double beta, int q, bool valid,
                                             const int* __restrict__ test_sites, double* lds,
-                                            int stride) {
+                                            int stride, double* __restrict__ dE_raw = nullptr) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
   double* Z = lds + g * stride;
@@ -252,6 +255,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   if (lane == 0) {
     dE[q] = accept ? a : 0.0;
     acc[q] = accept;
+    if (dE_raw) dE_raw[q] = a;
   }
 }
 
@@ -261,13 +265,15 @@ __global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__
                                                const double* __restrict__ prop,
                                                double* __restrict__ dE, int* __restrict__ acc,
                                                const ChainDev* __restrict__ st, int q_lo,
-                                               int q_n, const int* __restrict__ test_sites) {
+                                               int q_n, const int* __restrict__ test_sites,
+                                               double* __restrict__ dE_raw = nullptr) {
   constexpr int G = TPB / L;
   extern __shared__ double lds[];
   const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / L;
   const double beta = TEST ? 0.0 : st->beta;
   metro_group<L, LMAX, TEST>(T, config, zrows, prop, dE, acc, beta, q_lo + qi, qi < q_n,
-                             test_sites, lds, T.nb_max * (LMAX + 1) * (LMAX + 1) + 1 + (LMAX + 1) * (LMAX + 1));
+                             test_sites, lds, T.nb_max * (LMAX + 1) * (LMAX + 1) + 1 + (LMAX + 1) * (LMAX + 1),
+                             dE_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -388,7 +394,8 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
 // deterministic tree reduction: chunks of 512 (zero padded) folded by offsets
 // 256..1; a multi-chunk input recurses over the chunk sums.
 // ---------------------------------------------------------------------------
-enum { MODE_METRO = 0, MODE_OR = 1, MODE_RENORM = 2, MODE_TOTAL = 3, MODE_MEASURE = 4, MODE_TEST = 5 };
+enum { MODE_METRO = 0, MODE_OR = 1, MODE_RENORM = 2, MODE_TOTAL = 3, MODE_MEASURE = 4, MODE_TEST = 5,
+       MODE_DA = 6 };
 
 __device__ double block_tree(double x0, double x1, double* sh) {
   // 256 threads, v[t] = x0 + x1 is the offset-256 step
@@ -430,10 +437,10 @@ __device__ void chunk_body(long long b, const double* __restrict__ x, long long 
     const long long i0 = b * TREE_B + t, i1 = i0 + 256;
     const double v = block_tree(i0 < n ? x[i0] : 0.0, i1 < n ? x[i1] : 0.0, sh);
     if (t == 0) part[b] = v;
-    if (MODE == MODE_METRO || MODE == MODE_OR) {
+    if (MODE == MODE_METRO || MODE == MODE_OR || MODE == MODE_DA) {
       long long ca = (i0 < n ? iacc[i0] : 0) + (i1 < n ? iacc[i1] : 0);
       long long ct = 0;
-      if (MODE == MODE_OR) ct = (i0 < n ? iatt[i0] : 0) + (i1 < n ? iatt[i1] : 0);
+      if (MODE == MODE_OR || MODE == MODE_DA) ct = (i0 < n ? iatt[i0] : 0) + (i1 < n ? iatt[i1] : 0);
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
         ca += __shfl_down(ca, off, 64);
@@ -487,7 +494,7 @@ __device__ void final_body(double* part, long long n, const long long* ipart, lo
   double sum;
   double m3[3] = {0.0, 0.0, 0.0};
   long long nacc = 0, natt = 0;
-  if (MODE == MODE_METRO || MODE == MODE_OR) {
+  if (MODE == MODE_METRO || MODE == MODE_OR || MODE == MODE_DA) {
     // integer counts: order-free sum over the chunk partials
     long long ca = 0, ct = 0;
     for (long long b = t; b < nbx; b += 256) {
@@ -543,12 +550,13 @@ __device__ void final_body(double* part, long long n, const long long* ipart, lo
     }
   }
   if (t != 0) return;
-  if (MODE == MODE_METRO) {
+  if (MODE == MODE_METRO || MODE == MODE_DA) {
     st->energy = st->energy + sum;
     st->acc_metro += nacc;
     st->att_metro += n_att;
     st->nacc_last = nacc;
     st->sweep_index += 1;
+    if (MODE == MODE_DA) st->acc_stage1 += natt;
   } else if (MODE == MODE_OR) {
     st->energy = st->energy + sum;
     st->acc_or += nacc;
@@ -933,6 +941,7 @@ struct sce_tables {
   std::vector<int32_t> color_ptr;  // 0-based offsets, n_colors + 1
   int32_t* d_color_ptr = nullptr;
   int max_class = 0;
+  std::vector<int> sub_F;  // surrogate descriptor width per sublattice (0: inactive)
   int n_colors = 0;
   int64_t total_walk = 0;
   int w_max = 0;
@@ -1044,6 +1053,12 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   if ((r = dupload(t->mem, &pn, reinterpret_cast<const int4*>(L.nb.data()), L.nb.size() / 4)) != SCE_OK)
     return fail(r);
   D.nb = pn;
+  if ((r = dupload(t->mem, &p32, L.feat_ptr.data(), L.feat_ptr.size())) != SCE_OK) return fail(r);
+  D.feat_ptr = p32;
+  if ((r = dupload(t->mem, &pn, reinterpret_cast<const int4*>(L.feat.data()), L.feat.size() / 4)) != SCE_OK)
+    return fail(r);
+  D.feat = pn;
+  D.nfeat_max = L.nfeat_max;
   // energy tables, 0-based
   std::vector<int32_t> v;
   v.resize(d->n_inst);
@@ -1083,12 +1098,16 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   if ((r = dupload(t->mem, &t->d_color_ptr, t->color_ptr.data(), t->color_ptr.size())) != SCE_OK)
     return fail(r);
   t->total_walk = (int64_t)L.walk.size();
+  t->sub_F.assign(L.na, 0);
+  for (int a = 0; a < L.na; ++a)
+    if (d->site_active[a])
+      t->sub_F[a] = 2 * (L.nb_ptr[a + 1] - L.nb_ptr[a]) + (L.feat_ptr[a + 1] - L.feat_ptr[a]);
   t->w_max = L.w_max;
   *out = t;
   return SCE_OK;
 }
 
-extern "C" int32_t sce_layout_check(const sce_tables_desc* d, int64_t o[4]) {
+extern "C" int32_t sce_layout_check(const sce_tables_desc* d, int64_t o[5]) {
   if (!d || !o) return sce_fail(SCE_EINVAL, "sce_layout_check: NULL argument");
   HostLayout L;
   CHK(sce_derive_layout(d, &L));
@@ -1096,6 +1115,7 @@ extern "C" int32_t sce_layout_check(const sce_tables_desc* d, int64_t o[4]) {
   o[1] = L.nb_max;
   o[2] = L.w_max;
   o[3] = (int64_t)L.walk.size();
+  o[4] = L.nfeat_max;
   return SCE_OK;
 }
 
@@ -1104,12 +1124,13 @@ extern "C" int32_t sce_tables_destroy(sce_tables* t) {
   return SCE_OK;
 }
 
-extern "C" int32_t sce_tables_layout_info(const sce_tables* t, int64_t o[4]) {
+extern "C" int32_t sce_tables_layout_info(const sce_tables* t, int64_t o[5]) {
   if (!t || !o) return sce_fail(SCE_EINVAL, "NULL argument");
   o[0] = t->dev.na;
   o[1] = t->dev.nb_max;
   o[2] = t->w_max;
   o[3] = t->total_walk;
+  o[4] = t->dev.nfeat_max;
   return SCE_OK;
 }
 
@@ -1433,6 +1454,7 @@ extern "C" int32_t sce_chain_info_get(sce_chain* c, sce_chain_info* o) {
   o->acc_or = h.acc_or;
   o->att_or = h.att_or;
   o->seed = h.seed;
+  o->acc_stage1 = h.acc_stage1;
   return SCE_OK;
 }
 
@@ -1449,7 +1471,7 @@ extern "C" int32_t sce_chain_reset_counters(sce_chain* c) {
   CHK(chain_check(c));
   ChainDev h;
   CHK(read_state(c, &h));
-  h.acc_metro = h.att_metro = h.acc_or = h.att_or = 0;
+  h.acc_metro = h.att_metro = h.acc_or = h.att_or = h.acc_stage1 = 0;
   h.max_drift = 0.0;
   HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
   return SCE_OK;
@@ -2369,6 +2391,7 @@ extern "C" int32_t sce_batch_info(sce_batch* b, int32_t k, sce_chain_info* o) {
   o->acc_or = h.acc_or;
   o->att_or = h.att_or;
   o->seed = h.seed;
+  o->acc_stage1 = h.acc_stage1;
   return SCE_OK;
 }
 
@@ -2415,5 +2438,912 @@ extern "C" int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc
 extern "C" int32_t sce_batch_set_graph(sce_batch* b, int32_t on) {
   if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
   b->use_graph = on != 0;
+  return SCE_OK;
+}
+
+// ===========================================================================
+// Learned delayed acceptance.
+//
+// Stage 1 (Christen & Fox 2005): a surrogate site energy g(phi(e)) — an MLP
+// (F -> 64 -> 64 -> 1, tanh, plus a linear path v.phi) of a rotation-invariant
+// descriptor phi of the site's spin against its neighbour spins — gives
+// dE_s = g(phi(e')) - g(phi(e)), exactly antisymmetric under e <-> e'.  The
+// proposal passes with min(1, exp(-beta dE_s)) (uniform: slot 0 words 3-4).
+// Stage 2: the exact FP64 walk dE decides with min(1, exp(-beta (dE - dE_s)))
+// (uniform: slot 3 words 1-2).  A stage-1 rejection skips the neighbour-row
+// staging and the walk.  Detailed balance holds for any surrogate; the exact
+// FP64 dE always makes the final decision.
+//
+// Inference runs one site per wavefront in FP32 (lane j = hidden unit j) with
+// a fixed operation order (restated by the oracle); training uses MFMA FP32
+// GEMMs over minibatches of harvested (phi(e), phi(e'), exact dE) tuples.
+// ===========================================================================
+constexpr int SUR_H = 64;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct SurNet {
+  const float *W1, *b1, *W2, *b2, *w3, *v;  // W1: H x F row-major, W2: H x H row-major
+  int F;
+  double scale;
+};
+
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// Descriptor feature f of spin e against the staged neighbour spins E (NB x 3).
+__device__ __forceinline__ float sur_feature(int f, int NB, const int4* __restrict__ spec,
+                                             const double* E, const double* e) {
+  if (f < NB) return (float)dot3(e, E + 3 * f);
+  if (f < 2 * NB) {
+    const double a = dot3(e, E + 3 * (f - NB));
+    return (float)(a * a);
+  }
+  const int4 sp = spec[f - 2 * NB];
+  if (sp.x == 1) {
+    const double* x = E + 3 * sp.y;
+    const double* y = E + 3 * sp.z;
+    const double cr[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2], x[0] * y[1] - x[1] * y[0]};
+    return (float)dot3(e, cr);
+  }
+  if (sp.x == 2) {
+    const double* x = E + 3 * sp.y;
+    const double* y = E + 3 * sp.z;
+    const double* z = E + 3 * sp.w;
+    return (float)((dot3(e, x) * dot3(y, z) + dot3(e, y) * dot3(x, z)) + dot3(e, z) * dot3(x, y));
+  }
+  return (float)e[sp.y];
+}
+
+// tanh in FP32 from the deterministic exp: sign(x)(1 - t)/(1 + t), t = e^{-2|x|}
+__device__ __forceinline__ float det_tanhf(float z) {
+  const double x = (double)z;
+  const double t = sce::det_exp(-2.0 * fabs(x));
+  const double r = (1.0 - t) / (1.0 + t);
+  return (float)(x < 0.0 ? -r : r);
+}
+
+// Stages the neighbour spins of site s (atom a) into E (NB x 3).
+template <int L>
+__device__ __forceinline__ void stage_spins(const DevTables& T, const double* __restrict__ config, int s,
+                                            int lane, double* E, int* NB_out) {
+  const int atom = s % T.na;
+  const int cell = s / T.na;
+  const int c1 = cell % T.d0;
+  const int r = cell / T.d0;
+  const int c2 = r % T.d1;
+  const int c3 = r / T.d1;
+  const int nb0 = T.nb_ptr[atom];
+  const int NB = T.nb_ptr[atom + 1] - nb0;
+  for (int idx = lane; idx < 3 * NB; idx += L) {
+    const int nbi = idx / 3;
+    const int4 o = T.nb[nb0 + nbi];
+    int x = c1 + o.y, y = c2 + o.z, z = c3 + o.w;
+    if (x >= T.d0) x -= T.d0;
+    if (y >= T.d1) y -= T.d1;
+    if (z >= T.d2) z -= T.d2;
+    const int n = o.x + T.na * (x + T.d0 * (y + T.d1 * z));
+    E[idx] = config[3 * (size_t)n + (idx - 3 * nbi)];
+  }
+  *NB_out = NB;
+}
+
+// The surrogate difference g(phiB) - g(phiA) of one site, one wavefront (64
+// lanes, lane j = hidden unit j); phiA/phiB/h1 live in the group's LDS.
+// Block barriers inside: every thread of the block calls it.
+__device__ double sur_delta(const SurNet& net, const float* phiA, const float* phiB, float* h1A,
+                            float* h1B, bool valid) {
+  const int j = threadIdx.x % 64;
+  float oA = 0.0f, oB = 0.0f, sA = 0.0f, sB = 0.0f;
+  if (valid) {
+    const int F = net.F;
+    float zA = net.b1[j], zB = net.b1[j];
+    const float* w = net.W1 + (size_t)j * F;
+    for (int f = 0; f < F; ++f) {
+      const float wf = w[f];
+      zA = zA + wf * phiA[f];
+      zB = zB + wf * phiB[f];
+    }
+    h1A[j] = det_tanhf(zA);
+    h1B[j] = det_tanhf(zB);
+  }
+  __syncthreads();
+  if (valid) {
+    float zA = net.b2[j], zB = net.b2[j];
+    const float* w = net.W2 + (size_t)j * SUR_H;
+    for (int k = 0; k < SUR_H; ++k) {
+      const float wk = w[k];
+      zA = zA + wk * h1A[k];
+      zB = zB + wk * h1B[k];
+    }
+    oA = net.w3[j] * det_tanhf(zA);
+    oB = net.w3[j] * det_tanhf(zB);
+    for (int f = j; f < net.F; f += 64) {
+      sA = sA + net.v[f] * phiA[f];
+      sB = sB + net.v[f] * phiB[f];
+    }
+  }
+  float tA = oA + sA, tB = oB + sB;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    tA = tA + __shfl_down(tA, off, 64);
+    tB = tB + __shfl_down(tB, off, 64);
+  }
+  __syncthreads();
+  return ((double)tB - (double)tA) * net.scale;  // valid in lane 0
+}
+
+// LDS of one DA group (doubles): walk block (stride) | spins (3 NBmax) |
+// phiA, phiB (F floats each) | h1A, h1B (64 floats each)
+__host__ __device__ inline int da_stride(int walk_stride, int nb_max, int fmax) {
+  return walk_stride + 3 * nb_max + (2 * fmax + 2 * SUR_H + 1) / 2 + 1;
+}
+
+// Two-stage delayed-acceptance update of one site (64 lanes, one wavefront).
+template <int LMAX>
+__device__ void da_group(const DevTables& T, double* __restrict__ config, double* __restrict__ zrows,
+                         const double* __restrict__ prop, double* __restrict__ dE,
+                         int* __restrict__ acc, int* __restrict__ acc1, const ChainDev* st,
+                         const SurNet* __restrict__ nets, int q, bool valid, double* lds,
+                         int walk_stride, int gstride) {
+  constexpr int L = 64, NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  double* base = lds + g * gstride;
+  double* Z = base;
+  double* dz = Z + T.nb_max * NLM + 1;
+  double* E = base + walk_stride;
+  float* phiA = (float*)(E + 3 * T.nb_max);
+  float* phiB = phiA + T.nfeat_max;
+  float* h1A = phiB + T.nfeat_max;
+  float* h1B = h1A + SUR_H;
+  int s = 0, atom = 0, NB = 0;
+  const double* pq = prop + (size_t)q * PS;
+  double e[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
+  SurNet net{};
+  if (valid) {
+    s = T.color_sites[q];
+    atom = s % T.na;
+    net = nets[atom];
+    e[0] = config[3 * (size_t)s];
+    e[1] = config[3 * (size_t)s + 1];
+    e[2] = config[3 * (size_t)s + 2];
+    e2[0] = pq[0];
+    e2[1] = pq[1];
+    e2[2] = pq[2];
+    stage_spins<L>(T, config, s, lane, E, &NB);
+  }
+  __syncthreads();
+  if (valid) {
+    const int4* spec = T.feat + T.feat_ptr[atom];
+    for (int f = lane; f < net.F; f += L) {
+      phiA[f] = sur_feature(f, NB, spec, E, e);
+      phiB[f] = sur_feature(f, NB, spec, E, e2);
+    }
+  }
+  __syncthreads();
+  const double dEs = sur_delta(net, phiA, phiB, h1A, h1B, valid);
+  int pass = 0;
+  if (valid && lane == 0) pass = (dEs <= 0.0 || pq[3] < sce::det_exp(-st->beta * dEs)) ? 1 : 0;
+  pass = __shfl(pass, 0, L);
+  const bool walk = valid && pass;
+  if (walk) {
+    stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+    for (int k = lane; k < NLM; k += L) dz[k] = pq[4 + k] - zrows[(size_t)s * NLM + k];
+  }
+  __syncthreads();
+  double a = 0.0;
+  if (walk) {
+    const int w0 = T.walk_ptr[atom];
+    a = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
+  }
+#pragma unroll
+  for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
+  __syncthreads();
+  if (!valid) return;
+  int accept = 0;
+  if (lane == 0 && walk) {
+    const double d2 = a - dEs;
+    const sce::u32x4 b = sce::philox_block(st->seed, s + 1, (int32_t)(st->sweep_index + 1), sce::SLOT_DA2, 0u);
+    accept = (d2 <= 0.0 || sce::uniform01(b.x, b.y) < sce::det_exp(-st->beta * d2)) ? 1 : 0;
+  }
+  accept = __shfl(accept, 0, L);
+  if (accept) {
+    for (int k = lane; k < NLM; k += L) zrows[(size_t)s * NLM + k] = pq[4 + k];
+    if (lane == 0) {
+      config[3 * (size_t)s] = pq[0];
+      config[3 * (size_t)s + 1] = pq[1];
+      config[3 * (size_t)s + 2] = pq[2];
+    }
+  }
+  if (lane == 0) {
+    dE[q] = accept ? a : 0.0;
+    acc[q] = accept;
+    acc1[q] = pass;
+  }
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(TPB) void k_da(DevTables T, double* config, double* zrows, const double* prop,
+                                            double* dE, int* acc, int* acc1, const ChainDev* st,
+                                            const SurNet* nets, int q_lo, int q_n, int walk_stride,
+                                            int gstride) {
+  constexpr int G = TPB / 64;
+  extern __shared__ double lds[];
+  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / 64;
+  da_group<LMAX>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo + qi, qi < q_n, lds, walk_stride,
+                 gstride);
+}
+
+// Descriptor rows of a colour slice (for harvesting and tests): one wavefront
+// per site, phi(e) and phi(e') written as FP32 rows of width fmax.
+template <bool FROM_PROP>
+__global__ __launch_bounds__(TPB) void k_features(DevTables T, const double* __restrict__ config,
+                                                  const double* __restrict__ prop, int PS,
+                                                  const int* __restrict__ sites, const double* __restrict__ e2s,
+                                                  int q_lo, int q_n, float* __restrict__ outA,
+                                                  float* __restrict__ outB, int8_t* __restrict__ sub) {
+  constexpr int L = 64, G = TPB / L;
+  extern __shared__ double lds[];
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  const int qi = blockIdx.x * G + g;
+  const bool valid = qi < q_n;
+  double* E = lds + g * 3 * T.nb_max;
+  int s = 0, NB = 0;
+  if (valid) {
+    s = FROM_PROP ? T.color_sites[q_lo + qi] : sites[qi];
+    stage_spins<L>(T, config, s, lane, E, &NB);
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int atom = s % T.na;
+  const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
+  const double* p2 = FROM_PROP ? prop + (size_t)(q_lo + qi) * PS : e2s + 3 * (size_t)qi;
+  const double e2[3] = {p2[0], p2[1], p2[2]};
+  const int4* spec = T.feat + T.feat_ptr[atom];
+  const int F = 2 * NB + (T.feat_ptr[atom + 1] - T.feat_ptr[atom]);
+  float* ra = outA + (size_t)qi * T.nfeat_max;
+  float* rb = outB + (size_t)qi * T.nfeat_max;
+  for (int f = lane; f < T.nfeat_max; f += L) {
+    ra[f] = f < F ? sur_feature(f, NB, spec, E, e) : 0.0f;
+    rb[f] = f < F ? sur_feature(f, NB, spec, E, e2) : 0.0f;
+  }
+  if (lane == 0 && sub) sub[qi] = (int8_t)atom;
+}
+
+// Surrogate dE of given descriptor rows through the inference path (tests).
+__global__ __launch_bounds__(TPB) void k_sur_eval(const SurNet* nets, int sub, int n, int fmax,
+                                                  const float* A, const float* B, double* out) {
+  constexpr int G = TPB / 64;
+  __shared__ float buf[G][2 * SUR_H];
+  extern __shared__ double lds[];
+  const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int i = blockIdx.x * G + g;
+  const bool valid = i < n;
+  float* phiA = (float*)lds + (size_t)g * 2 * fmax;
+  float* phiB = phiA + fmax;
+  if (valid)
+    for (int f = lane; f < fmax; f += 64) {
+      phiA[f] = A[(size_t)i * fmax + f];
+      phiB[f] = B[(size_t)i * fmax + f];
+    }
+  __syncthreads();
+  const SurNet net = nets[sub];
+  const double d = sur_delta(net, phiA, phiB, buf[g], buf[g] + SUR_H, valid);
+  if (valid && lane == 0) out[i] = d;
+}
+
+// ---- training: MFMA FP32 GEMMs ---------------------------------------------
+// C (M x N, ldc) = op(A) (M x K) . op(B) (K x N) [+ C when accumulate]; each
+// 64-thread wavefront computes one 16 x 16 tile with v_mfma_f32_16x16x4f32.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* __restrict__ A, int lda,
+                                              const float* __restrict__ B, int ldb, float* __restrict__ C,
+                                              int ldc, int accumulate) {
+  const int lane = threadIdx.x % 64;
+  const int wave = blockIdx.x * 4 + threadIdx.x / 64;
+  const int tiles_n = (N + 15) / 16;
+  const int tm = wave / tiles_n, tn = wave % tiles_n;
+  if (tm * 16 >= M) return;
+  const int m0 = tm * 16, n0 = tn * 16;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int ra = m0 + lane % 16, cb = n0 + lane % 16, kk = lane / 16;
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int k = k0 + kk;
+    float a = 0.f, b = 0.f;
+    if (ra < M && k < K) a = TA ? A[(size_t)k * lda + ra] : A[(size_t)ra * lda + k];
+    if (cb < N && k < K) b = TB ? B[(size_t)cb * ldb + k] : B[(size_t)k * ldb + cb];
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = m0 + 4 * (lane / 16) + i, c = n0 + lane % 16;
+    if (r < M && c < N) C[(size_t)r * ldc + c] = accumulate ? C[(size_t)r * ldc + c] + acc[i] : acc[i];
+  }
+}
+
+static void gemm(bool ta, bool tb, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
+                 float* C, int ldc, int accumulate, hipStream_t s) {
+  const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
+  const int nb = (tiles + 3) / 4;
+  if (!ta && !tb) k_gemm<false, false><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+  else if (ta && !tb) k_gemm<true, false><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+  else if (!ta && tb) k_gemm<false, true><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+  else k_gemm<true, true><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+}
+
+// Z (rows x H) += bias; H = tanh(Z) (in place)
+__global__ void k_bias_tanh(float* Z, const float* b, long long rows, int H) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= rows * H) return;
+  Z[i] = tanhf(Z[i] + b[i % H]);
+}
+
+// g_r = H2_r . w3 + X_r . v; residual and output-gradient of the loss
+// mean((scale (g_B - g_A) - y)^2) / scale^2 over B pairs (rows r and B + r).
+__global__ void k_head(const float* H2, const float* w3, const float* X, const float* v, int F, int fmax,
+                       int Bn, const double* y, double scale, float* dg, double* loss_terms) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= Bn) return;
+  double gA = 0.0, gB = 0.0;
+  for (int j = 0; j < SUR_H; ++j) {
+    gA += (double)H2[(size_t)r * SUR_H + j] * w3[j];
+    gB += (double)H2[(size_t)(Bn + r) * SUR_H + j] * w3[j];
+  }
+  for (int f = 0; f < F; ++f) {
+    gA += (double)X[(size_t)r * fmax + f] * v[f];
+    gB += (double)X[(size_t)(Bn + r) * fmax + f] * v[f];
+  }
+  const double res = (gB - gA) - y[r] / scale;
+  loss_terms[r] = res * res;
+  const float d = (float)(2.0 * res / Bn);
+  dg[r] = -d;
+  dg[Bn + r] = d;
+}
+
+// dZ2 = (dg w3^T) * (1 - H2^2); also dH2 stored nowhere (fused)
+__global__ void k_back2(const float* H2, const float* w3, const float* dg, long long rows, float* dZ2) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= rows * SUR_H) return;
+  const float h = H2[i];
+  dZ2[i] = dg[i / SUR_H] * w3[i % SUR_H] * (1.0f - h * h);
+}
+
+__global__ void k_dtanh(float* dZ, const float* Hh, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float h = Hh[i];
+  dZ[i] = dZ[i] * (1.0f - h * h);
+}
+
+// out[c] = sum_r M[r, c] (+ optional second matrix weighting) — column sums, one block per column
+__global__ void k_colsum(const float* Mx, long long rows, int cols, int ld, float* out) {
+  __shared__ double sh[256];
+  const int c = blockIdx.x;
+  double a = 0.0;
+  for (long long r = threadIdx.x; r < rows; r += 256) a += Mx[(size_t)r * ld + c];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 128; off >= 1; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = (float)sh[0];
+}
+
+// grad = sum_r coef[r] * M[r, c]  (GEMV M^T coef), one block per column
+__global__ void k_gemv_t(const float* Mx, const float* coef, long long rows, int cols, int ld, float* out) {
+  __shared__ double sh[256];
+  const int c = blockIdx.x;
+  double a = 0.0;
+  for (long long r = threadIdx.x; r < rows; r += 256) a += (double)Mx[(size_t)r * ld + c] * coef[r];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 128; off >= 1; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = (float)sh[0];
+}
+
+__global__ void k_adam(float* p, const float* g, float* m, float* v2, long long n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  m[i] = b1 * m[i] + (1.f - b1) * gi;
+  v2[i] = b2 * v2[i] + (1.f - b2) * gi * gi;
+  p[i] -= lr * (m[i] / c1) / (sqrtf(v2[i] / c2) + eps);
+}
+
+// Gather the minibatch rows idx[0..B) of the dataset: X = [A rows; B rows], y.
+__global__ void k_gather(const float* dA, const float* dB, const double* dy, const long long* idx, int Bn,
+                         int fmax, float* X, double* y) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= (long long)Bn * fmax) return;
+  const int r = (int)(i / fmax), f = (int)(i % fmax);
+  const long long row = idx[r];
+  X[(size_t)r * fmax + f] = dA[(size_t)row * fmax + f];
+  X[(size_t)(Bn + r) * fmax + f] = dB[(size_t)row * fmax + f];
+  if (f == 0) y[r] = dy[row];
+}
+
+// ---- surrogate host side ------------------------------------------------------
+struct SurSub {
+  int F = 0;
+  size_t np = 0;
+  float *params = nullptr, *grads = nullptr, *m = nullptr, *v2 = nullptr;
+  double scale = 1.0;
+  long long adam_t = 0;
+};
+
+struct sce_surrogate {
+  sce_tables* t = nullptr;
+  int na = 0, fmax = 0;
+  std::vector<SurSub> subs;
+  SurNet* d_nets = nullptr;
+  DevBuf mem;
+  // harvested dataset: rows of phi(e), phi(e') (fmax floats each), exact dE, sublattice
+  float *dA = nullptr, *dB = nullptr;
+  double* dy = nullptr;
+  int8_t* dsub = nullptr;
+  long long n_rows = 0, cap = 0;
+  // training scratch for the current batch size
+  int bsz = 0;
+  float *X = nullptr, *Z1 = nullptr, *Z2 = nullptr, *dg = nullptr, *dZ2 = nullptr, *dH1 = nullptr;
+  double *yb = nullptr, *loss_terms = nullptr;
+  long long* idx = nullptr;
+  long long idx_cap = 0;
+  DevBuf scratch;
+  hipStream_t stream = nullptr;
+  ~sce_surrogate() {
+    if (stream) (void)hipStreamSynchronize(stream), (void)hipStreamDestroy(stream);
+  }
+};
+
+static size_t sur_np(int F) { return (size_t)SUR_H * F + SUR_H + (size_t)SUR_H * SUR_H + SUR_H + SUR_H + F; }
+
+static int32_t sur_upload_nets(sce_surrogate* s) {
+  std::vector<SurNet> h(s->na);
+  for (int a = 0; a < s->na; ++a) {
+    SurSub& u = s->subs[a];
+    SurNet& n = h[a];
+    n.F = u.F;
+    n.scale = u.scale;
+    if (!u.params) continue;
+    n.W1 = u.params;
+    n.b1 = n.W1 + (size_t)SUR_H * u.F;
+    n.W2 = n.b1 + SUR_H;
+    n.b2 = n.W2 + SUR_H * SUR_H;
+    n.w3 = n.b2 + SUR_H;
+    n.v = n.w3 + SUR_H;
+  }
+  HIPCHK(hipMemcpy(s->d_nets, h.data(), h.size() * sizeof(SurNet), hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_sur_create(sce_tables* t, uint64_t seed, sce_surrogate** out) {
+  if (!t || !out) return sce_fail(SCE_EINVAL, "sce_sur_create: NULL argument");
+  *out = nullptr;
+  auto* s = new sce_surrogate();
+  auto fail = [&](int32_t r) {
+    delete s;
+    return r;
+  };
+  s->t = t;
+  s->na = t->dev.na;
+  s->fmax = t->dev.nfeat_max;
+  s->subs.resize(s->na);
+  uint64_t x = seed;
+  auto unif = [&]() {  // splitmix64 -> (0,1)
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * 0x1p-53;
+  };
+  int32_t r;
+  for (int a = 0; a < s->na; ++a) {
+    SurSub& u = s->subs[a];
+    u.F = t->sub_F[a];
+    if (u.F == 0) continue;
+    u.np = sur_np(u.F);
+    std::vector<float> p(u.np, 0.0f);
+    const double a1 = std::sqrt(6.0 / (u.F + SUR_H)), a2 = std::sqrt(6.0 / (2 * SUR_H));
+    size_t o = 0;
+    for (size_t i = 0; i < (size_t)SUR_H * u.F; ++i) p[o++] = (float)(a1 * (2 * unif() - 1));
+    o += SUR_H;
+    for (size_t i = 0; i < (size_t)SUR_H * SUR_H; ++i) p[o++] = (float)(a2 * (2 * unif() - 1));
+    o += SUR_H;
+    for (int i = 0; i < SUR_H; ++i) p[o++] = (float)(0.1 * (2 * unif() - 1));
+    if ((r = dupload(s->mem, &u.params, p.data(), p.size())) != SCE_OK) return fail(r);
+    std::vector<float> z(u.np, 0.0f);
+    if ((r = dupload(s->mem, &u.grads, z.data(), z.size())) != SCE_OK) return fail(r);
+    if ((r = dupload(s->mem, &u.m, z.data(), z.size())) != SCE_OK) return fail(r);
+    if ((r = dupload(s->mem, &u.v2, z.data(), z.size())) != SCE_OK) return fail(r);
+  }
+  if ((r = dalloc(s->mem, &s->d_nets, (size_t)s->na)) != SCE_OK) return fail(r);
+  if ((r = sur_upload_nets(s)) != SCE_OK) return fail(r);
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(sce_fail(SCE_EHIP, "hipStreamCreate failed"));
+  *out = s;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_sur_destroy(sce_surrogate* s) {
+  delete s;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_sur_info(const sce_surrogate* s, int32_t sub, int32_t* F, int64_t* n_params,
+                                double* scale, int64_t* n_rows) {
+  if (!s || sub < 0 || sub >= s->na) return sce_fail(SCE_EINVAL, "bad surrogate or sublattice");
+  if (F) *F = s->subs[sub].F;
+  if (n_params) *n_params = (int64_t)s->subs[sub].np;
+  if (scale) *scale = s->subs[sub].scale;
+  if (n_rows) *n_rows = s->n_rows;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_sur_get_params(sce_surrogate* s, int32_t sub, float* out) {
+  if (!s || !out || sub < 0 || sub >= s->na || !s->subs[sub].params) return sce_fail(SCE_EINVAL, "bad sublattice");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(out, s->subs[sub].params, s->subs[sub].np * 4, hipMemcpyDeviceToHost));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_sur_set_params(sce_surrogate* s, int32_t sub, const float* in, double scale) {
+  if (!s || !in || sub < 0 || sub >= s->na || !s->subs[sub].params) return sce_fail(SCE_EINVAL, "bad sublattice");
+  if (!(scale > 0.0 && std::isfinite(scale))) return sce_fail(SCE_EINVAL, "scale must be finite and > 0");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(s->subs[sub].params, in, s->subs[sub].np * 4, hipMemcpyHostToDevice));
+  s->subs[sub].scale = scale;
+  return sur_upload_nets(s);
+}
+
+static int32_t sur_reserve(sce_surrogate* s, long long rows) {
+  if (rows <= s->cap) return SCE_OK;
+  long long cap = std::max(rows, 2 * s->cap);
+  float *A, *B;
+  double* y;
+  int8_t* sb;
+  CHK(dalloc(s->mem, &A, (size_t)cap * s->fmax));
+  CHK(dalloc(s->mem, &B, (size_t)cap * s->fmax));
+  CHK(dalloc(s->mem, &y, (size_t)cap));
+  CHK(dalloc(s->mem, &sb, (size_t)cap));
+  if (s->n_rows) {
+    HIPCHK(hipMemcpy(A, s->dA, (size_t)s->n_rows * s->fmax * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(B, s->dB, (size_t)s->n_rows * s->fmax * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(y, s->dy, (size_t)s->n_rows * 8, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(sb, s->dsub, (size_t)s->n_rows, hipMemcpyDeviceToDevice));
+  }
+  s->dA = A;
+  s->dB = B;
+  s->dy = y;
+  s->dsub = sb;
+  s->cap = cap;
+  return SCE_OK;
+}
+
+template <int LMAX>
+static void launch_harvest_walk(const DevTables& D, double* config, double* zrows, const double* prop, double* dE,
+                                int* acc, const ChainDev* st, int lo, int n, double* raw, hipStream_t s) {
+  constexpr int G = TPB / 64;
+  const int nb = (n + G - 1) / G;
+  if (nb)
+    k_metro<64, LMAX, false><<<nb, TPB, walk_lds_bytes(D, 64), s>>>(D, config, zrows, prop, dE, acc, st, lo, n,
+                                                                    nullptr, raw);
+}
+
+// Plain Metropolis sweeps of `c` that also record, for every attempted site,
+// (phi(e), phi(e'), exact dE): the surrogate's training tuples.
+extern "C" int32_t sce_sur_harvest(sce_surrogate* s, sce_chain* c, double beta, int64_t nsweeps) {
+  if (!s || !c || c->t != s->t) return sce_fail(SCE_EINVAL, "harvest: chain and surrogate on different tables");
+  CHK(check_lds(c->t->dev, 64));
+  CHK(set_beta(c, beta));
+  const DevTables& D = c->t->dev;
+  CHK(sur_reserve(s, s->n_rows + nsweeps * (long long)D.n_active));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const int PS = D.nlm + 4;
+  for (int64_t i = 0; i < nsweeps; ++i) {
+    if (c->sweeps_done >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
+    const long long row0 = s->n_rows;
+    SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, c->stream);
+    for (int col = 0; col < c->t->n_colors; ++col) {
+      const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
+      if (n == 0) continue;
+      const int nb = (n + 3) / 4;
+      k_features<true><<<nb, TPB, 4 * 3 * (size_t)D.nb_max * 8, c->stream>>>(
+          D, c->config, c->prop, PS, nullptr, nullptr, lo, n, s->dA + (size_t)(row0 + lo) * s->fmax,
+          s->dB + (size_t)(row0 + lo) * s->fmax, s->dsub + row0 + lo);
+      SCE_DISPATCH_LMAX(D.lmax, launch_harvest_walk, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n,
+                        s->dy + row0, c->stream);
+    }
+    launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config, D.active,
+                              D.n_sites, 0, c->st, nullptr, D.n_active, (double)D.n_active, c->stream);
+    c->sweeps_done += 1;
+    s->n_rows += D.n_active;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SCE_OK;
+}
+
+static int32_t sur_scratch(sce_surrogate* s, int bsz) {
+  if (s->bsz == bsz) return SCE_OK;
+  s->scratch.~DevBuf();
+  new (&s->scratch) DevBuf();
+  const size_t rows = 2 * (size_t)bsz;
+  CHK(dalloc(s->scratch, &s->X, rows * s->fmax));
+  CHK(dalloc(s->scratch, &s->Z1, rows * SUR_H));
+  CHK(dalloc(s->scratch, &s->Z2, rows * SUR_H));
+  CHK(dalloc(s->scratch, &s->dg, rows));
+  CHK(dalloc(s->scratch, &s->dZ2, rows * SUR_H));
+  CHK(dalloc(s->scratch, &s->dH1, rows * SUR_H));
+  CHK(dalloc(s->scratch, &s->yb, (size_t)bsz));
+  CHK(dalloc(s->scratch, &s->loss_terms, (size_t)bsz));
+  s->bsz = bsz;
+  return SCE_OK;
+}
+
+// Loss and gradient of one minibatch already gathered into s->X / s->yb.
+static void sur_grad_batch(sce_surrogate* s, int sub, int bsz) {
+  SurSub& u = s->subs[sub];
+  const int F = u.F, fm = s->fmax, rows = 2 * bsz;
+  hipStream_t st = s->stream;
+  float* W1 = u.params;
+  float* b1 = W1 + (size_t)SUR_H * F;
+  float* W2 = b1 + SUR_H;
+  float* b2 = W2 + SUR_H * SUR_H;
+  float* w3 = b2 + SUR_H;
+  float* v = w3 + SUR_H;
+  float* gW1 = u.grads;
+  float* gb1 = gW1 + (size_t)SUR_H * F;
+  float* gW2 = gb1 + SUR_H;
+  float* gb2 = gW2 + SUR_H * SUR_H;
+  float* gw3 = gb2 + SUR_H;
+  float* gv = gw3 + SUR_H;
+  const long long nh = (long long)rows * SUR_H;
+  gemm(false, true, rows, SUR_H, F, s->X, fm, W1, F, s->Z1, SUR_H, 0, st);
+  k_bias_tanh<<<(unsigned)((nh + 255) / 256), 256, 0, st>>>(s->Z1, b1, rows, SUR_H);
+  gemm(false, true, rows, SUR_H, SUR_H, s->Z1, SUR_H, W2, SUR_H, s->Z2, SUR_H, 0, st);
+  k_bias_tanh<<<(unsigned)((nh + 255) / 256), 256, 0, st>>>(s->Z2, b2, rows, SUR_H);
+  k_head<<<(bsz + 255) / 256, 256, 0, st>>>(s->Z2, w3, s->X, v, F, fm, bsz, s->yb, u.scale, s->dg, s->loss_terms);
+  k_gemv_t<<<SUR_H, 256, 0, st>>>(s->Z2, s->dg, rows, SUR_H, SUR_H, gw3);
+  k_gemv_t<<<F, 256, 0, st>>>(s->X, s->dg, rows, F, fm, gv);
+  k_back2<<<(unsigned)((nh + 255) / 256), 256, 0, st>>>(s->Z2, w3, s->dg, rows, s->dZ2);
+  gemm(true, false, SUR_H, SUR_H, rows, s->dZ2, SUR_H, s->Z1, SUR_H, gW2, SUR_H, 0, st);
+  k_colsum<<<SUR_H, 256, 0, st>>>(s->dZ2, rows, SUR_H, SUR_H, gb2);
+  gemm(false, false, rows, SUR_H, SUR_H, s->dZ2, SUR_H, W2, SUR_H, s->dH1, SUR_H, 0, st);
+  k_dtanh<<<(unsigned)((nh + 255) / 256), 256, 0, st>>>(s->dH1, s->Z1, nh);
+  gemm(true, false, SUR_H, F, rows, s->dH1, SUR_H, s->X, fm, gW1, F, 0, st);
+  k_colsum<<<SUR_H, 256, 0, st>>>(s->dH1, rows, SUR_H, SUR_H, gb1);
+}
+
+// Train sublattice `sub`'s network: `epochs` passes of Adam over its harvested
+// rows in shuffled minibatches of `bsz` pairs.  The output scale is fixed to the
+// rms of the rows' exact dE at the first call.  Writes the mean loss (in units
+// of scale^2) of every epoch.
+extern "C" int32_t sce_sur_train(sce_surrogate* s, int32_t sub, int32_t epochs, int32_t bsz, double lr,
+                                 uint64_t seed, double* loss_out) {
+  if (!s || sub < 0 || sub >= s->na || !s->subs[sub].params) return sce_fail(SCE_EINVAL, "bad sublattice");
+  if (bsz < 16 || epochs < 0) return sce_fail(SCE_EINVAL, "batch must be >= 16 and epochs >= 0");
+  SurSub& u = s->subs[sub];
+  HIPCHK(hipStreamSynchronize(s->stream));
+  std::vector<int8_t> hs((size_t)s->n_rows);
+  std::vector<double> hy((size_t)s->n_rows);
+  if (s->n_rows) {
+    HIPCHK(hipMemcpy(hs.data(), s->dsub, hs.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hy.data(), s->dy, hy.size() * 8, hipMemcpyDeviceToHost));
+  }
+  std::vector<long long> rows;
+  double ss = 0.0;
+  for (long long r = 0; r < s->n_rows; ++r)
+    if (hs[r] == sub) {
+      rows.push_back(r);
+      ss += hy[r] * hy[r];
+    }
+  if ((long long)rows.size() < bsz) return sce_fail(SCE_EINVAL, "sublattice %d has %zu rows; batch %d", sub, rows.size(), bsz);
+  if (u.adam_t == 0) {
+    u.scale = std::sqrt(ss / rows.size());
+    if (!(u.scale > 0.0)) u.scale = 1.0;
+    CHK(sur_upload_nets(s));
+  }
+  CHK(sur_scratch(s, bsz));
+  if ((long long)rows.size() > s->idx_cap) {
+    CHK(dalloc(s->mem, &s->idx, rows.size()));
+    s->idx_cap = (long long)rows.size();
+  }
+  uint64_t x = seed;
+  auto next = [&]() {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  const long long nb = (long long)rows.size() / bsz;
+  std::vector<double> lt((size_t)bsz);
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (size_t i = rows.size() - 1; i > 0; --i) std::swap(rows[i], rows[next() % (i + 1)]);
+    HIPCHK(hipMemcpyAsync(s->idx, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, s->stream));
+    double lsum = 0.0;
+    for (long long bi = 0; bi < nb; ++bi) {
+      const long long tot = (long long)bsz * s->fmax;
+      k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, s->stream>>>(s->dA, s->dB, s->dy, s->idx + bi * bsz, bsz,
+                                                                    s->fmax, s->X, s->yb);
+      sur_grad_batch(s, sub, bsz);
+      u.adam_t += 1;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.f - std::pow(b1, (float)u.adam_t), c2 = 1.f - std::pow(b2, (float)u.adam_t);
+      k_adam<<<(unsigned)((u.np + 255) / 256), 256, 0, s->stream>>>(u.params, u.grads, u.m, u.v2, (long long)u.np,
+                                                                    (float)lr, b1, b2, 1e-8f, c1, c2);
+      if (bi == nb - 1 || bi % 64 == 0) {
+        HIPCHK(hipMemcpyAsync(lt.data(), s->loss_terms, lt.size() * 8, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        double a = 0.0;
+        for (double v : lt) a += v;
+        lsum = a / bsz;
+      }
+    }
+    if (loss_out) loss_out[ep] = lsum;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SCE_OK;
+}
+
+// Loss gradient of given rows through the training kernels (parity hook).
+extern "C" int32_t sce_sur_grad(sce_surrogate* s, int32_t sub, int32_t n, const float* A, const float* B,
+                                const double* y, float* grad_out, double* loss) {
+  if (!s || sub < 0 || sub >= s->na || !s->subs[sub].params) return sce_fail(SCE_EINVAL, "bad sublattice");
+  CHK(sur_scratch(s, n));
+  std::vector<float> X((size_t)2 * n * s->fmax);
+  std::memcpy(X.data(), A, (size_t)n * s->fmax * 4);
+  std::memcpy(X.data() + (size_t)n * s->fmax, B, (size_t)n * s->fmax * 4);
+  HIPCHK(hipMemcpy(s->X, X.data(), X.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->yb, y, (size_t)n * 8, hipMemcpyHostToDevice));
+  sur_grad_batch(s, sub, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(grad_out, s->subs[sub].grads, s->subs[sub].np * 4, hipMemcpyDeviceToHost));
+  std::vector<double> lt((size_t)n);
+  HIPCHK(hipMemcpy(lt.data(), s->loss_terms, lt.size() * 8, hipMemcpyDeviceToHost));
+  double a = 0.0;
+  for (double v : lt) a += v;
+  if (loss) *loss = a / n;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_sur_eval(sce_surrogate* s, int32_t sub, int32_t n, const float* A, const float* B,
+                                double* out) {
+  if (!s || sub < 0 || sub >= s->na || !s->subs[sub].params) return sce_fail(SCE_EINVAL, "bad sublattice");
+  Scratch<float> a, b;
+  Scratch<double> o;
+  CHK(a.up(A, (size_t)n * s->fmax));
+  CHK(b.up(B, (size_t)n * s->fmax));
+  CHK(o.alloc(n));
+  const int nb = (n + 3) / 4;
+  if (nb) k_sur_eval<<<nb, TPB, 4 * 2 * (size_t)s->fmax * 4, s->stream>>>(s->d_nets, sub, n, s->fmax, a.p, b.p, o.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return o.down(out, n);
+}
+
+extern "C" int32_t sce_sur_features(sce_tables* t, const double* config, int32_t n, const int32_t* sites,
+                                    const double* e2, float* outA, float* outB) {
+  if (!t) return sce_fail(SCE_EINVAL, "NULL tables");
+  const DevTables& D = t->dev;
+  std::vector<int32_t> s0;
+  CHK(check_sites(t, n, sites, s0));
+  Scratch<double> cf, ee;
+  Scratch<int32_t> si;
+  Scratch<float> a, b;
+  CHK(cf.up(config, 3 * (size_t)D.n_sites));
+  CHK(ee.up(e2, 3 * (size_t)n));
+  CHK(si.up(s0.data(), n));
+  CHK(a.alloc((size_t)n * D.nfeat_max));
+  CHK(b.alloc((size_t)n * D.nfeat_max));
+  const int nb = (n + 3) / 4;
+  if (nb)
+    k_features<false><<<nb, TPB, 4 * 3 * (size_t)std::max(D.nb_max, 1) * 8>>>(D, cf.p, nullptr, 0, si.p, ee.p, 0, n,
+                                                                              a.p, b.p, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  CHK(a.down(outA, (size_t)n * D.nfeat_max));
+  return b.down(outB, (size_t)n * D.nfeat_max);
+}
+
+// ---- the delayed-acceptance sweep ----------------------------------------------
+template <int LMAX>
+static void launch_da(const DevTables& D, sce_chain* c, const SurNet* nets, int lo, int n, hipStream_t s) {
+  constexpr int G = TPB / 64, NLM = (LMAX + 1) * (LMAX + 1);
+  const int ws = D.nb_max * NLM + 1 + NLM;
+  const int gs = da_stride(ws, D.nb_max, D.nfeat_max);
+  const int nb = (n + G - 1) / G;
+  if (nb)
+    k_da<LMAX><<<nb, TPB, (size_t)G * gs * 8, s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc, c->att, c->st,
+                                                   nets, lo, n, ws, gs);
+}
+
+static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas) {
+  const DevTables& D = c->t->dev;
+  const int ws = D.nb_max * D.nlm + 1 + D.nlm;
+  if ((size_t)4 * da_stride(ws, D.nb_max, D.nfeat_max) * 8 > 65536)
+    return sce_fail(SCE_EINVAL, "delayed-acceptance stencil too large for one block");
+  if (c->sweeps_done >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
+  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, c->stream);
+  for (int col = 0; col < c->t->n_colors; ++col) {
+    const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
+    if (n == 0) continue;
+    SCE_DISPATCH_LMAX(D.lmax, launch_da, D, c, s->d_nets, lo, n, c->stream);
+  }
+  launch_reduce<MODE_DA>(c->dE, D.n_active, c->acc, c->att, c->partial, c->ip64, c->config, D.active, D.n_sites,
+                         meas, c->st, c->obs, D.n_active, (double)D.n_active, c->stream);
+  c->sweeps_done += 1;
+  return SCE_OK;
+}
+
+// Delayed-acceptance sweeps (surrogate stage + exact FP64 stage) of chain c.
+// Same renormalisation / measurement semantics as sce_run_sweeps.
+extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta, int64_t nsweeps,
+                                     int64_t renorm_interval, int64_t measure_interval, double* obs,
+                                     int64_t obs_cap, int64_t* n_obs) {
+  CHK(chain_check(c));
+  if (!s || s->t != c->t) return sce_fail(SCE_EINVAL, "surrogate and chain on different tables");
+  if (nsweeps < 0) return sce_fail(SCE_EINVAL, "nsweeps must be >= 0");
+  const int64_t want = measure_interval > 0 ? nsweeps / measure_interval : 0;
+  if (want > 0 && (!obs || obs_cap < want)) return sce_fail(SCE_EDIM, "obs buffer too small");
+  if (c->sweeps_done + nsweeps >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
+  if (want > 0 && (size_t)want > c->obs_cap) {
+    double* p;
+    CHK(dalloc(c->mem, &p, 4 * (size_t)want));
+    c->obs = p;
+    c->obs_cap = (size_t)want;
+    c->drop_graphs();
+  }
+  CHK(set_beta(c, beta));
+  {
+    ChainDev h;
+    CHK(read_state(c, &h));
+    h.n_obs = 0;
+    HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  // capture one DA sweep per measurement flavour
+  hipGraphExec_t ge[2] = {nullptr, nullptr};
+  for (int k = 0; k < 2; ++k) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const long long before = c->sweeps_done;
+    const int32_t r = enqueue_da(c, s, k);
+    c->sweeps_done = before;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != SCE_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      for (auto& x : ge)
+        if (x) (void)hipGraphExecDestroy(x);
+      return r != SCE_OK ? r : sce_fail(SCE_EHIP, "DA capture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&ge[k], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  int32_t rc = SCE_OK;
+  for (int64_t i = 1; i <= nsweeps && rc == SCE_OK; ++i) {
+    const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
+    const bool meas = measure_interval > 0 && i % measure_interval == 0;
+    const bool fused = meas && !renorm;
+    if (hipGraphLaunch(ge[fused ? 1 : 0], c->stream) != hipSuccess) rc = sce_fail(SCE_EHIP, "hipGraphLaunch failed");
+    c->sweeps_done += 1;
+    if (renorm) enqueue_renorm(c);
+    if (meas && !fused) enqueue_measure(c);
+  }
+  for (auto& x : ge)
+    if (x) (void)hipGraphExecDestroy(x);
+  CHK(rc);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (want > 0) HIPCHK(hipMemcpy(obs, c->obs, 4 * (size_t)want * 8, hipMemcpyDeviceToHost));
+  if (n_obs) *n_obs = want;
   return SCE_OK;
 }

@@ -37,7 +37,14 @@ struct HostLayout {
   std::vector<WalkEntry> walk;
   std::vector<int32_t> nb_ptr;    // na + 1
   std::vector<int32_t> nb;        // 4 ints per neighbour: atom (0-based), dc1, dc2, dc3
-  std::vector<int32_t> walk_len_l1; // unused placeholder for future layouts
+  // rotation-invariant surrogate descriptor of a site (per sublattice):
+  // features [e.e_n for the NB stencil neighbours, (e.e_n)^2 for them, then one
+  // per cluster spec]; spec (kind, x, y, z): kind 1 = e.(e_x x e_y) (triplet),
+  // kind 2 = (e.e_x)(e_y.e_z) + (e.e_y)(e_x.e_z) + (e.e_z)(e_x.e_y) (body 4),
+  // kind 3 = spin component x of e (body-1 terms, 3 specs)
+  std::vector<int32_t> feat_ptr;  // na + 1, into feat (units of specs)
+  std::vector<int32_t> feat;      // 4 ints per spec
+  int nfeat_max = 0;              // largest 2*NB + specs
 };
 
 // Derives the layout from the reference tables and verifies, for every active

@@ -60,7 +60,7 @@ class ChainInfo(C.Structure):
     _fields_ = [("energy", C.c_double), ("step", C.c_double), ("max_drift", C.c_double),
                 ("last_drift", C.c_double), ("sweep_index", C.c_int64),
                 ("acc_metro", C.c_int64), ("att_metro", C.c_int64), ("acc_or", C.c_int64),
-                ("att_or", C.c_int64), ("seed", C.c_uint64)]
+                ("att_or", C.c_int64), ("seed", C.c_uint64), ("acc_stage1", C.c_int64)]
 
 
 # (name, length-expression, dtype) of every array field of TablesDesc
@@ -95,6 +95,9 @@ EXPORTS = [
     "sce_batch_create", "sce_batch_destroy", "sce_batch_run", "sce_batch_download",
     "sce_batch_info", "sce_batch_set_step", "sce_batch_reset_counters", "sce_batch_swap_stats",
     "sce_batch_set_graph", "sce_batch_renormalize",
+    "sce_sur_create", "sce_sur_destroy", "sce_sur_info", "sce_sur_get_params",
+    "sce_sur_set_params", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
+    "sce_sur_features", "sce_da_run_sweeps",
 ]
 
 
@@ -137,6 +140,14 @@ def lib():
             "sce_batch_set_step": [P, i32, d], "sce_batch_reset_counters": [P],
             "sce_batch_swap_stats": [P, P, P], "sce_batch_set_graph": [P, i32],
             "sce_batch_renormalize": [P],
+            "sce_sur_create": [P, u64, P], "sce_sur_destroy": [P],
+            "sce_sur_info": [P, i32, P, P, P, P], "sce_sur_get_params": [P, i32, P],
+            "sce_sur_set_params": [P, i32, P, d], "sce_sur_harvest": [P, P, d, i64],
+            "sce_sur_train": [P, i32, i32, i32, d, u64, P],
+            "sce_sur_grad": [P, i32, i32, P, P, P, P, P],
+            "sce_sur_eval": [P, i32, i32, P, P, P],
+            "sce_sur_features": [P, P, i32, P, P, P, P],
+            "sce_da_run_sweeps": [P, P, d, i64, i64, i64, P, i64, P],
         }
         for name, args in sig.items():
             f = getattr(L, name)

@@ -37,9 +37,10 @@ class GPUTiledHamiltonian:
         h = C.c_void_p()
         L.check(L.lib().sce_tables_create(C.byref(H.desc), C.byref(h)))
         self._h = h
-        info = np.zeros(4, np.int64)
+        info = np.zeros(5, np.int64)
         L.check(L.lib().sce_tables_layout_info(h, L.ptr(info)))
-        self.n_sublattices, self.nb_max, self.walk_max, self.walk_total = (int(x) for x in info)
+        (self.n_sublattices, self.nb_max, self.walk_max, self.walk_total,
+         self.fmax) = (int(x) for x in info)
 
     @property
     def handle(self):

@@ -64,7 +64,7 @@ def test_builder_bitwise_equal_to_oracle(case):
 def test_device_layout_derives_and_verifies(case):
     name, na, terms, dims, mm, fl = case
     H = S.TiledHamiltonian(na, terms, dims, magmoms=mm, field=fl)
-    o = np.zeros(4, np.int64)
+    o = np.zeros(5, np.int64)
     L.check(L.lib().sce_layout_check(C.byref(H.desc), L.ptr(o)))
     assert o[0] == H.n_cell_atoms
     # the flattened walk of every active site equals its sublattice's program length
@@ -82,7 +82,7 @@ def test_layout_rejects_non_tiling_tables():
     j = int(np.nonzero(col < 0)[0][5])
     col[j] = -((-col[j]) % H.n_sites + 1)  # point one triplet at a wrong neighbour
     d.site_col = col.ctypes.data_as(L.i32p)
-    o = np.zeros(4, np.int64)
+    o = np.zeros(5, np.int64)
     rc = L.lib().sce_layout_check(C.byref(d), L.ptr(o))
     assert rc == -6 and "neighbour" in L.last_error()
 
@@ -110,7 +110,7 @@ def test_body5_rejected_by_device_layout():
     f = np.zeros((3,) * 5); f[(1,) * 5] = 1.0
     t = M.MultipoleTerm(0.1, [1] * 5, [(k, 0, 0) for k in range(5)], [1] * 5, f)
     H = S.TiledHamiltonian(1, [t], (6, 1, 1))
-    o = np.zeros(4, np.int64)
+    o = np.zeros(5, np.int64)
     rc = L.lib().sce_layout_check(C.byref(H.desc), L.ptr(o))
     assert rc == -1 and "body" in L.last_error()
 
