@@ -60,17 +60,75 @@ def cpu_baseline(na, terms, dims, cfg, kT, budget_s=12.0):
                       f"{T.n_sites}-spin model, {dt:.1f} s"}
 
 
+def _mean_err(x, nbins=20):
+    x = np.asarray(x)
+    b = x[: len(x) // nbins * nbins].reshape(nbins, -1).mean(1)
+    return float(x.mean()), float(b.std(ddof=1) / math.sqrt(nbins))
+
+
+def da_leg(args, S, H, gH, gst, beta, obs_plain, wall_plain):
+    """Learned delayed acceptance against the plain sweep at the same kT: harvest
+    (phi(e), phi(e'), exact dE) tuples from plain sweeps of the equilibrated chain,
+    train the surrogate on the GPU, then time `steps` DA sweeps from the same state
+    with E recorded every sweep."""
+    from sce_amd import surrogate as SG
+
+    sur = SG.Surrogate(gH, seed=424242)
+    n_harvest = max(1, int(math.ceil(args.da_rows / H.n_active)))
+    t0 = time.perf_counter()
+    sur.harvest(gst, beta, n_harvest)
+    t_harvest = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = sur.train_all(epochs=args.da_epochs, batch=args.da_batch, lr=args.da_lr)
+    t_train = time.perf_counter() - t0
+    cfg0, z0, e0 = gst.download()
+    dst = S.GPUChainState(gH, S.ChainState(H, cfg0, gst.step), seed=777)
+    SG.gpu_da_run_sweeps(dst, sur, None, beta, max(args.warmup // 4, 1), renorm_interval=1000)
+    dst.reset_counters()
+    t0 = time.perf_counter()
+    obs = SG.gpu_da_run_sweeps(dst, sur, None, beta, args.steps, renorm_interval=1000, measure_interval=1)
+    wall = time.perf_counter() - t0
+    inf = dst.info()
+    ess = S.effective_samples(obs[:, 0])
+    ess_p = S.effective_samples(obs_plain[:, 0])
+    e_da, de_da = _mean_err(obs[:, 0] / H.n_sites)
+    e_pl, de_pl = _mean_err(obs_plain[:, 0] / H.n_sites)
+    ess_s = ess / wall if math.isfinite(ess) else None
+    ess_s_p = ess_p / wall_plain if math.isfinite(ess_p) else None
+    return {
+        "spin_updates_per_s": H.n_active * args.steps / wall,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "effective_samples_per_s_E": ess_s,
+        "ess_per_s_ratio_vs_plain": (ess_s / ess_s_p) if ess_s and ess_s_p else None,
+        "tau_int_E": (args.steps / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
+        "acceptance": inf.acc_metro / max(inf.att_metro, 1),
+        "stage1_pass": inf.acc_stage1 / max(inf.att_metro, 1),
+        "stage2_acceptance": inf.acc_metro / max(inf.acc_stage1, 1),
+        "E_per_spin": [e_da, de_da], "E_per_spin_plain": [e_pl, de_pl],
+        "E_diff_sigma": abs(e_da - e_pl) / math.hypot(de_da, de_pl) if de_da > 0 else None,
+        "harvest_tuples": n_harvest * H.n_active, "harvest_s": t_harvest,
+        "train_s": t_train, "train_loss_last_epoch": {str(a): float(l[-1]) for a, l in losses.items()},
+        "surrogate": "per-sublattice MLP F->64->64->1 (tanh) + linear path, FP32; "
+                     f"F = {[sur.info(a)['F'] for a in sur.active_sublattices()]}",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=3000)
     ap.add_argument("--workload", default="bcc32")
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--kT", type=float, default=2.0e-3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--persistent", action="store_true",
                     help="one cooperative launch per renormalisation segment")
+    ap.add_argument("--no-da", action="store_true", help="skip the delayed-acceptance leg")
+    ap.add_argument("--da-rows", type=float, default=1e7, help="harvested training tuples")
+    ap.add_argument("--da-epochs", type=int, default=2)
+    ap.add_argument("--da-batch", type=int, default=4096)
+    ap.add_argument("--da-lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box has "
@@ -134,6 +192,8 @@ def main():
                      "kernel": "k_metro (colour walk)", "bytes_per_update": b_site,
                      "walk_launch_ms_avg": walk_ms_avg, "walk_launches": walk_n},
     }
+    if not args.no_da:
+        line["delayed_acceptance"] = da_leg(args, S, H, gH, gst, beta, obs, wall)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(na, terms, dims, cfg, args.kT)
     print(json.dumps(line), flush=True)

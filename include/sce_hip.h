@@ -119,6 +119,11 @@ int32_t sce_chain_destroy(sce_chain* c);
 int32_t sce_chain_info_get(sce_chain* c, sce_chain_info* out);
 int32_t sce_chain_set_step(sce_chain* c, double step);
 int32_t sce_chain_reset_counters(sce_chain* c);
+/* Restores the lane-owned counters of a chain (gpu_state.jl:19-47: sweep_index
+ * keys the next sweep's Philox counter as sweep_index + 1) — with sce_upload,
+ * what resuming a checkpointed chain needs. */
+int32_t sce_chain_set_counters(sce_chain* c, int64_t sweep_index, int64_t acc_metro,
+                               int64_t att_metro, int64_t acc_or, int64_t att_or);
 /* to_host!(st, gst): config (3 x n), zrows (nlm x n), energy; any may be NULL. */
 int32_t sce_download(sce_chain* c, double* config, double* zrows, double* energy);
 /* _from_host!(gst, st) */

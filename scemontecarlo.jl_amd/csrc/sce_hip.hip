@@ -1477,6 +1477,25 @@ extern "C" int32_t sce_chain_reset_counters(sce_chain* c) {
   return SCE_OK;
 }
 
+extern "C" int32_t sce_chain_set_counters(sce_chain* c, int64_t sweep_index, int64_t acc_metro,
+                                          int64_t att_metro, int64_t acc_or, int64_t att_or) {
+  CHK(chain_check(c));
+  if (sweep_index < 0 || sweep_index >= 2147483647LL - 1)
+    return sce_fail(SCE_ERANGE_COUNTER, "sweep_index %lld outside the 32-bit sweep counter", (long long)sweep_index);
+  if (acc_metro < 0 || att_metro < acc_metro || acc_or < 0 || att_or < acc_or)
+    return sce_fail(SCE_EINVAL, "counters must satisfy 0 <= acc <= att");
+  ChainDev h;
+  CHK(read_state(c, &h));
+  h.sweep_index = sweep_index;
+  h.acc_metro = acc_metro;
+  h.att_metro = att_metro;
+  h.acc_or = acc_or;
+  h.att_or = att_or;
+  HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
+  c->sweeps_done = sweep_index;
+  return SCE_OK;
+}
+
 extern "C" int32_t sce_download(sce_chain* c, double* config, double* zrows, double* energy) {
   CHK(chain_check(c));
   const DevTables& D = c->t->dev;

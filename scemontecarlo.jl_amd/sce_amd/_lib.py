@@ -97,7 +97,7 @@ EXPORTS = [
     "sce_batch_set_graph", "sce_batch_renormalize",
     "sce_sur_create", "sce_sur_destroy", "sce_sur_info", "sce_sur_get_params",
     "sce_sur_set_params", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
-    "sce_sur_features", "sce_da_run_sweeps",
+    "sce_sur_features", "sce_da_run_sweeps", "sce_chain_set_counters",
 ]
 
 
@@ -148,6 +148,7 @@ def lib():
             "sce_sur_eval": [P, i32, i32, P, P, P],
             "sce_sur_features": [P, P, i32, P, P, P, P],
             "sce_da_run_sweeps": [P, P, d, i64, i64, i64, P, i64, P],
+            "sce_chain_set_counters": [P, i64, i64, i64, i64, i64],
         }
         for name, args in sig.items():
             f = getattr(L, name)

@@ -107,6 +107,12 @@ class GPUChainState:
     def att_metro(self) -> int:
         return self.info().att_metro
 
+    def set_counters(self, sweep_index: int, acc_metro: int = 0, att_metro: int = 0,
+                     acc_or: int = 0, att_or: int = 0) -> None:
+        """Restore the lane-owned counters (resume; the next sweep is sweep_index + 1)."""
+        L.check(L.lib().sce_chain_set_counters(self._h, int(sweep_index), int(acc_metro),
+                                               int(att_metro), int(acc_or), int(att_or)))
+
     def reset_counters(self) -> None:
         L.check(L.lib().sce_chain_reset_counters(self._h))
 

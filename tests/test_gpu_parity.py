@@ -12,6 +12,7 @@ import pytest
 import oracle as O
 import sce_amd as S
 from sce_amd import gpu as G
+from sce_amd._lib import SceError
 from fixtures import build_pair, fixture_cases, rand_config, bcc_small
 
 pytestmark = pytest.mark.gpu
@@ -241,3 +242,30 @@ def test_persistent_equals_graph_path_and_oracle(case, lanes):
     assert np.array_equal(ca, c2) and np.array_equal(za, z2) and Ea == E
     ia, ib = a.info(), b.info()
     assert (ia.acc_metro, ia.att_metro, ia.sweep_index) == (ib.acc_metro, ib.att_metro, ib.sweep_index)
+
+
+def test_resume_from_downloaded_state_and_counters_is_bitwise():
+    """A chain rebuilt from to_host! output plus its counters continues the same
+    keyed stream (the lane-owned state of gpu_state.jl:19-47)."""
+    na, terms, dims = bcc_small(6)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 3), 0.6)
+    a = S.GPUChainState(gH, st, seed=4242)
+    S.gpu_run_sweeps(a, gH, None, 20.0, 10, renorm_interval=0)
+    cfg, z, E = a.download()
+    inf = a.info()
+    st2 = S.ChainState(H, cfg, 0.6)
+    st2.zrows[...] = z
+    st2.energy = E
+    b = S.GPUChainState(gH, st2, seed=4242)
+    b.set_counters(inf.sweep_index, inf.acc_metro, inf.att_metro)
+    S.gpu_run_sweeps(a, gH, None, 20.0, 7, renorm_interval=0)
+    S.gpu_run_sweeps(b, gH, None, 20.0, 7, renorm_interval=0)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    ia, ib = a.info(), b.info()
+    assert (ia.sweep_index, ia.acc_metro, ia.att_metro) == (ib.sweep_index, ib.acc_metro, ib.att_metro)
+    with pytest.raises(SceError):
+        b.set_counters(2**31, 0, 0)
