@@ -129,7 +129,20 @@ def main():
     ap.add_argument("--da-epochs", type=int, default=2)
     ap.add_argument("--da-batch", type=int, default=4096)
     ap.add_argument("--da-lr", type=float, default=1e-3)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES five times (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box has "
               "one MI355X; replicas shard across GPUs by design only)", file=sys.stderr)

@@ -16,6 +16,7 @@
 //                        device-resident chain state
 // all on the chain's stream with no host synchronisation between sweeps; a
 // sweep is captured once as a hipGraph and replayed.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -175,9 +176,12 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
 
 // ---------------------------------------------------------------------------
 // the colour walk: `L` lanes per site, 256/L sites per block.
-// The walk entries of a lane are loaded UNROLL at a time ahead of their use
-// (independent loads in flight), then consumed in ascending entry order, so
-// the per-lane accumulation order is unchanged.
+// Lane `lane` visits entries i = lane, lane + L, lane + 2L, ... in ascending
+// order (the per-lane accumulation order every path and the oracle share); the
+// entries are loaded WALK_UNROLL at a time ahead of their use.  (Holding the
+// first entries in registers from the start of the site's update shortens its
+// chain of dependent loads but costs occupancy: ~160 VGPRs, 3 waves per SIMD,
+// measured 1.5x slower per sweep at 32^3.)
 // ---------------------------------------------------------------------------
 constexpr int WALK_UNROLL = 8;
 
@@ -219,19 +223,24 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
   double* Z = lds + g * stride;
   double* dz = Z + T.nb_max * NLM + 1;
-  int s = 0, atom = 0;
+  int s = 0, atom = 0, W = 0;
+  const WalkEntry* wl = nullptr;
+  double u_acc = 0.0;
   if (valid) {
     s = TEST ? test_sites[q] : T.color_sites[q];
+    // independent of the staging: issued with it rather than after the barrier
+    const int a0 = s % T.na;
+    const int w0 = T.walk_ptr[a0];
+    W = T.walk_ptr[a0 + 1] - w0;
+    wl = T.walk + w0;
+    if (!TEST) u_acc = prop[(size_t)q * PS + 3];
     stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
     const double* pz = prop + (size_t)q * PS + 4;
     for (int k = lane; k < NLM; k += L) dz[k] = pz[k] - zrows[(size_t)s * NLM + k];
   }
   __syncthreads();
   double a = 0.0;
-  if (valid) {
-    const int w0 = T.walk_ptr[atom];
-    a = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
-  }
+  if (valid) a = walk_delta<L, NLM>(wl, W, lane, Z, dz);
 #pragma unroll
   for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
   __syncthreads();  // the block's LDS is reused by the next site group
@@ -242,7 +251,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   }
   int accept = 0;
   const double* pq = prop + (size_t)q * PS;
-  if (lane == 0) accept = (a <= 0.0 || pq[3] < sce::det_exp(-beta * a)) ? 1 : 0;
+  if (lane == 0) accept = (a <= 0.0 || u_acc < sce::det_exp(-beta * a)) ? 1 : 0;
   accept = __shfl(accept, 0, L);
   if (accept) {
     for (int k = lane; k < NLM; k += L) zrows[(size_t)s * NLM + k] = pq[4 + k];
@@ -1177,25 +1186,33 @@ static void launch_propose_range(const DevTables& D, const double* config, doubl
   if (nb) k_propose<LMAX><<<nb, 128, 0, s>>>(D, config, prop, st, q_lo, q_n);
 }
 
+// With events e0/e1 the launch goes through hipExtLaunchKernelGGL, whose
+// events carry the dispatch's own start and end timestamps (the kernel's
+// execution time, without the queueing that separately recorded events add).
 template <int L, int LMAX, bool TEST>
 static void launch_metro_l(const DevTables& D, double* config, double* zrows, const double* prop,
                            double* dE, int* acc, const ChainDev* st, int q_lo, int q_n,
-                           const int* ts, hipStream_t s) {
+                           const int* ts, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   constexpr int G = TPB / L;
   const int nb = (q_n + G - 1) / G;
-  if (nb) k_metro<L, LMAX, TEST><<<nb, TPB, walk_lds_bytes(D, L), s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts);
+  if (!nb) return;
+  if (e0)
+    hipExtLaunchKernelGGL(k_metro<L, LMAX, TEST>, dim3(nb), dim3(TPB), (uint32_t)walk_lds_bytes(D, L), s, e0, e1,
+                          0, D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, (double*)nullptr);
+  else
+    k_metro<L, LMAX, TEST><<<nb, TPB, walk_lds_bytes(D, L), s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts);
 }
 
 template <int LMAX, bool TEST>
 static void launch_metro(int L, const DevTables& D, double* config, double* zrows,
                          const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo,
-                         int q_n, const int* ts, hipStream_t s) {
-  if (L == 16) launch_metro_l<16, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s);
-  else if (L == 32) launch_metro_l<32, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s);
-  else launch_metro_l<64, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s);
+                         int q_n, const int* ts, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (L == 16) launch_metro_l<16, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
+  else if (L == 32) launch_metro_l<32, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
+  else launch_metro_l<64, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
 }
-template <int LMAX> static void launch_metro_run(int L, const DevTables& D, double* config, double* zrows, const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo, int q_n, hipStream_t s) {
-  launch_metro<LMAX, false>(L, D, config, zrows, prop, dE, acc, st, q_lo, q_n, nullptr, s);
+template <int LMAX> static void launch_metro_run(int L, const DevTables& D, double* config, double* zrows, const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo, int q_n, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  launch_metro<LMAX, false>(L, D, config, zrows, prop, dE, acc, st, q_lo, q_n, nullptr, s, e0, e1);
 }
 template <int LMAX> static void launch_metro_test(int L, const DevTables& D, double* zrows, const double* prop, double* dE, int q_n, const int* ts, hipStream_t s) {
   launch_metro<LMAX, true>(L, D, nullptr, zrows, prop, dE, nullptr, nullptr, 0, q_n, ts, s);
@@ -1296,13 +1313,10 @@ static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, hipStream_t side
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
       e0 = next_event(c);
-      (void)hipEventRecord(e0, s);
-    }
-    SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n, s);
-    if (c->timing) {
       e1 = next_event(c);
-      (void)hipEventRecord(e1, s);
     }
+    SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n, s,
+                      e0, e1);
   }
   launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config,
                             D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s);
