@@ -37,40 +37,58 @@ def algorithmic_bytes_per_update(H) -> float:
             + 0.5 * (8 * nlm + 36))
 
 
-def cpu_baseline(na, terms, dims, cfg, kT, budget_s=12.0):
+def cpu_baseline(na, terms, dims, cfg, kT, budget_s=4.0):
+    """The reference CPU path (oracle restatement of metropolis_sweep!: per-site
+    xoshiro256++ streams, colour classes split over threads with a barrier) on the
+    same model, timed at 1, 4 and up to 16 threads for about `budget_s` each."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     T = O.Tiled(na, terms, dims)
-    nthreads = max(1, min(16, os.cpu_count() or 1))
-    ch = O.CpuChain(T, cfg, 12345, 0.6)
-    ch.metropolis_sweep(1.0 / kT, nthreads)     # warm
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        ch.metropolis_sweep(1.0 / kT, nthreads)
-        n += 1
-        if time.perf_counter() - t0 > budget_s or n >= 200:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": T.n_active * n / dt, "unit": "spin updates/s", "cores": nthreads,
-            "kind": "port",
-            "sample": f"{n} Metropolis sweeps of the reference CPU path (site_coeffs! + "
-                      f"delta_energy, colour classes over {nthreads} threads) on the same "
-                      f"{T.n_sites}-spin model, {dt:.1f} s"}
+    ncpu = os.cpu_count() or 1
+    counts = sorted({1, min(4, ncpu), min(16, ncpu)})
+    by = {}
+    detail = []
+    for nthreads in counts:
+        ch = O.CpuChain(T, cfg, 12345, 0.6)
+        ch.metropolis_sweep(1.0 / kT, nthreads)     # warm
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            ch.metropolis_sweep(1.0 / kT, nthreads)
+            n += 1
+            if time.perf_counter() - t0 > budget_s or n >= 200:
+                break
+        dt = time.perf_counter() - t0
+        by[str(nthreads)] = T.n_active * n / dt
+        detail.append(f"{n} sweeps/{dt:.1f} s at {nthreads} thread(s)")
+    best = max(counts)
+    return {"value": by[str(best)], "unit": "spin updates/s", "cores": best, "kind": "port",
+            "by_threads": by,
+            "sample": f"Metropolis sweeps of the reference CPU path (site_coeffs! + delta_energy, "
+                      f"colour classes over the threads) on the same {T.n_sites}-spin model: "
+                      + "; ".join(detail)}
 
 
-def _mean_err(x, nbins=20):
-    x = np.asarray(x)
-    b = x[: len(x) // nbins * nbins].reshape(nbins, -1).mean(1)
-    return float(x.mean()), float(b.std(ddof=1) / math.sqrt(nbins))
+def measured_traffic(workload):
+    """HBM bytes per walk launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a known-size copy)."""
+    path = os.path.join(ROOT, "profiles", f"k_metro_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if "```json" not in text:
+        return None
+    blob = text.split("```json", 1)[1].split("```", 1)[0]
+    try:
+        return json.loads(blob).get("traffic_bytes_per_launch")
+    except ValueError:
+        return None
 
 
-def da_leg(args, S, H, gH, gst, beta, obs_plain, wall_plain):
-    """Learned delayed acceptance against the plain sweep at the same kT: harvest
-    (phi(e), phi(e'), exact dE) tuples from plain sweeps of the equilibrated chain,
-    train the surrogate on the GPU, then time `steps` DA sweeps from the same state
-    with E recorded every sweep."""
+def train_surrogate(args, S, H, gH, gst, beta):
+    """Harvest (phi(e), phi(e'), exact dE) tuples from plain sweeps of the
+    equilibrated chain and train the surrogate on the GPU."""
     from sce_amd import surrogate as SG
 
     sur = SG.Surrogate(gH, seed=424242)
@@ -81,18 +99,31 @@ def da_leg(args, S, H, gH, gst, beta, obs_plain, wall_plain):
     t0 = time.perf_counter()
     losses = sur.train_all(epochs=args.da_epochs, batch=args.da_batch, lr=args.da_lr)
     t_train = time.perf_counter() - t0
-    cfg0, z0, e0 = gst.download()
-    dst = S.GPUChainState(gH, S.ChainState(H, cfg0, gst.step), seed=777)
-    SG.gpu_da_run_sweeps(dst, sur, None, beta, max(args.warmup // 4, 1), renorm_interval=1000)
-    dst.reset_counters()
+    return sur, {"harvest_tuples": n_harvest * H.n_active, "harvest_s": t_harvest, "train_s": t_train,
+                 "train_loss_last_epoch": {str(a): float(l[-1]) for a, l in losses.items()}}
+
+
+def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, obs_plain, wall_plain):
+    """Delayed acceptance against the plain sweep at the same kT: `steps` DA sweeps
+    from the same equilibrated snapshot the plain timed region started from, with
+    E recorded every sweep."""
+    from sce_amd import surrogate as SG
+
+    dst = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=777)
     t0 = time.perf_counter()
     obs = SG.gpu_da_run_sweeps(dst, sur, None, beta, args.steps, renorm_interval=1000, measure_interval=1)
     wall = time.perf_counter() - t0
     inf = dst.info()
     ess = S.effective_samples(obs[:, 0])
     ess_p = S.effective_samples(obs_plain[:, 0])
-    e_da, de_da = _mean_err(obs[:, 0] / H.n_sites)
-    e_pl, de_pl = _mean_err(obs_plain[:, 0] / H.n_sites)
+    kT = 1.0 / beta
+    sd = S.standard_stats(obs, kT, H.n_active)
+    sp = S.standard_stats(obs_plain, kT, H.n_active)
+    agree = {}
+    for k in ("energy", "specific_heat", "absm", "susceptibility", "binder"):
+        sig = math.hypot(sd[k].err, sp[k].err)
+        agree[k] = {"da": sd[k].mean, "plain": sp[k].mean, "sigma": sig,
+                    "diff_in_sigma": abs(sd[k].mean - sp[k].mean) / sig if sig > 0 else None}
     ess_s = ess / wall if math.isfinite(ess) else None
     ess_s_p = ess_p / wall_plain if math.isfinite(ess_p) else None
     return {
@@ -104,10 +135,8 @@ def da_leg(args, S, H, gH, gst, beta, obs_plain, wall_plain):
         "acceptance": inf.acc_metro / max(inf.att_metro, 1),
         "stage1_pass": inf.acc_stage1 / max(inf.att_metro, 1),
         "stage2_acceptance": inf.acc_metro / max(inf.acc_stage1, 1),
-        "E_per_spin": [e_da, de_da], "E_per_spin_plain": [e_pl, de_pl],
-        "E_diff_sigma": abs(e_da - e_pl) / math.hypot(de_da, de_pl) if de_da > 0 else None,
-        "harvest_tuples": n_harvest * H.n_active, "harvest_s": t_harvest,
-        "train_s": t_train, "train_loss_last_epoch": {str(a): float(l[-1]) for a, l in losses.items()},
+        "observables_vs_plain": agree,
+        **train_info,
         "surrogate": "per-sublattice MLP F->64->64->1 (tanh) + linear path, FP32; "
                      f"F = {[sur.info(a)['F'] for a in sur.active_sublattices()]}",
     }
@@ -117,14 +146,18 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=5000)
     ap.add_argument("--workload", default="bcc32")
     ap.add_argument("--lanes", type=int, default=64)
-    ap.add_argument("--kT", type=float, default=2.0e-3)
+    ap.add_argument("--kT", type=float, default=0.04,
+                    help="model units (eV); at 0.04 the 32^3 chain equilibrates (tau_int(E) ~ 4 "
+                         "sweeps), at <= 0.02 the random-sign couplings freeze into a glass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--persistent", action="store_true",
                     help="one cooperative launch per renormalisation segment")
     ap.add_argument("--no-da", action="store_true", help="skip the delayed-acceptance leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every sweep's kernels directly instead of replaying hipGraphs")
     ap.add_argument("--da-rows", type=float, default=1e7, help="harvested training tuples")
     ap.add_argument("--da-epochs", type=int, default=2)
     ap.add_argument("--da-batch", type=int, default=4096)
@@ -159,14 +192,27 @@ def main():
     gst = S.GPUChainState(gH, st, seed=20261203)
     beta = 1.0 / args.kT
     gst.set_persistent(args.persistent)
-    S.gpu_run_sweeps(gst, gH, None, beta, args.warmup, renorm_interval=1000, lanes=args.lanes)
-    # kernel timing pass: HIP events around every colour-walk launch on the
-    # chain's stream (sweeps launched one by one, not replayed as graphs)
-    gst.set_timing(True)
-    S.gpu_run_sweeps(gst, gH, None, beta, max(args.steps // 10, 20), renorm_interval=1000,
-                     lanes=args.lanes, measure_interval=1)
-    tot_ms_t, walk_ms_avg, walk_n = gst.timing()
-    gst.set_timing(False)
+    gst.set_graph(not args.no_graph)
+    # warm-up with run_mc's step adaptation (src/run.jl:172: every 50 sweeps toward
+    # acceptance 0.5), then the step stays frozen for everything measured
+    from sce_amd.batch import _adapt
+    done = 0
+    while done < args.warmup:
+        n = min(50, args.warmup - done)
+        gst.reset_counters()
+        S.gpu_run_sweeps(gst, gH, None, beta, n, renorm_interval=0, lanes=args.lanes)
+        done += n
+        if n == 50:
+            inf = gst.info()
+            gst.step = _adapt(gst.step, inf.acc_metro, inf.att_metro, 0.5)
+    S.gpu_renormalize(gst)
+    sur = train_info = None
+    if not args.no_da:   # harvesting is plain sweeps: it continues the equilibration
+        sur, train_info = train_surrogate(args, S, H, gH, gst, beta)
+    snapshot = gst.download()[0]
+    # kernel timing: the colour walks of 20 passes launched back to back between
+    # one pair of HIP events on the chain's stream (state restored afterwards)
+    walk_ms_avg, walk_n = gst.time_walk(args.lanes, 20)
     gst.reset_counters()
     # the timed region: `steps` sweeps with E and m recorded on the device every sweep
     t0 = time.perf_counter()
@@ -177,9 +223,8 @@ def main():
     ups = H.n_active * args.steps / wall
     ess = S.effective_samples(obs[:, 0])
     b_site = algorithmic_bytes_per_update(H)
-    n_timed = max(args.steps // 10, 20)
-    walk_s = walk_ms_avg * walk_n / 1e3
-    achieved = b_site * H.n_active * n_timed / walk_s / 1e9 if walk_s > 0 else float("nan")
+    b_launch = b_site * H.n_active / H.n_colors
+    achieved = b_launch / (walk_ms_avg * 1e-3) / 1e9 if walk_ms_avg > 0 else float("nan")
     line = {
         "metric": "spin updates/s + effective samples/s of E at 32^3 (plain device sweep)",
         "value": ups,
@@ -195,18 +240,24 @@ def main():
         "data": "synthetic (seeded random spins, seeded N(0, 1 meV) isotropic SALC couplings)",
         "config": {"workload": f"{args.workload}: bcc {dims[0]}^3 x 2 atoms = {H.n_sites} spins, "
                                "body-2 l<=2 to 3rd shell + l=1 triangles + l=1 tetrahedra",
-                   "kT_eV": args.kT, "lanes_per_site": args.lanes, "colours": H.n_colors},
+                   "kT_eV": args.kT, "lanes_per_site": args.lanes, "colours": H.n_colors,
+                   "step": gst.step},
         "effective_samples_per_s_E": ess / wall if math.isfinite(ess) else None,
         "tau_int_E": (args.steps / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
         "acceptance": inf.acc_metro / max(inf.att_metro, 1),
-        "launch_mode": "persistent" if args.persistent else "hipGraph per sweep",
+        "launch_mode": ("persistent" if args.persistent else
+                        "direct launches" if args.no_graph else "hipGraph per sweep"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_metro (colour walk)", "bytes_per_update": b_site,
-                     "walk_launch_ms_avg": walk_ms_avg, "walk_launches": walk_n},
+                     "bytes_per_launch": b_launch,
+                     "walk_launch_ms_avg": walk_ms_avg, "walk_launches": walk_n,
+                     "duration_source": "HIP events around 20 back-to-back passes of the colour "
+                                        "walks on the chain's stream, just before the timed region"},
     }
     if not args.no_da:
-        line["delayed_acceptance"] = da_leg(args, S, H, gH, gst, beta, obs, wall)
+        line["delayed_acceptance"] = da_leg(args, S, H, gH, sur, train_info, snapshot, gst.step, beta,
+                                            obs, wall)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(na, terms, dims, cfg, args.kT)
     print(json.dumps(line), flush=True)

@@ -150,9 +150,12 @@ int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps,
                        int64_t measure_interval, int32_t lanes, double* obs,
                        int64_t obs_cap, int64_t* n_obs);
 /* Run the Metropolis-only sweeps of sce_run_sweeps as one persistent
- * cooperative launch per renormalisation segment (on != 0, the default) or as
- * per-colour launches (0).  Both give bit-identical results. */
+ * cooperative launch per renormalisation segment (on != 0) or as per-colour
+ * launches (0, the default).  Both give bit-identical results. */
 int32_t sce_chain_set_persistent(sce_chain* c, int32_t on);
+/* Replay each sweep of sce_run_sweeps as a captured hipGraph (on != 0, the
+ * default) or launch its kernels directly; results are identical. */
+int32_t sce_chain_set_graph(sce_chain* c, int32_t on);
 /* _renormalize!(st, H) on the device: normalise active spins, rebuild rows,
  * recompute the FP64 total energy, re-anchor.  Writes the drift. */
 int32_t sce_renormalize(sce_chain* c, double* drift);
@@ -165,6 +168,13 @@ int32_t sce_chain_set_timing(sce_chain* c, int32_t on);
  * the average duration of its Metropolis walk launches (both from HIP events). */
 int32_t sce_chain_timing(sce_chain* c, double* total_ms, double* walk_ms_avg,
                          int64_t* walk_launches);
+
+/* Average duration of the colour-walk kernel over `reps` back-to-back passes of
+ * every colour class between one pair of HIP events (a diagnostic with no
+ * reference counterpart; the chain's config, rows, energy and counters are left
+ * as they were). */
+int32_t sce_chain_time_walk(sce_chain* c, int32_t lanes, int32_t reps, double* ms_avg,
+                            int64_t* launches);
 
 /* ---- batches: many chains in one launch per colour --------------------------
  * n_chains chains, each on its own tables (a replica ladder shares one tables

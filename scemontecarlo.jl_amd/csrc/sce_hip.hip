@@ -1660,6 +1660,12 @@ extern "C" int32_t sce_chain_set_persistent(sce_chain* c, int32_t on) {
   return SCE_OK;
 }
 
+extern "C" int32_t sce_chain_set_graph(sce_chain* c, int32_t on) {
+  CHK(chain_check(c));
+  c->use_graph = on != 0;
+  return SCE_OK;
+}
+
 extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, int32_t or_per_metropolis,
                                   int64_t renorm_interval, int64_t measure_interval, int32_t lanes,
                                   double* obs, int64_t obs_cap, int64_t* n_obs) {
@@ -1819,6 +1825,50 @@ struct Scratch {
     return SCE_OK;
   }
 };
+
+// Average duration of the colour-walk kernel: `reps` passes over every colour
+// class launched back to back on the chain's stream between one pair of HIP
+// events (the GPU runs them without gaps, so elapsed / launches is the kernel's
+// own duration).  The walks re-use the sweep's proposal buffer; config and rows
+// are saved before and restored after, and no counter or energy changes.
+extern "C" int32_t sce_chain_time_walk(sce_chain* c, int32_t lanes, int32_t reps, double* ms_avg,
+                                       int64_t* launches) {
+  CHK(chain_check(c));
+  CHK(check_lanes(lanes));
+  CHK(check_lds(c->t->dev, lanes));
+  if (reps < 1) return sce_fail(SCE_EINVAL, "reps must be >= 1");
+  const DevTables& D = c->t->dev;
+  const size_t ncfg = 3 * (size_t)D.n_sites, nz = (size_t)D.nlm * D.n_sites;
+  Scratch<double> cfg, z;
+  CHK(cfg.alloc(ncfg));
+  CHK(z.alloc(nz));
+  HIPCHK(hipMemcpyAsync(cfg.p, c->config, ncfg * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(z.p, c->zrows, nz * 8, hipMemcpyDeviceToDevice, c->stream));
+  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, c->stream);
+  hipEvent_t e0 = next_event(c), e1 = next_event(c);
+  c->ev_used = 0;
+  int64_t n = 0;
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r)
+    for (int col = 0; col < c->t->n_colors; ++col) {
+      const int lo = c->t->color_ptr[col], cnt = c->t->color_ptr[col + 1] - lo;
+      if (cnt == 0) continue;
+      SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, lanes, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st,
+                        lo, cnt, c->stream);
+      ++n;
+    }
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipMemcpyAsync(c->config, cfg.p, ncfg * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->zrows, z.p, nz * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if (ms_avg) *ms_avg = n ? ms / (double)n : 0.0;
+  if (launches) *launches = n;
+  return SCE_OK;
+}
+
 
 static int32_t need_device() {
   int n = 0;

@@ -97,7 +97,8 @@ EXPORTS = [
     "sce_batch_set_graph", "sce_batch_renormalize",
     "sce_sur_create", "sce_sur_destroy", "sce_sur_info", "sce_sur_get_params",
     "sce_sur_set_params", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
-    "sce_sur_features", "sce_da_run_sweeps", "sce_chain_set_counters",
+    "sce_sur_features", "sce_da_run_sweeps", "sce_chain_set_counters", "sce_chain_time_walk",
+    "sce_chain_set_graph",
 ]
 
 
@@ -149,6 +150,7 @@ def lib():
             "sce_sur_features": [P, P, i32, P, P, P, P],
             "sce_da_run_sweeps": [P, P, d, i64, i64, i64, P, i64, P],
             "sce_chain_set_counters": [P, i64, i64, i64, i64, i64],
+            "sce_chain_time_walk": [P, i32, i32, P, P], "sce_chain_set_graph": [P, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)

@@ -107,6 +107,13 @@ class GPUChainState:
     def att_metro(self) -> int:
         return self.info().att_metro
 
+    def time_walk(self, lanes: int = DEFAULT_LANES, reps: int = 20):
+        """(average colour-walk kernel duration in ms, launches timed); the chain
+        state is left unchanged."""
+        ms, n = C.c_double(), C.c_int64()
+        L.check(L.lib().sce_chain_time_walk(self._h, int(lanes), int(reps), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def set_counters(self, sweep_index: int, acc_metro: int = 0, att_metro: int = 0,
                      acc_or: int = 0, att_or: int = 0) -> None:
         """Restore the lane-owned counters (resume; the next sweep is sweep_index + 1)."""
@@ -127,6 +134,10 @@ class GPUChainState:
     def set_persistent(self, on: bool) -> None:
         """Persistent cooperative launches (default) or per-colour launches."""
         L.check(L.lib().sce_chain_set_persistent(self._h, 1 if on else 0))
+
+    def set_graph(self, on: bool) -> None:
+        """Replay sweeps as hipGraphs (default) or launch their kernels directly."""
+        L.check(L.lib().sce_chain_set_graph(self._h, 1 if on else 0))
 
     def set_timing(self, on: bool) -> None:
         L.check(L.lib().sce_chain_set_timing(self._h, 1 if on else 0))

@@ -269,3 +269,38 @@ def test_resume_from_downloaded_state_and_counters_is_bitwise():
     assert (ia.sweep_index, ia.acc_metro, ia.att_metro) == (ib.sweep_index, ib.acc_metro, ib.att_metro)
     with pytest.raises(SceError):
         b.set_counters(2**31, 0, 0)
+
+
+def test_walk_timing_leaves_the_chain_unchanged():
+    na, terms, dims = bcc_small(7)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 5), 0.6)
+    a = S.GPUChainState(gH, st, seed=99)
+    b = S.GPUChainState(gH, st, seed=99)
+    S.gpu_run_sweeps(a, gH, None, 20.0, 3, renorm_interval=0)
+    S.gpu_run_sweeps(b, gH, None, 20.0, 3, renorm_interval=0)
+    ms, n = a.time_walk(64, 3)
+    assert ms > 0 and n == 3 * sum(1 for k in range(H.n_colors) if H.color_ptr[k + 1] > H.color_ptr[k])
+    S.gpu_run_sweeps(a, gH, None, 20.0, 4, renorm_interval=0)
+    S.gpu_run_sweeps(b, gH, None, 20.0, 4, renorm_interval=0)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    assert (a.info().acc_metro, a.info().sweep_index) == (b.info().acc_metro, b.info().sweep_index)
+
+
+def test_direct_launches_equal_graph_replay():
+    na, terms, dims = bcc_small(8)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 6), 0.6)
+    a = S.GPUChainState(gH, st, seed=5)
+    b = S.GPUChainState(gH, st, seed=5)
+    b.set_graph(False)
+    oa = S.gpu_run_sweeps(a, gH, None, 20.0, 9, renorm_interval=4, measure_interval=2)
+    ob = S.gpu_run_sweeps(b, gH, None, 20.0, 9, renorm_interval=4, measure_interval=2)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    assert np.array_equal(oa, ob)
