@@ -142,6 +142,97 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, obs_plain, wal
     }
 
 
+def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
+    """Batched parallel tempering: 64 replicas of fcc 12^3 (6912 spins, body-2 l<=2 to
+    the 4th shell) on a geometric kT ladder spanning 0.2-2.0 J1, replica exchange
+    on the device after every sweep, one launch per colour for all replicas.
+    J1 = 3 x rms raw coefficient of the first-shell (1,1) bonds (the bond energy
+    of a (1,1) template is 4 pi N1^2 c = 3c times a unit-norm tensor form)."""
+    na, terms, dims = M.config_model("fcc12", seed=0)
+    first = [t for t in terms if len(t.atoms) == 2 and list(t.ls) == [1, 1]
+             and np.linalg.norm(np.asarray(M.BASIS["fcc"][t.atoms[1] - 1]) + np.asarray(t.shifts[1])
+                                - np.asarray(M.BASIS["fcc"][t.atoms[0] - 1]) - np.asarray(t.shifts[0]))
+             < M.shell_distances("fcc", 1)[0] + 1e-9]
+    J1 = 3.0 * math.sqrt(np.mean([t.coef ** 2 for t in first]))
+    R = 64
+    kts = J1 * np.geomspace(0.2, 2.0, R)
+    H = S.TiledHamiltonian(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    from sce_amd.batch import GPUBatch, derive_seeds, _adapt
+    init, exch, dev = derive_seeds(2026, R)
+    states = [S.ChainState(H, S.random_config(H.n_sites, np.random.default_rng(init[r])), 0.6) for r in range(R)]
+    b = GPUBatch([gH] * R, states, dev, kts, exch)
+    steps = [0.6] * R
+    done = 0
+    while done < sweeps_therm:            # untimed: exchange + step adaptation (gpu_pt.jl:115-140)
+        n = min(50, sweeps_therm - done)
+        b.run(n, 64, 1000, 0, 1, done, sweeps_therm)
+        done += n
+        for k in range(R):
+            inf = b.info(k)
+            steps[k] = _adapt(steps[k], inf.acc_metro, inf.att_metro, 0.5)
+            b.set_step(k, steps[k])
+        b.reset_counters()
+    b.renormalize()
+    t0 = time.perf_counter()
+    obs = b.run(sweeps_measure, 64, 1000, 1, 1, 0, sweeps_measure)
+    wall = time.perf_counter() - t0
+    att, acc = b.swap_stats()
+    sw = np.where(att > 0, acc / np.maximum(att, 1), np.nan)
+    return {"workload": f"64 replicas x fcc 12^3 = {R * H.n_sites} spins, ladder {kts[0]:.2e}-{kts[-1]:.2e} eV "
+                        f"(J1 = {J1:.3e} eV), exchange every sweep on the device",
+            "spin_updates_per_s": R * H.n_active * sweeps_measure / wall,
+            "ms_per_sweep_all_replicas": 1e3 * wall / sweeps_measure, "sweeps": sweeps_measure,
+            "lanes": 64, "measurements": int(obs.shape[1]),
+            "swap_acceptance_mean": float(np.nanmean(sw)), "swap_acceptance_min": float(np.nanmin(sw)),
+            "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep"}
+
+
+def uneven_leg(S, M, sweeps=100, n_chains=256):
+    """Uneven batch: 256 chains, sc or bcc with N^3, N uniform in 6..20, each with
+    its own seeded body-2 l<=2 (+ body-3 l=1 triangles on bcc) model, kT and a
+    Zeeman term (2 mu_B moments, field 0-10 T along a random axis).  Chains run in
+    three size classes (N <= 10, <= 15, <= 20), one batch per class, with lanes
+    per class picked by the autotuner."""
+    from sce_amd.batch import GPUBatch, autotune_lanes
+
+    rng = np.random.default_rng(20261203)
+    classes = {0: [], 1: [], 2: []}
+    total_sites = 0
+    for k in range(n_chains):
+        lat = "sc" if rng.random() < 0.5 else "bcc"
+        N = int(rng.integers(6, 21))
+        na, terms = M.lattice_model(lat, pair_shells=2, pair_lmax=2, triangles=(lat == "bcc"), seed=1000 + k)
+        B = rng.uniform(0.0, 10.0) * (lambda v: v / np.linalg.norm(v))(rng.standard_normal(3))
+        H = S.TiledHamiltonian(na, terms, (N, N, N), magmoms=[2.0] * na, field=tuple(B))
+        st = S.ChainState(H, S.random_config(H.n_sites, rng), 0.6)
+        classes[0 if N <= 10 else 1 if N <= 15 else 2].append((H, st, int(rng.integers(1, 2**62)),
+                                                                float(rng.uniform(0.005, 0.05))))
+        total_sites += H.n_active
+    out = {}
+    t_all = 0.0
+    for cls, chains in classes.items():
+        gHs = [S.GPUTiledHamiltonian(H) for H, _, _, _ in chains]
+
+        def make():
+            return GPUBatch(gHs, [st for _, st, _, _ in chains], [sd for _, _, sd, _ in chains],
+                            [kt for _, _, _, kt in chains])
+        lanes = autotune_lanes(make, sweeps=10)
+        b = make()
+        b.run(5, lanes, 0)
+        t0 = time.perf_counter()
+        b.run(sweeps, lanes, 1000)
+        dt = time.perf_counter() - t0
+        t_all += dt
+        n_sites = sum(H.n_active for H, _, _, _ in chains)
+        out[f"class_{cls}"] = {"chains": len(chains), "spins": n_sites, "lanes": lanes,
+                               "ms_per_sweep": 1e3 * dt / sweeps, "spin_updates_per_s": n_sites * sweeps / dt}
+        del b
+    return {"workload": f"{n_chains} chains, sc/bcc N^3 with N in 6..20, {total_sites} spins, per-chain model, "
+                        "kT and Zeeman field", "spin_updates_per_s": total_sites * sweeps / t_all,
+            "sweeps": sweeps, "classes": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +247,8 @@ def main():
     ap.add_argument("--persistent", action="store_true",
                     help="one cooperative launch per renormalisation segment")
     ap.add_argument("--no-da", action="store_true", help="skip the delayed-acceptance leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched legs (64-replica PT, 256-chain uneven batch)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every sweep's kernels directly instead of replaying hipGraphs")
     ap.add_argument("--da-rows", type=float, default=1e7, help="harvested training tuples")
@@ -258,6 +351,8 @@ def main():
     if not args.no_da:
         line["delayed_acceptance"] = da_leg(args, S, H, gH, sur, train_info, snapshot, gst.step, beta,
                                             obs, wall)
+    if not args.no_batch:
+        line["batched"] = {"parallel_tempering": pt_leg(S, M), "uneven_batch": uneven_leg(S, M)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(na, terms, dims, cfg, args.kT)
     print(json.dumps(line), flush=True)

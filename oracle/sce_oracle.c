@@ -1226,7 +1226,14 @@ static float ora_tanhf(float z) {
   return (float)(x < 0.0 ? -r : r);
 }
 
-/* params: [W1 (64 x F), b1 (64), W2 (64 x 64), b2 (64), w3 (64), v (F)] */
+/* params: [W1 (64 x F), b1 (64), W2 (64 x 64), b2 (64), w3 (64), v (F)].
+ * Hidden pre-activations are fused multiply-add chains in ascending input
+ * index starting from the bias, over the input zero-padded to a multiple of 4
+ * (F4): z = fma(x[F4-1], w[F4-1], ... fma(x[0], w[0], b)).  This is what the
+ * device's matrix-core tiles (v_mfma_f32_16x16x4f32: four fused multiply-adds
+ * in k order per instruction, instructions in ascending k) compute.  The head
+ * w3 . h2 + v . phi keeps separate multiplies and adds, folded over 64 lanes
+ * (lane j: w3[j] h2[j] + sum over f = j, j+64, ... of v[f] phi[f]). */
 double ora_sur_delta(int F, const float* params, double scale, const float* phiA, const float* phiB) {
   const float* W1 = params;
   const float* b1 = W1 + SUR_H * F;
@@ -1234,13 +1241,19 @@ double ora_sur_delta(int F, const float* params, double scale, const float* phiA
   const float* b2 = W2 + SUR_H * SUR_H;
   const float* w3 = b2 + SUR_H;
   const float* v = w3 + SUR_H;
+  const int F4 = (F + 3) & ~3;
+  float xA[520], xB[520];
+  for (int f = 0; f < F4; ++f) {
+    xA[f] = f < F ? phiA[f] : 0.0f;
+    xB[f] = f < F ? phiB[f] : 0.0f;
+  }
   float h1A[SUR_H], h1B[SUR_H], tA[SUR_H], tB[SUR_H];
   for (int j = 0; j < SUR_H; ++j) {
     float zA = b1[j], zB = b1[j];
-    for (int f = 0; f < F; ++f) {
-      float w = W1[j * F + f];
-      zA = zA + w * phiA[f];
-      zB = zB + w * phiB[f];
+    for (int f = 0; f < F4; ++f) {
+      const float w = f < F ? W1[j * F + f] : 0.0f;
+      zA = fmaf(xA[f], w, zA);
+      zB = fmaf(xB[f], w, zB);
     }
     h1A[j] = ora_tanhf(zA);
     h1B[j] = ora_tanhf(zB);
@@ -1248,9 +1261,9 @@ double ora_sur_delta(int F, const float* params, double scale, const float* phiA
   for (int j = 0; j < SUR_H; ++j) {
     float zA = b2[j], zB = b2[j];
     for (int k = 0; k < SUR_H; ++k) {
-      float w = W2[j * SUR_H + k];
-      zA = zA + w * h1A[k];
-      zB = zB + w * h1B[k];
+      const float w = W2[j * SUR_H + k];
+      zA = fmaf(h1A[k], w, zA);
+      zB = fmaf(h1B[k], w, zB);
     }
     float oA = w3[j] * ora_tanhf(zA), oB = w3[j] * ora_tanhf(zB);
     float sA = 0.0f, sB = 0.0f;

@@ -2547,6 +2547,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct SurNet {
   const float *W1, *b1, *W2, *b2, *w3, *v;  // W1: H x F row-major, W2: H x H row-major
+  const float *W1T, *W2T;                    // inference copies: W1T[f][j], W2T[k][j] (coalesced over j)
   int F;
   double scale;
 };
@@ -2612,21 +2613,27 @@ __device__ __forceinline__ void stage_spins(const DevTables& T, const double* __
   *NB_out = NB;
 }
 
-// The surrogate difference g(phiB) - g(phiA) of one site, one wavefront (64
-// lanes, lane j = hidden unit j); phiA/phiB/h1 live in the group's LDS.
-// Block barriers inside: every thread of the block calls it.
-__device__ double sur_delta(const SurNet& net, const float* phiA, const float* phiB, float* h1A,
-                            float* h1B, bool valid) {
+// The surrogate difference of one site on one wavefront (lane j = hidden unit
+// j).  phiA/phiB: the site's descriptor rows in LDS, zero padded to F4 = F
+// rounded up to 4; W1T/W2T: transposed weights (rows f >= F zero), so the loads
+// of one step are coalesced over j.  Each pre-activation is the fused
+// multiply-add chain z = fma(x[K-1], w[K-1], ... fma(x[0], w[0], b)) in
+// ascending input index over the padded K — the same value a matrix-core tile
+// (v_mfma_f32_16x16x4f32: four fused multiply-adds in k order per instruction)
+// produces, and the order the oracle restates.  The head (w3 . h2 + v . phi)
+// keeps separate multiplies and adds and folds over the 64 lanes.
+// Contains a block barrier: every thread of the block calls it.
+__device__ double sur_delta(const SurNet& net, const float* phiA, const float* phiB, float* h1A, float* h1B,
+                            bool valid) {
   const int j = threadIdx.x % 64;
-  float oA = 0.0f, oB = 0.0f, sA = 0.0f, sB = 0.0f;
+  float tA = 0.0f, tB = 0.0f;
   if (valid) {
-    const int F = net.F;
+    const int F4 = (net.F + 3) & ~3;
     float zA = net.b1[j], zB = net.b1[j];
-    const float* w = net.W1 + (size_t)j * F;
-    for (int f = 0; f < F; ++f) {
-      const float wf = w[f];
-      zA = zA + wf * phiA[f];
-      zB = zB + wf * phiB[f];
+    for (int f = 0; f < F4; ++f) {
+      const float w = net.W1T[(size_t)f * SUR_H + j];
+      zA = __builtin_fmaf(phiA[f], w, zA);
+      zB = __builtin_fmaf(phiB[f], w, zB);
     }
     h1A[j] = det_tanhf(zA);
     h1B[j] = det_tanhf(zB);
@@ -2634,51 +2641,57 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
   __syncthreads();
   if (valid) {
     float zA = net.b2[j], zB = net.b2[j];
-    const float* w = net.W2 + (size_t)j * SUR_H;
     for (int k = 0; k < SUR_H; ++k) {
-      const float wk = w[k];
-      zA = zA + wk * h1A[k];
-      zB = zB + wk * h1B[k];
+      const float w = net.W2T[k * SUR_H + j];
+      zA = __builtin_fmaf(h1A[k], w, zA);
+      zB = __builtin_fmaf(h1B[k], w, zB);
     }
-    oA = net.w3[j] * det_tanhf(zA);
-    oB = net.w3[j] * det_tanhf(zB);
+    const float oA = net.w3[j] * det_tanhf(zA);
+    const float oB = net.w3[j] * det_tanhf(zB);
+    float sA = 0.0f, sB = 0.0f;
     for (int f = j; f < net.F; f += 64) {
       sA = sA + net.v[f] * phiA[f];
       sB = sB + net.v[f] * phiB[f];
     }
+    tA = oA + sA;
+    tB = oB + sB;
   }
-  float tA = oA + sA, tB = oB + sB;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     tA = tA + __shfl_down(tA, off, 64);
     tB = tB + __shfl_down(tB, off, 64);
   }
-  __syncthreads();
   return ((double)tB - (double)tA) * net.scale;  // valid in lane 0
 }
 
-// LDS of one DA group (doubles): walk block (stride) | spins (3 NBmax) |
-// phiA, phiB (F floats each) | h1A, h1B (64 floats each)
-__host__ __device__ inline int da_stride(int walk_stride, int nb_max, int fmax) {
-  return walk_stride + 3 * nb_max + (2 * fmax + 2 * SUR_H + 1) / 2 + 1;
+// LDS of one DA group (doubles): walk block (walk_stride) | spins (3 nb_max) |
+// phiA, phiB (FP floats each) | h1A, h1B (64 floats each).
+__host__ __device__ inline int da_group_stride(int walk_stride, int nb_max, int FP) {
+  return walk_stride + 3 * nb_max + (2 * FP + 2 * SUR_H + 1) / 2 + 1;
+}
+__host__ __device__ inline size_t da_lds_bytes(int walk_stride, int nb_max, int FP) {
+  return (size_t)(TPB / 64) * da_group_stride(walk_stride, nb_max, FP) * 8;
 }
 
-// Two-stage delayed-acceptance update of one site (64 lanes, one wavefront).
+// Two-stage delayed-acceptance update: one site per wavefront (64 lanes).
 template <int LMAX>
-__device__ void da_group(const DevTables& T, double* __restrict__ config, double* __restrict__ zrows,
-                         const double* __restrict__ prop, double* __restrict__ dE,
-                         int* __restrict__ acc, int* __restrict__ acc1, const ChainDev* st,
-                         const SurNet* __restrict__ nets, int q, bool valid, double* lds,
-                         int walk_stride, int gstride) {
-  constexpr int L = 64, NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+__global__ __launch_bounds__(TPB) void k_da(DevTables T, double* config, double* zrows, const double* prop,
+                                            double* dE, int* acc, int* acc1, const ChainDev* st,
+                                            const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP) {
+  constexpr int L = 64, G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  extern __shared__ double lds[];
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
+  const bool valid = qi < q_n;
+  const int q = q_lo + qi;
+  const int gstride = da_group_stride(walk_stride, T.nb_max, FP);
   double* base = lds + g * gstride;
   double* Z = base;
   double* dz = Z + T.nb_max * NLM + 1;
   double* E = base + walk_stride;
-  float* phiA = (float*)(E + 3 * T.nb_max);
-  float* phiB = phiA + T.nfeat_max;
-  float* h1A = phiB + T.nfeat_max;
+  float* phiA = reinterpret_cast<float*>(E + 3 * T.nb_max);
+  float* phiB = phiA + FP;
+  float* h1A = phiB + FP;
   float* h1B = h1A + SUR_H;
   int s = 0, atom = 0, NB = 0;
   const double* pq = prop + (size_t)q * PS;
@@ -2699,9 +2712,10 @@ __device__ void da_group(const DevTables& T, double* __restrict__ config, double
   __syncthreads();
   if (valid) {
     const int4* spec = T.feat + T.feat_ptr[atom];
-    for (int f = lane; f < net.F; f += L) {
-      phiA[f] = sur_feature(f, NB, spec, E, e);
-      phiB[f] = sur_feature(f, NB, spec, E, e2);
+    const int F4 = (net.F + 3) & ~3;
+    for (int f = lane; f < F4; f += L) {
+      phiA[f] = f < net.F ? sur_feature(f, NB, spec, E, e) : 0.0f;
+      phiB[f] = f < net.F ? sur_feature(f, NB, spec, E, e2) : 0.0f;
     }
   }
   __syncthreads();
@@ -2722,7 +2736,6 @@ __device__ void da_group(const DevTables& T, double* __restrict__ config, double
   }
 #pragma unroll
   for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
-  __syncthreads();
   if (!valid) return;
   int accept = 0;
   if (lane == 0 && walk) {
@@ -2744,18 +2757,6 @@ __device__ void da_group(const DevTables& T, double* __restrict__ config, double
     acc[q] = accept;
     acc1[q] = pass;
   }
-}
-
-template <int LMAX>
-__global__ __launch_bounds__(TPB) void k_da(DevTables T, double* config, double* zrows, const double* prop,
-                                            double* dE, int* acc, int* acc1, const ChainDev* st,
-                                            const SurNet* nets, int q_lo, int q_n, int walk_stride,
-                                            int gstride) {
-  constexpr int G = TPB / 64;
-  extern __shared__ double lds[];
-  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / 64;
-  da_group<LMAX>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo + qi, qi < q_n, lds, walk_stride,
-                 gstride);
 }
 
 // Descriptor rows of a colour slice (for harvesting and tests): one wavefront
@@ -2795,24 +2796,23 @@ __global__ __launch_bounds__(TPB) void k_features(DevTables T, const double* __r
 }
 
 // Surrogate dE of given descriptor rows through the inference path (tests).
-__global__ __launch_bounds__(TPB) void k_sur_eval(const SurNet* nets, int sub, int n, int fmax,
+__global__ __launch_bounds__(TPB) void k_sur_eval(const SurNet* nets, int subl, int n, int fmax, int FP,
                                                   const float* A, const float* B, double* out) {
-  constexpr int G = TPB / 64;
-  __shared__ float buf[G][2 * SUR_H];
   extern __shared__ double lds[];
   const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int i = blockIdx.x * G + g;
+  const int i = blockIdx.x * (TPB / 64) + g;
   const bool valid = i < n;
-  float* phiA = (float*)lds + (size_t)g * 2 * fmax;
-  float* phiB = phiA + fmax;
-  if (valid)
-    for (int f = lane; f < fmax; f += 64) {
-      phiA[f] = A[(size_t)i * fmax + f];
-      phiB[f] = B[(size_t)i * fmax + f];
-    }
+  float* phiA = reinterpret_cast<float*>(lds) + (size_t)g * (2 * FP + 2 * SUR_H);
+  float* phiB = phiA + FP;
+  float* h1A = phiB + FP;
+  float* h1B = h1A + SUR_H;
+  for (int f = lane; f < FP; f += 64) {
+    phiA[f] = valid && f < fmax ? A[(size_t)i * fmax + f] : 0.0f;
+    phiB[f] = valid && f < fmax ? B[(size_t)i * fmax + f] : 0.0f;
+  }
   __syncthreads();
-  const SurNet net = nets[sub];
-  const double d = sur_delta(net, phiA, phiB, buf[g], buf[g] + SUR_H, valid);
+  const SurNet net = nets[subl];
+  const double d = sur_delta(net, phiA, phiB, h1A, h1B, valid);
   if (valid && lane == 0) out[i] = d;
 }
 
@@ -2951,11 +2951,27 @@ __global__ void k_gather(const float* dA, const float* dB, const double* dy, con
   if (f == 0) y[r] = dy[row];
 }
 
+// Inference copies of the weight matrices: W1T[f][j] = W1[j][f] for f < F and
+// 0 for F <= f < F4 (F rounded up to 4), then W2T[k][j] = W2[j][k].
+__global__ void k_sur_transpose(const float* __restrict__ W1, const float* __restrict__ W2, int F, int F4,
+                                float* __restrict__ W1T, float* __restrict__ W2T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < SUR_H * F4) {
+    const int f = i / SUR_H, j = i % SUR_H;
+    W1T[i] = f < F ? W1[(size_t)j * F + f] : 0.0f;
+  }
+  if (i < SUR_H * SUR_H) {
+    const int k = i / SUR_H, j = i % SUR_H;
+    W2T[i] = W2[j * SUR_H + k];
+  }
+}
+
 // ---- surrogate host side ------------------------------------------------------
 struct SurSub {
   int F = 0;
   size_t np = 0;
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v2 = nullptr;
+  float* wT = nullptr;  // W1T (H x F) then W2T (H x H)
   double scale = 1.0;
   long long adam_t = 0;
 };
@@ -3000,8 +3016,23 @@ static int32_t sur_upload_nets(sce_surrogate* s) {
     n.b2 = n.W2 + SUR_H * SUR_H;
     n.w3 = n.b2 + SUR_H;
     n.v = n.w3 + SUR_H;
+    n.W1T = u.wT;
+    n.W2T = u.wT + (size_t)SUR_H * ((u.F + 3) & ~3);
   }
   HIPCHK(hipMemcpy(s->d_nets, h.data(), h.size() * sizeof(SurNet), hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+// Refreshes the inference copies of sublattice `a` after its parameters changed.
+static int32_t sur_refresh(sce_surrogate* s, int a) {
+  SurSub& u = s->subs[a];
+  if (!u.params) return SCE_OK;
+  const int F4 = (u.F + 3) & ~3;
+  const int n = std::max(SUR_H * F4, SUR_H * SUR_H);
+  k_sur_transpose<<<(n + 255) / 256, 256, 0, s->stream>>>(u.params, u.params + (size_t)SUR_H * u.F + SUR_H, u.F,
+                                                        F4, u.wT, u.wT + (size_t)SUR_H * F4);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
   return SCE_OK;
 }
 
@@ -3044,11 +3075,14 @@ extern "C" int32_t sce_sur_create(sce_tables* t, uint64_t seed, sce_surrogate** 
     if ((r = dupload(s->mem, &u.grads, z.data(), z.size())) != SCE_OK) return fail(r);
     if ((r = dupload(s->mem, &u.m, z.data(), z.size())) != SCE_OK) return fail(r);
     if ((r = dupload(s->mem, &u.v2, z.data(), z.size())) != SCE_OK) return fail(r);
+    if ((r = dalloc(s->mem, &u.wT, (size_t)SUR_H * ((u.F + 3) & ~3) + SUR_H * SUR_H)) != SCE_OK) return fail(r);
   }
   if ((r = dalloc(s->mem, &s->d_nets, (size_t)s->na)) != SCE_OK) return fail(r);
   if ((r = sur_upload_nets(s)) != SCE_OK) return fail(r);
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(sce_fail(SCE_EHIP, "hipStreamCreate failed"));
+  for (int a = 0; a < s->na; ++a)
+    if ((r = sur_refresh(s, a)) != SCE_OK) return fail(r);
   *out = s;
   return SCE_OK;
 }
@@ -3081,6 +3115,7 @@ extern "C" int32_t sce_sur_set_params(sce_surrogate* s, int32_t sub, const float
   HIPCHK(hipStreamSynchronize(s->stream));
   HIPCHK(hipMemcpy(s->subs[sub].params, in, s->subs[sub].np * 4, hipMemcpyHostToDevice));
   s->subs[sub].scale = scale;
+  CHK(sur_refresh(s, sub));
   return sur_upload_nets(s);
 }
 
@@ -3272,7 +3307,7 @@ extern "C" int32_t sce_sur_train(sce_surrogate* s, int32_t sub, int32_t epochs, 
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
-  return SCE_OK;
+  return sur_refresh(s, sub);
 }
 
 // Loss gradient of given rows through the training kernels (parity hook).
@@ -3306,7 +3341,10 @@ extern "C" int32_t sce_sur_eval(sce_surrogate* s, int32_t sub, int32_t n, const 
   CHK(b.up(B, (size_t)n * s->fmax));
   CHK(o.alloc(n));
   const int nb = (n + 3) / 4;
-  if (nb) k_sur_eval<<<nb, TPB, 4 * 2 * (size_t)s->fmax * 4, s->stream>>>(s->d_nets, sub, n, s->fmax, a.p, b.p, o.p);
+  const int FP = (s->fmax + 3) & ~3;
+  if (nb)
+    k_sur_eval<<<nb, TPB, (size_t)(TPB / 64) * (2 * FP + 2 * SUR_H) * 4, s->stream>>>(s->d_nets, sub, n, s->fmax,
+                                                                                      FP, a.p, b.p, o.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return o.down(out, n);
@@ -3341,17 +3379,17 @@ template <int LMAX>
 static void launch_da(const DevTables& D, sce_chain* c, const SurNet* nets, int lo, int n, hipStream_t s) {
   constexpr int G = TPB / 64, NLM = (LMAX + 1) * (LMAX + 1);
   const int ws = D.nb_max * NLM + 1 + NLM;
-  const int gs = da_stride(ws, D.nb_max, D.nfeat_max);
+  const int FP = (D.nfeat_max + 3) & ~3;
   const int nb = (n + G - 1) / G;
   if (nb)
-    k_da<LMAX><<<nb, TPB, (size_t)G * gs * 8, s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc, c->att, c->st,
-                                                   nets, lo, n, ws, gs);
+    k_da<LMAX><<<nb, TPB, da_lds_bytes(ws, D.nb_max, FP), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc,
+                                                               c->att, c->st, nets, lo, n, ws, FP);
 }
 
 static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas) {
   const DevTables& D = c->t->dev;
   const int ws = D.nb_max * D.nlm + 1 + D.nlm;
-  if ((size_t)4 * da_stride(ws, D.nb_max, D.nfeat_max) * 8 > 65536)
+  if (da_lds_bytes(ws, D.nb_max, (D.nfeat_max + 3) & ~3) > 65536)
     return sce_fail(SCE_EINVAL, "delayed-acceptance stencil too large for one block");
   if (c->sweeps_done >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
   SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, c->stream);
