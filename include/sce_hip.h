@@ -283,6 +283,9 @@ int32_t sce_test_walk_coeffs(sce_tables* t, const double* zrows, int64_t n,
                              const int32_t* sites, int32_t lanes, double* out);
 /* Device deterministic tree sum of x (the sweep/energy reduction). */
 int32_t sce_test_reduce(int64_t n, const double* x, double* out);
+/* One v_mfma_f32_16x16x4f32 per test t: out[t] = c[t] + sum_k a[4t+k] b[4t+k] as
+ * the matrix core computes it (the accumulation order the oracle restates). */
+int32_t sce_test_mfma(int64_t n, const float* a4, const float* b4, const float* c, float* out);
 
 #ifdef __cplusplus
 }

@@ -2816,6 +2816,20 @@ __global__ __launch_bounds__(TPB) void k_sur_eval(const SurNet* nets, int subl, 
   if (valid && lane == 0) out[i] = d;
 }
 
+// One v_mfma_f32_16x16x4f32 per test (diagnostic): A[0][k] = a[4t + k],
+// B[k][0] = b[4t + k], C[0][0] = c[t], all else 0; writes D[0][0].
+__global__ void k_test_mfma(const float* a, const float* b, const float* c, float* out, int n) {
+  const int t = blockIdx.x;
+  if (t >= n) return;
+  const int l = threadIdx.x, i = l % 16, k = l / 16;
+  const float av = i == 0 ? a[4 * t + k] : 0.0f;
+  const float bv = i == 0 ? b[4 * t + k] : 0.0f;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  if (l == 0) acc[0] = c[t];
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+  if (l == 0) out[t] = acc[0];
+}
+
 // ---- training: MFMA FP32 GEMMs ---------------------------------------------
 // C (M x N, ldc) = op(A) (M x K) . op(B) (K x N) [+ C when accumulate]; each
 // 64-thread wavefront computes one 16 x 16 tile with v_mfma_f32_16x16x4f32.
@@ -3467,4 +3481,18 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
   if (want > 0) HIPCHK(hipMemcpy(obs, c->obs, 4 * (size_t)want * 8, hipMemcpyDeviceToHost));
   if (n_obs) *n_obs = want;
   return SCE_OK;
+}
+
+extern "C" int32_t sce_test_mfma(int64_t n, const float* a4, const float* b4, const float* c, float* out) {
+  CHK(need_device());
+  if (n < 0 || n > (1 << 20)) return sce_fail(SCE_EINVAL, "n out of range");
+  Scratch<float> a, b, cc, o;
+  CHK(a.up(a4, 4 * (size_t)n));
+  CHK(b.up(b4, 4 * (size_t)n));
+  CHK(cc.up(c, (size_t)n));
+  CHK(o.alloc((size_t)n));
+  if (n) k_test_mfma<<<(unsigned)n, 64>>>(a.p, b.p, cc.p, o.p, (int)n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return o.down(out, (size_t)n);
 }

@@ -280,3 +280,13 @@ def dev_reduce(x):
     o = C.c_double()
     L.check(L.lib().sce_test_reduce(len(x), L.ptr(x), C.byref(o)))
     return o.value
+
+
+def dev_mfma(a4, b4, c):
+    """One matrix-core instruction per row: c + sum_k a[k] b[k] (diagnostic)."""
+    a = np.ascontiguousarray(a4, np.float32).reshape(-1, 4)
+    b = np.ascontiguousarray(b4, np.float32).reshape(-1, 4)
+    cc = np.ascontiguousarray(c, np.float32)
+    out = np.zeros(max(len(cc), 1), np.float32)
+    L.check(L.lib().sce_test_mfma(len(cc), L.ptr(a), L.ptr(b), L.ptr(cc), L.ptr(out)))
+    return out[: len(cc)]

@@ -304,3 +304,26 @@ def test_direct_launches_equal_graph_replay():
     cb, zb, Eb = b.download()
     assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
     assert np.array_equal(oa, ob)
+
+
+def test_checkpoint_resume_is_bitwise(tmp_path):
+    from sce_amd import checkpoint as CK
+    na, terms, dims = bcc_small(9)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 2), 0.7)
+    a = S.GPUChainState(gH, st, seed=2718)
+    S.gpu_run_sweeps(a, gH, None, 20.0, 11, renorm_interval=5)
+    path = str(tmp_path / "chain.npz")
+    CK.save_chain(path, a)
+    b = CK.load_chain(path, gH)
+    S.gpu_run_sweeps(a, gH, None, 20.0, 9, renorm_interval=5)
+    S.gpu_run_sweeps(b, gH, None, 20.0, 9, renorm_interval=5)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    ia, ib = a.info(), b.info()
+    assert (ia.sweep_index, ia.acc_metro, ia.att_metro, ia.step) == (ib.sweep_index, ib.acc_metro, ib.att_metro, ib.step)
+    other = S.GPUTiledHamiltonian(S.TiledHamiltonian(*bcc_small(10)))
+    with pytest.raises(ValueError, match="different model"):
+        CK.load_chain(path, other)

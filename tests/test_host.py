@@ -183,3 +183,19 @@ def test_free_spin_closed_forms_from_iid_measurements():
     chi, U = st["susceptibility"], st["binder"]
     assert abs(chi.mean - (1 - 8 / (3 * math.pi)) / kT) < 5 * chi.err
     assert abs(U.mean - 5 / 3) < 5 * U.err
+
+
+def test_model_fingerprint_detects_changes():
+    from sce_amd import checkpoint as CK
+    na, terms = M.lattice_model("bcc", pair_shells=2, pair_lmax=1, seed=3)
+    H1 = S.TiledHamiltonian(na, terms, (3, 3, 3))
+    H2 = S.TiledHamiltonian(na, terms, (3, 3, 3))
+    assert CK.model_fingerprint(H1) == CK.model_fingerprint(H2)
+    flipped = list(terms)
+    t0 = flipped[0]
+    flipped[0] = M.MultipoleTerm(-t0.coef, t0.atoms, t0.shifts, t0.ls, t0.folded)
+    assert CK.model_fingerprint(S.TiledHamiltonian(na, flipped, (3, 3, 3))) != CK.model_fingerprint(H1)
+    assert CK.model_fingerprint(S.TiledHamiltonian(na, terms, (3, 3, 4))) != CK.model_fingerprint(H1)
+    Hf = S.TiledHamiltonian(na, terms, (3, 3, 3), magmoms=[2.0, 2.0], field=(0, 0, 1.0))
+    Hg = S.TiledHamiltonian(na, terms, (3, 3, 3), magmoms=[2.0, 2.0], field=(0, 0, -1.0))
+    assert CK.model_fingerprint(Hf) != CK.model_fingerprint(Hg)
