@@ -2675,12 +2675,14 @@ __host__ __device__ inline size_t da_lds_bytes(int walk_stride, int nb_max, int 
 
 // Two-stage delayed-acceptance update: one site per wavefront (64 lanes).
 template <int LMAX>
-__global__ __launch_bounds__(TPB) void k_da(DevTables T, double* config, double* zrows, const double* prop,
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da(DevTables T, double* config, double* zrows, const double* prop,
                                             double* dE, int* acc, int* acc1, const ChainDev* st,
                                             const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP) {
   constexpr int L = 64, G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   extern __shared__ double lds[];
-  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  // one site per wavefront: its index, site and sublattice are wave-uniform
+  // (scalar registers, scalar loads)
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / L), lane = threadIdx.x % L;
   const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
   const bool valid = qi < q_n;
   const int q = q_lo + qi;
@@ -2698,7 +2700,7 @@ __global__ __launch_bounds__(TPB) void k_da(DevTables T, double* config, double*
   double e[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
   SurNet net{};
   if (valid) {
-    s = T.color_sites[q];
+    s = __builtin_amdgcn_readfirstlane(T.color_sites[q]);
     atom = s % T.na;
     net = nets[atom];
     e[0] = config[3 * (size_t)s];
