@@ -233,6 +233,24 @@ def uneven_leg(S, M, sweeps=100, n_chains=256):
             "sweeps": sweeps, "classes": out}
 
 
+def device_smoke(S, H, gH):
+    """On-device gates before timing (bench/bench_gpu.jl:112-132): two chains with one
+    seed must agree bitwise after 20 sweeps, and the incremental energy must stay
+    within 1e-8 max(1, |E|) of a full recompute.  Aborts the run otherwise."""
+    cfg = S.random_config(H.n_sites, np.random.default_rng(99))
+    a = S.GPUChainState(gH, S.ChainState(H, cfg, 0.6), seed=5)
+    b = S.GPUChainState(gH, S.ChainState(H, cfg, 0.6), seed=5)
+    S.gpu_run_sweeps(a, gH, None, 1 / 0.04, 20, renorm_interval=0)
+    S.gpu_run_sweeps(b, gH, None, 1 / 0.04, 20, renorm_interval=0)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    E_full = a.total_energy()
+    if not (np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb):
+        sys.exit("device smoke: repeated runs differ")
+    if abs(Ea - E_full) > 1e-8 * max(1.0, abs(E_full)):
+        sys.exit(f"device smoke: drift {abs(Ea - E_full):.3e} above 1e-8 max(1, |E|)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +298,7 @@ def main():
     na, terms, dims = M.config_model(args.workload, seed=0)
     H = S.TiledHamiltonian(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
+    device_smoke(S, H, gH)
     cfg = S.random_config(H.n_sites, np.random.default_rng(0))
     st = S.ChainState(H, cfg, 0.6)
     gst = S.GPUChainState(gH, st, seed=20261203)

@@ -43,6 +43,46 @@ def test_philox_counter_layout_and_stream_separation():
     assert tuple(O.philox_block(1, 3, 5, 0)) == ref
 
 
+def test_proposal_reads_its_named_slots_value_for_value():
+    """The proposal restated from the slot constants outside the proposal function
+    (test_gpu.jl:219-251): slot 0 = (flip uniform, accept uniform), slot 1 = (n1, n2),
+    slot 2 = (n3, angle n4); flip fraction 0.2; e' = cos t e + sin t (a x e) +
+    (1 - cos t)(a.e) a with a = normalize(n1, n2, n3), t = step n4.  Collapsing two
+    slots inside the function moves device and oracle together and only this pin
+    sees it."""
+    seed, step = 0xBEEF, 0.6
+    e = np.array([0.0, 0.0, 1.0])
+    nflip = 0
+    for site in range(1, 25):
+        for sw in range(1, 5):
+            b0 = O.philox_block(seed, site, sw, 0)
+            u_flip, u_acc = O.uniform(int(b0[0]), int(b0[1])), O.uniform(int(b0[2]), int(b0[3]))
+            e2, ua = O.keyed_proposal(seed, site, sw, e, step)
+            assert ua == u_acc
+            if u_flip < 0.2:
+                nflip += 1
+                assert np.array_equal(e2, -e)
+            else:
+                n1, n2 = O.normal2(O.philox_block(seed, site, sw, 1))
+                n3, n4 = O.normal2(O.philox_block(seed, site, sw, 2))
+                ax = np.array([n1, n2, n3]) / math.sqrt(n1 * n1 + n2 * n2 + n3 * n3)
+                t = step * n4
+                want = math.cos(t) * e + math.sin(t) * np.cross(ax, e) + (1 - math.cos(t)) * np.dot(ax, e) * ax
+                np.testing.assert_allclose(e2, want, atol=1e-14)
+    assert 0 < nflip < 96
+
+
+def test_reflection_is_an_involution():
+    rng = np.random.default_rng(2)
+    for _ in range(5):
+        e, a = rng.standard_normal(3), rng.standard_normal(3)
+        e /= np.linalg.norm(e)
+        a /= np.linalg.norm(a)
+        S = lambda x: 2 * np.dot(x, a) * a - x
+        np.testing.assert_allclose(S(S(e)), e, atol=1e-15)
+        assert abs(np.linalg.norm(S(e)) - 1.0) < 1e-15
+
+
 def test_uniform_open_interval_exact_endpoints():
     assert O.uniform(0, 0) == 2.0 ** -53
     assert O.uniform(0xffffffff, 0xffffffff) == 1.0 - 2.0 ** -53
