@@ -37,6 +37,20 @@ def algorithmic_bytes_per_update(H) -> float:
             + 0.5 * (8 * nlm + 36))
 
 
+def fp64_flops_per_update(H) -> float:
+    """FP64 flops of the walk of one site update: 3 per pair program entry, 4 per
+    triplet entry, 6 per general (body-4) entry, as counted on the reference
+    layout (product, weight, fused multiply-add)."""
+    s = 0 if H.n_active == 0 else int(H.color_sites[0])
+    flops = 0
+    for j in range(int(H.site_ptr[s - 1]), int(H.site_ptr[s])):
+        pid = int(H.site_prog[j - 1])
+        n = int(H.sprog_ptr[pid]) - int(H.sprog_ptr[pid - 1])
+        col = int(H.site_col[j - 1])
+        flops += n * (3 if col > 0 else 4 if col < 0 else 6)
+    return float(flops)
+
+
 def cpu_baseline(na, terms, dims, cfg, kT, budget_s=4.0):
     """The reference CPU path (oracle restatement of metropolis_sweep!: per-site
     xoshiro256++ streams, colour classes split over threads with a barrier) on the
@@ -336,6 +350,8 @@ def main():
     ess = S.effective_samples(obs[:, 0])
     b_site = algorithmic_bytes_per_update(H)
     b_launch = b_site * H.n_active / H.n_colors
+    f_site = fp64_flops_per_update(H)
+    fp64_tflops = f_site * H.n_active / H.n_colors / (walk_ms_avg * 1e-3) / 1e12 if walk_ms_avg > 0 else float("nan")
     achieved = b_launch / (walk_ms_avg * 1e-3) / 1e9 if walk_ms_avg > 0 else float("nan")
     line = {
         "metric": "spin updates/s + effective samples/s of E at 32^3 (plain device sweep)",
@@ -365,7 +381,9 @@ def main():
                      "bytes_per_launch": b_launch,
                      "walk_launch_ms_avg": walk_ms_avg, "walk_launches": walk_n,
                      "duration_source": "HIP events around 20 back-to-back passes of the colour "
-                                        "walks on the chain's stream, just before the timed region"},
+                                        "walks on the chain's stream, just before the timed region",
+                     "fp64": {"flops_per_update": f_site, "achieved": fp64_tflops, "peak": 78.6,
+                              "unit": "TFLOP/s", "frac": fp64_tflops / 78.6}},
     }
     if not args.no_da:
         line["delayed_acceptance"] = da_leg(args, S, H, gH, sur, train_info, snapshot, gst.step, beta,
