@@ -918,10 +918,25 @@ __global__ void k_test_proposal(long long n, uint64_t seed, const int32_t* sites
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Owner of device allocations: everything dalloc'd into it is freed with it;
+// release() frees one allocation early (a buffer replaced by a larger one).
 struct DevBuf {
   std::vector<void*> ptrs;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
     for (void* p : ptrs) (void)hipFree(p);
+  }
+  void release(void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < ptrs.size(); ++i)
+      if (ptrs[i] == p) {
+        (void)hipFree(p);
+        ptrs[i] = ptrs.back();
+        ptrs.pop_back();
+        return;
+      }
   }
 };
 
@@ -994,6 +1009,8 @@ struct sce_chain {
     g_lanes = 0;
   }
   ~sce_chain() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
     drop_graphs();
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_col) (void)hipEventDestroy(e);
@@ -1681,9 +1698,10 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   if (want_obs > 0 && (size_t)want_obs > c->obs_cap) {
     double* p;
     CHK(dalloc(c->mem, &p, 4 * (size_t)want_obs));
+    c->drop_graphs();
+    c->mem.release(c->obs);
     c->obs = p;
     c->obs_cap = (size_t)want_obs;
-    c->drop_graphs();
   }
   CHK(set_beta(c, beta));
   // reset the observation counter
@@ -2409,12 +2427,13 @@ extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, i
   if (want > 0 && (size_t)want > b->obs_cap) {
     double* p;
     CHK(dalloc(b->mem, &p, 4 * (size_t)want * (size_t)b->R));
+    b->drop_graphs();
+    b->mem.release(b->obs);
     b->obs = p;
     b->obs_cap = (size_t)want;
     CHK(bsync_descriptors(b));
     for (int k = 0; k < b->R; ++k) b->h_ch[k].obs = p + 4 * (size_t)want * (size_t)k;
     HIPCHK(hipMemcpy(b->d_ch, b->h_ch.data(), b->h_ch.size() * sizeof(BChain), hipMemcpyHostToDevice));
-    b->drop_graphs();
   }
   k_breset_obs<<<(b->R + 255) / 256, 256, 0, b->stream>>>(b->d_ch, b->R);
   for (int64_t i = 1; i <= nsweeps; ++i) {
@@ -3151,6 +3170,10 @@ static int32_t sur_reserve(sce_surrogate* s, long long rows) {
     HIPCHK(hipMemcpy(y, s->dy, (size_t)s->n_rows * 8, hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(sb, s->dsub, (size_t)s->n_rows, hipMemcpyDeviceToDevice));
   }
+  s->mem.release(s->dA);
+  s->mem.release(s->dB);
+  s->mem.release(s->dy);
+  s->mem.release(s->dsub);
   s->dA = A;
   s->dB = B;
   s->dy = y;
@@ -3285,6 +3308,7 @@ extern "C" int32_t sce_sur_train(sce_surrogate* s, int32_t sub, int32_t epochs, 
   }
   CHK(sur_scratch(s, bsz));
   if ((long long)rows.size() > s->idx_cap) {
+    s->mem.release(s->idx);
     CHK(dalloc(s->mem, &s->idx, rows.size()));
     s->idx_cap = (long long)rows.size();
   }
@@ -3434,9 +3458,10 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
   if (want > 0 && (size_t)want > c->obs_cap) {
     double* p;
     CHK(dalloc(c->mem, &p, 4 * (size_t)want));
+    c->drop_graphs();
+    c->mem.release(c->obs);
     c->obs = p;
     c->obs_cap = (size_t)want;
-    c->drop_graphs();
   }
   CHK(set_beta(c, beta));
   {

@@ -123,3 +123,32 @@ def test_checkpoint_does_not_perturb_the_run(tmp_path):
     ca, za, Ea = a.download()
     cb, zb, Eb = b.download()
     assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+
+
+def test_buffers_regrow_across_calls():
+    """Observation buffers and the harvested dataset grow across calls (old device
+    buffers are released) and keep working: measurements, a second harvest,
+    training on the merged rows, DA sweeps."""
+    from sce_amd import surrogate as SG
+    na, terms, dims = bcc_tet_small(7, dims=(2, 2, 2), isotropic=True)
+    H = S.TiledHamiltonian(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    g = S.GPUChainState(gH, S.ChainState(H, rand_config(H.n_sites, 3), 0.6), seed=8)
+    for n in (5, 20, 60):
+        obs = S.gpu_run_sweeps(g, gH, None, 1 / 0.05, n, renorm_interval=0, measure_interval=1)
+        assert obs.shape == (n, 4)
+        assert obs[-1, 0] == g.download()[2]
+    sur = SG.Surrogate(gH, seed=2)
+    r1 = sur.harvest(g, 1 / 0.05, 30)
+    r2 = sur.harvest(g, 1 / 0.05, 90)
+    assert r2 == r1 + 90 * H.n_active
+    losses = sur.train_all(epochs=2, batch=64, lr=1e-3)
+    assert all(np.isfinite(l).all() for l in losses.values())
+    for n in (4, 40):
+        obs = SG.gpu_da_run_sweeps(g, sur, None, 1 / 0.05, n, renorm_interval=0, measure_interval=1)
+        assert obs.shape == (n, 4)
+    R = 3
+    b = GPUBatch([gH] * R, [S.ChainState(H, rand_config(H.n_sites, 10 + k), 0.6) for k in range(R)],
+                 [1, 2, 3], [0.05, 0.06, 0.07])
+    for n in (3, 30):
+        assert b.run(n, 64, 0, 1).shape == (R, n, 4)
