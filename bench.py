@@ -124,6 +124,7 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, obs_plain, wal
     from sce_amd import surrogate as SG
 
     dst = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=777)
+    dst.set_graph(not args.no_graph)
     t0 = time.perf_counter()
     obs = SG.gpu_da_run_sweeps(dst, sur, None, beta, args.steps, renorm_interval=1000, measure_interval=1)
     wall = time.perf_counter() - t0

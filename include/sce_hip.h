@@ -257,8 +257,9 @@ int32_t sce_sur_eval(sce_surrogate* s, int32_t sub, int32_t n, const float* A, c
 int32_t sce_sur_features(sce_tables* t, const double* config, int32_t n, const int32_t* sites,
                          const double* e2, float* outA, float* outB);
 /* nsweeps delayed-acceptance sweeps of chain c with the same renormalisation
- * and measurement schedule as sce_run_sweeps; acc_stage1 in sce_chain_info
- * counts the proposals that passed the surrogate stage. */
+ * and measurement schedule as sce_run_sweeps (hipGraph replay unless
+ * sce_chain_set_graph(c, 0)); acc_stage1 in sce_chain_info counts the proposals
+ * that passed the surrogate stage. */
 int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta, int64_t nsweeps,
                           int64_t renorm_interval, int64_t measure_interval, double* obs,
                           int64_t obs_cap, int64_t* n_obs);

@@ -2880,8 +2880,62 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
   }
 }
 
+// Split-K form for long reductions (the weight gradients, K = 2 x batch rows):
+// wave w computes tile w % tiles over the K-chunk w / tiles into its own
+// partial P[chunk][M][N]; k_sum_parts adds the chunks in ascending order.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm_part(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                   const float* __restrict__ B, int ldb, float* __restrict__ P,
+                                                   int kchunk, int nchunks) {
+  const int lane = threadIdx.x % 64;
+  const int wave = blockIdx.x * 4 + threadIdx.x / 64;
+  const int tiles_n = (N + 15) / 16, tiles = ((M + 15) / 16) * tiles_n;
+  if (wave >= tiles * nchunks) return;
+  const int chunk = wave / tiles, t = wave % tiles;
+  const int m0 = (t / tiles_n) * 16, n0 = (t % tiles_n) * 16;
+  const int kb = chunk * kchunk, ke = min(K, kb + kchunk);
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int ra = m0 + lane % 16, cb = n0 + lane % 16, kk = lane / 16;
+  for (int k0 = kb; k0 < ke; k0 += 4) {
+    const int k = k0 + kk;
+    float a = 0.f, b = 0.f;
+    if (ra < M && k < ke) a = TA ? A[(size_t)k * lda + ra] : A[(size_t)ra * lda + k];
+    if (cb < N && k < ke) b = TB ? B[(size_t)cb * ldb + k] : B[(size_t)k * ldb + cb];
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+  float* out = P + (size_t)chunk * M * N;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = m0 + 4 * (lane / 16) + i, c = n0 + lane % 16;
+    if (r < M && c < N) out[(size_t)r * N + c] = acc[i];
+  }
+}
+
+__global__ void k_sum_parts(const float* __restrict__ P, int nchunks, int M, int N, float* __restrict__ C, int ldc,
+                            int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * N) return;
+  float a = 0.0f;
+  for (int k = 0; k < nchunks; ++k) a += P[(size_t)k * M * N + i];
+  const int r = i / N, c = i % N;
+  C[(size_t)r * ldc + c] = accumulate ? C[(size_t)r * ldc + c] + a : a;
+}
+
+constexpr int GEMM_KCHUNK = 512;
+
 static void gemm(bool ta, bool tb, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
-                 float* C, int ldc, int accumulate, hipStream_t s) {
+                 float* C, int ldc, int accumulate, hipStream_t s, float* part = nullptr) {
+  if (part && K > 2 * GEMM_KCHUNK) {
+    const int nchunks = (K + GEMM_KCHUNK - 1) / GEMM_KCHUNK;
+    const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
+    const int nb = (tiles * nchunks + 3) / 4;
+    if (ta && !tb) k_gemm_part<true, false><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, part, GEMM_KCHUNK, nchunks);
+    else if (!ta && !tb) k_gemm_part<false, false><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, part, GEMM_KCHUNK, nchunks);
+    else if (!ta && tb) k_gemm_part<false, true><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, part, GEMM_KCHUNK, nchunks);
+    else k_gemm_part<true, true><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, part, GEMM_KCHUNK, nchunks);
+    k_sum_parts<<<(M * N + 255) / 256, 256, 0, s>>>(part, nchunks, M, N, C, ldc, accumulate);
+    return;
+  }
   const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
   const int nb = (tiles + 3) / 4;
   if (!ta && !tb) k_gemm<false, false><<<nb, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, accumulate);
@@ -3025,6 +3079,7 @@ struct sce_surrogate {
   // training scratch for the current batch size
   int bsz = 0;
   float *X = nullptr, *Z1 = nullptr, *Z2 = nullptr, *dg = nullptr, *dZ2 = nullptr, *dH1 = nullptr;
+  float* part = nullptr;  // split-K partial tiles of the weight gradients
   double *yb = nullptr, *loss_terms = nullptr;
   long long* idx = nullptr;
   long long idx_cap = 0;
@@ -3238,6 +3293,8 @@ static int32_t sur_scratch(sce_surrogate* s, int bsz) {
   CHK(dalloc(s->scratch, &s->dZ2, rows * SUR_H));
   CHK(dalloc(s->scratch, &s->dH1, rows * SUR_H));
   CHK(dalloc(s->scratch, &s->yb, (size_t)bsz));
+  const size_t nchunks = (rows + GEMM_KCHUNK - 1) / GEMM_KCHUNK;
+  CHK(dalloc(s->scratch, &s->part, nchunks * SUR_H * (size_t)std::max(SUR_H, s->fmax)));
   CHK(dalloc(s->scratch, &s->loss_terms, (size_t)bsz));
   s->bsz = bsz;
   return SCE_OK;
@@ -3269,11 +3326,11 @@ static void sur_grad_batch(sce_surrogate* s, int sub, int bsz) {
   k_gemv_t<<<SUR_H, 256, 0, st>>>(s->Z2, s->dg, rows, SUR_H, SUR_H, gw3);
   k_gemv_t<<<F, 256, 0, st>>>(s->X, s->dg, rows, F, fm, gv);
   k_back2<<<(unsigned)((nh + 255) / 256), 256, 0, st>>>(s->Z2, w3, s->dg, rows, s->dZ2);
-  gemm(true, false, SUR_H, SUR_H, rows, s->dZ2, SUR_H, s->Z1, SUR_H, gW2, SUR_H, 0, st);
+  gemm(true, false, SUR_H, SUR_H, rows, s->dZ2, SUR_H, s->Z1, SUR_H, gW2, SUR_H, 0, st, s->part);
   k_colsum<<<SUR_H, 256, 0, st>>>(s->dZ2, rows, SUR_H, SUR_H, gb2);
   gemm(false, false, rows, SUR_H, SUR_H, s->dZ2, SUR_H, W2, SUR_H, s->dH1, SUR_H, 0, st);
   k_dtanh<<<(unsigned)((nh + 255) / 256), 256, 0, st>>>(s->dH1, s->Z1, nh);
-  gemm(true, false, SUR_H, F, rows, s->dH1, SUR_H, s->X, fm, gW1, F, 0, st);
+  gemm(true, false, SUR_H, F, rows, s->dH1, SUR_H, s->X, fm, gW1, F, 0, st, s->part);
   k_colsum<<<SUR_H, 256, 0, st>>>(s->dH1, rows, SUR_H, SUR_H, gb1);
 }
 
@@ -3471,6 +3528,22 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
     HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
+  if (!c->use_graph) {
+    // direct launches (the chain's graph replay switched off)
+    for (int64_t i = 1; i <= nsweeps; ++i) {
+      const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
+      const bool meas = measure_interval > 0 && i % measure_interval == 0;
+      const bool fused = meas && !renorm;
+      CHK(enqueue_da(c, s, fused ? 1 : 0));
+      if (renorm) enqueue_renorm(c);
+      if (meas && !fused) enqueue_measure(c);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (want > 0) HIPCHK(hipMemcpy(obs, c->obs, 4 * (size_t)want * 8, hipMemcpyDeviceToHost));
+    if (n_obs) *n_obs = want;
+    return SCE_OK;
+  }
   // capture one DA sweep per measurement flavour
   hipGraphExec_t ge[2] = {nullptr, nullptr};
   for (int k = 0; k < 2; ++k) {

@@ -76,7 +76,8 @@ def test_inference_path_bitwise():
             assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
-def test_training_gradient_matches_fp64_backprop():
+@pytest.mark.parametrize("n", [250, 1500])   # 1500 pairs: K = 3000, the split-K weight gradients
+def test_training_gradient_matches_fp64_backprop(n):
     na, terms, dims = bcc_tet_small(2)
     H, T = build_pair(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
@@ -87,7 +88,7 @@ def test_training_gradient_matches_fp64_backprop():
     sur.set_params(a, p, 0.05)
     cfg = rand_config(H.n_sites, 2)
     sites = np.arange(a + 1, H.n_sites + 1, na, dtype=np.int32)
-    sites = np.concatenate([sites] * 12)[:250].astype(np.int32)
+    sites = np.concatenate([sites] * (n // len(sites) + 1))[:n].astype(np.int32)
     e2 = rand_config(len(sites), 3)
     A, B = SG.dev_features(gH, cfg, sites, e2)
     y = np.random.default_rng(1).normal(0, 0.05, len(sites))
@@ -191,3 +192,21 @@ def test_trained_surrogate_screens_better_than_untrained():
     # a trained surrogate keeps the plain acceptance while its stage 2 rarely rejects
     assert acc_t > 0.9 * plain and s2_t > 0.9
     assert acc_t > acc_u and s2_t > s2_u
+
+
+def test_da_direct_launches_equal_graph_replay():
+    na, terms, dims = bcc_tet_small(2)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    sur = SG.Surrogate(gH, seed=4)
+    st = S.ChainState(H, rand_config(H.n_sites, 12), 0.6)
+    a = S.GPUChainState(gH, st, seed=9)
+    b = S.GPUChainState(gH, st, seed=9)
+    b.set_graph(False)
+    oa = SG.gpu_da_run_sweeps(a, sur, None, 20.0, 7, renorm_interval=3, measure_interval=2)
+    ob = SG.gpu_da_run_sweeps(b, sur, None, 20.0, 7, renorm_interval=3, measure_interval=2)
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    assert np.array_equal(oa, ob)
+    assert a.info().acc_stage1 == b.info().acc_stage1
