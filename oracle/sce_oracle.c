@@ -1179,7 +1179,8 @@ static int site_factor_sites(const ora_tiled* t, int j, int* fs) {
   return n;
 }
 
-/* Features of site s for spin e (3) against config; returns F. */
+/* Features of site s for spin e (3) against config; returns F.  Needs
+ * lmax >= 1 (the neighbour spins are read back from their l = 1 rows). */
 int ora_sur_features(const ora_tiled* t, const double* config, int s, const double* e, float* out) {
   int n, nb[256], nnb = 0, spec[512][4], nspec = 0, body1 = 0;
   int* js = sorted_adjacency(t, s, &n);
@@ -1199,8 +1200,18 @@ int ora_sur_features(const ora_tiled* t, const double* config, int s, const doub
   }
   free(js);
   if (body1) for (int x = 0; x < 3; ++x) { spec[nspec][0] = 3; spec[nspec][1] = x; ++nspec; }
+  /* neighbour spins read back from their tesseral rows: Z_{1,-1} = N1 y,
+   * Z_{1,0} = N1 z, Z_{1,1} = N1 x, so e_n = (Z[3], Z[1], Z[2]) * (1 / N1) */
+  static const double inv_n1 = 0x1.05f8bd37c0e62p+1;
+  double Es[256][3], zr[49];
   const double* E[256];
-  for (int k = 0; k < nnb; ++k) E[k] = config + 3 * (size_t)(nb[k] - 1);
+  for (int k = 0; k < nnb; ++k) {
+    ora_zlm_row(config + 3 * (size_t)(nb[k] - 1), t->lmax, zr);
+    Es[k][0] = zr[3] * inv_n1;
+    Es[k][1] = zr[1] * inv_n1;
+    Es[k][2] = zr[2] * inv_n1;
+    E[k] = Es[k];
+  }
   int F = 0;
   for (int k = 0; k < nnb; ++k) out[F++] = (float)dot3d(e, E[k]);
   for (int k = 0; k < nnb; ++k) { double a = dot3d(e, E[k]); out[F++] = (float)(a * a); }

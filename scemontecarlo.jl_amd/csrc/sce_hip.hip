@@ -147,9 +147,16 @@ __global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __re
 }
 
 // Stage the neighbour rows of site s into Z (nb*NLM + row) and 1.0 at NB*NLM.
+// Each lane fetches STAGE_ROUND(L) entries per round with every load issued
+// before the first is used: indices past the end are clamped to the last entry
+// (an in-bounds load whose value is written to that entry's own slot again), so
+// no lane-dependent branch splits the round into serial load-use pairs.
+__host__ __device__ constexpr int stage_round(int L) { return L >= 64 ? 4 : 8; }
+
 template <int L, int NLM>
 __device__ __forceinline__ void stage_neighbours(const DevTables& T, const double* __restrict__ zrows,
                                                  int s, int lane, double* Z, int* atom_out) {
+  constexpr int U = stage_round(L);
   const int atom = s % T.na;
   const int cell = s / T.na;
   const int c1 = cell % T.d0;
@@ -157,22 +164,51 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
   const int c2 = r % T.d1;
   const int c3 = r / T.d1;
   const int nb0 = T.nb_ptr[atom];
-  const int NB = T.nb_ptr[atom + 1] - nb0;
-#pragma unroll 4
-  for (int idx = lane; idx < NB * NLM; idx += L) {
-    const int nbi = idx / NLM;
-    const int row = idx - nbi * NLM;
-    const int4 o = T.nb[nb0 + nbi];
-    int x = c1 + o.y, y = c2 + o.z, z = c3 + o.w;
-    if (x >= T.d0) x -= T.d0;
-    if (y >= T.d1) y -= T.d1;
-    if (z >= T.d2) z -= T.d2;
-    const int n = o.x + T.na * (x + T.d0 * (y + T.d1 * z));
-    Z[idx] = zrows[(size_t)n * NLM + row];
+  const int total = (T.nb_ptr[atom + 1] - nb0) * NLM;
+  for (int b = 0; b < total; b += U * L) {
+    int at[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = b + u * L + lane;
+      at[u] = i < total ? i : total - 1;
+      const int nbi = at[u] / NLM;
+      const int row = at[u] - nbi * NLM;
+      const int4 o = T.nb[nb0 + nbi];
+      int x = c1 + o.y, y = c2 + o.z, z = c3 + o.w;
+      if (x >= T.d0) x -= T.d0;
+      if (y >= T.d1) y -= T.d1;
+      if (z >= T.d2) z -= T.d2;
+      const int n = o.x + T.na * (x + T.d0 * (y + T.d1 * z));
+      v[u] = zrows[(size_t)n * NLM + row];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) Z[at[u]] = v[u];
   }
-  if (lane == 0) Z[NB * NLM] = 1.0;
+  if (lane == 0) Z[total] = 1.0;
   *atom_out = atom;
 }
+
+// dz = proposed row - current row of site s, for the walk.  load() issues the
+// reads before the neighbour staging (indices clamped like the staging's),
+// store() writes the differences after it.
+template <int L, int NLM>
+struct RowDelta {
+  static constexpr int U = (NLM + L - 1) / L;
+  double p[U], c[U];
+  __device__ __forceinline__ void load(const double* __restrict__ pz, const double* __restrict__ zs, int lane) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = lane + u * L < NLM ? lane + u * L : NLM - 1;
+      p[u] = pz[k];
+      c[u] = zs[k];
+    }
+  }
+  __device__ __forceinline__ void store(double* dz, int lane) const {
+#pragma unroll
+    for (int u = 0; u < U; ++u) dz[lane + u * L < NLM ? lane + u * L : NLM - 1] = p[u] - c[u];
+  }
+};
 
 // ---------------------------------------------------------------------------
 // the colour walk: `L` lanes per site, 256/L sites per block.
@@ -191,18 +227,19 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
   double a = 0.0;
   for (int base = 0; base < W; base += WALK_UNROLL * L) {
     WalkEntry en[WALK_UNROLL];
+    // every load of the batch is issued before the first use: entries past the
+    // end are clamped to the last one and their terms discarded by a select
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
       const int i = base + u * L + lane;
-      if (i < W) en[u] = wl[i];
+      en[u] = wl[i < W ? i : W - 1];
     }
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
       const int i = base + u * L + lane;
-      if (i < W) {
-        const double p = (Z[en[u].f0] * Z[en[u].f1]) * Z[en[u].f2];
-        if (p != 0.0) a = __builtin_fma(en[u].w * p, dz[en[u].tgt], a);
-      }
+      const double p = (Z[en[u].f0] * Z[en[u].f1]) * Z[en[u].f2];
+      const double t = __builtin_fma(en[u].w * p, dz[en[u].tgt], a);
+      a = (i < W && p != 0.0) ? t : a;
     }
   }
   return a;
@@ -234,9 +271,10 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
     W = T.walk_ptr[a0 + 1] - w0;
     wl = T.walk + w0;
     if (!TEST) u_acc = prop[(size_t)q * PS + 3];
+    RowDelta<L, NLM> rd;
+    rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
     stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
-    const double* pz = prop + (size_t)q * PS + 4;
-    for (int k = lane; k < NLM; k += L) dz[k] = pz[k] - zrows[(size_t)s * NLM + k];
+    rd.store(dz, lane);
   }
   __syncthreads();
   double a = 0.0;
@@ -332,19 +370,16 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
 #pragma unroll
       for (int u = 0; u < WALK_UNROLL; ++u) {
         const int i = base + u * L + lane;
-        if (i < W) en[u] = wl[i];
+        en[u] = wl[i < W ? i : W - 1];  // clamped: the batch's loads issue together
       }
 #pragma unroll
       for (int u = 0; u < WALK_UNROLL; ++u) {
         const int i = base + u * L + lane;
-        if (i >= W) continue;
         const double p = (Z[en[u].f0] * Z[en[u].f1]) * Z[en[u].f2];
-        if (p != 0.0) {
-          const double v = en[u].w * p;
+        const bool use = i < W && p != 0.0;
+        const double v = en[u].w * p;
 #pragma unroll
-          for (int k = 0; k < NLM; ++k)
-            if (k == en[u].tgt) c[k] = c[k] + v;
-        }
+        for (int k = 0; k < NLM; ++k) c[k] = (use && k == en[u].tgt) ? c[k] + v : c[k];
       }
     }
   }
@@ -2564,9 +2599,20 @@ constexpr int SUR_H = 64;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Inference reads the hidden layers' weights in quad-packed form:
+// WQ[(q * 64 + j) * 4 + r] = W[j][4q + r], so lane j's four weights of input quad
+// q are one 16-byte load and a wavefront's load of quad q is 1 KiB contiguous.
+// Layer 1 is zero padded to SUR_QB quads (64 inputs) per batch pair; the
+// descriptor rows in LDS are zero padded alike.
+constexpr int SUR_BQ = 4;              // quads per weight batch (16 floats per lane)
+constexpr int SUR_QB = 2 * SUR_BQ;     // quads per double batch
+constexpr int SUR_FQ = 4 * SUR_QB;     // inputs per double batch (64)
+
+__host__ __device__ inline int sur_fpad(int F) { return (F + SUR_FQ - 1) / SUR_FQ * SUR_FQ; }
+
 struct SurNet {
   const float *W1, *b1, *W2, *b2, *w3, *v;  // W1: H x F row-major, W2: H x H row-major
-  const float *W1T, *W2T;                    // inference copies: W1T[f][j], W2T[k][j] (coalesced over j)
+  const float *W1Q, *W2Q;                    // inference copies, quad-packed (W1Q: sur_fpad(F) inputs)
   int F;
   double scale;
 };
@@ -2575,28 +2621,96 @@ __device__ __forceinline__ double dot3(const double* a, const double* b) {
   return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
 
-// Descriptor feature f of spin e against the staged neighbour spins E (NB x 3).
-__device__ __forceinline__ float sur_feature(int f, int NB, const int4* __restrict__ spec,
-                                             const double* E, const double* e) {
-  if (f < NB) return (float)dot3(e, E + 3 * f);
+// The descriptor reads each neighbour's spin back from its tesseral row: with
+// Z_{1,-1} = N1 y, Z_{1,0} = N1 z, Z_{1,1} = N1 x (N1 = sqrt(3 / 4pi)), the spin
+// is (Z[3], Z[1], Z[2]) * (1 / N1), each a double multiply (tables need l >= 1).
+// Rows are what the walk stages anyway, so the delayed-acceptance kernel gathers
+// every neighbour once for both stages.  Spin component c comes from row
+// sur_spin_row(c).
+__host__ __device__ constexpr int sur_spin_row(int c) { return c == 0 ? 3 : c; }
+
+struct SpinsE {  // spins staged as NB x 3
+  const double* E;
+  __device__ __forceinline__ void get(int k, double* v) const {
+    v[0] = E[3 * k];
+    v[1] = E[3 * k + 1];
+    v[2] = E[3 * k + 2];
+  }
+};
+struct SpinsZ {  // rows staged as NB x nlm
+  const double* Z;
+  int nlm;
+  __device__ __forceinline__ void get(int k, double* v) const {
+    const double* z = Z + k * nlm;
+    v[0] = z[sur_spin_row(0)] * sce::INV_N1;
+    v[1] = z[sur_spin_row(1)] * sce::INV_N1;
+    v[2] = z[sur_spin_row(2)] * sce::INV_N1;
+  }
+};
+
+// Spec of descriptor feature f (meaningful for f >= 2 NB): an in-bounds load
+// for every f, so a wavefront's loads issue together.
+__device__ __forceinline__ int4 sur_spec(const int4* __restrict__ spec, int nspec, int f, int NB) {
+  if (nspec == 0) return int4{0, 0, 0, 0};
+  const int k = f - 2 * NB;
+  return spec[k < 0 ? 0 : (k < nspec ? k : nspec - 1)];
+}
+
+// Descriptor feature f (spec q) of the current spin e and the proposed spin e2
+// against the neighbour spins `sp`: *a = phi_f(e), *b = phi_f(e2).
+template <class S>
+__device__ __forceinline__ void sur_feature_pair(int f, int NB, int4 q, const S& sp, const double* e,
+                                                 const double* e2, float* a, float* b) {
+  double x[3], y[3], z[3];
+  if (f < NB) {
+    sp.get(f, x);
+    *a = (float)dot3(e, x);
+    *b = (float)dot3(e2, x);
+    return;
+  }
   if (f < 2 * NB) {
-    const double a = dot3(e, E + 3 * (f - NB));
-    return (float)(a * a);
+    sp.get(f - NB, x);
+    const double ta = dot3(e, x), tb = dot3(e2, x);
+    *a = (float)(ta * ta);
+    *b = (float)(tb * tb);
+    return;
   }
-  const int4 sp = spec[f - 2 * NB];
-  if (sp.x == 1) {
-    const double* x = E + 3 * sp.y;
-    const double* y = E + 3 * sp.z;
+  if (q.x == 1) {
+    sp.get(q.y, x);
+    sp.get(q.z, y);
     const double cr[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2], x[0] * y[1] - x[1] * y[0]};
-    return (float)dot3(e, cr);
+    *a = (float)dot3(e, cr);
+    *b = (float)dot3(e2, cr);
+    return;
   }
-  if (sp.x == 2) {
-    const double* x = E + 3 * sp.y;
-    const double* y = E + 3 * sp.z;
-    const double* z = E + 3 * sp.w;
-    return (float)((dot3(e, x) * dot3(y, z) + dot3(e, y) * dot3(x, z)) + dot3(e, z) * dot3(x, y));
+  if (q.x == 2) {
+    sp.get(q.y, x);
+    sp.get(q.z, y);
+    sp.get(q.w, z);
+    const double yz = dot3(y, z), xz = dot3(x, z), xy = dot3(x, y);
+    *a = (float)((dot3(e, x) * yz + dot3(e, y) * xz) + dot3(e, z) * xy);
+    *b = (float)((dot3(e2, x) * yz + dot3(e2, y) * xz) + dot3(e2, z) * xy);
+    return;
   }
-  return (float)e[sp.y];
+  *a = (float)e[q.y];
+  *b = (float)e2[q.y];
+}
+
+// Both descriptor rows of one site, phi(e) -> ra and phi(e2) -> rb, over
+// f < width (zero for F <= f), two features per lane per round.
+template <int L, class S>
+__device__ __forceinline__ void sur_rows(int lane, int F, int width, int NB, const int4* __restrict__ spec,
+                                         int nspec, const S& sp, const double* e, const double* e2, float* ra,
+                                         float* rb) {
+  for (int f0 = 0; f0 < width; f0 += 2 * L) {
+    const int f1 = f0 + lane, f2 = f1 + L;
+    const int4 q1 = sur_spec(spec, nspec, f1, NB), q2 = sur_spec(spec, nspec, f2, NB);
+    float a1 = 0.0f, b1 = 0.0f, a2 = 0.0f, b2 = 0.0f;
+    if (f1 < F) sur_feature_pair(f1, NB, q1, sp, e, e2, &a1, &b1);
+    if (f2 < F) sur_feature_pair(f2, NB, q2, sp, e, e2, &a2, &b2);
+    if (f1 < width) ra[f1] = a1, rb[f1] = b1;
+    if (f2 < width) ra[f2] = a2, rb[f2] = b2;
+  }
 }
 
 // tanh in FP32 from the deterministic exp: sign(x)(1 - t)/(1 + t), t = e^{-2|x|}
@@ -2607,9 +2721,9 @@ __device__ __forceinline__ float det_tanhf(float z) {
   return (float)(x < 0.0 ? -r : r);
 }
 
-// Stages the neighbour spins of site s (atom a) into E (NB x 3).
+// Stages the neighbour spins of site s (read back from their rows) into E (NB x 3).
 template <int L>
-__device__ __forceinline__ void stage_spins(const DevTables& T, const double* __restrict__ config, int s,
+__device__ __forceinline__ void stage_spins(const DevTables& T, const double* __restrict__ zrows, int s,
                                             int lane, double* E, int* NB_out) {
   const int atom = s % T.na;
   const int cell = s / T.na;
@@ -2619,6 +2733,7 @@ __device__ __forceinline__ void stage_spins(const DevTables& T, const double* __
   const int c3 = r / T.d1;
   const int nb0 = T.nb_ptr[atom];
   const int NB = T.nb_ptr[atom + 1] - nb0;
+#pragma unroll 2
   for (int idx = lane; idx < 3 * NB; idx += L) {
     const int nbi = idx / 3;
     const int4 o = T.nb[nb0 + nbi];
@@ -2627,50 +2742,89 @@ __device__ __forceinline__ void stage_spins(const DevTables& T, const double* __
     if (y >= T.d1) y -= T.d1;
     if (z >= T.d2) z -= T.d2;
     const int n = o.x + T.na * (x + T.d0 * (y + T.d1 * z));
-    E[idx] = config[3 * (size_t)n + (idx - 3 * nbi)];
+    E[idx] = zrows[(size_t)n * T.nlm + sur_spin_row(idx - 3 * nbi)] * sce::INV_N1;
   }
   *NB_out = NB;
 }
 
+__device__ __forceinline__ void fma_quad(const floatx4 w, const floatx4 a, const floatx4 b, float& zA, float& zB) {
+  zA = __builtin_fmaf(a[0], w[0], zA);
+  zB = __builtin_fmaf(b[0], w[0], zB);
+  zA = __builtin_fmaf(a[1], w[1], zA);
+  zB = __builtin_fmaf(b[1], w[1], zB);
+  zA = __builtin_fmaf(a[2], w[2], zA);
+  zB = __builtin_fmaf(b[2], w[2], zB);
+  zA = __builtin_fmaf(a[3], w[3], zA);
+  zB = __builtin_fmaf(b[3], w[3], zB);
+}
+
+// Pre-activations of hidden unit j (this lane) for two inputs xA, xB (LDS, nq
+// quads, nq a multiple of SUR_QB): the fused chains
+// z = fma(x[4nq-1], W[j][4nq-1], ... fma(x[0], W[j][0], z)).  wq = WQ + j in
+// quads.  Weights arrive in batches of SUR_BQ quads, one batch in flight while
+// the previous one is used.
+typedef const __attribute__((address_space(1))) floatx4 gfloatx4;
+typedef const __attribute__((address_space(1))) float gfloat;
+
+__device__ __forceinline__ void hidden_pair(const floatx4* wq_generic, int nq, const floatx4* xA,
+                                            const floatx4* xB, float& zA, float& zB) {
+  // the weights are global memory: global loads count on the in-order vector
+  // memory counter only, so each batch is waited for separately
+  gfloatx4* __restrict__ wq = (gfloatx4*)wq_generic;
+  floatx4 p[SUR_BQ], r[SUR_BQ];
+#pragma unroll
+  for (int u = 0; u < SUR_BQ; ++u) p[u] = wq[u * SUR_H];
+  for (int q0 = 0; q0 < nq; q0 += SUR_QB) {
+#pragma unroll
+    for (int u = 0; u < SUR_BQ; ++u) r[u] = wq[(q0 + SUR_BQ + u) * SUR_H];
+#pragma unroll
+    for (int u = 0; u < SUR_BQ; ++u) fma_quad(p[u], xA[q0 + u], xB[q0 + u], zA, zB);
+    if (q0 + SUR_QB < nq) {
+#pragma unroll
+      for (int u = 0; u < SUR_BQ; ++u) p[u] = wq[(q0 + SUR_QB + u) * SUR_H];
+    }
+#pragma unroll
+    for (int u = 0; u < SUR_BQ; ++u) fma_quad(r[u], xA[q0 + SUR_BQ + u], xB[q0 + SUR_BQ + u], zA, zB);
+  }
+}
+
 // The surrogate difference of one site on one wavefront (lane j = hidden unit
-// j).  phiA/phiB: the site's descriptor rows in LDS, zero padded to F4 = F
-// rounded up to 4; W1T/W2T: transposed weights (rows f >= F zero), so the loads
-// of one step are coalesced over j.  Each pre-activation is the fused
-// multiply-add chain z = fma(x[K-1], w[K-1], ... fma(x[0], w[0], b)) in
-// ascending input index over the padded K — the same value a matrix-core tile
-// (v_mfma_f32_16x16x4f32: four fused multiply-adds in k order per instruction)
-// produces, and the order the oracle restates.  The head (w3 . h2 + v . phi)
-// keeps separate multiplies and adds and folds over the 64 lanes.
-// Contains a block barrier: every thread of the block calls it.
+// j).  phiA/phiB: the site's descriptor rows in LDS (16-byte aligned), zero
+// padded to sur_fpad(F).  Each pre-activation is the fused multiply-add chain
+// z = fma(x[K-1], w[K-1], ... fma(x[0], w[0], b)) in ascending input index — the
+// value a matrix-core tile (v_mfma_f32_16x16x4f32: four fused multiply-adds in k
+// order per instruction) produces, and the order the oracle restates over the
+// input padded to a multiple of 4.  The longer zero padding here appends
+// fma(+0, +0, z) steps, which return z except that a z of -0 becomes +0; tanh
+// maps both zeros to +0, so the hidden activations are the same bits.  The head
+// (w3 . h2 + v . phi) keeps separate multiplies and adds and folds over the 64
+// lanes.  Contains a block barrier: every thread of the block calls it.
 __device__ double sur_delta(const SurNet& net, const float* phiA, const float* phiB, float* h1A, float* h1B,
                             bool valid) {
   const int j = threadIdx.x % 64;
   float tA = 0.0f, tB = 0.0f;
   if (valid) {
-    const int F4 = (net.F + 3) & ~3;
-    float zA = net.b1[j], zB = net.b1[j];
-    for (int f = 0; f < F4; ++f) {
-      const float w = net.W1T[(size_t)f * SUR_H + j];
-      zA = __builtin_fmaf(phiA[f], w, zA);
-      zB = __builtin_fmaf(phiB[f], w, zB);
-    }
+    float zA = ((gfloat*)net.b1)[j], zB = zA;
+    hidden_pair(reinterpret_cast<const floatx4*>(net.W1Q) + j, sur_fpad(net.F) / 4,
+                reinterpret_cast<const floatx4*>(phiA), reinterpret_cast<const floatx4*>(phiB), zA, zB);
+    __asm__ volatile("" : "+v"(zA), "+v"(zB));  // both chains complete before either tanh
     h1A[j] = det_tanhf(zA);
     h1B[j] = det_tanhf(zB);
   }
   __syncthreads();
   if (valid) {
-    float zA = net.b2[j], zB = net.b2[j];
-    for (int k = 0; k < SUR_H; ++k) {
-      const float w = net.W2T[k * SUR_H + j];
-      zA = __builtin_fmaf(h1A[k], w, zA);
-      zB = __builtin_fmaf(h1B[k], w, zB);
-    }
-    const float oA = net.w3[j] * det_tanhf(zA);
-    const float oB = net.w3[j] * det_tanhf(zB);
+    float zA = ((gfloat*)net.b2)[j], zB = zA;
+    hidden_pair(reinterpret_cast<const floatx4*>(net.W2Q) + j, SUR_H / 4, reinterpret_cast<const floatx4*>(h1A),
+                reinterpret_cast<const floatx4*>(h1B), zA, zB);
+    __asm__ volatile("" : "+v"(zA), "+v"(zB));
+    const float w3 = ((gfloat*)net.w3)[j];
+    const float oA = w3 * det_tanhf(zA);
+    const float oB = w3 * det_tanhf(zB);
     float sA = 0.0f, sB = 0.0f;
     for (int f = j; f < net.F; f += 64) {
-      sA = sA + net.v[f] * phiA[f];
-      sB = sB + net.v[f] * phiB[f];
+      const float v = ((gfloat*)net.v)[f];
+      sA = sA + v * phiA[f];
+      sB = sB + v * phiB[f];
     }
     tA = oA + sA;
     tB = oB + sB;
@@ -2683,20 +2837,24 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
   return ((double)tB - (double)tA) * net.scale;  // valid in lane 0
 }
 
-// LDS of one DA group (doubles): walk block (walk_stride) | spins (3 nb_max) |
-// phiA, phiB (FP floats each) | h1A, h1B (64 floats each).
-__host__ __device__ inline int da_group_stride(int walk_stride, int nb_max, int FP) {
-  return walk_stride + 3 * nb_max + (2 * FP + 2 * SUR_H + 1) / 2 + 1;
+// LDS of one DA group (doubles): neighbour rows + dz (walk_stride, rounded up to
+// even) | phiA, phiB (FP floats each, FP a multiple of 64) | h1A, h1B (64 floats
+// each).  Every float block starts 16-byte aligned.
+__host__ __device__ inline int da_phi_offset(int walk_stride) { return (walk_stride + 1) & ~1; }
+__host__ __device__ inline int da_group_stride(int walk_stride, int FP) {
+  return da_phi_offset(walk_stride) + FP + SUR_H;
 }
-__host__ __device__ inline size_t da_lds_bytes(int walk_stride, int nb_max, int FP) {
-  return (size_t)(TPB / 64) * da_group_stride(walk_stride, nb_max, FP) * 8;
+__host__ __device__ inline size_t da_lds_bytes(int walk_stride, int FP) {
+  return (size_t)(TPB / 64) * da_group_stride(walk_stride, FP) * 8;
 }
 
-// Two-stage delayed-acceptance update: one site per wavefront (64 lanes).
+// Two-stage delayed-acceptance update: one site per wavefront (64 lanes).  The
+// neighbour rows are staged once: the descriptor reads the spins from them and
+// the stage-1 survivors walk them.
 template <int LMAX>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da(DevTables T, double* config, double* zrows, const double* prop,
-                                            double* dE, int* acc, int* acc1, const ChainDev* st,
-                                            const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP) {
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da(
+    DevTables T, double* config, double* zrows, const double* prop, double* dE, int* acc, int* acc1,
+    const ChainDev* st, const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP) {
   constexpr int L = 64, G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   extern __shared__ double lds[];
   // one site per wavefront: its index, site and sublattice are wave-uniform
@@ -2705,12 +2863,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
   const bool valid = qi < q_n;
   const int q = q_lo + qi;
-  const int gstride = da_group_stride(walk_stride, T.nb_max, FP);
-  double* base = lds + g * gstride;
+  double* base = lds + g * da_group_stride(walk_stride, FP);
   double* Z = base;
   double* dz = Z + T.nb_max * NLM + 1;
-  double* E = base + walk_stride;
-  float* phiA = reinterpret_cast<float*>(E + 3 * T.nb_max);
+  float* phiA = reinterpret_cast<float*>(base + da_phi_offset(walk_stride));
   float* phiB = phiA + FP;
   float* h1A = phiB + FP;
   float* h1B = h1A + SUR_H;
@@ -2728,16 +2884,17 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     e2[0] = pq[0];
     e2[1] = pq[1];
     e2[2] = pq[2];
-    stage_spins<L>(T, config, s, lane, E, &NB);
+    RowDelta<L, NLM> rd;
+    rd.load(pq + 4, zrows + (size_t)s * NLM, lane);
+    stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+    rd.store(dz, lane);
+    NB = T.nb_ptr[atom + 1] - T.nb_ptr[atom];
   }
   __syncthreads();
   if (valid) {
-    const int4* spec = T.feat + T.feat_ptr[atom];
-    const int F4 = (net.F + 3) & ~3;
-    for (int f = lane; f < F4; f += L) {
-      phiA[f] = f < net.F ? sur_feature(f, NB, spec, E, e) : 0.0f;
-      phiB[f] = f < net.F ? sur_feature(f, NB, spec, E, e2) : 0.0f;
-    }
+    const int f0 = T.feat_ptr[atom];
+    sur_rows<L>(lane, net.F, sur_fpad(net.F), NB, T.feat + f0, T.feat_ptr[atom + 1] - f0, SpinsZ{Z, NLM}, e, e2,
+                phiA, phiB);
   }
   __syncthreads();
   const double dEs = sur_delta(net, phiA, phiB, h1A, h1B, valid);
@@ -2745,11 +2902,6 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (valid && lane == 0) pass = (dEs <= 0.0 || pq[3] < sce::det_exp(-st->beta * dEs)) ? 1 : 0;
   pass = __shfl(pass, 0, L);
   const bool walk = valid && pass;
-  if (walk) {
-    stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
-    for (int k = lane; k < NLM; k += L) dz[k] = pq[4 + k] - zrows[(size_t)s * NLM + k];
-  }
-  __syncthreads();
   double a = 0.0;
   if (walk) {
     const int w0 = T.walk_ptr[atom];
@@ -2784,6 +2936,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
 // per site, phi(e) and phi(e') written as FP32 rows of width fmax.
 template <bool FROM_PROP>
 __global__ __launch_bounds__(TPB) void k_features(DevTables T, const double* __restrict__ config,
+                                                  const double* __restrict__ zrows,
                                                   const double* __restrict__ prop, int PS,
                                                   const int* __restrict__ sites, const double* __restrict__ e2s,
                                                   int q_lo, int q_n, float* __restrict__ outA,
@@ -2797,22 +2950,18 @@ __global__ __launch_bounds__(TPB) void k_features(DevTables T, const double* __r
   int s = 0, NB = 0;
   if (valid) {
     s = FROM_PROP ? T.color_sites[q_lo + qi] : sites[qi];
-    stage_spins<L>(T, config, s, lane, E, &NB);
+    stage_spins<L>(T, zrows, s, lane, E, &NB);
   }
   __syncthreads();
   if (!valid) return;
+  const SpinsE sp{E};
   const int atom = s % T.na;
   const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
   const double* p2 = FROM_PROP ? prop + (size_t)(q_lo + qi) * PS : e2s + 3 * (size_t)qi;
   const double e2[3] = {p2[0], p2[1], p2[2]};
-  const int4* spec = T.feat + T.feat_ptr[atom];
-  const int F = 2 * NB + (T.feat_ptr[atom + 1] - T.feat_ptr[atom]);
-  float* ra = outA + (size_t)qi * T.nfeat_max;
-  float* rb = outB + (size_t)qi * T.nfeat_max;
-  for (int f = lane; f < T.nfeat_max; f += L) {
-    ra[f] = f < F ? sur_feature(f, NB, spec, E, e) : 0.0f;
-    rb[f] = f < F ? sur_feature(f, NB, spec, E, e2) : 0.0f;
-  }
+  const int f0 = T.feat_ptr[atom], nspec = T.feat_ptr[atom + 1] - f0;
+  sur_rows<L>(lane, 2 * NB + nspec, T.nfeat_max, NB, T.feat + f0, nspec, sp, e, e2,
+              outA + (size_t)qi * T.nfeat_max, outB + (size_t)qi * T.nfeat_max);
   if (lane == 0 && sub) sub[qi] = (int8_t)atom;
 }
 
@@ -3040,19 +3189,14 @@ __global__ void k_gather(const float* dA, const float* dB, const double* dy, con
   if (f == 0) y[r] = dy[row];
 }
 
-// Inference copies of the weight matrices: W1T[f][j] = W1[j][f] for f < F and
-// 0 for F <= f < F4 (F rounded up to 4), then W2T[k][j] = W2[j][k].
-__global__ void k_sur_transpose(const float* __restrict__ W1, const float* __restrict__ W2, int F, int F4,
-                                float* __restrict__ W1T, float* __restrict__ W2T) {
+// Quad-packed inference copies (layout in SurNet): W1Q over FQ = sur_fpad(F)
+// inputs (zero for f >= F), then W2Q.
+__global__ void k_sur_pack(const float* __restrict__ W1, const float* __restrict__ W2, int F, int FQ,
+                           float* __restrict__ W1Q, float* __restrict__ W2Q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < SUR_H * F4) {
-    const int f = i / SUR_H, j = i % SUR_H;
-    W1T[i] = f < F ? W1[(size_t)j * F + f] : 0.0f;
-  }
-  if (i < SUR_H * SUR_H) {
-    const int k = i / SUR_H, j = i % SUR_H;
-    W2T[i] = W2[j * SUR_H + k];
-  }
+  const int j = (i / 4) % SUR_H, f = 4 * (i / (4 * SUR_H)) + i % 4;
+  if (i < SUR_H * FQ) W1Q[i] = f < F ? W1[(size_t)j * F + f] : 0.0f;
+  if (i < SUR_H * SUR_H) W2Q[i] = W2[j * SUR_H + f];
 }
 
 // ---- surrogate host side ------------------------------------------------------
@@ -3060,7 +3204,7 @@ struct SurSub {
   int F = 0;
   size_t np = 0;
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v2 = nullptr;
-  float* wT = nullptr;  // W1T (H x F) then W2T (H x H)
+  float* wQ = nullptr;  // W1Q (H x sur_fpad(F)) then W2Q (H x H), quad-packed
   double scale = 1.0;
   long long adam_t = 0;
 };
@@ -3106,8 +3250,8 @@ static int32_t sur_upload_nets(sce_surrogate* s) {
     n.b2 = n.W2 + SUR_H * SUR_H;
     n.w3 = n.b2 + SUR_H;
     n.v = n.w3 + SUR_H;
-    n.W1T = u.wT;
-    n.W2T = u.wT + (size_t)SUR_H * ((u.F + 3) & ~3);
+    n.W1Q = u.wQ;
+    n.W2Q = u.wQ + (size_t)SUR_H * sur_fpad(u.F);
   }
   HIPCHK(hipMemcpy(s->d_nets, h.data(), h.size() * sizeof(SurNet), hipMemcpyHostToDevice));
   return SCE_OK;
@@ -3117,10 +3261,10 @@ static int32_t sur_upload_nets(sce_surrogate* s) {
 static int32_t sur_refresh(sce_surrogate* s, int a) {
   SurSub& u = s->subs[a];
   if (!u.params) return SCE_OK;
-  const int F4 = (u.F + 3) & ~3;
-  const int n = std::max(SUR_H * F4, SUR_H * SUR_H);
-  k_sur_transpose<<<(n + 255) / 256, 256, 0, s->stream>>>(u.params, u.params + (size_t)SUR_H * u.F + SUR_H, u.F,
-                                                        F4, u.wT, u.wT + (size_t)SUR_H * F4);
+  const int FQ = sur_fpad(u.F);
+  const int n = std::max(SUR_H * FQ, SUR_H * SUR_H);
+  k_sur_pack<<<(n + 255) / 256, 256, 0, s->stream>>>(u.params, u.params + (size_t)SUR_H * u.F + SUR_H, u.F, FQ,
+                                                    u.wQ, u.wQ + (size_t)SUR_H * FQ);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return SCE_OK;
@@ -3129,6 +3273,7 @@ static int32_t sur_refresh(sce_surrogate* s, int a) {
 extern "C" int32_t sce_sur_create(sce_tables* t, uint64_t seed, sce_surrogate** out) {
   if (!t || !out) return sce_fail(SCE_EINVAL, "sce_sur_create: NULL argument");
   *out = nullptr;
+  if (t->dev.lmax < 1) return sce_fail(SCE_EINVAL, "sce_sur_create: the descriptor needs lmax >= 1");
   auto* s = new sce_surrogate();
   auto fail = [&](int32_t r) {
     delete s;
@@ -3165,7 +3310,7 @@ extern "C" int32_t sce_sur_create(sce_tables* t, uint64_t seed, sce_surrogate** 
     if ((r = dupload(s->mem, &u.grads, z.data(), z.size())) != SCE_OK) return fail(r);
     if ((r = dupload(s->mem, &u.m, z.data(), z.size())) != SCE_OK) return fail(r);
     if ((r = dupload(s->mem, &u.v2, z.data(), z.size())) != SCE_OK) return fail(r);
-    if ((r = dalloc(s->mem, &u.wT, (size_t)SUR_H * ((u.F + 3) & ~3) + SUR_H * SUR_H)) != SCE_OK) return fail(r);
+    if ((r = dalloc(s->mem, &u.wQ, (size_t)SUR_H * sur_fpad(u.F) + SUR_H * SUR_H)) != SCE_OK) return fail(r);
   }
   if ((r = dalloc(s->mem, &s->d_nets, (size_t)s->na)) != SCE_OK) return fail(r);
   if ((r = sur_upload_nets(s)) != SCE_OK) return fail(r);
@@ -3266,7 +3411,7 @@ extern "C" int32_t sce_sur_harvest(sce_surrogate* s, sce_chain* c, double beta, 
       if (n == 0) continue;
       const int nb = (n + 3) / 4;
       k_features<true><<<nb, TPB, 4 * 3 * (size_t)D.nb_max * 8, c->stream>>>(
-          D, c->config, c->prop, PS, nullptr, nullptr, lo, n, s->dA + (size_t)(row0 + lo) * s->fmax,
+          D, c->config, c->zrows, c->prop, PS, nullptr, nullptr, lo, n, s->dA + (size_t)(row0 + lo) * s->fmax,
           s->dB + (size_t)(row0 + lo) * s->fmax, s->dsub + row0 + lo);
       SCE_DISPATCH_LMAX(D.lmax, launch_harvest_walk, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n,
                         s->dy + row0, c->stream);
@@ -3438,7 +3583,7 @@ extern "C" int32_t sce_sur_eval(sce_surrogate* s, int32_t sub, int32_t n, const 
   CHK(b.up(B, (size_t)n * s->fmax));
   CHK(o.alloc(n));
   const int nb = (n + 3) / 4;
-  const int FP = (s->fmax + 3) & ~3;
+  const int FP = sur_fpad(s->fmax);
   if (nb)
     k_sur_eval<<<nb, TPB, (size_t)(TPB / 64) * (2 * FP + 2 * SUR_H) * 4, s->stream>>>(s->d_nets, sub, n, s->fmax,
                                                                                       FP, a.p, b.p, o.p);
@@ -3451,19 +3596,23 @@ extern "C" int32_t sce_sur_features(sce_tables* t, const double* config, int32_t
                                     const double* e2, float* outA, float* outB) {
   if (!t) return sce_fail(SCE_EINVAL, "NULL tables");
   const DevTables& D = t->dev;
+  if (D.lmax < 1) return sce_fail(SCE_EINVAL, "the surrogate descriptor needs lmax >= 1");
   std::vector<int32_t> s0;
   CHK(check_sites(t, n, sites, s0));
-  Scratch<double> cf, ee;
+  std::vector<double> zr((size_t)D.n_sites * D.nlm);
+  CHK(sce_host_zrows(D.lmax, D.n_sites, config, zr.data()));
+  Scratch<double> cf, zz, ee;
   Scratch<int32_t> si;
   Scratch<float> a, b;
   CHK(cf.up(config, 3 * (size_t)D.n_sites));
+  CHK(zz.up(zr.data(), zr.size()));
   CHK(ee.up(e2, 3 * (size_t)n));
   CHK(si.up(s0.data(), n));
   CHK(a.alloc((size_t)n * D.nfeat_max));
   CHK(b.alloc((size_t)n * D.nfeat_max));
   const int nb = (n + 3) / 4;
   if (nb)
-    k_features<false><<<nb, TPB, 4 * 3 * (size_t)std::max(D.nb_max, 1) * 8>>>(D, cf.p, nullptr, 0, si.p, ee.p, 0, n,
+    k_features<false><<<nb, TPB, 4 * 3 * (size_t)std::max(D.nb_max, 1) * 8>>>(D, cf.p, zz.p, nullptr, 0, si.p, ee.p, 0, n,
                                                                               a.p, b.p, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
@@ -3476,17 +3625,17 @@ template <int LMAX>
 static void launch_da(const DevTables& D, sce_chain* c, const SurNet* nets, int lo, int n, hipStream_t s) {
   constexpr int G = TPB / 64, NLM = (LMAX + 1) * (LMAX + 1);
   const int ws = D.nb_max * NLM + 1 + NLM;
-  const int FP = (D.nfeat_max + 3) & ~3;
+  const int FP = sur_fpad(D.nfeat_max);
   const int nb = (n + G - 1) / G;
   if (nb)
-    k_da<LMAX><<<nb, TPB, da_lds_bytes(ws, D.nb_max, FP), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc,
+    k_da<LMAX><<<nb, TPB, da_lds_bytes(ws, FP), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc,
                                                                c->att, c->st, nets, lo, n, ws, FP);
 }
 
 static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas) {
   const DevTables& D = c->t->dev;
   const int ws = D.nb_max * D.nlm + 1 + D.nlm;
-  if (da_lds_bytes(ws, D.nb_max, (D.nfeat_max + 3) & ~3) > 65536)
+  if (da_lds_bytes(ws, sur_fpad(D.nfeat_max)) > 65536)
     return sce_fail(SCE_EINVAL, "delayed-acceptance stencil too large for one block");
   if (c->sweeps_done >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
   SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, c->stream);

@@ -301,6 +301,11 @@ SCE_HD void keyed_proposal(uint64_t seed, int32_t site, int32_t sweep, const dou
 // (philox.jl:65-67); 3 is the delayed-acceptance second-stage uniform; 16+r is
 // the accept uniform of overrelaxation pass r of a compound sweep.
 constexpr uint32_t SLOT_DA2 = 3u;
+
+// 1 / N1, N1 = sqrt(3 / (4 pi)) the l = 1 tesseral normalisation, rounded to
+// double: the surrogate descriptor reads a neighbour's spin back from its row as
+// (Z[3], Z[1], Z[2]) * INV_N1.
+constexpr double INV_N1 = 0x1.05f8bd37c0e62p+1;
 constexpr uint32_t SLOT_OR0 = 16u;
 
 }  // namespace sce
