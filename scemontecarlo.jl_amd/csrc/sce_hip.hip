@@ -210,6 +210,29 @@ struct RowDelta {
   }
 };
 
+// Fold of a group's lane partials into its first lane: the shuffle-down tree
+// a += a[lane + off] for off = L/2, ..., 1.  The steps of 8 and fewer stay
+// inside a 16-lane row and move data with DPP row shifts (row_shl: lane i
+// reads lane i + off) instead of LDS permutes; the lanes that feed the first
+// lane add the same values in the same order as the shuffle tree.
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int L>
+__device__ __forceinline__ double fold_lanes(double a) {
+#pragma unroll
+  for (int off = L / 2; off >= 16; off >>= 1) a = a + __shfl_down(a, off, L);
+  a = a + dpp_double<0x108>(a);
+  a = a + dpp_double<0x104>(a);
+  a = a + dpp_double<0x102>(a);
+  a = a + dpp_double<0x101>(a);
+  return a;
+}
+
 // ---------------------------------------------------------------------------
 // the colour walk: `L` lanes per site, 256/L sites per block.
 // Lane `lane` visits entries i = lane, lane + L, lane + 2L, ... in ascending
@@ -258,6 +281,12 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
                                             int stride, double* __restrict__ dE_raw = nullptr) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  if constexpr (L == 64) {
+    // one site per wavefront: its index, site and sublattice tables are
+    // wave-uniform (scalar registers)
+    q = __builtin_amdgcn_readfirstlane(q);
+    valid = __builtin_amdgcn_readfirstlane((int)valid) != 0;
+  }
   double* Z = lds + g * stride;
   double* dz = Z + T.nb_max * NLM + 1;
   int s = 0, atom = 0, W = 0;
@@ -265,6 +294,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   double u_acc = 0.0;
   if (valid) {
     s = TEST ? test_sites[q] : T.color_sites[q];
+    if constexpr (L == 64) s = __builtin_amdgcn_readfirstlane(s);
     // independent of the staging: issued with it rather than after the barrier
     const int a0 = s % T.na;
     const int w0 = T.walk_ptr[a0];
@@ -279,8 +309,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   __syncthreads();
   double a = 0.0;
   if (valid) a = walk_delta<L, NLM>(wl, W, lane, Z, dz);
-#pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
+  a = fold_lanes<L>(a);
   __syncthreads();  // the block's LDS is reused by the next site group
   if (!valid) return;
   if (TEST) {
@@ -1018,12 +1047,8 @@ struct sce_chain {
   long long* ip64 = nullptr;  // integer partials of the reductions
   bool persistent = false;  // run sweeps in one cooperative launch (opt-in)
   bool use_graph = true;    // replay captured sweeps as hipGraphs
-  bool graph_fork = false;  // proposals as a parallel graph branch per colour
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // [plain, measured] sweep
   int g_lanes = 0;
-  hipStream_t side = nullptr;           // proposal branch of a captured sweep
-  hipEvent_t ev_fork = nullptr;
-  std::vector<hipEvent_t> ev_col;       // one per colour
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
   long long sweeps_done = 0;  // host mirror of sweep_index
@@ -1045,12 +1070,8 @@ struct sce_chain {
   }
   ~sce_chain() {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (side) (void)hipStreamSynchronize(side);
     drop_graphs();
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    for (hipEvent_t e : ev_col) (void)hipEventDestroy(e);
-    if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -1231,12 +1252,6 @@ static void launch_propose(const DevTables& D, const double* config, double* pro
   if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st, 0, D.n_active);
 }
 
-template <int LMAX>
-static void launch_propose_range(const DevTables& D, const double* config, double* prop,
-                                 const ChainDev* st, int q_lo, int q_n, hipStream_t s) {
-  const int nb = (q_n + 127) / 128;
-  if (nb) k_propose<LMAX><<<nb, 128, 0, s>>>(D, config, prop, st, q_lo, q_n);
-}
 
 // With events e0/e1 the launch goes through hipExtLaunchKernelGGL, whose
 // events carry the dispatch's own start and end timestamps (the kernel's
@@ -1337,30 +1352,19 @@ static int32_t set_beta(sce_chain* c, double beta) {
   return SCE_OK;
 }
 
-// Enqueue one Metropolis sweep (no synchronisation).
-// With `side` (during graph capture) the proposals of colour c are a separate
-// kernel on a side stream that only colour c's walk waits for, so proposing
-// overlaps with the walks of the colours before it.
-static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, hipStream_t side = nullptr) {
+// Enqueue one Metropolis sweep (no synchronisation).  (Proposing each colour
+// on a forked graph branch, to overlap it with the earlier colours' walks,
+// measured 40% slower per sweep: the cross-branch dependencies cost more than
+// the proposal kernel.)
+static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0) {
   CHK(check_lds(c->t->dev, L));
   if (c->sweeps_done >= 2147483647LL - 1)
     return sce_fail(SCE_ERANGE_COUNTER, "sweep_index exhausted the 32-bit RNG counter word");
   const DevTables& D = c->t->dev;
   hipStream_t s = c->stream;
-  if (side) {
-    HIPCHK(hipEventRecord(c->ev_fork, s));
-    HIPCHK(hipStreamWaitEvent(side, c->ev_fork, 0));
-    for (int col = 0; col < c->t->n_colors; ++col) {
-      const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
-      SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, lo, n, side);
-      HIPCHK(hipEventRecord(c->ev_col[col], side));
-    }
-  } else {
-    SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
-  }
+  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
   for (int col = 0; col < c->t->n_colors; ++col) {
     const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
-    if (side) HIPCHK(hipStreamWaitEvent(s, c->ev_col[col], 0));
     if (n == 0) continue;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
@@ -1424,7 +1428,7 @@ static int32_t ensure_graphs(sce_chain* c, int L) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     const long long before = c->sweeps_done;
-    const int32_t r = enqueue_metro(c, L, k, c->graph_fork ? c->side : nullptr);
+    const int32_t r = enqueue_metro(c, L, k);
     c->sweeps_done = before;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (r != SCE_OK) {
@@ -1485,15 +1489,8 @@ extern "C" int32_t sce_chain_create(sce_tables* t, const double* config, const d
   h.seed = seed;
   if (hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess)
     return fail(sce_fail(SCE_EHIP, "upload of chain state failed"));
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(sce_fail(SCE_EHIP, "hipStreamCreate failed"));
-  if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
-    return fail(sce_fail(SCE_EHIP, "hipEventCreate failed"));
-  c->ev_col.resize((size_t)t->n_colors);
-  for (auto& e : c->ev_col)
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-      return fail(sce_fail(SCE_EHIP, "hipEventCreate failed"));
   c->beta_dev = 0.0;
   *out = c;
   return SCE_OK;
@@ -2907,8 +2904,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int w0 = T.walk_ptr[atom];
     a = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
   }
-#pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1) a = a + __shfl_down(a, off, L);
+  a = fold_lanes<L>(a);
   if (!valid) return;
   int accept = 0;
   if (lane == 0 && walk) {

@@ -132,7 +132,7 @@ class GPUChainState:
         return cfg, z, e.value
 
     def set_persistent(self, on: bool) -> None:
-        """Persistent cooperative launches (default) or per-colour launches."""
+        """Persistent cooperative launches (opt-in) or per-colour launches (default)."""
         L.check(L.lib().sce_chain_set_persistent(self._h, 1 if on else 0))
 
     def set_graph(self, on: bool) -> None:
