@@ -6,7 +6,7 @@
 // :168-185), src/energy.jl (_total_energy, :55-75).
 //
 // One Metropolis sweep is
-//   k_propose            one lane per active site: keyed proposal + Z row
+//   k_propose            four lanes per active site: keyed proposal + Z row
 //   k_metro   x colours  `lanes` lanes per site: neighbour rows staged in LDS,
 //                        the site's sublattice walk dealt round-robin over the
 //                        lanes, fma-accumulated ΔE, shuffle-tree fold, FP64
@@ -120,22 +120,54 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
 }
 
 // ---------------------------------------------------------------------------
-// proposals: one lane per active site, at sweep start (a site's spin and row
-// change only at its own colour step, so this equals proposing there)
+// proposals at sweep start (a site's spin and row change only at its own
+// colour step, so this equals proposing there): four lanes per active site.
+// Lanes 4i .. 4i+3 evaluate the radius and the angle of both Box-Muller pairs
+// of sce::keyed_proposal side by side (philox slots 1 and 2), exchange them,
+// and lane 4i completes the proposal and its Z row.  The operations are those
+// of sce::keyed_proposal / normal2, so the bits are the same.
 // ---------------------------------------------------------------------------
+constexpr int PROPOSE_LANES = 4;
+
 template <int LMAX>
 __global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __restrict__ config,
                                                  double* __restrict__ prop,
                                                  const ChainDev* __restrict__ st, int q_lo,
                                                  int q_n) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
-  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (qi >= q_n) return;
-  const int q = q_lo + qi;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int qi = t / PROPOSE_LANES, part = t % PROPOSE_LANES;
+  const bool valid = qi < q_n;
+  const int q = q_lo + (valid ? qi : q_n - 1);  // every lane takes part in the exchange
   const int s = T.color_sites[q];
+  const uint64_t seed = st->seed;
+  const int32_t sweep = (int32_t)(st->sweep_index + 1);
   const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
-  double e2[3], u, z[NLM];
-  sce::keyed_proposal(st->seed, s + 1, (int32_t)(st->sweep_index + 1), e, st->step, e2, &u);
+  // this lane's half of a Box-Muller pair: the radius (even part) or the angle
+  const sce::u32x4 b = sce::philox_block(seed, s + 1, sweep, 1u + (uint32_t)(part >> 1), 0u);
+  double x0 = 0.0, x1 = 0.0;
+  if ((part & 1) == 0)
+    x0 = __builtin_sqrt(-2.0 * sce::det_log(sce::uniform01(b.x, b.y)));
+  else
+    sce::det_sincos(6.283185307179586 * sce::uniform01(b.z, b.w), &x0, &x1);
+  const int l0 = (int)(threadIdx.x % 64) & ~(PROPOSE_LANES - 1);
+  const double r1 = __shfl(x0, l0), s1 = __shfl(x0, l0 + 1), c1 = __shfl(x1, l0 + 1);
+  const double r2 = __shfl(x0, l0 + 2), s2 = __shfl(x0, l0 + 3), c2 = __shfl(x1, l0 + 3);
+  if (!valid || part != 0) return;
+  const sce::u32x4 b0 = sce::philox_block(seed, s + 1, sweep, 0u, 0u);
+  const double u_flip = sce::uniform01(b0.x, b0.y);
+  const double u = sce::uniform01(b0.z, b0.w);
+  double e2[3], z[NLM];
+  if (u_flip < 0.2) {
+    e2[0] = -e[0];
+    e2[1] = -e[1];
+    e2[2] = -e[2];
+  } else {
+    const double n1 = r1 * c1, n2 = r1 * s1, n3 = r2 * c2, n4 = r2 * s2;
+    const double nn = __builtin_sqrt((n1 * n1 + n2 * n2) + n3 * n3);
+    const double ax[3] = {n1 / nn, n2 / nn, n3 / nn};
+    sce::rotate3(e, ax, st->step * n4, e2);
+  }
   sce::zlm_row<LMAX>(e2[0], e2[1], e2[2], z);
   double* o = prop + (size_t)q * PS;
   o[0] = e2[0];
@@ -1248,7 +1280,7 @@ static int32_t check_lds(const DevTables& D, int L) {
 template <int LMAX>
 static void launch_propose(const DevTables& D, const double* config, double* prop,
                            const ChainDev* st, hipStream_t s) {
-  const int nb = (D.n_active + 255) / 256;
+  const int nb = (int)(((long long)D.n_active * PROPOSE_LANES + 255) / 256);
   if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st, 0, D.n_active);
 }
 
