@@ -156,6 +156,19 @@ int32_t sce_chain_set_persistent(sce_chain* c, int32_t on);
 /* Replay each sweep of sce_run_sweeps as a captured hipGraph (on != 0, the
  * default) or launch its kernels directly; results are identical. */
 int32_t sce_chain_set_graph(sce_chain* c, int32_t on);
+/* Streaming error analysis on the device, fed by every measurement row of
+ * later sce_run_sweeps / sce_da_run_sweeps calls: the LogBinner of
+ * src/binning.jl:25-79 and the BinStore of :138-188 over the five series
+ * E, E^2, |m|, m^2, m^4 (m^2 = (m_x^2 + m_y^2) + m_z^2).  reset zeroes the
+ * state and turns it on (bin_size >= 1 and 2 <= nbins <= 1024, or both 0 for
+ * the LogBinner alone); off turns it off. */
+int32_t sce_chain_binner_reset(sce_chain* c, int32_t bin_size, int32_t nbins);
+int32_t sce_chain_binner_off(sce_chain* c);
+/* The binner state: n pushes; count[62]; sums[62][5], sums2[62][5] (row
+ * major); nfull completed BinStore bins and their means[nfull][5] (means_cap
+ * rows available). */
+int32_t sce_chain_binner_get(sce_chain* c, int64_t* n, int64_t* count, double* sums, double* sums2,
+                             int32_t* nfull, double* means, int32_t means_cap);
 /* _renormalize!(st, H) on the device: normalise active spins, rebuild rows,
  * recompute the FP64 total energy, re-anchor.  Writes the drift. */
 int32_t sce_renormalize(sce_chain* c, double* drift);

@@ -520,6 +520,69 @@ __device__ double block_tree(double x0, double x1, double* sh) {
 }
 
 // ---------------------------------------------------------------------------
+// Streaming error analysis on the device: the LogBinner (src/binning.jl:25-79)
+// and BinStore (:138-188) of the five standard series of a measurement row
+// (E, m): E, E^2, |m|, m^2, m^4 with m^2 = (m_x^2 + m_y^2) + m_z^2, |m| its
+// square root, m^4 = m^2 m^2.  The measurement's final reduction thread pushes
+// each row, so a run's statistics need no download of the rows.
+// ---------------------------------------------------------------------------
+constexpr int BIN_COMP = 5, BIN_LEVELS = 62, BIN_MAXSTORE = 1024;
+
+struct DevBinner {
+  long long n;
+  long long count[BIN_LEVELS];
+  int pending_full[BIN_LEVELS];
+  double sums[BIN_LEVELS][BIN_COMP], sums2[BIN_LEVELS][BIN_COMP], pending[BIN_LEVELS][BIN_COMP];
+  int bin_size, nbins, nfull, nacc;
+  double acc[BIN_COMP];
+  double means[BIN_MAXSTORE][BIN_COMP];
+};
+
+__device__ void binner_push(DevBinner* b, double E, double mx, double my, double mz) {
+  const double m2 = (mx * mx + my * my) + mz * mz;
+  const double x[BIN_COMP] = {E, E * E, __builtin_sqrt(m2), m2, m2 * m2};
+  double cur[BIN_COMP];
+#pragma unroll
+  for (int j = 0; j < BIN_COMP; ++j) cur[j] = x[j];
+  b->n += 1;
+  for (int k = 0; k < BIN_LEVELS; ++k) {
+    b->count[k] += 1;
+#pragma unroll
+    for (int j = 0; j < BIN_COMP; ++j) {
+      b->sums[k][j] = b->sums[k][j] + cur[j];
+      b->sums2[k][j] = b->sums2[k][j] + cur[j] * cur[j];
+    }
+    if (b->pending_full[k]) {
+#pragma unroll
+      for (int j = 0; j < BIN_COMP; ++j) {
+        const double pj = 0.5 * (b->pending[k][j] + cur[j]);
+        b->pending[k][j] = pj;
+        cur[j] = pj;
+      }
+      b->pending_full[k] = 0;
+    } else {
+#pragma unroll
+      for (int j = 0; j < BIN_COMP; ++j) b->pending[k][j] = cur[j];
+      b->pending_full[k] = 1;
+      break;
+    }
+  }
+  if (b->bin_size > 0 && b->nfull < b->nbins) {
+#pragma unroll
+    for (int j = 0; j < BIN_COMP; ++j) b->acc[j] = b->acc[j] + x[j];
+    if (++b->nacc == b->bin_size) {
+#pragma unroll
+      for (int j = 0; j < BIN_COMP; ++j) {
+        b->means[b->nfull][j] = b->acc[j] / (double)b->bin_size;
+        b->acc[j] = 0.0;
+      }
+      b->nfull += 1;
+      b->nacc = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // fence-free two-kernel reduction: k_chunks writes the 512-element chunk trees
 // (and integer counts, and optionally the three mean-spin components); one
 // block of k_final folds them (the kernel boundary makes the partials visible
@@ -591,7 +654,7 @@ __device__ double final_tree(const double* buf, long long m, double* scratch, do
 template <int MODE>
 __device__ void final_body(double* part, long long n, const long long* ipart, long long n_sites,
                            int meas, ChainDev* st, double* __restrict__ out, long long n_att,
-                           double n_div, double* scratch) {
+                           double n_div, double* scratch, DevBinner* bin = nullptr) {
   __shared__ double sh[256];
   const int t = threadIdx.x;
   const long long nbx = (n + TREE_B - 1) / TREE_B > 0 ? (n + TREE_B - 1) / TREE_B : 1;
@@ -679,11 +742,13 @@ __device__ void final_body(double* part, long long n, const long long* ipart, lo
   }
   if (meas) {
     const int row = st->n_obs;
-    out[4 * (size_t)row] = st->energy;
-    out[4 * (size_t)row + 1] = m3[0] / n_div;
-    out[4 * (size_t)row + 2] = m3[1] / n_div;
-    out[4 * (size_t)row + 3] = m3[2] / n_div;
+    const double E = st->energy, mx = m3[0] / n_div, my = m3[1] / n_div, mz = m3[2] / n_div;
+    out[4 * (size_t)row] = E;
+    out[4 * (size_t)row + 1] = mx;
+    out[4 * (size_t)row + 2] = my;
+    out[4 * (size_t)row + 3] = mz;
     st->n_obs = row + 1;
+    if (bin) binner_push(bin, E, mx, my, mz);
   }
 }
 
@@ -691,8 +756,8 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_final(double* part, long long n, const long long* ipart,
                                                long long n_sites, int meas, ChainDev* st,
                                                double* __restrict__ out, long long n_att,
-                                               double n_div, double* scratch) {
-  final_body<MODE>(part, n, ipart, n_sites, meas, st, out, n_att, n_div, scratch);
+                                               double n_div, double* scratch, DevBinner* bin) {
+  final_body<MODE>(part, n, ipart, n_sites, meas, st, out, n_att, n_div, scratch, bin);
 }
 
 // ---------------------------------------------------------------------------
@@ -1079,6 +1144,8 @@ struct sce_chain {
   long long* ip64 = nullptr;  // integer partials of the reductions
   bool persistent = false;  // run sweeps in one cooperative launch (opt-in)
   bool use_graph = true;    // replay captured sweeps as hipGraphs
+  DevBinner* bin = nullptr;  // device binner fed by every measurement (null: off)
+  DevBinner* bin_mem = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // [plain, measured] sweep
   int g_lanes = 0;
   size_t obs_cap = 0;
@@ -1352,13 +1419,15 @@ template <int MODE>
 static void launch_reduce(const double* x, long long n, const int* iacc, const int* iatt,
                           double* part, long long* ipart, const double* config,
                           const uint8_t* active, long long n_sites, int meas, ChainDev* st,
-                          double* out, long long n_att, double n_div, hipStream_t s) {
+                          double* out, long long n_att, double n_div, hipStream_t s,
+                          DevBinner* bin = nullptr) {
   const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1);
   const long long nchm = meas ? (n_sites + TREE_B - 1) / TREE_B : 0;
   const long long grid = std::max(nbx, nchm);
   double* scratch = part + nbx + 3 * std::max<long long>(nchm, 1);
   k_chunks<MODE><<<(unsigned)grid, 256, 0, s>>>(x, n, iacc, iatt, part, ipart, config, active, n_sites, meas);
-  k_final<MODE><<<1, 256, 0, s>>>(part, n, ipart, n_sites, meas, st, out, n_att, n_div, scratch);
+  k_final<MODE><<<1, 256, 0, s>>>(part, n, ipart, n_sites, meas, st, out, n_att, n_div, scratch,
+                                   meas ? bin : nullptr);
 }
 
 static int32_t chain_check(sce_chain* c) {
@@ -1407,7 +1476,7 @@ static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0) {
                       e0, e1);
   }
   launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config,
-                            D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s);
+                            D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s, c->bin);
   c->sweeps_done += 1;
   return SCE_OK;
 }
@@ -1447,7 +1516,7 @@ static void enqueue_renorm(sce_chain* c) {
 static void enqueue_measure(sce_chain* c) {
   const DevTables& D = c->t->dev;
   launch_reduce<MODE_MEASURE>(c->dE, 0, nullptr, nullptr, c->partial, c->ip64, c->config, D.active,
-                              D.n_sites, 1, c->st, c->obs, 0, (double)D.n_active, c->stream);
+                              D.n_sites, 1, c->st, c->obs, 0, (double)D.n_active, c->stream, c->bin);
 }
 
 // Capture one Metropolis sweep (plain and with the fused measurement) as
@@ -1741,6 +1810,62 @@ extern "C" int32_t sce_chain_set_persistent(sce_chain* c, int32_t on) {
   return SCE_OK;
 }
 
+extern "C" int32_t sce_chain_binner_reset(sce_chain* c, int32_t bin_size, int32_t nbins) {
+  CHK(chain_check(c));
+  if (bin_size < 0 || nbins < 0 || nbins > BIN_MAXSTORE || (bin_size > 0) != (nbins > 0) || nbins == 1)
+    return sce_fail(SCE_EINVAL, "binner: need bin_size >= 1 and 2 <= nbins <= %d, or both 0", BIN_MAXSTORE);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->bin_mem) CHK(dalloc(c->mem, &c->bin_mem, 1));
+  DevBinner* h = new DevBinner();
+  std::memset(h, 0, sizeof *h);
+  h->bin_size = bin_size;
+  h->nbins = nbins;
+  const hipError_t e = hipMemcpy(c->bin_mem, h, sizeof *h, hipMemcpyHostToDevice);
+  delete h;
+  if (e != hipSuccess) return sce_fail(SCE_EHIP, "binner upload failed");
+  if (!c->bin) c->drop_graphs();  // captured sweeps carry the binner pointer
+  c->bin = c->bin_mem;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_binner_off(sce_chain* c) {
+  CHK(chain_check(c));
+  if (c->bin) c->drop_graphs();
+  c->bin = nullptr;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_chain_binner_get(sce_chain* c, int64_t* n, int64_t* count, double* sums, double* sums2,
+                                        int32_t* nfull, double* means, int32_t means_cap) {
+  CHK(chain_check(c));
+  if (!c->bin_mem) return sce_fail(SCE_EINVAL, "binner: never reset on this chain");
+  if (!n || !count || !sums || !sums2 || !nfull) return sce_fail(SCE_EINVAL, "binner: NULL output");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  DevBinner* h = new DevBinner();
+  const hipError_t e = hipMemcpy(h, c->bin_mem, sizeof *h, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    delete h;
+    return sce_fail(SCE_EHIP, "binner download failed");
+  }
+  if (h->nfull > 0 && (!means || means_cap < h->nfull)) {
+    delete h;
+    return sce_fail(SCE_EDIM, "binner: means buffer holds %d bins; %d needed", means_cap, h->nfull);
+  }
+  *n = h->n;
+  *nfull = h->nfull;
+  for (int k = 0; k < BIN_LEVELS; ++k) {
+    count[k] = h->count[k];
+    for (int j = 0; j < BIN_COMP; ++j) {
+      sums[k * BIN_COMP + j] = h->sums[k][j];
+      sums2[k * BIN_COMP + j] = h->sums2[k][j];
+    }
+  }
+  for (int i = 0; i < h->nfull; ++i)
+    for (int j = 0; j < BIN_COMP; ++j) means[i * BIN_COMP + j] = h->means[i][j];
+  delete h;
+  return SCE_OK;
+}
+
 extern "C" int32_t sce_chain_set_graph(sce_chain* c, int32_t on) {
   CHK(chain_check(c));
   c->use_graph = on != 0;
@@ -1754,6 +1879,8 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   CHK(check_lanes(lanes));
   if (nsweeps < 0) return sce_fail(SCE_EINVAL, "nsweeps must be >= 0");
   if (or_per_metropolis < 0) return sce_fail(SCE_EINVAL, "or_per_metropolis must be >= 0");
+  if (c->bin && c->persistent && or_per_metropolis == 0 && measure_interval > 0)
+    return sce_fail(SCE_EINVAL, "the device binner is fed by per-colour launches; disable persistent sweeps");
   const int64_t want_obs = measure_interval > 0 ? nsweeps / measure_interval : 0;
   if (want_obs > 0 && (!obs || obs_cap < want_obs))
     return sce_fail(SCE_EDIM, "obs buffer holds %lld rows; %lld needed", (long long)obs_cap, (long long)want_obs);
@@ -3673,7 +3800,7 @@ static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas) {
     SCE_DISPATCH_LMAX(D.lmax, launch_da, D, c, s->d_nets, lo, n, c->stream);
   }
   launch_reduce<MODE_DA>(c->dE, D.n_active, c->acc, c->att, c->partial, c->ip64, c->config, D.active, D.n_sites,
-                         meas, c->st, c->obs, D.n_active, (double)D.n_active, c->stream);
+                         meas, c->st, c->obs, D.n_active, (double)D.n_active, c->stream, c->bin);
   c->sweeps_done += 1;
   return SCE_OK;
 }

@@ -135,6 +135,45 @@ class GPUChainState:
         """Persistent cooperative launches (opt-in) or per-colour launches (default)."""
         L.check(L.lib().sce_chain_set_persistent(self._h, 1 if on else 0))
 
+    def binner_reset(self, bin_size: int = 0, nbins: int = 0) -> None:
+        """Zero the device binner and feed it every later measurement row
+        (LogBinner of the five standard series; a BinStore of `nbins` bins of
+        `bin_size` rows when both are given)."""
+        L.check(L.lib().sce_chain_binner_reset(self._h, int(bin_size), int(nbins)))
+
+    def binner_off(self) -> None:
+        L.check(L.lib().sce_chain_binner_off(self._h))
+
+    def binner_state(self):
+        """(LogBinner per series, BinStore means per series) as the device holds
+        them, in the host classes of ``observables``."""
+        from .observables import SERIES, LogBinner, MAX_LEVELS
+        n = C.c_int64()
+        nfull = C.c_int32()
+        count = np.zeros(MAX_LEVELS, np.int64)
+        sums = np.zeros((MAX_LEVELS, len(SERIES)))
+        sums2 = np.zeros((MAX_LEVELS, len(SERIES)))
+        means = np.zeros((1024, len(SERIES)))
+        L.check(L.lib().sce_chain_binner_get(self._h, C.byref(n), L.ptr(count), L.ptr(sums), L.ptr(sums2),
+                                             C.byref(nfull), L.ptr(means), means.shape[0]))
+        binners, bin_means = {}, {}
+        for j, name in enumerate(SERIES):
+            b = LogBinner(1)
+            b.n = n.value
+            b.count[:] = count
+            b.sums[:, 0] = sums[:, j]
+            b.sums2[:, 0] = sums2[:, j]
+            binners[name] = b
+            bin_means[name] = means[: nfull.value, j].copy()
+        return binners, bin_means
+
+    def device_stats(self, kT: float):
+        """The standard observables from the device binner (no measurement rows
+        are downloaded)."""
+        from .observables import stats_from_binners
+        binners, means = self.binner_state()
+        return stats_from_binners(binners, means, kT, self.gH.host.n_active)
+
     def set_graph(self, on: bool) -> None:
         """Replay sweeps as hipGraphs (default) or launch their kernels directly."""
         L.check(L.lib().sce_chain_set_graph(self._h, 1 if on else 0))

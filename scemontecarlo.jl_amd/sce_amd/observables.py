@@ -142,40 +142,65 @@ class ObservableStat:
     count: int
 
 
-def standard_stats(obs: np.ndarray, kT: float, n_active: int, nbins: int = 32) -> Dict[str, ObservableStat]:
-    """Finalise the standard observables (observables.jl:85-98, 160-166, 207-249)
-    from device measurement rows (E, m_x, m_y, m_z)."""
+SERIES = ("energy", "energy2", "absm", "m2", "m4")
+
+
+def standard_series(obs: np.ndarray) -> Dict[str, np.ndarray]:
+    """The five series binned per measurement row (E, m_x, m_y, m_z), with the
+    device binner's operation order: m^2 = (m_x^2 + m_y^2) + m_z^2."""
     obs = np.asarray(obs, np.float64)
     E = obs[:, 0]
-    m = obs[:, 1:4]
-    absm = np.linalg.norm(m, axis=1)
-    m2 = np.sum(m * m, axis=1)
-    m4 = m2 * m2
-    series = {"energy": E, "energy2": E * E, "absm": absm, "m2": m2, "m4": m4}
+    mx, my, mz = obs[:, 1], obs[:, 2], obs[:, 3]
+    m2 = (mx * mx + my * my) + mz * mz
+    return {"energy": E, "energy2": E * E, "absm": np.sqrt(m2), "m2": m2, "m4": m2 * m2}
+
+
+def stats_from_binners(binners: Dict[str, LogBinner], bin_means: Dict[str, np.ndarray], kT: float,
+                       n_active: int) -> Dict[str, ObservableStat]:
+    """Finalise the standard observables (observables.jl:85-98, 160-166,
+    207-249) from the LogBinner and BinStore means of the five series."""
     stats: Dict[str, ObservableStat] = {}
-    bin_size = max(1, len(E) // nbins)
-    stores = {}
-    for name, x in series.items():
-        b = LogBinner(1).push_series(x)
+    for name in SERIES:
+        b = binners[name]
         stats[name] = ObservableStat(name, float(b.mean()[0]), float(b.std_error()[0]),
                                      float(b.tau_int()[0]), b.n)
-        s = BinStore(1, bin_size, nbins)
-        for v in x:
-            s.push(v)
-        stores[name] = s.bin_means()[:, 0]
     evals = {
         "specific_heat": (("energy", "energy2"), lambda e, e2: (e2 - e * e) / (n_active * kT * kT)),
         "susceptibility": (("m2", "absm"), lambda q2, a: n_active * (q2 - a * a) / kT),
         "binder": (("m2", "m4"), lambda q2, q4: q4 / (q2 * q2)),
     }
     for name, (inputs, f) in evals.items():
-        cols = [stores[i] for i in inputs]
+        cols = [bin_means[i] for i in inputs]
         if len(cols[0]) < 2:
             stats[name] = ObservableStat(name, math.nan, math.nan, math.nan, 0)
             continue
         est, err = jackknife(f, cols)
         stats[name] = ObservableStat(name, est, err, math.nan, len(cols[0]))
     return stats
+
+
+def host_binners(obs: np.ndarray, nbins: int = 32):
+    """(LogBinner per series, BinStore means per series) of measurement rows,
+    BinStore bins of max(1, n // nbins) rows."""
+    series = standard_series(obs)
+    n = len(series["energy"])
+    bin_size = max(1, n // nbins)
+    binners, means = {}, {}
+    for name in SERIES:
+        x = series[name]
+        binners[name] = LogBinner(1).push_series(x)
+        s = BinStore(1, bin_size, nbins)
+        for v in x:
+            s.push(v)
+        means[name] = s.bin_means()[:, 0]
+    return binners, means
+
+
+def standard_stats(obs: np.ndarray, kT: float, n_active: int, nbins: int = 32) -> Dict[str, ObservableStat]:
+    """Finalise the standard observables (observables.jl:85-98, 160-166, 207-249)
+    from device measurement rows (E, m_x, m_y, m_z)."""
+    binners, means = host_binners(obs, nbins)
+    return stats_from_binners(binners, means, kT, n_active)
 
 
 def effective_samples(series: np.ndarray) -> float:
