@@ -277,6 +277,24 @@ int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta, int64_t n
                           int64_t renorm_interval, int64_t measure_interval, double* obs,
                           int64_t obs_cap, int64_t* n_obs);
 
+/* ---- all-site energy gradient ----------------------------------------------
+ * gpu_energy_gradient!(dG, gH, dconfig, gsc; workgroupsize = lanes,
+ * refresh_zrows = true) — src/gpu/gpu_gradient.jl:187-212: the tangent-projected
+ * dE/de_s of every site (3 x n_sites, model units, intercept excluded; inactive
+ * sites exactly zero) for the configuration `config` (3 x n_sites, unit spins),
+ * whose rows are rebuilt on the device.  The arithmetic is the reference's lane
+ * contract (_gradient_lane_ref!, :242-279) with `lanes` strided shares
+ * (64, 128 or 256; the reference's default is 128), so the result is the same
+ * bits on any backend. */
+int32_t sce_energy_gradient(sce_tables* t, const double* config, double* G, int32_t lanes);
+/* gpu_energy_gradient!(dG, gst, gH, gsc) — :222-226: the same on the chain's
+ * device spins and rows (no rebuild: the rows are Z(config) by the sweep
+ * invariant). */
+int32_t sce_chain_energy_gradient(sce_chain* c, double* G, int32_t lanes);
+/* Average gradient-kernel duration over reps back-to-back launches on the
+ * chain's rows (measurement; no reference counterpart). */
+int32_t sce_chain_time_gradient(sce_chain* c, int32_t lanes, int32_t reps, double* ms_avg);
+
 /* ---- diagnostics (no reference counterpart): the device arithmetic exposed
  * for bit-exact parity tests ------------------------------------------------- */
 int32_t sce_test_philox(int64_t n, const uint64_t* seeds, const uint32_t* ctr4,

@@ -52,6 +52,8 @@ def lib():
             "ora_log": (d, [d]),
             "ora_sincos": (None, [d, P, P]),
             "ora_zlm_row": (None, [P, C.c_int, P]),
+            "ora_grad_zlm_row": (None, [P, C.c_int, P]),
+            "ora_energy_gradient": (None, [P, P, P, C.c_int, P]),
             "ora_tiled_build": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P,
                                           P, C.c_char_p, C.c_size_t]),
             "ora_tiled_free": (None, [P]),
@@ -145,6 +147,14 @@ def zlm_row(u, lmax):
     return z
 
 
+def grad_zlm_row(u, lmax):
+    """Tangent-projected gradients of the Z row at u: nlm x 3."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    g = np.zeros((lmax + 1) ** 2 * 3)
+    lib().ora_grad_zlm_row(_p(u), lmax, _p(g))
+    return g.reshape(-1, 3)
+
+
 def reduce_tree(x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     return lib().ora_reduce_tree(len(x), _p(x))
@@ -229,6 +239,15 @@ class Tiled:
     def total_energy_tree(self, zrows):
         z = np.ascontiguousarray(zrows, np.float64)
         return lib().ora_total_energy_tree(self._h, _p(z))
+
+    def energy_gradient(self, config, zrows=None, lanes=64):
+        """All-site tangent-projected gradient dE/de_s (n_sites x 3), the
+        reference's lane definition with `lanes` strided shares."""
+        cfg = np.ascontiguousarray(config, np.float64)
+        z = np.ascontiguousarray(self.zrows(cfg) if zrows is None else zrows, np.float64)
+        G = np.zeros((self.n_sites, 3))
+        lib().ora_energy_gradient(self._h, _p(cfg), _p(z), int(lanes), _p(G))
+        return G
 
     def site_coeffs(self, s, zrows):
         z = np.ascontiguousarray(zrows, np.float64)

@@ -260,6 +260,55 @@ void ora_zlm_row(const double u[3], int lmax, double* z) { /* :254-263 */
     for (int m = -l; m <= l; ++m) z[i++] = zlm_one(l, m, u, cache);
 }
 
+/* The tangent-projected gradient of one tesseral harmonic, grad Z_lm(u)
+ * (gpu/grad_device.jl:30-66, the device twin of Harmonics.grad_Zlm_unsafe):
+ * plm and dplm are the order-n and order-(n+1) derivatives of P_l at u_z with
+ * the parity and normalisation factors (the order-(n+1) derivative is a +0.0
+ * literal when n + 1 > l), then the Cartesian derivative of
+ * c P_l^(n)(z) Re/Im (x + iy)^n with the radial part u (u . dZ) / |u|^2
+ * removed.  Only + - * / and a correctly rounded sqrt. */
+static void grad_zlm_one(int l, int m, const double u[3], double* cache, double g[3]) {
+  const double x = u[0], y = u[1], z = u[2];
+  const int n = m < 0 ? -m : m;
+  const double par = (n & 1) ? -1.0 : 1.0;
+  const double plm = par * zlm_plm_norm(l, n) * zlm_dnpl(z, l, n, cache);
+  const double d1 = l < n + 1 ? 0.0 : zlm_dnpl(z, l, n + 1, cache);
+  const double dplm = par * zlm_plm_norm(l, n) * d1;
+  const double r2 = x * x + y * y + z * z;
+  if (m == 0) {
+    const double zz = z * dplm / r2;
+    g[0] = -x * zz;
+    g[1] = -y * zz;
+    g[2] = dplm - z * zz;
+    return;
+  }
+  const double c = par * sqrt(2.0);
+  cplx xy = {x, y};
+  const cplx pn = zlm_cpow(xy, n), pn1 = zlm_cpow(xy, n - 1);
+  double dx, dy, dz;
+  if (m > 0) {
+    dx = c * (double)n * plm * pn1.re;
+    dy = -c * (double)n * plm * pn1.im;
+    dz = c * dplm * pn.re;
+  } else {
+    dx = c * (double)n * plm * pn1.im;
+    dy = c * (double)n * plm * pn1.re;
+    dz = c * dplm * pn.im;
+  }
+  const double zz = (x * dx + y * dy + z * dz) / r2;
+  g[0] = dx - x * zz;
+  g[1] = dy - y * zz;
+  g[2] = dz - z * zz;
+}
+
+/* grad Z rows of u in lm order, component fastest: grow[3k + d]. */
+void ora_grad_zlm_row(const double u[3], int lmax, double* grow) {
+  double cache[8];
+  int i = 0;
+  for (int l = 0; l <= lmax; ++l)
+    for (int m = -l; m <= l; ++m, ++i) grad_zlm_one(l, m, u, cache, grow + 3 * i);
+}
+
 /* ======================================================================
  * Tiled Hamiltonian — src/hamiltonian.jl:99-181 (programs), :267-393 (ctor),
  * :437-448 (Zeeman templates), :456-499 (greedy colouring), :520-524 (site id).
@@ -696,6 +745,50 @@ void ora_site_coeffs(const ora_tiled* t, int s, const double* zrows, double* c) 
       c[t->sent_tgt.a[e - 1] - 1] += t->sent_w.a[e - 1] * p;
     }
   }
+}
+
+/* The all-site energy gradient G[s] = dE/de_s, tangent-projected (the
+ * reference's _gradient_lane_ref!, gpu/gpu_gradient.jl:242-279, with
+ * _entry_walk_grad :92-126): the adjacency programs of site s are dealt to ws
+ * lanes (lane l takes j = site_ptr[s] + l, + ws, ...); within a program each
+ * entry with a nonzero product p (the sweep's skips) adds (w p) grad Z_tgt(e_s)
+ * to the lane's 3-vector, w p formed first, then one multiply and one add per
+ * component; the lane partials are added in lane order.  config: 3 x n_sites;
+ * zrows the rows of config; G: 3 x n_sites. */
+void ora_energy_gradient(const ora_tiled* t, const double* config, const double* zrows, int ws, double* G) {
+  double grow[3 * 49];
+  double* part = malloc((size_t)ws * 3 * sizeof(double));
+  for (int s = 1; s <= t->n_sites; ++s) {
+    ora_grad_zlm_row(config + 3 * (size_t)(s - 1), t->lmax, grow);
+    for (int lane = 0; lane < ws; ++lane) {
+      double a[3] = {0.0, 0.0, 0.0};
+      for (int j = t->site_ptr.a[s - 1] + lane; j < t->site_ptr.a[s]; j += ws) {
+        int pid = t->site_prog.a[j - 1];
+        for (int e = t->sprog_ptr.a[pid - 1]; e < t->sprog_ptr.a[pid]; ++e) {
+          double p = entry_product(t, j, e, zrows);
+          if (p == 0.0) continue;
+          int k = t->sent_tgt.a[e - 1] - 1;
+          double w = t->sent_w.a[e - 1] * p;
+          a[0] = a[0] + w * grow[3 * k];
+          a[1] = a[1] + w * grow[3 * k + 1];
+          a[2] = a[2] + w * grow[3 * k + 2];
+        }
+      }
+      part[3 * lane] = a[0];
+      part[3 * lane + 1] = a[1];
+      part[3 * lane + 2] = a[2];
+    }
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int lane = 0; lane < ws; ++lane) {
+      gx += part[3 * lane];
+      gy += part[3 * lane + 1];
+      gz += part[3 * lane + 2];
+    }
+    G[3 * (size_t)(s - 1)] = gx;
+    G[3 * (size_t)(s - 1) + 1] = gy;
+    G[3 * (size_t)(s - 1) + 2] = gz;
+  }
+  free(part);
 }
 
 double ora_delta_energy(int nlm, const double* c, const double* zold, const double* znew) {

@@ -35,6 +35,7 @@ void ora_sincos(double x, double* s, double* c);
 
 /* ---- tesseral row (src/gpu/zlm_device.jl) ---- */
 void ora_zlm_row(const double u[3], int lmax, double* z);
+void ora_grad_zlm_row(const double u[3], int lmax, double* grow);
 
 /* ---- tiled Hamiltonian (src/hamiltonian.jl) ---- */
 typedef struct ora_tiled ora_tiled;
@@ -62,6 +63,7 @@ void ora_zrows(const ora_tiled* t, const double* config, double* zrows);
 double ora_total_energy(const ora_tiled* t, const double* zrows);
 double ora_total_energy_tree(const ora_tiled* t, const double* zrows);
 void ora_site_coeffs(const ora_tiled* t, int s, const double* zrows, double* c);
+void ora_energy_gradient(const ora_tiled* t, const double* config, const double* zrows, int ws, double* G);
 double ora_delta_energy(int nlm, const double* c, const double* zold, const double* znew);
 double ora_reduce_tree(int64_t n, const double* x);
 

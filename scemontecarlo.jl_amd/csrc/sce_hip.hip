@@ -108,6 +108,18 @@ struct DevTables {
   int nfeat_max;
 };
 
+// The reference's adjacency programs, 1-based as the TiledHamiltonian holds
+// them (src/hamiltonian.jl site programs), for the all-site energy gradient,
+// whose arithmetic contract deals whole programs to lanes.
+struct GradTables {
+  int n_sites, nlm;
+  const int32_t *site_ptr, *site_inst, *site_prog, *site_col, *site_col2;
+  const int32_t *sprog_ptr, *sent_tgt, *sfac_ptr, *sfac_row, *pent_row, *pent_row2;
+  const int32_t *inst_ptr, *inst_sites;
+  const int8_t* sfac_slot;
+  const double* sent_w;
+};
+
 constexpr int TPB = 256;      // threads per block of the walk kernels
 constexpr int TREE_B = 512;   // elements per tree chunk
 
@@ -1002,6 +1014,94 @@ __global__ void k_renorm(DevTables T, double* __restrict__ config, double* __res
   for (int k = 0; k < NLM; ++k) zrows[(size_t)s * NLM + k] = z[k];
 }
 
+// Tesseral rows of every site from a configuration (3 x n_sites), one lane
+// per site (gpu_gradient.jl:41-51: the rows the gradient reads).
+template <int LMAX>
+__global__ void k_rows_of(int n_sites, const double* __restrict__ config, double* __restrict__ zrows) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sites) return;
+  double z[NLM];
+  sce::zlm_row<LMAX>(config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2], z);
+#pragma unroll
+  for (int k = 0; k < NLM; ++k) zrows[(size_t)s * NLM + k] = z[k];
+}
+
+// The all-site energy gradient G[s] = dE/de_s, tangent-projected — the
+// arithmetic contract of the reference's _grad_kernel! / _gradient_lane_ref!
+// (gpu/gpu_gradient.jl:92-164, 242-279): one wavefront per site; VL strided
+// shares (the reference's workgroup size), share v taking adjacency programs
+// j = site_ptr[s] + v, + VL, ... and, per entry with a nonzero product p (the
+// sweep's skips), adding (w p) grad Z_tgt(e_s) — w p formed first, then one
+// multiply and one add per component; the shares are added in share order.
+// Lane k < nlm computes grad Z_k(e_s) into LDS (each element is independent of
+// the others).  Writes G[s] only; reads config and rows.
+template <int LMAX, int VL>
+__global__ __launch_bounds__(TPB) void k_grad(GradTables T, const double* __restrict__ config,
+                                              const double* __restrict__ zrows, double* __restrict__ G) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), W = TPB / 64, K = VL / 64;
+  __shared__ double grow[W][3 * NLM];
+  __shared__ double part[W][3 * VL];
+  const int w = threadIdx.x / 64, h = threadIdx.x % 64;
+  const int s = blockIdx.x * W + w;  // 0-based site
+  const bool valid = s < T.n_sites;
+  if (valid && h < NLM) {
+    int l, m;
+    sce::lm_of(h, &l, &m);
+    double g[3];
+    sce::grad_zlm_one<LMAX>(l, m, config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2], g);
+    grow[w][3 * h] = g[0];
+    grow[w][3 * h + 1] = g[1];
+    grow[w][3 * h + 2] = g[2];
+  }
+  __syncthreads();
+  if (valid) {
+    const int j0 = T.site_ptr[s], j1 = T.site_ptr[s + 1];  // 1-based [j0, j1)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int v = h + 64 * k;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      for (int j = j0 + v; j < j1; j += VL) {
+        const int pid = T.site_prog[j - 1], col = T.site_col[j - 1];
+        const int e0 = T.sprog_ptr[pid - 1], e1 = T.sprog_ptr[pid];
+        const int col2 = col < 0 ? T.site_col2[j - 1] : 0;
+        const int off = col == 0 ? T.inst_ptr[T.site_inst[j - 1] - 1] - 1 : 0;
+        for (int e = e0; e < e1; ++e) {
+          double p;
+          if (col > 0) {
+            p = zrows[(size_t)(col - 1) * T.nlm + (T.pent_row[e - 1] - 1)];
+          } else if (col < 0) {
+            p = zrows[(size_t)(-col - 1) * T.nlm + (T.pent_row[e - 1] - 1)] *
+                zrows[(size_t)(col2 - 1) * T.nlm + (T.pent_row2[e - 1] - 1)];
+          } else {
+            p = 1.0;
+            for (int f = T.sfac_ptr[e - 1]; f < T.sfac_ptr[e]; ++f) {
+              const int mm = T.inst_sites[off + T.sfac_slot[f - 1] - 1];
+              p *= zrows[(size_t)(mm - 1) * T.nlm + (T.sfac_row[f - 1] - 1)];
+            }
+          }
+          if (p == 0.0) continue;
+          const int t = T.sent_tgt[e - 1] - 1;
+          const double wp = T.sent_w[e - 1] * p;
+          a0 = a0 + wp * grow[w][3 * t];
+          a1 = a1 + wp * grow[w][3 * t + 1];
+          a2 = a2 + wp * grow[w][3 * t + 2];
+        }
+      }
+      part[w][3 * v] = a0;
+      part[w][3 * v + 1] = a1;
+      part[w][3 * v + 2] = a2;
+    }
+  }
+  __syncthreads();
+  if (valid && h < 3) {
+    // component h of the share-ordered fold
+    double g = 0.0;
+    for (int v = 0; v < VL; ++v) g += part[w][3 * v + h];
+    G[3 * (size_t)s + h] = g;
+  }
+}
+
 __global__ void k_inst_energy(DevTables T, const double* __restrict__ zrows,
                               double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1120,6 +1220,15 @@ static int32_t dupload(DevBuf& b, T** p, const T* src, size_t n) {
   return SCE_OK;
 }
 
+// As dupload, into an allocation of at least one element (a valid pointer for
+// empty tables).
+template <class T>
+static int32_t dupload1(DevBuf& b, T** p, const T* src, size_t n) {
+  CHK(dalloc(b, p, std::max<size_t>(n, 1)));
+  if (n) HIPCHK(hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
 struct sce_tables {
   DevBuf mem;
   DevTables dev{};
@@ -1130,6 +1239,7 @@ struct sce_tables {
   int n_colors = 0;
   int64_t total_walk = 0;
   int w_max = 0;
+  GradTables grad{};
 };
 
 struct sce_chain {
@@ -1183,6 +1293,36 @@ static size_t partial_capacity(long long n) {
     m = (m + TREE_B - 1) / TREE_B;
   }
   return 3 * cap + 16;
+}
+
+// The reference's adjacency programs for the energy gradient, uploaded as the
+// descriptor holds them (1-based).
+static int32_t upload_grad_tables(sce_tables* t, const sce_tables_desc* d) {
+  GradTables& g = t->grad;
+  g.n_sites = d->n_sites;
+  g.nlm = d->nlm;
+  const size_t na = (size_t)d->n_inst_sites, np = (size_t)d->n_prog, ns = (size_t)d->n_sent;
+  size_t n_inst_members = 0;
+  if (d->n_inst > 0) n_inst_members = (size_t)(d->inst_ptr[d->n_inst] - 1);
+  int32_t* q;
+  int8_t* q8;
+  double* qd;
+  CHK(dupload(t->mem, &q, d->site_ptr, (size_t)d->n_sites + 1)); g.site_ptr = q;
+  CHK(dupload1(t->mem, &q, d->site_inst, na)); g.site_inst = q;
+  CHK(dupload1(t->mem, &q, d->site_prog, na)); g.site_prog = q;
+  CHK(dupload1(t->mem, &q, d->site_col, na)); g.site_col = q;
+  CHK(dupload1(t->mem, &q, d->site_col2, na)); g.site_col2 = q;
+  CHK(dupload(t->mem, &q, d->sprog_ptr, np + 1)); g.sprog_ptr = q;
+  CHK(dupload1(t->mem, &q, d->sent_tgt, ns)); g.sent_tgt = q;
+  CHK(dupload(t->mem, &q, d->sfac_ptr, ns + 1)); g.sfac_ptr = q;
+  CHK(dupload1(t->mem, &q, d->sfac_row, (size_t)d->n_sfac)); g.sfac_row = q;
+  CHK(dupload1(t->mem, &q8, d->sfac_slot, (size_t)d->n_sfac)); g.sfac_slot = q8;
+  CHK(dupload1(t->mem, &q, d->pent_row, ns)); g.pent_row = q;
+  CHK(dupload1(t->mem, &q, d->pent_row2, ns)); g.pent_row2 = q;
+  CHK(dupload(t->mem, &q, d->inst_ptr, (size_t)d->n_inst + 1)); g.inst_ptr = q;
+  CHK(dupload1(t->mem, &q, d->inst_sites, n_inst_members)); g.inst_sites = q;
+  CHK(dupload1(t->mem, &qd, d->sent_w, ns)); g.sent_w = qd;
+  return SCE_OK;
 }
 
 extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out) {
@@ -1284,6 +1424,7 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
     if (d->site_active[a])
       t->sub_F[a] = 2 * (L.nb_ptr[a + 1] - L.nb_ptr[a]) + (L.feat_ptr[a + 1] - L.feat_ptr[a]);
   t->w_max = L.w_max;
+  if ((r = upload_grad_tables(t, d)) != SCE_OK) return fail(r);
   *out = t;
   return SCE_OK;
 }
@@ -2034,6 +2175,81 @@ struct Scratch {
     return SCE_OK;
   }
 };
+
+// ---- all-site energy gradient (gpu_energy_gradient!, gpu/gpu_gradient.jl) -----
+template <int LMAX>
+static void launch_rows_of(int n, const double* config, double* zrows, hipStream_t s) {
+  if (n > 0) k_rows_of<LMAX><<<(n + 127) / 128, 128, 0, s>>>(n, config, zrows);
+}
+
+template <int LMAX>
+static void launch_grad(const GradTables& T, const double* config, const double* zrows, double* G, int lanes,
+                        hipStream_t s) {
+  const int nb = (T.n_sites + TPB / 64 - 1) / (TPB / 64);
+  if (nb == 0) return;
+  if (lanes == 64) k_grad<LMAX, 64><<<nb, TPB, 0, s>>>(T, config, zrows, G);
+  else if (lanes == 128) k_grad<LMAX, 128><<<nb, TPB, 0, s>>>(T, config, zrows, G);
+  else k_grad<LMAX, 256><<<nb, TPB, 0, s>>>(T, config, zrows, G);
+}
+
+static int32_t check_grad_lanes(int lanes) {
+  if (lanes != 64 && lanes != 128 && lanes != 256)
+    return sce_fail(SCE_EINVAL, "gradient lanes (workgroup size) must be 64, 128 or 256 (got %d)", lanes);
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_energy_gradient(sce_tables* t, const double* config, double* G, int32_t lanes) {
+  if (!t || !config || !G) return sce_fail(SCE_EINVAL, "sce_energy_gradient: NULL argument");
+  CHK(check_grad_lanes(lanes));
+  const DevTables& D = t->dev;
+  const size_t n = (size_t)D.n_sites;
+  Scratch<double> cf, z, g;
+  CHK(cf.up(config, 3 * n));
+  CHK(z.alloc((size_t)D.nlm * n));
+  CHK(g.alloc(3 * n));
+  SCE_DISPATCH_LMAX(D.lmax, launch_rows_of, D.n_sites, cf.p, z.p, (hipStream_t)0);
+  SCE_DISPATCH_LMAX(D.lmax, launch_grad, t->grad, cf.p, z.p, g.p, lanes, (hipStream_t)0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return g.down(G, 3 * n);
+}
+
+extern "C" int32_t sce_chain_energy_gradient(sce_chain* c, double* G, int32_t lanes) {
+  CHK(chain_check(c));
+  if (!G) return sce_fail(SCE_EINVAL, "sce_chain_energy_gradient: NULL output");
+  CHK(check_grad_lanes(lanes));
+  const DevTables& D = c->t->dev;
+  Scratch<double> g;
+  CHK(g.alloc(3 * (size_t)D.n_sites));
+  SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, c->config, c->zrows, g.p, lanes, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return g.down(G, 3 * (size_t)D.n_sites);
+}
+
+// Average duration of the gradient kernel over `reps` back-to-back launches on
+// the chain's rows (HIP events on the chain's stream; nothing is changed).
+extern "C" int32_t sce_chain_time_gradient(sce_chain* c, int32_t lanes, int32_t reps, double* ms_avg) {
+  CHK(chain_check(c));
+  CHK(check_grad_lanes(lanes));
+  if (reps < 1 || !ms_avg) return sce_fail(SCE_EINVAL, "time_gradient: reps >= 1 and an output needed");
+  const DevTables& D = c->t->dev;
+  Scratch<double> g;
+  CHK(g.alloc(3 * (size_t)D.n_sites));
+  SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, c->config, c->zrows, g.p, lanes, c->stream);
+  hipEvent_t e0 = next_event(c), e1 = next_event(c);
+  c->ev_used = 0;
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r)
+    SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, c->config, c->zrows, g.p, lanes, c->stream);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_avg = (double)ms / reps;
+  return SCE_OK;
+}
 
 // Average duration of the colour-walk kernel: `reps` passes over every colour
 // class launched back to back on the chain's stream between one pair of HIP

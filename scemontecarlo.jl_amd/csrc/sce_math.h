@@ -263,6 +263,56 @@ SCE_HD void zlm_row_dyn(int lmax, double ux, double uy, double uz, double* z) {
 }
 
 // ---------------------------------------------------------------------------
+// Tangent-projected gradient of one tesseral harmonic, grad Z_lm(u) — the
+// operation order of src/gpu/grad_device.jl:30-66 (Harmonics.grad_Zlm_unsafe):
+// the order-n and order-(n+1) dnPl at u_z with parity and normalisation (the
+// order-(n+1) derivative is a +0.0 literal when n + 1 > l), the Cartesian
+// derivative of c P^(n)(z) Re/Im (x + iy)^n, and the radial part
+// u (u . dZ) / |u|^2 removed.  Only + - * / and a correctly rounded sqrt.
+// ---------------------------------------------------------------------------
+template <int LMAX>
+SCE_HD void grad_zlm_one(int l, int m, double x, double y, double z, double g[3]) {
+  double cache[LMAX + 2];
+  const int n = m < 0 ? -m : m;
+  const double par = (n & 1) ? -1.0 : 1.0;
+  const double plm = par * plm_norm(l, n) * dnpl<LMAX>(z, l, n, cache);
+  const double d1 = l < n + 1 ? 0.0 : dnpl<LMAX>(z, l, n + 1, cache);
+  const double dplm = par * plm_norm(l, n) * d1;
+  const double r2 = x * x + y * y + z * z;
+  if (m == 0) {
+    const double zz = z * dplm / r2;
+    g[0] = -x * zz;
+    g[1] = -y * zz;
+    g[2] = dplm - z * zz;
+    return;
+  }
+  const double c = par * __builtin_sqrt(2.0);
+  const cplx pn = cpow_int(cplx{x, y}, n), pn1 = cpow_int(cplx{x, y}, n - 1);
+  double dx, dy, dz;
+  if (m > 0) {
+    dx = c * (double)n * plm * pn1.re;
+    dy = -c * (double)n * plm * pn1.im;
+    dz = c * dplm * pn.re;
+  } else {
+    dx = c * (double)n * plm * pn1.im;
+    dy = c * (double)n * plm * pn1.re;
+    dz = c * dplm * pn.im;
+  }
+  const double zz = (x * dx + y * dy + z * dz) / r2;
+  g[0] = dx - x * zz;
+  g[1] = dy - y * zz;
+  g[2] = dz - z * zz;
+}
+
+// lm index k (lm order) -> (l, m)
+SCE_HD void lm_of(int k, int* l, int* m) {
+  int ll = 0;
+  while ((ll + 1) * (ll + 1) <= k) ++ll;
+  *l = ll;
+  *m = k - ll * ll - ll;
+}
+
+// ---------------------------------------------------------------------------
 // Keyed Metropolis proposal — src/gpu/gpu_sweep.jl:32-41 with
 // _FLIP_FRACTION = 0.2 and _rotate (src/updates.jl:20,25-29).
 // normalize(v) = v / sqrt((x*x + y*y) + z*z).

@@ -235,6 +235,37 @@ def gpu_run_sweeps(gst: GPUChainState, gH: GPUTiledHamiltonian, st: Optional[Cha
     return obs[: n.value]
 
 
+GRADIENT_LANES = 128  # the reference's default workgroup size for the gradient
+
+
+def gpu_energy_gradient(gH: GPUTiledHamiltonian, config, lanes: int = GRADIENT_LANES) -> np.ndarray:
+    """gpu_energy_gradient!(dG, gH, dconfig, gsc; workgroupsize = lanes)
+    (src/gpu/gpu_gradient.jl:187-212): the tangent-projected dE/de_s of every
+    site (n_sites x 3) for unit spins `config`, rows rebuilt on the device."""
+    cfg = np.ascontiguousarray(config, np.float64)
+    H = gH.host
+    if cfg.shape != (H.n_sites, 3):
+        raise ValueError(f"config has shape {cfg.shape}; expected ({H.n_sites}, 3)")
+    G = np.zeros((H.n_sites, 3))
+    L.check(L.lib().sce_energy_gradient(gH.handle, L.ptr(cfg), L.ptr(G), int(lanes)))
+    return G
+
+
+def gpu_chain_energy_gradient(gst: "GPUChainState", lanes: int = GRADIENT_LANES) -> np.ndarray:
+    """gpu_energy_gradient!(dG, gst, gH, gsc) (:222-226): the same on the
+    chain's device spins and rows."""
+    G = np.zeros((gst.gH.host.n_sites, 3))
+    L.check(L.lib().sce_chain_energy_gradient(gst.handle, L.ptr(G), int(lanes)))
+    return G
+
+
+def gpu_time_gradient(gst: "GPUChainState", lanes: int = GRADIENT_LANES, reps: int = 20) -> float:
+    """Average gradient-kernel duration (ms) over `reps` back-to-back launches."""
+    ms = C.c_double()
+    L.check(L.lib().sce_chain_time_gradient(gst.handle, int(lanes), int(reps), C.byref(ms)))
+    return ms.value
+
+
 def gpu_renormalize(gst: GPUChainState) -> float:
     d = C.c_double()
     L.check(L.lib().sce_renormalize(gst.handle, C.byref(d)))
