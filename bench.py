@@ -386,6 +386,15 @@ def main():
                      "fp64": {"flops_per_update": f_site, "achieved": fp64_tflops, "peak": 78.6,
                               "unit": "TFLOP/s", "frac": fp64_tflops / 78.6}},
     }
+    # the all-site energy gradient (gpu_energy_gradient! twin) on the chain's rows
+    grad_ms = S.gpu_time_gradient(gst, S.GRADIENT_LANES, 20)
+    adj_bytes = 16 * int(H.site_prog.size)
+    line["energy_gradient"] = {
+        "kernel": f"k_grad (all sites, {S.GRADIENT_LANES} strided shares per site)",
+        "ms_per_call": grad_ms, "site_gradients_per_s": H.n_sites / (grad_ms * 1e-3),
+        "adjacency_bytes_per_call": adj_bytes,
+        "adjacency_GBs": adj_bytes / (grad_ms * 1e-3) / 1e9,
+        "duration_source": "HIP events around 20 back-to-back launches on the chain's stream"}
     if not args.no_da:
         line["delayed_acceptance"] = da_leg(args, S, H, gH, sur, train_info, snapshot, gst.step, beta,
                                             obs, wall)
