@@ -1065,7 +1065,16 @@ __global__ __launch_bounds__(TPB) void k_grad(GradTables T, const double* __rest
         const int pid = T.site_prog[j - 1], col = T.site_col[j - 1];
         const int e0 = T.sprog_ptr[pid - 1], e1 = T.sprog_ptr[pid];
         const int col2 = col < 0 ? T.site_col2[j - 1] : 0;
-        const int off = col == 0 ? T.inst_ptr[T.site_inst[j - 1] - 1] - 1 : 0;
+        // a general program's instance members (1-based sites), read once
+        // (bodies have at most 4 members: the table upload's layout check)
+        int mem[4] = {1, 1, 1, 1};
+        if (col == 0) {
+          const int inst = T.site_inst[j - 1];
+          const int off = T.inst_ptr[inst - 1] - 1, body = T.inst_ptr[inst] - 1 - off;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (q < body) mem[q] = T.inst_sites[off + q];
+        }
         for (int e = e0; e < e1; ++e) {
           double p;
           if (col > 0) {
@@ -1076,7 +1085,8 @@ __global__ __launch_bounds__(TPB) void k_grad(GradTables T, const double* __rest
           } else {
             p = 1.0;
             for (int f = T.sfac_ptr[e - 1]; f < T.sfac_ptr[e]; ++f) {
-              const int mm = T.inst_sites[off + T.sfac_slot[f - 1] - 1];
+              const int sl = T.sfac_slot[f - 1];
+              const int mm = sl == 1 ? mem[0] : sl == 2 ? mem[1] : sl == 3 ? mem[2] : mem[3];
               p *= zrows[(size_t)(mm - 1) * T.nlm + (T.sfac_row[f - 1] - 1)];
             }
           }
