@@ -141,19 +141,19 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
 // ---------------------------------------------------------------------------
 constexpr int PROPOSE_LANES = 4;
 
+// Thread t of the proposal pass (PROPOSE_LANES per site) for the sweep whose
+// 1-based index is the chain's completed sweeps + sweep_off.
 template <int LMAX>
-__global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __restrict__ config,
-                                                 double* __restrict__ prop,
-                                                 const ChainDev* __restrict__ st, int q_lo,
-                                                 int q_n) {
+__device__ __forceinline__ void propose_body(const DevTables& T, const double* __restrict__ config,
+                                             double* __restrict__ prop, const ChainDev* __restrict__ st,
+                                             int q_lo, int q_n, int t, int sweep_off) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int qi = t / PROPOSE_LANES, part = t % PROPOSE_LANES;
   const bool valid = qi < q_n;
   const int q = q_lo + (valid ? qi : q_n - 1);  // every lane takes part in the exchange
   const int s = T.color_sites[q];
   const uint64_t seed = st->seed;
-  const int32_t sweep = (int32_t)(st->sweep_index + 1);
+  const int32_t sweep = (int32_t)(st->sweep_index + sweep_off);
   const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
   // this lane's half of a Box-Muller pair: the radius (even part) or the angle
   const sce::u32x4 b = sce::philox_block(seed, s + 1, sweep, 1u + (uint32_t)(part >> 1), 0u);
@@ -188,6 +188,14 @@ __global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __re
   o[3] = u;
 #pragma unroll
   for (int k = 0; k < NLM; ++k) o[4 + k] = z[k];
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __restrict__ config,
+                                                 double* __restrict__ prop,
+                                                 const ChainDev* __restrict__ st, int q_lo,
+                                                 int q_n) {
+  propose_body<LMAX>(T, config, prop, st, q_lo, q_n, blockIdx.x * blockDim.x + threadIdx.x, 1);
 }
 
 // Stage the neighbour rows of site s into Z (nb*NLM + row) and 1.0 at NB*NLM.
@@ -659,6 +667,26 @@ __global__ __launch_bounds__(256) void k_chunks(const double* __restrict__ x, lo
                                                 const uint8_t* __restrict__ active,
                                                 long long n_sites, int meas) {
   chunk_body<MODE>(blockIdx.x, x, n, iacc, iatt, part, ipart, config, active, n_sites, meas);
+}
+
+// The first reduction level of a Metropolis sweep fused with the proposals of
+// the next sweep: blocks below nchunk reduce, the others propose (the sweep's
+// walks are complete, so every spin is final for the sweep; the proposals use
+// sweep index completed + 2, as k_final has not counted this sweep yet).
+template <int LMAX>
+__global__ __launch_bounds__(256) void k_chunks_propose(const double* __restrict__ x, long long n,
+                                                        const int* __restrict__ iacc, double* __restrict__ part,
+                                                        long long* __restrict__ ipart,
+                                                        const double* __restrict__ config,
+                                                        const uint8_t* __restrict__ active, long long n_sites,
+                                                        int meas, long long nchunk, DevTables T,
+                                                        double* __restrict__ prop, const ChainDev* __restrict__ st) {
+  if ((long long)blockIdx.x < nchunk) {
+    chunk_body<MODE_METRO>(blockIdx.x, x, n, iacc, nullptr, part, ipart, config, active, n_sites, meas);
+    return;
+  }
+  const int t = (int)(((long long)blockIdx.x - nchunk) * blockDim.x + threadIdx.x);
+  propose_body<LMAX>(T, config, prop, st, 0, T.n_active, t, 2);
 }
 
 __device__ double final_tree(const double* buf, long long m, double* scratch, double* sh);
@@ -1266,7 +1294,8 @@ struct sce_chain {
   bool use_graph = true;    // replay captured sweeps as hipGraphs
   DevBinner* bin = nullptr;  // device binner fed by every measurement (null: off)
   DevBinner* bin_mem = nullptr;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // [plain, measured] sweep
+  // captured sweeps: [measured + 2 need_prop + 4 next_prop]
+  hipGraphExec_t gexec[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int g_lanes = 0;
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
@@ -1581,6 +1610,23 @@ static void launch_reduce(const double* x, long long n, const int* iacc, const i
                                    meas ? bin : nullptr);
 }
 
+// launch_reduce<MODE_METRO> with the next sweep's proposals in the first level.
+template <int LMAX>
+static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s) {
+  const DevTables& D = c->t->dev;
+  const long long n = D.n_active;
+  const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1);
+  const long long nchm = meas ? (D.n_sites + TREE_B - 1) / TREE_B : 0;
+  const long long nchunk = std::max(nbx, nchm);
+  const long long nprop = ((long long)D.n_active * PROPOSE_LANES + 255) / 256;
+  double* scratch = c->partial + nbx + 3 * std::max<long long>(nchm, 1);
+  k_chunks_propose<LMAX><<<(unsigned)(nchunk + nprop), 256, 0, s>>>(c->dE, n, c->acc, c->partial, c->ip64,
+                                                                   c->config, D.active, D.n_sites, meas, nchunk, D,
+                                                                   c->prop, c->st);
+  k_final<MODE_METRO><<<1, 256, 0, s>>>(c->partial, n, c->ip64, D.n_sites, meas, c->st, c->obs, D.n_active,
+                                        (double)D.n_active, scratch, meas ? c->bin : nullptr);
+}
+
 static int32_t chain_check(sce_chain* c) {
   if (!c || !c->t) return sce_fail(SCE_EINVAL, "NULL chain");
   return SCE_OK;
@@ -1608,13 +1654,15 @@ static int32_t set_beta(sce_chain* c, double beta) {
 // on a forked graph branch, to overlap it with the earlier colours' walks,
 // measured 40% slower per sweep: the cross-branch dependencies cost more than
 // the proposal kernel.)
-static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0) {
+// need_prop: propose this sweep first (otherwise the previous sweep's
+// reduction did); next_prop: propose the next sweep in this sweep's reduction.
+static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop = true, bool next_prop = false) {
   CHK(check_lds(c->t->dev, L));
   if (c->sweeps_done >= 2147483647LL - 1)
     return sce_fail(SCE_ERANGE_COUNTER, "sweep_index exhausted the 32-bit RNG counter word");
   const DevTables& D = c->t->dev;
   hipStream_t s = c->stream;
-  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
+  if (need_prop) SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
   for (int col = 0; col < c->t->n_colors; ++col) {
     const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
     if (n == 0) continue;
@@ -1626,8 +1674,12 @@ static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0) {
     SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n, s,
                       e0, e1);
   }
-  launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config,
-                            D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s, c->bin);
+  if (next_prop) {
+    SCE_DISPATCH_LMAX(D.lmax, launch_reduce_propose, c, meas, s);
+  } else {
+    launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config,
+                              D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s, c->bin);
+  }
   c->sweeps_done += 1;
   return SCE_OK;
 }
@@ -1673,26 +1725,26 @@ static void enqueue_measure(sce_chain* c) {
 // Capture one Metropolis sweep (plain and with the fused measurement) as
 // hipGraphs.  Every kernel reads the sweep counter, β and step from the
 // device-resident chain state, so a captured sweep replays unchanged.
-static int32_t ensure_graphs(sce_chain* c, int L) {
-  if (c->gexec[0] && c->g_lanes == L) return SCE_OK;
-  c->drop_graphs();
-  for (int k = 0; k < 2; ++k) {
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const long long before = c->sweeps_done;
-    const int32_t r = enqueue_metro(c, L, k);
-    c->sweeps_done = before;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (r != SCE_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return r;
-    }
-    if (e != hipSuccess) return sce_fail(SCE_EHIP, "sweep capture failed: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+static int32_t ensure_graph(sce_chain* c, int L, int k) {
+  if (c->g_lanes != L) {
+    c->drop_graphs();
+    c->g_lanes = L;
   }
-  c->g_lanes = L;
+  if (c->gexec[k]) return SCE_OK;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const long long before = c->sweeps_done;
+  const int32_t r = enqueue_metro(c, L, k & 1, (k & 2) != 0, (k & 4) != 0);
+  c->sweeps_done = before;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (r != SCE_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return sce_fail(SCE_EHIP, "sweep capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
   return SCE_OK;
 }
 
@@ -2083,19 +2135,27 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
     }
   } else {
     const bool graphs = c->use_graph && !c->timing && or_per_metropolis == 0;
-    if (graphs) CHK(ensure_graphs(c, lanes));
+    // Proposals depend only on each site's own spin at the sweep's start, so a
+    // sweep's reduction may make the next sweep's (fused launch) when nothing
+    // changes spins in between: not across renormalisations or overrelaxation,
+    // and never across calls (the step or state may change between them).
+    bool have_prop = false;
     for (int64_t i = 1; i <= nsweeps; ++i) {
       const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
       const bool meas = measure_interval > 0 && i % measure_interval == 0;
       // a measurement rides on the sweep's own reduction unless something
       // (overrelaxation, renormalisation) comes between them (run.jl:191-200)
       const bool fused = meas && !renorm && or_per_metropolis == 0;
+      const bool next_prop = or_per_metropolis == 0 && !renorm && i < nsweeps;
+      const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0);
       if (graphs) {
-        HIPCHK(hipGraphLaunch(c->gexec[fused ? 1 : 0], c->stream));
+        CHK(ensure_graph(c, lanes, k));
+        HIPCHK(hipGraphLaunch(c->gexec[k], c->stream));
         c->sweeps_done += 1;
       } else {
-        CHK(enqueue_metro(c, lanes, fused ? 1 : 0));
+        CHK(enqueue_metro(c, lanes, fused ? 1 : 0, !have_prop, next_prop));
       }
+      have_prop = next_prop;
       for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
       if (renorm) enqueue_renorm(c);
       if (meas && !fused) enqueue_measure(c);
