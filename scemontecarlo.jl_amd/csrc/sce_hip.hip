@@ -2109,7 +2109,6 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   c->ev_used = 0;
   hipEvent_t t0 = next_event(c);
   HIPCHK(hipEventRecord(t0, c->stream));
-  const DevTables& D = c->t->dev;
   if (c->persistent && or_per_metropolis == 0) {
     // segments end at renormalisation points; each is one cooperative launch
     int64_t i = 0, row = 0;
