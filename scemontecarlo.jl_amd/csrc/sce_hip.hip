@@ -95,6 +95,7 @@ struct DevTables {
   const WalkEntry* walk;
   const int32_t* nb_ptr;
   const int4* nb;
+  const int32_t* nbq;  // per colour position: nb_max neighbour site ids (padded with the site's own)
   const int32_t* inst_term;  // 0-based
   const int32_t* inst_ptr;   // 0-based offsets
   const int32_t* inst_sites; // 0-based
@@ -241,6 +242,39 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
   *atom_out = atom;
 }
 
+// The same staging for site s at colour position q, from the per-position
+// neighbour list: every slot of nb_max rows is filled (padding repeats the
+// site's own row, which no walk entry reads), so the row loads wait on one
+// load, issued first, whose address is the position.  between() runs once,
+// after the first list loads are issued (loads that need s go there).
+// Returns the sublattice.
+template <int L, int NLM, class F>
+__device__ __forceinline__ int stage_neighbours_q(const DevTables& T, const double* __restrict__ zrows,
+                                                  int q, int s, int lane, double* Z, F&& between) {
+  constexpr int U = stage_round(L);
+  const int32_t* nq = T.nbq + (size_t)q * T.nb_max;
+  const int total = T.nb_max * NLM;
+  for (int b = 0; b < total; b += U * L) {
+    int at[U], n[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = b + u * L + lane;
+      at[u] = i < total ? i : total - 1;
+      n[u] = nq[at[u] / NLM];
+    }
+    if (b == 0) between();
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = zrows[(size_t)n[u] * NLM + (at[u] - (at[u] / NLM) * NLM)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) Z[at[u]] = v[u];
+  }
+  if (total == 0) between();  // no neighbours (single-site terms only)
+  const int atom = s % T.na;
+  if (lane == 0) Z[(T.nb_ptr[atom + 1] - T.nb_ptr[atom]) * NLM] = 1.0;
+  return atom;
+}
+
 // dz = proposed row - current row of site s, for the walk.  load() issues the
 // reads before the neighbour staging (indices clamped like the staging's),
 // store() writes the differences after it.
@@ -352,10 +386,16 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
     const int w0 = T.walk_ptr[a0];
     W = T.walk_ptr[a0 + 1] - w0;
     wl = T.walk + w0;
-    if (!TEST) u_acc = prop[(size_t)q * PS + 3];
     RowDelta<L, NLM> rd;
-    rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
-    stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+    if constexpr (TEST) {
+      rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
+      stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+    } else {
+      u_acc = prop[(size_t)q * PS + 3];
+      atom = stage_neighbours_q<L, NLM>(T, zrows, q, s, lane, Z, [&] {
+        rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
+      });
+    }
     rd.store(dz, lane);
   }
   __syncthreads();
@@ -437,7 +477,12 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
   }
   const int stride = T.nb_max * NLM + 1 + NLM;
   double* Z = lds + g * stride;
-  if (valid) stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+  if (valid) {
+    if constexpr (TEST)
+      stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+    else
+      atom = stage_neighbours_q<L, NLM>(T, zrows, q, s, lane, Z, [] {});
+  }
   __syncthreads();
   double c[NLM];
 #pragma unroll
@@ -1413,6 +1458,27 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   if ((r = dupload(t->mem, &pn, reinterpret_cast<const int4*>(L.nb.data()), L.nb.size() / 4)) != SCE_OK)
     return fail(r);
   D.nb = pn;
+  {
+    // the neighbour sites of every colour position, resolved once here so that
+    // a walk's row loads depend on one load whose address is its position
+    std::vector<int32_t> nq((size_t)d->n_active * L.nb_max);
+    for (int q = 0; q < d->n_active; ++q) {
+      const int s = cs[q], a = s % L.na, cell = s / L.na;
+      const int c1 = cell % d->dims[0], c2 = (cell / d->dims[0]) % d->dims[1],
+                c3 = cell / (d->dims[0] * d->dims[1]);
+      for (int j = 0; j < L.nb_max; ++j) {
+        int n = s;
+        if (j < L.nb_ptr[a + 1] - L.nb_ptr[a]) {
+          const int32_t* o = &L.nb[4 * (size_t)(L.nb_ptr[a] + j)];
+          n = o[0] + L.na * ((c1 + o[1]) % d->dims[0] +
+                             d->dims[0] * ((c2 + o[2]) % d->dims[1] + d->dims[1] * ((c3 + o[3]) % d->dims[2])));
+        }
+        nq[(size_t)q * L.nb_max + j] = n;
+      }
+    }
+    if ((r = dupload(t->mem, &p32, nq.data(), nq.size())) != SCE_OK) return fail(r);
+    D.nbq = p32;
+  }
   if ((r = dupload(t->mem, &p32, L.feat_ptr.data(), L.feat_ptr.size())) != SCE_OK) return fail(r);
   D.feat_ptr = p32;
   if ((r = dupload(t->mem, &pn, reinterpret_cast<const int4*>(L.feat.data()), L.feat.size() / 4)) != SCE_OK)
@@ -3326,8 +3392,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     e2[1] = pq[1];
     e2[2] = pq[2];
     RowDelta<L, NLM> rd;
-    rd.load(pq + 4, zrows + (size_t)s * NLM, lane);
-    stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
+    stage_neighbours_q<L, NLM>(T, zrows, q, s, lane, Z, [&] { rd.load(pq + 4, zrows + (size_t)s * NLM, lane); });
     rd.store(dz, lane);
     NB = T.nb_ptr[atom + 1] - T.nb_ptr[atom];
   }
