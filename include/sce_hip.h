@@ -224,6 +224,20 @@ int32_t sce_batch_renormalize(sce_batch* b);
 int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc);
 /* Replay captured sweeps as hipGraphs (on != 0, default) or launch directly. */
 int32_t sce_batch_set_graph(sce_batch* b, int32_t on);
+/* Checkpoint restore of a "gpu_pt" run (resume, src/checkpoint.jl:555-635).
+ * set_counters: the lane-owned state of chain k that batch creation does not
+ * take — its keyed-RNG sweep index, acceptance counters and drift record
+ * (gst.sweep_index / acc_metro / att_metro and st.max_drift restored at
+ * src/checkpoint.jl:584-597).  get/set_exchange: the exchange schedule's state
+ * — exchanges attempted so far (the keyed exchange counter), the parity of the
+ * next attempt, and the per-pair statistics (n_chains - 1 each); the
+ * reference stores its exchange Xoshiro words, parity and swap_att/swap_acc
+ * (src/checkpoint.jl:396-400). */
+int32_t sce_batch_set_counters(sce_batch* b, int32_t k, int64_t sweep_index, int64_t acc_metro,
+                               int64_t att_metro, int64_t acc_or, int64_t att_or, double max_drift);
+int32_t sce_batch_get_exchange(sce_batch* b, int64_t* exch_count, int32_t* parity);
+int32_t sce_batch_set_exchange(sce_batch* b, int64_t exch_count, int32_t parity, const int64_t* att,
+                               const int64_t* acc);
 
 /* ---- learned delayed acceptance (no reference counterpart: an extension of the
  * device sweep for callers that want more effective samples per second) ------

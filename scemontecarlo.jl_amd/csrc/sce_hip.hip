@@ -3002,7 +3002,10 @@ extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, i
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
-  if (want > 0) HIPCHK(hipMemcpy(obs, b->obs, 4 * (size_t)want * (size_t)b->R * 8, hipMemcpyDeviceToHost));
+  // device rows: b->obs_cap per chain; the caller's: obs_cap per chain
+  if (want > 0)
+    HIPCHK(hipMemcpy2D(obs, 4 * (size_t)obs_cap * 8, b->obs, 4 * b->obs_cap * 8, 4 * (size_t)want * 8,
+                       (size_t)b->R, hipMemcpyDeviceToHost));
   if (n_obs) *n_obs = want;
   return SCE_OK;
 }
@@ -3076,6 +3079,60 @@ extern "C" int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc
   HIPCHK(hipStreamSynchronize(b->stream));
   if (att) HIPCHK(hipMemcpy(att, b->sw_att, (size_t)b->R * 8, hipMemcpyDeviceToHost));
   if (acc) HIPCHK(hipMemcpy(acc, b->sw_acc, (size_t)b->R * 8, hipMemcpyDeviceToHost));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_set_counters(sce_batch* b, int32_t k, int64_t sweep_index, int64_t acc_metro,
+                                          int64_t att_metro, int64_t acc_or, int64_t att_or, double max_drift) {
+  if (!b || k < 0 || k >= b->R) return sce_fail(SCE_EINVAL, "bad batch or chain index");
+  if (sweep_index < 0 || sweep_index >= 2147483647LL - 1)
+    return sce_fail(SCE_ERANGE_COUNTER, "sweep_index %lld outside the 32-bit sweep counter", (long long)sweep_index);
+  if (acc_metro < 0 || att_metro < acc_metro || acc_or < 0 || att_or < acc_or)
+    return sce_fail(SCE_EINVAL, "counters must satisfy 0 <= acc <= att");
+  if (!(max_drift >= 0.0)) return sce_fail(SCE_EINVAL, "max_drift must be >= 0");
+  ChainDev h;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(&h, b->h_ch[k].st, sizeof h, hipMemcpyDeviceToHost));
+  h.sweep_index = sweep_index;
+  h.acc_metro = acc_metro;
+  h.att_metro = att_metro;
+  h.acc_or = acc_or;
+  h.att_or = att_or;
+  h.max_drift = max_drift;
+  HIPCHK(hipMemcpy(b->h_ch[k].st, &h, sizeof h, hipMemcpyHostToDevice));
+  b->sweeps_done = std::max<int64_t>(b->sweeps_done, sweep_index);
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_get_exchange(sce_batch* b, int64_t* exch_count, int32_t* parity) {
+  if (!b || !exch_count || !parity) return sce_fail(SCE_EINVAL, "NULL argument");
+  BatchDev h;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(&h, b->bd, sizeof h, hipMemcpyDeviceToHost));
+  *exch_count = h.exch_count;
+  *parity = h.parity;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_set_exchange(sce_batch* b, int64_t exch_count, int32_t parity,
+                                          const int64_t* att, const int64_t* acc) {
+  if (!b || !att || !acc) return sce_fail(SCE_EINVAL, "NULL argument");
+  if (exch_count < 0 || exch_count >= 2147483647LL - 1)
+    return sce_fail(SCE_ERANGE_COUNTER, "exchange count %lld outside the 32-bit counter", (long long)exch_count);
+  if (parity != 0 && parity != 1) return sce_fail(SCE_EINVAL, "parity must be 0 or 1");
+  for (int i = 0; i + 1 < b->R; ++i)
+    if (att[i] < 0 || acc[i] < 0 || acc[i] > att[i])
+      return sce_fail(SCE_EINVAL, "exchange statistics must satisfy 0 <= acc <= att");
+  BatchDev h;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(&h, b->bd, sizeof h, hipMemcpyDeviceToHost));
+  h.exch_count = exch_count;
+  h.parity = parity;
+  HIPCHK(hipMemcpy(b->bd, &h, sizeof h, hipMemcpyHostToDevice));
+  if (b->R > 1) {
+    HIPCHK(hipMemcpy(b->sw_att, att, (size_t)(b->R - 1) * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->sw_acc, acc, (size_t)(b->R - 1) * 8, hipMemcpyHostToDevice));
+  }
   return SCE_OK;
 }
 

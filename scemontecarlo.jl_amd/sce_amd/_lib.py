@@ -94,7 +94,8 @@ EXPORTS = [
     "sce_test_proposal", "sce_test_walk_delta", "sce_test_walk_coeffs", "sce_test_reduce",
     "sce_batch_create", "sce_batch_destroy", "sce_batch_run", "sce_batch_download",
     "sce_batch_info", "sce_batch_set_step", "sce_batch_reset_counters", "sce_batch_swap_stats",
-    "sce_batch_set_graph", "sce_batch_renormalize",
+    "sce_batch_set_graph", "sce_batch_renormalize", "sce_batch_set_counters",
+    "sce_batch_get_exchange", "sce_batch_set_exchange",
     "sce_sur_create", "sce_sur_destroy", "sce_sur_info", "sce_sur_get_params",
     "sce_sur_set_params", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
     "sce_sur_features", "sce_da_run_sweeps", "sce_chain_set_counters", "sce_chain_time_walk",
@@ -143,6 +144,8 @@ def lib():
             "sce_batch_set_step": [P, i32, d], "sce_batch_reset_counters": [P],
             "sce_batch_swap_stats": [P, P, P], "sce_batch_set_graph": [P, i32],
             "sce_batch_renormalize": [P],
+            "sce_batch_set_counters": [P, i32, i64, i64, i64, i64, i64, d],
+            "sce_batch_get_exchange": [P, P, P], "sce_batch_set_exchange": [P, i64, i32, P, P],
             "sce_sur_create": [P, u64, P], "sce_sur_destroy": [P],
             "sce_sur_info": [P, i32, P, P, P, P], "sce_sur_get_params": [P, i32, P],
             "sce_sur_set_params": [P, i32, P, d], "sce_sur_harvest": [P, P, d, i64],

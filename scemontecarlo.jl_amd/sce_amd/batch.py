@@ -34,6 +34,7 @@ class GPUBatch:
         self.gHs = list(gHs)
         self.R = R
         self.kts = np.asarray(kts, np.float64)
+        self.exchange_seed = int(exchange_seed) & (2**64 - 1)
         self._keep = [np.ascontiguousarray(st.config, np.float64) for st in states]
         self._keepz = [np.ascontiguousarray(st.zrows, np.float64) for st in states]
         tabs = (C.c_void_p * R)(*[g.handle.value for g in gHs])
@@ -95,6 +96,26 @@ class GPUBatch:
     def set_graph(self, on: bool) -> None:
         L.check(L.lib().sce_batch_set_graph(self._h, 1 if on else 0))
 
+    def set_counters(self, k: int, sweep_index: int, acc_metro: int, att_metro: int,
+                     acc_or: int = 0, att_or: int = 0, max_drift: float = 0.0) -> None:
+        """Restore chain k's keyed-RNG sweep index, counters and drift record."""
+        L.check(L.lib().sce_batch_set_counters(self._h, int(k), int(sweep_index), int(acc_metro),
+                                               int(att_metro), int(acc_or), int(att_or), float(max_drift)))
+
+    def exchange_state(self):
+        """(exchanges attempted so far, parity of the next attempt)."""
+        n = C.c_int64()
+        par = C.c_int32()
+        L.check(L.lib().sce_batch_get_exchange(self._h, C.byref(n), C.byref(par)))
+        return n.value, par.value
+
+    def set_exchange(self, count: int, parity: int, att, acc) -> None:
+        att = np.ascontiguousarray(att, np.int64)
+        acc = np.ascontiguousarray(acc, np.int64)
+        if att.shape != (self.R - 1,) or acc.shape != (self.R - 1,):
+            raise ValueError(f"exchange statistics need {self.R - 1} entries each")
+        L.check(L.lib().sce_batch_set_exchange(self._h, int(count), int(parity), L.ptr(att), L.ptr(acc)))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and L._lib is not None:
@@ -138,13 +159,53 @@ def _adapt(step, acc, att, target):
     return step
 
 
-def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_interval: int = 10,
-               sweeps_therm: int = 2000, sweeps_measure: int = 10000, measure_interval: int = 1,
-               step: float = 0.6, adapt_target: float = 0.5, adapt_interval: int = 50,
-               renorm_interval: int = 1000, nbins: int = 32, init=None, seed: int = 0,
-               lanes: int = DEFAULT_LANES) -> PTResult:
-    """Replica exchange on the device — gpu_run_pt (src/gpu/gpu_pt.jl:261-319)."""
-    kts = resolve_kt(temperature, kT)
+@dataclass
+class PTPlan:
+    """What a ladder run is, as the checkpoint stores it (the reference's
+    UpdatePlan fields a gpu_pt file holds, src/checkpoint.jl:257-288)."""
+    kts: np.ndarray
+    exchange_interval: int
+    sweeps_therm: int
+    sweeps_measure: int
+    measure_interval: int
+    step: float
+    adapt_target: float
+    adapt_interval: int
+    renorm_interval: int
+    nbins: int
+    seed: int
+    lanes: int
+
+
+class _Checkpointer:
+    """Writes gpu_pt checkpoints every `interval` sweeps (0: only at the
+    thermalisation -> measurement boundary) — _Checkpointer/_ck_gpu_pt!
+    (src/checkpoint.jl:16-42, 428-437)."""
+
+    def __init__(self, path: str, interval: int):
+        if interval < 0:
+            raise ValueError(f"checkpoint_interval must be >= 0; got {interval}")
+        self.path = str(path)
+        self.interval = int(interval)
+        self.since = 0
+
+    def room(self, n: int) -> int:
+        """Sweeps the next run may take so that a due write lands at its end."""
+        return min(n, self.interval - self.since) if self.interval > 0 else n
+
+    def tick(self, n: int, write) -> None:
+        if self.interval <= 0:
+            return
+        self.since += n
+        if self.since >= self.interval:
+            self.write(write)
+
+    def write(self, write) -> None:
+        self.since = 0
+        write(self.path)
+
+
+def _validate_pt(kts, exchange_interval, sweeps_measure, measure_interval):
     R = len(kts)
     if R < 2:
         raise ValueError(f"parallel tempering needs a ladder of >= 2 temperatures; got {R}")
@@ -155,42 +216,124 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
         raise ValueError("exchange_interval must be >= 1")
     if sweeps_measure < 1 or not (1 <= measure_interval <= sweeps_measure):
         raise ValueError("measure_interval must be in 1:sweeps_measure")
+
+
+def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_interval: int = 10,
+               sweeps_therm: int = 2000, sweeps_measure: int = 10000, measure_interval: int = 1,
+               step: float = 0.6, adapt_target: float = 0.5, adapt_interval: int = 50,
+               renorm_interval: int = 1000, nbins: int = 32, init=None, seed: int = 0,
+               lanes: int = DEFAULT_LANES, checkpoint: Optional[str] = None,
+               checkpoint_interval: int = 0) -> PTResult:
+    """Replica exchange on the device — gpu_run_pt (src/gpu/gpu_pt.jl:261-319).
+
+    `checkpoint` / `checkpoint_interval` write restartable checkpoints (kind
+    "gpu_pt", interval in sweeps, 0: only at the thermalisation -> measurement
+    boundary); `resume_pt(path, gH)` continues one bit-identically.  As in the
+    reference there is no end-of-run write."""
+    kts = resolve_kt(temperature, kT)
+    _validate_pt(kts, exchange_interval, sweeps_measure, measure_interval)
+    plan = PTPlan(kts, int(exchange_interval), int(sweeps_therm), int(sweeps_measure),
+                  int(measure_interval), float(step), float(adapt_target), int(adapt_interval),
+                  int(renorm_interval), int(nbins), int(seed), int(lanes))
+    ck = _Checkpointer(checkpoint, checkpoint_interval) if checkpoint is not None else None
+    b = _pt_batch(gH, plan, init)
+    return _pt_run(b, gH, plan, "therm", 0, None, ck)
+
+
+def _pt_batch(gH: GPUTiledHamiltonian, plan: PTPlan, init=None) -> GPUBatch:
+    """The ladder's batch as a fresh run starts it (seed derivation of derive_seeds)."""
     H = gH.host
-    init_seeds, exch_seed, dev_seeds = derive_seeds(seed, R)
+    R = len(plan.kts)
+    init_seeds, exch_seed, dev_seeds = derive_seeds(plan.seed, R)
     states = []
     for r in range(R):
         cfg = init if init is not None else random_config(H.n_sites, np.random.default_rng(init_seeds[r]))
-        states.append(ChainState(H, cfg, step))
-    b = GPUBatch([gH] * R, states, dev_seeds, kts, exch_seed)
-    t0 = time.perf_counter()
-    steps = [step] * R
-    done = 0
-    while done < sweeps_therm:
-        n = min(adapt_interval - done % adapt_interval, sweeps_therm - done)
-        b.run(n, lanes, renorm_interval, 0, exchange_interval, done, sweeps_therm)
+        states.append(ChainState(H, cfg, plan.step))
+    return GPUBatch([gH] * R, states, dev_seeds, plan.kts, exch_seed)
+
+
+def _pt_therm(b: GPUBatch, plan: PTPlan, done: int, ck: Optional[_Checkpointer], write) -> None:
+    """The thermalisation phase from sweep `done`: exchanges, renormalisation,
+    and step adaptation on the counters every adapt_interval sweeps."""
+    total = plan.sweeps_therm
+    while done < total:
+        n = min(plan.adapt_interval - done % plan.adapt_interval, total - done)
+        if ck is not None:
+            n = ck.room(n)
+        b.run(n, plan.lanes, plan.renorm_interval, 0, plan.exchange_interval, done, total)
         done += n
-        if done % adapt_interval == 0:
-            for r in range(R):
+        if done % plan.adapt_interval == 0:
+            for r in range(b.R):
                 inf = b.info(r)
-                steps[r] = _adapt(steps[r], inf.acc_metro, inf.att_metro, adapt_target)
-                b.set_step(r, steps[r])
+                b.set_step(r, _adapt(inf.step, inf.acc_metro, inf.att_metro, plan.adapt_target))
             b.reset_counters()
-    b.renormalize()
-    b.reset_counters()
-    obs = b.run(sweeps_measure, lanes, renorm_interval, measure_interval, exchange_interval, 0,
-                sweeps_measure)
-    wall = time.perf_counter() - t0
+        if ck is not None:
+            ck.tick(n, lambda path, d=done: write(path, "therm", d, None))
+
+
+def _pt_run(b: GPUBatch, gH: GPUTiledHamiltonian, plan: PTPlan, phase: str, done: int,
+            rows: Optional[np.ndarray], ck: Optional[_Checkpointer], wall0: float = 0.0) -> PTResult:
+    """The phase driver shared by gpu_run_pt and resume_pt — _gpu_pt_run!
+    (src/gpu/gpu_pt.jl:141-224): `phase`/`done` enter mid-flight from a
+    checkpoint (a fresh run: "therm", 0); `rows` are the measurements a
+    measurement-phase checkpoint held."""
+    from .checkpoint import save_pt
+    H = gH.host
+    R = b.R
+    t0 = time.perf_counter()
+
+    def write(path, ph, d, obs):
+        save_pt(path, b, gH, plan, ph, d, obs, ck.interval if ck is not None else 0)
+
+    if phase == "therm":
+        _pt_therm(b, plan, done, ck, write)
+        # thermalisation -> measurement boundary: renormalise, fresh counters
+        b.renormalize()
+        b.reset_counters()
+        done = 0
+        rows = np.zeros((R, 0, 4))
+        if ck is not None:
+            ck.write(lambda path: write(path, "measure", 0, rows))
+    parts = [rows]
+    while done < plan.sweeps_measure:
+        n = plan.sweeps_measure - done
+        if ck is not None:
+            n = ck.room(n)
+        parts.append(b.run(n, plan.lanes, plan.renorm_interval, plan.measure_interval,
+                           plan.exchange_interval, done, plan.sweeps_measure))
+        done += n
+        if ck is not None:
+            ck.tick(n, lambda path, d=done: write(path, "measure", d, np.concatenate(parts, axis=1)))
+    obs = np.concatenate(parts, axis=1)
+    wall = wall0 + time.perf_counter() - t0
     points = []
     for r in range(R):
         inf = b.info(r)
-        points.append(TempResult(float(kts[r]), standard_stats(obs[r], float(kts[r]), H.n_active, nbins),
+        points.append(TempResult(float(plan.kts[r]),
+                                 standard_stats(obs[r], float(plan.kts[r]), H.n_active, plan.nbins),
                                  inf.acc_metro / inf.att_metro if inf.att_metro else math.nan,
                                  inf.step, inf.max_drift))
     att, acc = b.swap_stats()
     with np.errstate(invalid="ignore", divide="ignore"):
         swaps = np.where(att > 0, acc / np.maximum(att, 1), np.nan)
     finals = [b.download(r)[0] for r in range(R)]
-    return PTResult(points, swaps, finals, seed, wall)
+    return PTResult(points, swaps, finals, plan.seed, wall)
+
+
+def resume_pt(path: str, gH: GPUTiledHamiltonian, checkpoint: Optional[str] = "same",
+              checkpoint_interval: Optional[int] = None) -> PTResult:
+    """Continue a checkpointed gpu_run_pt and return the full run's result,
+    bit-identical to the uninterrupted run — resume(path, gH)
+    (src/checkpoint.jl:555-635).  By default the resumed run keeps writing to
+    `path` with the stored cadence (`checkpoint=None` disables,
+    `checkpoint_interval` overrides)."""
+    from .checkpoint import load_pt
+    st = load_pt(path, gH)
+    if checkpoint == "same":
+        checkpoint = path
+    interval = st["checkpoint_interval"] if checkpoint_interval is None else int(checkpoint_interval)
+    ck = _Checkpointer(checkpoint, interval) if checkpoint is not None else None
+    return _pt_run(st["batch"], gH, st["plan"], st["phase"], st["done"], st["rows"], ck)
 
 
 def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache: Optional[dict] = None,

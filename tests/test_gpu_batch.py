@@ -122,3 +122,27 @@ def test_batch_direct_launch_equals_graph():
     for r in range(R):
         assert np.array_equal(outs[0][r][0], outs[1][r][0]) and outs[0][r][2] == outs[1][r][2]
     assert np.array_equal(outs[0][R][1], outs[1][R][1])
+
+
+@pytest.mark.parametrize("renorm, exch", [(40, 7), (0, 0)])
+def test_split_runs_equal_one_run(renorm, exch):
+    """A phase run as several calls (as checkpointed ladder runs do) gives the
+    rows, payloads and exchange state of one call — including a later call
+    that takes fewer measurements than an earlier one (the device keeps its
+    larger row buffer; every chain's rows are copied from its own offset)."""
+    H = S.TiledHamiltonian(*bcc_tet_small(11, dims=(2, 2, 2)))
+    gH = S.GPUTiledHamiltonian(H)
+
+    def mk():
+        sts = [S.ChainState(H, rand_config(H.n_sites, 60 + k), 0.6) for k in range(3)]
+        return GPUBatch([gH] * 3, sts, [5, 6, 7], [0.05, 0.03, 0.02], 99)
+
+    b1, b2 = mk(), mk()
+    o1 = b1.run(300, 32, renorm, 3, exch, 0, 300)
+    o2 = np.concatenate([b2.run(n, 32, renorm, 3, exch, d, 300)
+                         for d, n in [(0, 120), (120, 120), (240, 60)]], axis=1)
+    assert o1.shape == o2.shape == (3, 100, 4)
+    assert np.array_equal(o1, o2)
+    assert b1.exchange_state() == b2.exchange_state()
+    for k in range(3):
+        assert np.array_equal(b1.download(k)[0], b2.download(k)[0])
