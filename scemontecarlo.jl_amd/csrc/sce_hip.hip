@@ -845,6 +845,41 @@ __global__ __launch_bounds__(256) void k_final(double* part, long long n, const 
   final_body<MODE>(part, n, ipart, n_sites, meas, st, out, n_att, n_div, scratch, bin);
 }
 
+// The previous sweep's final reduction (k_final<MODE_METRO> of that sweep),
+// deferred into block 0 of the next sweep's walk of its smallest colour class:
+// its operands (the chunk partials and counts) are complete at that launch's
+// start and are rewritten only by the sweep's own reduction after it, and no
+// walk block reads what it writes (the walk reads only st->beta).  Same
+// arithmetic as the standalone launch, one kernel fewer per sweep.  The final
+// needs more registers than the walk (fewer waves per SIMD), which the
+// smallest class does not feel.
+struct FinalArgs {
+  double* part;
+  const long long* ipart;
+  double* out;
+  double* scratch;
+  DevBinner* bin;
+  long long n, n_sites, n_att;
+  double n_div;
+  int meas;
+};
+
+template <int L, int LMAX>
+__global__ __launch_bounds__(TPB) void k_metro_fin(DevTables T, double* __restrict__ config,
+                                                   double* __restrict__ zrows, const double* __restrict__ prop,
+                                                   double* __restrict__ dE, int* __restrict__ acc, ChainDev* st,
+                                                   int q_lo, int q_n, FinalArgs f) {
+  if (blockIdx.x == 0) {
+    final_body<MODE_METRO>(f.part, f.n, f.ipart, f.n_sites, f.meas, st, f.out, f.n_att, f.n_div, f.scratch, f.bin);
+    return;
+  }
+  constexpr int G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1);
+  extern __shared__ double lds[];
+  const int qi = xcd_swizzle(blockIdx.x - 1, gridDim.x - 1) * G + (int)threadIdx.x / L;
+  metro_group<L, LMAX, false>(T, config, zrows, prop, dE, acc, st->beta, q_lo + qi, qi < q_n, nullptr, lds,
+                              T.nb_max * NLM + 1 + NLM);
+}
+
 // ---------------------------------------------------------------------------
 // persistent sweeps: one cooperative launch runs many Metropolis sweeps —
 // proposals, the colour steps and the deterministic reductions — separated by
@@ -1318,6 +1353,7 @@ struct sce_tables {
   std::vector<int32_t> color_ptr;  // 0-based offsets, n_colors + 1
   int32_t* d_color_ptr = nullptr;
   int max_class = 0;
+  int fin_col = -1;  // the smallest non-empty colour class: its launch carries a deferred final
   std::vector<int> sub_F;  // surrogate descriptor width per sublattice (0: inactive)
   int n_colors = 0;
   int64_t total_walk = 0;
@@ -1340,7 +1376,8 @@ struct sce_chain {
   DevBinner* bin = nullptr;  // device binner fed by every measurement (null: off)
   DevBinner* bin_mem = nullptr;
   // captured sweeps: [measured + 2 need_prop + 4 next_prop]
-  hipGraphExec_t gexec[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // [fused measurement + 2 need_prop + 4 next_prop (and deferred final) + 8 pending final's measurement]
+  hipGraphExec_t gexec[16] = {};
   int g_lanes = 0;
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
@@ -1519,8 +1556,11 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   t->n_colors = d->n_colors;
   t->color_ptr.resize((size_t)d->n_colors + 1);
   for (int c = 0; c <= d->n_colors; ++c) t->color_ptr[c] = d->color_ptr[c] - 1;
-  for (int c = 0; c < d->n_colors; ++c)
-    t->max_class = std::max(t->max_class, t->color_ptr[c + 1] - t->color_ptr[c]);
+  for (int c = 0; c < d->n_colors; ++c) {
+    const int n = t->color_ptr[c + 1] - t->color_ptr[c];
+    t->max_class = std::max(t->max_class, n);
+    if (n > 0 && (t->fin_col < 0 || n < t->color_ptr[t->fin_col + 1] - t->color_ptr[t->fin_col])) t->fin_col = c;
+  }
   if ((r = dupload(t->mem, &t->d_color_ptr, t->color_ptr.data(), t->color_ptr.size())) != SCE_OK)
     return fail(r);
   t->total_walk = (int64_t)L.walk.size();
@@ -1626,6 +1666,15 @@ static void launch_metro(int L, const DevTables& D, double* config, double* zrow
 template <int LMAX> static void launch_metro_run(int L, const DevTables& D, double* config, double* zrows, const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo, int q_n, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   launch_metro<LMAX, false>(L, D, config, zrows, prop, dE, acc, st, q_lo, q_n, nullptr, s, e0, e1);
 }
+template <int LMAX>
+static void launch_metro_fin(int L, const DevTables& D, double* config, double* zrows, const double* prop, double* dE,
+                             int* acc, ChainDev* st, int q_lo, int q_n, const FinalArgs& f, hipStream_t s) {
+  const int nb = (q_n + TPB / L - 1) / (TPB / L) + 1;
+  const size_t lb = walk_lds_bytes(D, L);
+  if (L == 16) k_metro_fin<16, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+  else if (L == 32) k_metro_fin<32, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+  else k_metro_fin<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+}
 template <int LMAX> static void launch_metro_test(int L, const DevTables& D, double* zrows, const double* prop, double* dE, int q_n, const int* ts, hipStream_t s) {
   launch_metro<LMAX, true>(L, D, nullptr, zrows, prop, dE, nullptr, nullptr, 0, q_n, ts, s);
 }
@@ -1678,7 +1727,7 @@ static void launch_reduce(const double* x, long long n, const int* iacc, const i
 
 // launch_reduce<MODE_METRO> with the next sweep's proposals in the first level.
 template <int LMAX>
-static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s) {
+static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s, bool defer_final) {
   const DevTables& D = c->t->dev;
   const long long n = D.n_active;
   const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1);
@@ -1689,8 +1738,30 @@ static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s) {
   k_chunks_propose<LMAX><<<(unsigned)(nchunk + nprop), 256, 0, s>>>(c->dE, n, c->acc, c->partial, c->ip64,
                                                                    c->config, D.active, D.n_sites, meas, nchunk, D,
                                                                    c->prop, c->st);
-  k_final<MODE_METRO><<<1, 256, 0, s>>>(c->partial, n, c->ip64, D.n_sites, meas, c->st, c->obs, D.n_active,
-                                        (double)D.n_active, scratch, meas ? c->bin : nullptr);
+  if (!defer_final)
+    k_final<MODE_METRO><<<1, 256, 0, s>>>(c->partial, n, c->ip64, D.n_sites, meas, c->st, c->obs, D.n_active,
+                                          (double)D.n_active, scratch, meas ? c->bin : nullptr);
+}
+
+// The operands of sweep-final reduction (k_final<MODE_METRO>) of a chain whose
+// sweep measured (meas) or not.
+static FinalArgs final_args(sce_chain* c, int meas) {
+  const DevTables& D = c->t->dev;
+  const long long n = D.n_active;
+  const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1);
+  const long long nchm = meas ? (D.n_sites + TREE_B - 1) / TREE_B : 0;
+  FinalArgs f;
+  f.part = c->partial;
+  f.ipart = c->ip64;
+  f.out = c->obs;
+  f.scratch = c->partial + nbx + 3 * std::max<long long>(nchm, 1);
+  f.bin = meas ? c->bin : nullptr;
+  f.n = n;
+  f.n_sites = D.n_sites;
+  f.n_att = D.n_active;
+  f.n_div = (double)D.n_active;
+  f.meas = meas;
+  return f;
 }
 
 static int32_t chain_check(sce_chain* c) {
@@ -1722,7 +1793,11 @@ static int32_t set_beta(sce_chain* c, double beta) {
 // the proposal kernel.)
 // need_prop: propose this sweep first (otherwise the previous sweep's
 // reduction did); next_prop: propose the next sweep in this sweep's reduction.
-static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop = true, bool next_prop = false) {
+// pending >= 0: the previous sweep's final reduction (with its measurement
+// flag) is still to run and rides in block 0 of the smallest class's launch.
+// defer_final (with next_prop): this sweep's final is left to the next sweep.
+static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop = true, bool next_prop = false,
+                             bool defer_final = false, int pending = -1) {
   CHK(check_lds(c->t->dev, L));
   if (c->sweeps_done >= 2147483647LL - 1)
     return sce_fail(SCE_ERANGE_COUNTER, "sweep_index exhausted the 32-bit RNG counter word");
@@ -1737,11 +1812,23 @@ static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop =
       e0 = next_event(c);
       e1 = next_event(c);
     }
-    SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n, s,
-                      e0, e1);
+    if (pending >= 0 && col == c->t->fin_col) {
+      const FinalArgs f = final_args(c, pending);
+      SCE_DISPATCH_LMAX(D.lmax, launch_metro_fin, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n,
+                        f, s);
+      pending = -1;
+    } else {
+      SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n,
+                        s, e0, e1);
+    }
+  }
+  if (pending >= 0) {  // no colour launch to carry it
+    const FinalArgs f = final_args(c, pending);
+    k_final<MODE_METRO><<<1, 256, 0, s>>>(f.part, f.n, f.ipart, f.n_sites, f.meas, c->st, f.out, f.n_att, f.n_div,
+                                          f.scratch, f.bin);
   }
   if (next_prop) {
-    SCE_DISPATCH_LMAX(D.lmax, launch_reduce_propose, c, meas, s);
+    SCE_DISPATCH_LMAX(D.lmax, launch_reduce_propose, c, meas, s, defer_final);
   } else {
     launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config,
                               D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s, c->bin);
@@ -1800,7 +1887,7 @@ static int32_t ensure_graph(sce_chain* c, int L, int k) {
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const long long before = c->sweeps_done;
-  const int32_t r = enqueue_metro(c, L, k & 1, (k & 2) != 0, (k & 4) != 0);
+  const int32_t r = enqueue_metro(c, L, k & 1, (k & 2) != 0, (k & 4) != 0, (k & 4) != 0, (k & 2) ? -1 : (k >> 3) & 1);
   c->sweeps_done = before;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (r != SCE_OK) {
@@ -2204,7 +2291,12 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
     // sweep's reduction may make the next sweep's (fused launch) when nothing
     // changes spins in between: not across renormalisations or overrelaxation,
     // and never across calls (the step or state may change between them).
+    // With the proposals, the sweep's final reduction moves into the next
+    // sweep's smallest-class launch (block 0 of k_metro_fin): nothing between
+    // the two reads or writes the chain scalars it updates.  Not in timing
+    // mode, whose per-launch events time the walks alone.
     bool have_prop = false;
+    int pending = -1;
     for (int64_t i = 1; i <= nsweeps; ++i) {
       const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
       const bool meas = measure_interval > 0 && i % measure_interval == 0;
@@ -2212,15 +2304,17 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
       // (overrelaxation, renormalisation) comes between them (run.jl:191-200)
       const bool fused = meas && !renorm && or_per_metropolis == 0;
       const bool next_prop = or_per_metropolis == 0 && !renorm && i < nsweeps;
-      const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0);
+      const bool defer = next_prop && !c->timing;
       if (graphs) {
+        const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0) + (pending == 1 ? 8 : 0);
         CHK(ensure_graph(c, lanes, k));
         HIPCHK(hipGraphLaunch(c->gexec[k], c->stream));
         c->sweeps_done += 1;
       } else {
-        CHK(enqueue_metro(c, lanes, fused ? 1 : 0, !have_prop, next_prop));
+        CHK(enqueue_metro(c, lanes, fused ? 1 : 0, !have_prop, next_prop, defer, pending));
       }
       have_prop = next_prop;
+      pending = defer ? (fused ? 1 : 0) : -1;
       for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
       if (renorm) enqueue_renorm(c);
       if (meas && !fused) enqueue_measure(c);
