@@ -195,9 +195,16 @@ template <int LMAX>
 __global__ __launch_bounds__(256) void k_propose(DevTables T, const double* __restrict__ config,
                                                  double* __restrict__ prop,
                                                  const ChainDev* __restrict__ st, int q_lo,
-                                                 int q_n) {
-  propose_body<LMAX>(T, config, prop, st, q_lo, q_n, blockIdx.x * blockDim.x + threadIdx.x, 1);
+                                                 int q_n, int sweep_off) {
+  propose_body<LMAX>(T, config, prop, st, q_lo, q_n, blockIdx.x * blockDim.x + threadIdx.x, sweep_off);
 }
+
+// Proposals of the colour positions [lo, lo + n) for the sweep whose 1-based
+// index is the completed sweeps + off, made as extra blocks of another launch.
+struct PropRange {
+  int lo, n, off;
+};
+__host__ __device__ inline int prop_blocks(int n) { return (int)(((long long)n * PROPOSE_LANES + 255) / 256); }
 
 // Stage the neighbour rows of site s into Z (nb*NLM + row) and 1.0 at NB*NLM.
 // Each lane fetches STAGE_ROUND(L) entries per round with every load issued
@@ -714,10 +721,9 @@ __global__ __launch_bounds__(256) void k_chunks(const double* __restrict__ x, lo
   chunk_body<MODE>(blockIdx.x, x, n, iacc, iatt, part, ipart, config, active, n_sites, meas);
 }
 
-// The first reduction level of a Metropolis sweep fused with the proposals of
-// the next sweep: blocks below nchunk reduce, the others propose (the sweep's
-// walks are complete, so every spin is final for the sweep; the proposals use
-// sweep index completed + 2, as k_final has not counted this sweep yet).
+// The first reduction level of a Metropolis sweep fused with proposals of the
+// next sweep: blocks below nchunk reduce, the others propose the range r (the
+// sweep's walks are complete, so every spin is final for the sweep).
 template <int LMAX>
 __global__ __launch_bounds__(256) void k_chunks_propose(const double* __restrict__ x, long long n,
                                                         const int* __restrict__ iacc, double* __restrict__ part,
@@ -725,13 +731,14 @@ __global__ __launch_bounds__(256) void k_chunks_propose(const double* __restrict
                                                         const double* __restrict__ config,
                                                         const uint8_t* __restrict__ active, long long n_sites,
                                                         int meas, long long nchunk, DevTables T,
-                                                        double* __restrict__ prop, const ChainDev* __restrict__ st) {
+                                                        double* __restrict__ prop, const ChainDev* __restrict__ st,
+                                                        PropRange r) {
   if ((long long)blockIdx.x < nchunk) {
     chunk_body<MODE_METRO>(blockIdx.x, x, n, iacc, nullptr, part, ipart, config, active, n_sites, meas);
     return;
   }
   const int t = (int)(((long long)blockIdx.x - nchunk) * blockDim.x + threadIdx.x);
-  propose_body<LMAX>(T, config, prop, st, 0, T.n_active, t, 2);
+  propose_body<LMAX>(T, config, prop, st, r.lo, r.n, t, r.off);
 }
 
 __device__ double final_tree(const double* buf, long long m, double* scratch, double* sh);
@@ -876,6 +883,31 @@ __global__ __launch_bounds__(TPB) void k_metro_fin(DevTables T, double* __restri
   constexpr int G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1);
   extern __shared__ double lds[];
   const int qi = xcd_swizzle(blockIdx.x - 1, gridDim.x - 1) * G + (int)threadIdx.x / L;
+  metro_group<L, LMAX, false>(T, config, zrows, prop, dE, acc, st->beta, q_lo + qi, qi < q_n, nullptr, lds,
+                              T.nb_max * NLM + 1 + NLM);
+}
+
+// A colour walk whose extra blocks (after the nwalk walk blocks) make
+// proposals of other colour classes: r0 then r1, each in whole blocks.  The
+// proposals read the spins of classes whose walks are complete and write
+// their own classes' proposal records; the walk reads only its own class's.
+template <int L, int LMAX>
+__global__ __launch_bounds__(TPB) void k_metro_prop(DevTables T, double* __restrict__ config,
+                                                    double* __restrict__ zrows, double* __restrict__ prop,
+                                                    double* __restrict__ dE, int* __restrict__ acc,
+                                                    const ChainDev* __restrict__ st, int q_lo, int q_n, int nwalk,
+                                                    PropRange r0, PropRange r1) {
+  if ((int)blockIdx.x >= nwalk) {
+    const int b = (int)blockIdx.x - nwalk, nb0 = prop_blocks(r0.n);
+    if (b < nb0)
+      propose_body<LMAX>(T, config, prop, st, r0.lo, r0.n, b * TPB + (int)threadIdx.x, r0.off);
+    else
+      propose_body<LMAX>(T, config, prop, st, r1.lo, r1.n, (b - nb0) * TPB + (int)threadIdx.x, r1.off);
+    return;
+  }
+  constexpr int G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1);
+  extern __shared__ double lds[];
+  const int qi = xcd_swizzle(blockIdx.x, nwalk) * G + (int)threadIdx.x / L;
   metro_group<L, LMAX, false>(T, config, zrows, prop, dE, acc, st->beta, q_lo + qi, qi < q_n, nullptr, lds,
                               T.nb_max * NLM + 1 + NLM);
 }
@@ -1375,8 +1407,8 @@ struct sce_chain {
   bool use_graph = true;    // replay captured sweeps as hipGraphs
   DevBinner* bin = nullptr;  // device binner fed by every measurement (null: off)
   DevBinner* bin_mem = nullptr;
-  // captured sweeps: [measured + 2 need_prop + 4 next_prop]
-  // [fused measurement + 2 need_prop + 4 next_prop (and deferred final) + 8 pending final's measurement]
+  // captured sweeps: [fused measurement + 2 need_prop + 4 next_prop (with the final
+  // deferred) + 8 measurement flag of the previous sweep's deferred final]
   hipGraphExec_t gexec[16] = {};
   int g_lanes = 0;
   size_t obs_cap = 0;
@@ -1634,7 +1666,7 @@ template <int LMAX>
 static void launch_propose(const DevTables& D, const double* config, double* prop,
                            const ChainDev* st, hipStream_t s) {
   const int nb = (int)(((long long)D.n_active * PROPOSE_LANES + 255) / 256);
-  if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st, 0, D.n_active);
+  if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st, 0, D.n_active, 1);
 }
 
 
@@ -1674,6 +1706,23 @@ static void launch_metro_fin(int L, const DevTables& D, double* config, double* 
   if (L == 16) k_metro_fin<16, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
   else if (L == 32) k_metro_fin<32, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
   else k_metro_fin<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+}
+template <int LMAX>
+static void launch_metro_prop(int L, const DevTables& D, double* config, double* zrows, double* prop, double* dE,
+                              int* acc, const ChainDev* st, int q_lo, int q_n, PropRange r0, PropRange r1,
+                              hipStream_t s) {
+  const int nwalk = (q_n + TPB / L - 1) / (TPB / L);
+  const int nb = nwalk + prop_blocks(r0.n) + prop_blocks(r1.n);
+  const size_t lb = walk_lds_bytes(D, L);
+  if (L == 16) k_metro_prop<16, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
+  else if (L == 32) k_metro_prop<32, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
+  else k_metro_prop<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
+}
+template <int LMAX>
+static void launch_propose_range(const DevTables& D, const double* config, double* prop, const ChainDev* st,
+                                 PropRange r, hipStream_t s) {
+  const int nb = prop_blocks(r.n);
+  if (nb) k_propose<LMAX><<<nb, 256, 0, s>>>(D, config, prop, st, r.lo, r.n, r.off);
 }
 template <int LMAX> static void launch_metro_test(int L, const DevTables& D, double* zrows, const double* prop, double* dE, int q_n, const int* ts, hipStream_t s) {
   launch_metro<LMAX, true>(L, D, nullptr, zrows, prop, dE, nullptr, nullptr, 0, q_n, ts, s);
@@ -1727,17 +1776,17 @@ static void launch_reduce(const double* x, long long n, const int* iacc, const i
 
 // launch_reduce<MODE_METRO> with the next sweep's proposals in the first level.
 template <int LMAX>
-static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s, bool defer_final) {
+static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s, bool defer_final, PropRange r) {
   const DevTables& D = c->t->dev;
   const long long n = D.n_active;
   const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1);
   const long long nchm = meas ? (D.n_sites + TREE_B - 1) / TREE_B : 0;
   const long long nchunk = std::max(nbx, nchm);
-  const long long nprop = ((long long)D.n_active * PROPOSE_LANES + 255) / 256;
+  const long long nprop = prop_blocks(r.n);
   double* scratch = c->partial + nbx + 3 * std::max<long long>(nchm, 1);
   k_chunks_propose<LMAX><<<(unsigned)(nchunk + nprop), 256, 0, s>>>(c->dE, n, c->acc, c->partial, c->ip64,
                                                                    c->config, D.active, D.n_sites, meas, nchunk, D,
-                                                                   c->prop, c->st);
+                                                                   c->prop, c->st, r);
   if (!defer_final)
     k_final<MODE_METRO><<<1, 256, 0, s>>>(c->partial, n, c->ip64, D.n_sites, meas, c->st, c->obs, D.n_active,
                                           (double)D.n_active, scratch, meas ? c->bin : nullptr);
@@ -1793,46 +1842,119 @@ static int32_t set_beta(sce_chain* c, double beta) {
 // the proposal kernel.)
 // need_prop: propose this sweep first (otherwise the previous sweep's
 // reduction did); next_prop: propose the next sweep in this sweep's reduction.
-// pending >= 0: the previous sweep's final reduction (with its measurement
-// flag) is still to run and rides in block 0 of the smallest class's launch.
-// defer_final (with next_prop): this sweep's final is left to the next sweep.
+// One Metropolis sweep.  Chaining within a call (next_prop, and in the
+// previous sweep when !need_prop):
+//  - proposals: a class's proposals for the next sweep depend only on its own
+//    spins after its walk, so they ride as extra blocks in a later walk launch
+//    of the same sweep (k_metro_prop); the last class's ride in the next
+//    sweep's first walk launch (carry_last), or all go into the reduction
+//    launch in timing mode, whose events time the walks alone;
+//  - the final reduction (defer_final; pending >= 0 with the previous sweep's
+//    measurement flag) rides in block 0 of the next sweep's smallest-class
+//    walk (k_metro_fin), a launch that then carries no proposals, because its
+//    final block advances the completed-sweep count the proposals key on.
 static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop = true, bool next_prop = false,
-                             bool defer_final = false, int pending = -1) {
+                             bool defer_final = false, int pending = -1, bool carry_last = false) {
   CHK(check_lds(c->t->dev, L));
   if (c->sweeps_done >= 2147483647LL - 1)
     return sce_fail(SCE_ERANGE_COUNTER, "sweep_index exhausted the 32-bit RNG counter word");
   const DevTables& D = c->t->dev;
+  const sce_tables* t = c->t;
   hipStream_t s = c->stream;
-  if (need_prop) SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
-  for (int col = 0; col < c->t->n_colors; ++col) {
-    const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
-    if (n == 0) continue;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
-      e0 = next_event(c);
-      e1 = next_event(c);
+  const bool walk_props = !c->timing;
+  std::vector<int> cols;  // the non-empty colour classes, in sweep order
+  for (int col = 0; col < t->n_colors; ++col)
+    if (t->color_ptr[col + 1] > t->color_ptr[col]) cols.push_back(col);
+  const int K = (int)cols.size();
+  int kf = -1;  // the walk launch that carries the pending final
+  if (pending >= 0)
+    for (int k = 0; k < K; ++k)
+      if (cols[k] == t->fin_col) kf = k;
+  // A proposal for this sweep (next = 0) or the next (next = 1) made at launch
+  // j (-1: before the walks, K: the reduction) keys on the completed-sweep
+  // count plus this offset; before the deferred final has run it lags by one.
+  auto off_at = [&](int j, int next) { return 1 + next + ((kf >= 0 && j < kf) ? 1 : 0); };
+  auto range_of = [&](int k) {
+    const int lo = t->color_ptr[cols[k]];
+    return PropRange{lo, t->color_ptr[cols[k] + 1] - lo, 0};
+  };
+  std::vector<std::vector<PropRange>> at((size_t)K + 1);
+  std::vector<PropRange> before, after;
+  auto put = [&](int j, PropRange r, int next) {
+    r.off = off_at(j, next);
+    std::vector<PropRange>& v = at[j];
+    if (!v.empty() && v.back().off == r.off && v.back().lo + v.back().n == r.lo)
+      v.back().n += r.n;  // the classes are contiguous in colour order
+    else
+      v.push_back(r);
+  };
+  if (carry_last && K > 0) {
+    // this sweep's proposals of its last class, made before that class's walk
+    int j = 0;
+    while (j < K - 1 && j == kf) ++j;
+    if (j < K - 1) {
+      put(j, range_of(K - 1), 0);
+    } else {
+      PropRange r = range_of(K - 1);
+      r.off = off_at(-1, 0);
+      before.push_back(r);
     }
-    if (pending >= 0 && col == c->t->fin_col) {
+  }
+  if (next_prop) {
+    for (int k = 0; k < K; ++k) {
+      if (walk_props && k == K - 1) break;  // carried into the next sweep
+      int j = walk_props ? k + 1 : K;
+      while (j < K && j == kf) ++j;
+      put(j, range_of(k), 1);
+    }
+  }
+  for (int j = 0; j < K; ++j)  // a walk launch carries at most two ranges
+    while (at[j].size() > 2) {
+      PropRange r = at[j].back();
+      at[j].pop_back();
+      if (r.off != off_at(j, 1)) return sce_fail(SCE_EINVAL, "internal: proposal placement");
+      r.off = off_at(K, 1);
+      after.push_back(r);
+    }
+  while (at[K].size() > 1) {
+    after.push_back(at[K].back());
+    at[K].pop_back();
+  }
+  for (const PropRange& r : before) SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, r, s);
+  if (need_prop) SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
+  for (int k = 0; k < K; ++k) {
+    const int lo = t->color_ptr[cols[k]], n = t->color_ptr[cols[k] + 1] - lo;
+    if (k == kf) {
       const FinalArgs f = final_args(c, pending);
       SCE_DISPATCH_LMAX(D.lmax, launch_metro_fin, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n,
                         f, s);
-      pending = -1;
+    } else if (!at[k].empty()) {
+      const PropRange none{0, 0, 0};
+      SCE_DISPATCH_LMAX(D.lmax, launch_metro_prop, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n,
+                        at[k][0], at[k].size() > 1 ? at[k][1] : none, s);
     } else {
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (c->timing) {
+        e0 = next_event(c);
+        e1 = next_event(c);
+      }
       SCE_DISPATCH_LMAX(D.lmax, launch_metro_run, L, D, c->config, c->zrows, c->prop, c->dE, c->acc, c->st, lo, n,
                         s, e0, e1);
     }
   }
-  if (pending >= 0) {  // no colour launch to carry it
+  if (pending >= 0 && kf < 0) {  // no colour launch to carry it
     const FinalArgs f = final_args(c, pending);
     k_final<MODE_METRO><<<1, 256, 0, s>>>(f.part, f.n, f.ipart, f.n_sites, f.meas, c->st, f.out, f.n_att, f.n_div,
                                           f.scratch, f.bin);
   }
   if (next_prop) {
-    SCE_DISPATCH_LMAX(D.lmax, launch_reduce_propose, c, meas, s, defer_final);
+    const PropRange r = at[K].empty() ? PropRange{0, 0, 0} : at[K][0];
+    SCE_DISPATCH_LMAX(D.lmax, launch_reduce_propose, c, meas, s, defer_final, r);
   } else {
     launch_reduce<MODE_METRO>(c->dE, D.n_active, c->acc, nullptr, c->partial, c->ip64, c->config,
                               D.active, D.n_sites, meas, c->st, c->obs, D.n_active, (double)D.n_active, s, c->bin);
   }
+  for (const PropRange& r : after) SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, r, s);
   c->sweeps_done += 1;
   return SCE_OK;
 }
@@ -1887,7 +2009,9 @@ static int32_t ensure_graph(sce_chain* c, int L, int k) {
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const long long before = c->sweeps_done;
-  const int32_t r = enqueue_metro(c, L, k & 1, (k & 2) != 0, (k & 4) != 0, (k & 4) != 0, (k & 2) ? -1 : (k >> 3) & 1);
+  const bool chained = (k & 2) == 0;  // the previous sweep made proposals and deferred its final
+  const int32_t r = enqueue_metro(c, L, k & 1, !chained, (k & 4) != 0, (k & 4) != 0, chained ? (k >> 3) & 1 : -1,
+                                  chained);
   c->sweeps_done = before;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (r != SCE_OK) {
@@ -2294,8 +2418,8 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
     // With the proposals, the sweep's final reduction moves into the next
     // sweep's smallest-class launch (block 0 of k_metro_fin): nothing between
     // the two reads or writes the chain scalars it updates.  Not in timing
-    // mode, whose per-launch events time the walks alone.
-    bool have_prop = false;
+    // mode, whose per-launch events time the walks alone (enqueue_metro).
+    bool have_prop = false, carry = false;
     int pending = -1;
     for (int64_t i = 1; i <= nsweeps; ++i) {
       const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
@@ -2311,10 +2435,11 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
         HIPCHK(hipGraphLaunch(c->gexec[k], c->stream));
         c->sweeps_done += 1;
       } else {
-        CHK(enqueue_metro(c, lanes, fused ? 1 : 0, !have_prop, next_prop, defer, pending));
+        CHK(enqueue_metro(c, lanes, fused ? 1 : 0, !have_prop, next_prop, defer, pending, carry));
       }
       have_prop = next_prop;
       pending = defer ? (fused ? 1 : 0) : -1;
+      carry = next_prop && !c->timing;
       for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
       if (renorm) enqueue_renorm(c);
       if (meas && !fused) enqueue_measure(c);
