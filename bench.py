@@ -160,7 +160,8 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, obs_plain, wal
 def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
     """Batched parallel tempering: 64 replicas of fcc 12^3 (6912 spins, body-2 l<=2 to
     the 4th shell) on a geometric kT ladder spanning 0.2-2.0 J1, replica exchange
-    on the device after every sweep, one launch per colour for all replicas.
+    on the device after every sweep, one launch per colour for all replicas, lanes
+    per site picked by the autotuner.
     J1 = 3 x rms raw coefficient of the first-shell (1,1) bonds (the bond energy
     of a (1,1) template is 4 pi N1^2 c = 3c times a unit-norm tensor form)."""
     na, terms, dims = M.config_model("fcc12", seed=0)
@@ -173,15 +174,21 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
     kts = J1 * np.geomspace(0.2, 2.0, R)
     H = S.TiledHamiltonian(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
-    from sce_amd.batch import GPUBatch, derive_seeds, _adapt
+    from sce_amd.batch import GPUBatch, derive_seeds, _adapt, autotune_lanes
     init, exch, dev = derive_seeds(2026, R)
-    states = [S.ChainState(H, S.random_config(H.n_sites, np.random.default_rng(init[r])), 0.6) for r in range(R)]
-    b = GPUBatch([gH] * R, states, dev, kts, exch)
+
+    def make():
+        states = [S.ChainState(H, S.random_config(H.n_sites, np.random.default_rng(init[r])), 0.6)
+                  for r in range(R)]
+        return GPUBatch([gH] * R, states, dev, kts, exch)
+
+    lanes = autotune_lanes(make, sweeps=20)   # lanes per site: a scratch batch, before the run
+    b = make()
     steps = [0.6] * R
     done = 0
     while done < sweeps_therm:            # untimed: exchange + step adaptation (gpu_pt.jl:115-140)
         n = min(50, sweeps_therm - done)
-        b.run(n, 64, 1000, 0, 1, done, sweeps_therm)
+        b.run(n, lanes, 1000, 0, 1, done, sweeps_therm)
         done += n
         for k in range(R):
             inf = b.info(k)
@@ -190,7 +197,7 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
         b.reset_counters()
     b.renormalize()
     t0 = time.perf_counter()
-    obs = b.run(sweeps_measure, 64, 1000, 1, 1, 0, sweeps_measure)
+    obs = b.run(sweeps_measure, lanes, 1000, 1, 1, 0, sweeps_measure)
     wall = time.perf_counter() - t0
     att, acc = b.swap_stats()
     sw = np.where(att > 0, acc / np.maximum(att, 1), np.nan)
@@ -198,7 +205,7 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
                         f"(J1 = {J1:.3e} eV), exchange every sweep on the device",
             "spin_updates_per_s": R * H.n_active * sweeps_measure / wall,
             "ms_per_sweep_all_replicas": 1e3 * wall / sweeps_measure, "sweeps": sweeps_measure,
-            "lanes": 64, "measurements": int(obs.shape[1]),
+            "lanes": lanes, "measurements": int(obs.shape[1]),
             "swap_acceptance_mean": float(np.nanmean(sw)), "swap_acceptance_min": float(np.nanmin(sw)),
             "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep"}
 

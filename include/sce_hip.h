@@ -124,6 +124,10 @@ int32_t sce_chain_reset_counters(sce_chain* c);
  * what resuming a checkpointed chain needs. */
 int32_t sce_chain_set_counters(sce_chain* c, int64_t sweep_index, int64_t acc_metro,
                                int64_t att_metro, int64_t acc_or, int64_t att_or);
+/* Restores the chain's drift record (the largest |incremental - recomputed|
+ * energy seen at a renormalisation), which a checkpoint carries with the
+ * chain state (max_drift, src/checkpoint.jl:165-190). */
+int32_t sce_chain_set_drift(sce_chain* c, double max_drift);
 /* to_host!(st, gst): config (3 x n), zrows (nlm x n), energy; any may be NULL. */
 int32_t sce_download(sce_chain* c, double* config, double* zrows, double* energy);
 /* _from_host!(gst, st) */

@@ -2140,6 +2140,16 @@ extern "C" int32_t sce_chain_set_counters(sce_chain* c, int64_t sweep_index, int
   return SCE_OK;
 }
 
+extern "C" int32_t sce_chain_set_drift(sce_chain* c, double max_drift) {
+  CHK(chain_check(c));
+  if (!(max_drift >= 0.0)) return sce_fail(SCE_EINVAL, "max_drift must be >= 0");
+  ChainDev h;
+  CHK(read_state(c, &h));
+  h.max_drift = max_drift;
+  HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
 extern "C" int32_t sce_download(sce_chain* c, double* config, double* zrows, double* energy) {
   CHK(chain_check(c));
   const DevTables& D = c->t->dev;

@@ -99,7 +99,7 @@ EXPORTS = [
     "sce_sur_create", "sce_sur_destroy", "sce_sur_info", "sce_sur_get_params",
     "sce_sur_set_params", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
     "sce_sur_features", "sce_da_run_sweeps", "sce_chain_set_counters", "sce_chain_time_walk",
-    "sce_chain_set_graph", "sce_chain_binner_reset", "sce_chain_binner_off", "sce_chain_binner_get",
+    "sce_chain_set_graph", "sce_chain_set_drift", "sce_chain_binner_reset", "sce_chain_binner_off", "sce_chain_binner_get",
     "sce_energy_gradient", "sce_chain_energy_gradient", "sce_chain_time_gradient",
     "sce_test_mfma",
 ]
@@ -156,6 +156,7 @@ def lib():
             "sce_da_run_sweeps": [P, P, d, i64, i64, i64, P, i64, P],
             "sce_chain_set_counters": [P, i64, i64, i64, i64, i64],
             "sce_chain_time_walk": [P, i32, i32, P, P], "sce_chain_set_graph": [P, i32],
+            "sce_chain_set_drift": [P, d],
             "sce_chain_binner_reset": [P, i32, i32], "sce_chain_binner_off": [P],
             "sce_chain_binner_get": [P, P, P, P, P, P, P, i32],
             "sce_energy_gradient": [P, P, P, i32], "sce_chain_energy_gradient": [P, P, i32],

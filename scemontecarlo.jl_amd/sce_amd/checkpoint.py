@@ -84,6 +84,7 @@ def load_chain(path: str, gH: GPUTiledHamiltonian) -> GPUChainState:
         gst = GPUChainState(gH, st, seed=int(f["seed"]))
         c = [int(x) for x in f["counters"]]
         gst.set_counters(int(f["sweep_index"]), c[0], c[1], c[2], c[3])
+        gst.set_drift(float(f["max_drift"]))
     return gst
 
 

@@ -120,6 +120,10 @@ class GPUChainState:
         L.check(L.lib().sce_chain_set_counters(self._h, int(sweep_index), int(acc_metro),
                                                int(att_metro), int(acc_or), int(att_or)))
 
+    def set_drift(self, max_drift: float) -> None:
+        """Restore the drift record (resume)."""
+        L.check(L.lib().sce_chain_set_drift(self._h, float(max_drift)))
+
     def reset_counters(self) -> None:
         L.check(L.lib().sce_chain_reset_counters(self._h))
 

@@ -324,6 +324,9 @@ def test_checkpoint_resume_is_bitwise(tmp_path):
     assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
     ia, ib = a.info(), b.info()
     assert (ia.sweep_index, ia.acc_metro, ia.att_metro, ia.step) == (ib.sweep_index, ib.acc_metro, ib.att_metro, ib.step)
+    assert ia.max_drift == ib.max_drift             # the drift record travels too
+    c = CK.load_chain(path, gH)
+    assert c.info().max_drift == np.load(path)["max_drift"] > 0.0
     other = S.GPUTiledHamiltonian(S.TiledHamiltonian(*bcc_small(10)))
     with pytest.raises(ValueError, match="different model"):
         CK.load_chain(path, other)
