@@ -216,6 +216,39 @@ def test_run_sweeps_matches_single_sweeps():
 
 
 @pytest.mark.parametrize("lanes", LANES)
+@pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
+def test_chained_sweeps_equal_single_sweeps(case, lanes):
+    """Sweeps chained within one call (the next sweep's proposals made during
+    the walks, the final reduction deferred into the next sweep's
+    smallest-class launch, every colour count and class-size order of the
+    fixture models) equal the same sweeps made one call each, which are
+    bitwise the oracle's; graph replay and direct launches alike, with the
+    measured energies of the deferred finals in the right rows."""
+    name, na, terms, dims, mm, fl = case
+    H, T = build_pair(na, terms, dims, mm, fl)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 21), 0.6)
+    ref = S.GPUChainState(gH, st, seed=404)
+    energies = []
+    for _ in range(7):
+        S.gpu_metropolis_sweep(ref, gH, 20.0, lanes=lanes)
+        energies.append(ref.download()[2])
+    cr, zr, Er = ref.download()
+    ir = ref.info()
+    for graph in (True, False):
+        g = S.GPUChainState(gH, st, seed=404)
+        g.set_graph(graph)
+        obs = S.gpu_run_sweeps(g, gH, None, 20.0, 7, renorm_interval=0, measure_interval=2, lanes=lanes)
+        c, z, E = g.download()
+        assert np.array_equal(c.view(np.uint64), cr.view(np.uint64)), graph
+        assert np.array_equal(z.view(np.uint64), zr.view(np.uint64)), graph
+        assert E == Er
+        i = g.info()
+        assert (i.sweep_index, i.acc_metro, i.att_metro) == (ir.sweep_index, ir.acc_metro, ir.att_metro)
+        assert obs.shape == (3, 4) and list(obs[:, 0]) == [energies[1], energies[3], energies[5]]
+
+
+@pytest.mark.parametrize("lanes", LANES)
 @pytest.mark.parametrize("case", [c for c in fixture_cases() if c[0] in ("dimer", "dimer+field", "bcc-tet", "fcc-l2", "sc-onsite")],
                          ids=lambda c: c[0])
 def test_persistent_equals_graph_path_and_oracle(case, lanes):
