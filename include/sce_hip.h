@@ -317,7 +317,7 @@ int32_t sce_chain_time_gradient(sce_chain* c, int32_t lanes, int32_t reps, doubl
  * for bit-exact parity tests ------------------------------------------------- */
 int32_t sce_test_philox(int64_t n, const uint64_t* seeds, const uint32_t* ctr4,
                         uint32_t* out4);
-/* op: 0 exp, 1 log, 2 sin, 3 cos */
+/* op: 0 exp, 1 log, 2 sin, 3 cos, 4 the surrogate's FP32 tanh (of x rounded to float) */
 int32_t sce_test_detmath(int64_t n, int32_t op, const double* x, double* out);
 int32_t sce_test_zlm(int64_t n, int32_t lmax, const double* dirs3, double* out);
 int32_t sce_test_proposal(int64_t n, uint64_t seed, const int32_t* sites,

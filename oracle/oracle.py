@@ -85,6 +85,7 @@ def lib():
             "ora_cpu_counters": (None, [P, P]),
             "ora_sur_features": (C.c_int, [P, P, C.c_int, P, P]),
             "ora_sur_delta": (d, [C.c_int, P, d, P, P]),
+            "ora_test_tanhf": (C.c_float, [C.c_float]),
             "ora_da_sweep": (i64, [P, P, P, P, d, d, u64, i32, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
@@ -132,6 +133,11 @@ def exp(x):
 
 def log(x):
     return lib().ora_log(float(x))
+
+
+def tanhf(x):
+    """The surrogate's FP32 tanh of x rounded to single precision."""
+    return lib().ora_test_tanhf(float(np.float32(x)))
 
 
 def sincos(x):

@@ -1323,12 +1323,41 @@ int ora_sur_features(const ora_tiled* t, const double* config, int s, const doub
   return F;
 }
 
+/* The surrogate's activation in IEEE single ops (the device's det_tanhf): odd
+ * Taylor polynomial of tanh through x^11 for |x| < 1/4; (1 - t)/(1 + t) with
+ * t = e^{-2|x|} for |x| <= 10, the exponential by Cody-Waite reduction
+ * (ln 2 = 0x1.63p-1 + -0x1.bd0106p-13), degree-7 Taylor polynomial and an exact
+ * 2^k; 1 beyond.  Pinned against tanh (<= 2.2 ulp), oddness, monotonicity and
+ * the two zeros by the oracle's tests. */
+static const float TANH_C[5] = {-0x1.555556p-2f, 0x1.111112p-3f, -0x1.ba1ba2p-5f, 0x1.664f48p-6f,
+                                -0x1.226e36p-7f}; /* x^3 .. x^11: -1/3, 2/15, -17/315, 62/2835, -1382/155925 */
+static const float EXPF_C[8] = {1.0f, 1.0f, 0.5f, 0x1.555556p-3f, 0x1.555556p-5f, 0x1.111112p-7f,
+                                0x1.6c16c2p-10f, 0x1.a01a02p-13f}; /* 1/n!, n = 0..7 */
+
 static float ora_tanhf(float z) {
-  double x = (double)z;
-  double t = ora_exp(-2.0 * fabs(x));
-  double r = (1.0 - t) / (1.0 + t);
-  return (float)(x < 0.0 ? -r : r);
+  float ax = fabsf(z), r;
+  if (ax < 0.25f) {
+    float x2 = ax * ax, p = TANH_C[4];
+    for (int k = 3; k >= 0; --k) p = p * x2 + TANH_C[k];
+    r = ax + ax * (x2 * p);
+  } else if (ax > 10.0f) {
+    r = 1.0f;
+  } else {
+    float y = -2.0f * ax;
+    float k = rintf(y * 0x1.715476p+0f);
+    float q = (y - k * 0x1.63p-1f) - k * -0x1.bd0106p-13f;
+    float p = EXPF_C[7];
+    for (int n = 6; n >= 0; --n) p = p * q + EXPF_C[n];
+    uint32_t b = (uint32_t)((int)k + 127) << 23;
+    float s;
+    memcpy(&s, &b, 4);
+    float t = p * s;
+    r = (1.0f - t) / (1.0f + t);
+  }
+  return z < 0.0f ? -r : r;
 }
+
+float ora_test_tanhf(float z) { return ora_tanhf(z); }
 
 /* params: [W1 (64 x F), b1 (64), W2 (64 x 64), b2 (64), w3 (64), v (F)].
  * Hidden pre-activations are fused multiply-add chains in ascending input

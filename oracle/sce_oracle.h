@@ -31,6 +31,8 @@ void ora_normal2(const uint32_t blk[4], double out[2]);
 /* ---- project deterministic math (Cody-Waite reduction + Taylor series) ---- */
 double ora_exp(double x);
 double ora_log(double x);
+/* the surrogate's FP32 activation (the device's det_tanhf) */
+float ora_test_tanhf(float z);
 void ora_sincos(double x, double* s, double* c);
 
 /* ---- tesseral row (src/gpu/zlm_device.jl) ---- */

@@ -1298,6 +1298,7 @@ __global__ void k_test_detmath(long long n, int op, const double* x, double* out
     case 0: out[i] = sce::det_exp(x[i]); break;
     case 1: out[i] = sce::det_log(x[i]); break;
     case 2: sce::det_sincos(x[i], &s, &c); out[i] = s; break;
+    case 4: out[i] = (double)sce::det_tanhf((float)x[i]); break;
     default: sce::det_sincos(x[i], &s, &c); out[i] = c; break;
   }
 }
@@ -3506,13 +3507,6 @@ __device__ __forceinline__ void sur_rows(int lane, int F, int width, int NB, con
   }
 }
 
-// tanh in FP32 from the deterministic exp: sign(x)(1 - t)/(1 + t), t = e^{-2|x|}
-__device__ __forceinline__ float det_tanhf(float z) {
-  const double x = (double)z;
-  const double t = sce::det_exp(-2.0 * fabs(x));
-  const double r = (1.0 - t) / (1.0 + t);
-  return (float)(x < 0.0 ? -r : r);
-}
 
 // Stages the neighbour spins of site s (read back from their rows) into E (NB x 3).
 template <int L>
@@ -3601,8 +3595,8 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
     hidden_pair(reinterpret_cast<const floatx4*>(net.W1Q) + j, sur_fpad(net.F) / 4,
                 reinterpret_cast<const floatx4*>(phiA), reinterpret_cast<const floatx4*>(phiB), zA, zB);
     __asm__ volatile("" : "+v"(zA), "+v"(zB));  // both chains complete before either tanh
-    h1A[j] = det_tanhf(zA);
-    h1B[j] = det_tanhf(zB);
+    h1A[j] = sce::det_tanhf(zA);
+    h1B[j] = sce::det_tanhf(zB);
   }
   __syncthreads();
   if (valid) {
@@ -3611,8 +3605,8 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
                 reinterpret_cast<const floatx4*>(h1B), zA, zB);
     __asm__ volatile("" : "+v"(zA), "+v"(zB));
     const float w3 = ((gfloat*)net.w3)[j];
-    const float oA = w3 * det_tanhf(zA);
-    const float oB = w3 * det_tanhf(zB);
+    const float oA = w3 * sce::det_tanhf(zA);
+    const float oB = w3 * sce::det_tanhf(zB);
     float sA = 0.0f, sB = 0.0f;
     for (int f = j; f < net.F; f += 64) {
       const float v = ((gfloat*)net.v)[f];

@@ -85,6 +85,43 @@ SCE_HD double det_exp(double x) {
   return p * pow2i(k);
 }
 
+// tanh in FP32 (the surrogate's activation): an odd Taylor polynomial through
+// x^11 below |x| = 1/4; (1 - t)/(1 + t) with t = e^{-2|x|} up to |x| = 10
+// (t from a Cody-Waite reduction by ln 2 with a 9-bit high part, the degree-7
+// Taylor exponential and an exact 2^k scale; 1 - t >= 0.39, no cancellation);
+// 1 beyond (t < 2.1e-9 rounds away).  Within 2.2 ulp of tanh; odd, with both
+// zeros mapped to +0.  IEEE single ops only, restated op for op by the oracle.
+SCE_HD float det_tanhf(float z) {
+  const float ax = __builtin_fabsf(z);
+  float r;
+  if (ax < 0.25f) {
+    const float x2 = ax * ax;
+    float p = -0x1.226e36p-7f;
+    p = p * x2 + 0x1.664f48p-6f;
+    p = p * x2 + -0x1.ba1ba2p-5f;
+    p = p * x2 + 0x1.111112p-3f;
+    p = p * x2 + -0x1.555556p-2f;
+    r = ax + ax * (x2 * p);
+  } else if (ax > 10.0f) {
+    r = 1.0f;
+  } else {
+    const float y = -2.0f * ax;
+    const float k = __builtin_rintf(y * 0x1.715476p+0f);
+    const float q = (y - k * 0x1.63p-1f) - k * -0x1.bd0106p-13f;
+    float p = 0x1.a01a02p-13f;
+    p = p * q + 0x1.6c16c2p-10f;
+    p = p * q + 0x1.111112p-7f;
+    p = p * q + 0x1.555556p-5f;
+    p = p * q + 0x1.555556p-3f;
+    p = p * q + 0.5f;
+    p = p * q + 1.0f;
+    p = p * q + 1.0f;
+    const float t = p * __builtin_bit_cast(float, (uint32_t)((int)k + 127) << 23);
+    r = (1.0f - t) / (1.0f + t);
+  }
+  return z < 0.0f ? -r : r;
+}
+
 SCE_HD double det_log(double x) {
   if (x != x || x < 0.0) return __builtin_nan("");
   if (x == 0.0) return -__builtin_inf();

@@ -47,9 +47,16 @@ def test_philox_kat_and_streams_on_device():
         assert O.philox4x32(c, (int(s) & 0xffffffff, int(s) >> 32)) == tuple(int(x) for x in d)
 
 
-@pytest.mark.parametrize("op", [0, 1, 2, 3])
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 4])
 def test_detmath_bitwise(op):
     rng = np.random.default_rng(op)
+    if op == 4:   # the surrogate's FP32 tanh, compared as single-precision bits
+        x = np.concatenate([rng.normal(0, 3, 5000), rng.uniform(-0.3, 0.3, 2000), [0.0, -0.0, 0.25, -10.0, 10.5]])
+        x = x.astype(np.float32).astype(np.float64)
+        dev = G.dev_detmath(op, x).astype(np.float32)
+        ref = np.array([O.tanhf(v) for v in x], np.float32)
+        assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
+        return
     if op == 0:
         x = np.concatenate([-rng.exponential(5, 5000), rng.uniform(-700, 700, 500), [0.0, -745.0, 709.0]])
         ref = [O.exp(v) for v in x]

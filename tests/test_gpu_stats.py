@@ -167,7 +167,7 @@ def test_pt_exchange_free_ladder_is_independent_chains():
         assert np.array_equal(gst.download()[0], r.final_configs[i])
 
 
-def test_delayed_acceptance_matches_plain_within_2_sigma():
+def test_delayed_acceptance_matches_plain_statistically():
     na, terms, dims = bcc_tet_small(4, dims=(6, 6, 6), isotropic=True)
     H = S.TiledHamiltonian(na, terms, dims)
     kT = 0.02
@@ -184,11 +184,18 @@ def test_delayed_acceptance_matches_plain_within_2_sigma():
                                            measure_interval=1), kT, H.n_active)
     sd = S.standard_stats(SG.gpu_da_run_sweeps(da, sur, None, 1 / kT, n, renorm_interval=1000,
                                                measure_interval=1), kT, H.n_active)
-    report = {}
+    # Delayed acceptance is exact whatever the surrogate, so the two chains
+    # sample one distribution and each z below is a draw of a unit normal.
+    # Five such gates at 2 sigma would fail together by chance about one run
+    # in five; the gate is every observable within 3 sigma and at most one of
+    # the five beyond 2 sigma (a chance failure rate of about 3.5%, while a
+    # bias of 3 sigma in any observable fails).
+    report, z = {}, []
     for k in ("energy", "specific_heat", "absm", "susceptibility", "binder"):
         a, b = sp[k], sd[k]
         sig = math.hypot(a.err, b.err)
         report[k] = (a.mean, b.mean, sig)
-        assert abs(a.mean - b.mean) < 2 * sig, (k, report)
-    print(report)
+        z.append(abs(a.mean - b.mean) / sig)
+    print(report, z)
+    assert max(z) < 3 and sum(x >= 2 for x in z) <= 1, (z, report)
     assert da.info().acc_stage1 < 0.8 * da.info().att_metro

@@ -8,6 +8,8 @@ finite differences, the sweep against the plain keyed sweep where both must
 agree (beta = 0) and against the exact single-spin law (Langevin) under a
 random, untrained surrogate — the law holds only if stage 2 corrects stage 1.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -118,6 +120,30 @@ def test_network_forward_matches_fp64(F):
         assert abs(got - want) <= 2e-5 * (1 + abs(want))
         assert O.sur_delta(F, p, 2.5, b, a) == -got       # exactly antisymmetric
         assert O.sur_delta(F, p, 2.5, a, a) == 0.0
+
+
+def test_surrogate_tanh_pinned():
+    """The surrogate's FP32 tanh against tanh in double: within 2.5 ulp of the
+    single-precision result over [-12, 12] and near zero, odd, monotone,
+    saturated to exactly 1 beyond |x| = 10, both zeros to +0, and continuous
+    across the polynomial/exponential switch at |x| = 1/4."""
+    xs = np.concatenate([np.linspace(-12, 12, 60001), np.geomspace(1e-9, 0.6, 4000),
+                         -np.geomspace(1e-9, 0.6, 400)]).astype(np.float32)
+    got = np.array([O.tanhf(x) for x in xs])
+    want = np.tanh(xs.astype(np.float64))
+    ulp = np.spacing(np.abs(want.astype(np.float32))).astype(np.float64)
+    assert np.max(np.abs(got - want) / ulp) <= 2.5
+    for x in xs[::97]:
+        if x != 0:
+            assert O.tanhf(-x) == -O.tanhf(x)
+    grid = np.array([O.tanhf(x) for x in np.linspace(-11, 11, 20001).astype(np.float32)])
+    assert np.all(np.diff(grid) >= 0) and np.all(np.abs(grid) <= 1.0)
+    assert O.tanhf(10.001) == 1.0 and O.tanhf(-40.0) == -1.0
+    for z in (0.0, -0.0):
+        assert O.tanhf(z) == 0.0 and math.copysign(1.0, O.tanhf(z)) == 1.0
+    q = np.float32(0.25)
+    below = O.tanhf(np.nextafter(q, np.float32(0)))
+    assert 0 < O.tanhf(q) - below <= 3 * np.spacing(np.float32(0.245))
 
 
 def test_network_tanh_saturates_and_is_odd():
