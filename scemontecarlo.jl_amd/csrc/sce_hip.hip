@@ -15,7 +15,10 @@
 //                        (+ the mean spin when measuring), folded into the
 //                        device-resident chain state
 // all on the chain's stream with no host synchronisation between sweeps; a
-// sweep is captured once as a hipGraph and replayed.
+// sweep is captured once as a hipGraph and replayed.  Within one call the
+// sweeps are chained (enqueue_metro): the next sweep's proposals ride in the
+// walk launches and the final reduction in the next sweep's smallest-class
+// walk, so a chained sweep is the colour walks plus one reduction launch.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -724,9 +727,10 @@ __global__ __launch_bounds__(256) void k_chunks(const double* __restrict__ x, lo
 // The first reduction level of a Metropolis sweep fused with proposals of the
 // next sweep: blocks below nchunk reduce, the others propose the range r (the
 // sweep's walks are complete, so every spin is final for the sweep).
-template <int LMAX>
+template <int LMAX, int MODE = MODE_METRO>
 __global__ __launch_bounds__(256) void k_chunks_propose(const double* __restrict__ x, long long n,
-                                                        const int* __restrict__ iacc, double* __restrict__ part,
+                                                        const int* __restrict__ iacc, const int* __restrict__ iatt,
+                                                        double* __restrict__ part,
                                                         long long* __restrict__ ipart,
                                                         const double* __restrict__ config,
                                                         const uint8_t* __restrict__ active, long long n_sites,
@@ -734,7 +738,7 @@ __global__ __launch_bounds__(256) void k_chunks_propose(const double* __restrict
                                                         double* __restrict__ prop, const ChainDev* __restrict__ st,
                                                         PropRange r) {
   if ((long long)blockIdx.x < nchunk) {
-    chunk_body<MODE_METRO>(blockIdx.x, x, n, iacc, nullptr, part, ipart, config, active, n_sites, meas);
+    chunk_body<MODE>(blockIdx.x, x, n, iacc, iatt, part, ipart, config, active, n_sites, meas);
     return;
   }
   const int t = (int)(((long long)blockIdx.x - nchunk) * blockDim.x + threadIdx.x);
@@ -1776,7 +1780,7 @@ static void launch_reduce(const double* x, long long n, const int* iacc, const i
 }
 
 // launch_reduce<MODE_METRO> with the next sweep's proposals in the first level.
-template <int LMAX>
+template <int LMAX, int MODE = MODE_METRO>
 static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s, bool defer_final, PropRange r) {
   const DevTables& D = c->t->dev;
   const long long n = D.n_active;
@@ -1785,12 +1789,18 @@ static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s, bool de
   const long long nchunk = std::max(nbx, nchm);
   const long long nprop = prop_blocks(r.n);
   double* scratch = c->partial + nbx + 3 * std::max<long long>(nchm, 1);
-  k_chunks_propose<LMAX><<<(unsigned)(nchunk + nprop), 256, 0, s>>>(c->dE, n, c->acc, c->partial, c->ip64,
+  k_chunks_propose<LMAX, MODE><<<(unsigned)(nchunk + nprop), 256, 0, s>>>(c->dE, n, c->acc,
+                                                                         MODE == MODE_DA ? c->att : nullptr,
+                                                                         c->partial, c->ip64,
                                                                    c->config, D.active, D.n_sites, meas, nchunk, D,
                                                                    c->prop, c->st, r);
   if (!defer_final)
-    k_final<MODE_METRO><<<1, 256, 0, s>>>(c->partial, n, c->ip64, D.n_sites, meas, c->st, c->obs, D.n_active,
-                                          (double)D.n_active, scratch, meas ? c->bin : nullptr);
+    k_final<MODE><<<1, 256, 0, s>>>(c->partial, n, c->ip64, D.n_sites, meas, c->st, c->obs, D.n_active,
+                                    (double)D.n_active, scratch, meas ? c->bin : nullptr);
+}
+template <int LMAX>
+static void launch_reduce_propose_da(sce_chain* c, int meas, hipStream_t s, bool defer_final, PropRange r) {
+  launch_reduce_propose<LMAX, MODE_DA>(c, meas, s, defer_final, r);
 }
 
 // The operands of sweep-final reduction (k_final<MODE_METRO>) of a chain whose
@@ -1843,6 +1853,80 @@ static int32_t set_beta(sce_chain* c, double beta) {
 // the proposal kernel.)
 // need_prop: propose this sweep first (otherwise the previous sweep's
 // reduction did); next_prop: propose the next sweep in this sweep's reduction.
+// Where a chained sweep's proposals and deferred final go (enqueue_metro,
+// enqueue_da): launch k (0..K-1) walks cols[k]; at[k] are the proposal ranges
+// it carries as extra blocks (at[K]: the reduction launch; at most two per
+// walk launch, one for the reduction); before/after: standalone proposal
+// launches; kf: the walk launch whose block carries the pending final.
+struct ChainPlan {
+  std::vector<int> cols;
+  int K = 0, kf = -1;
+  std::vector<std::vector<PropRange>> at;
+  std::vector<PropRange> before, after;
+};
+
+static int32_t plan_chain(const sce_tables* t, bool next_prop, int pending, bool carry_last, bool walk_props,
+                          ChainPlan* P) {
+  ChainPlan& c = *P;
+  for (int col = 0; col < t->n_colors; ++col)
+    if (t->color_ptr[col + 1] > t->color_ptr[col]) c.cols.push_back(col);
+  const int K = c.K = (int)c.cols.size();
+  if (pending >= 0)
+    for (int k = 0; k < K; ++k)
+      if (c.cols[k] == t->fin_col) c.kf = k;
+  const int kf = c.kf;
+  // A proposal for this sweep (next = 0) or the next (next = 1) made at launch
+  // j (-1: before the walks, K: the reduction) keys on the completed-sweep
+  // count plus this offset; before the deferred final has run it lags by one.
+  auto off_at = [&](int j, int next) { return 1 + next + ((kf >= 0 && j < kf) ? 1 : 0); };
+  auto range_of = [&](int k) {
+    const int lo = t->color_ptr[c.cols[k]];
+    return PropRange{lo, t->color_ptr[c.cols[k] + 1] - lo, 0};
+  };
+  c.at.assign((size_t)K + 1, {});
+  auto put = [&](int j, PropRange r, int next) {
+    r.off = off_at(j, next);
+    std::vector<PropRange>& v = c.at[j];
+    if (!v.empty() && v.back().off == r.off && v.back().lo + v.back().n == r.lo)
+      v.back().n += r.n;  // the classes are contiguous in colour order
+    else
+      v.push_back(r);
+  };
+  if (carry_last && K > 0) {
+    // this sweep's proposals of its last class, made before that class's walk
+    int j = 0;
+    while (j < K - 1 && j == kf) ++j;
+    if (j < K - 1) {
+      put(j, range_of(K - 1), 0);
+    } else {
+      PropRange r = range_of(K - 1);
+      r.off = off_at(-1, 0);
+      c.before.push_back(r);
+    }
+  }
+  if (next_prop) {
+    for (int k = 0; k < K; ++k) {
+      if (walk_props && k == K - 1) break;  // carried into the next sweep
+      int j = walk_props ? k + 1 : K;
+      while (j < K && j == kf) ++j;
+      put(j, range_of(k), 1);
+    }
+  }
+  for (int j = 0; j < K; ++j)  // a walk launch carries at most two ranges
+    while (c.at[j].size() > 2) {
+      PropRange r = c.at[j].back();
+      c.at[j].pop_back();
+      if (r.off != off_at(j, 1)) return sce_fail(SCE_EINVAL, "internal: proposal placement");
+      r.off = off_at(K, 1);
+      c.after.push_back(r);
+    }
+  while (c.at[K].size() > 1) {
+    c.after.push_back(c.at[K].back());
+    c.at[K].pop_back();
+  }
+  return SCE_OK;
+}
+
 // One Metropolis sweep.  Chaining within a call (next_prop, and in the
 // previous sweep when !need_prop):
 //  - proposals: a class's proposals for the next sweep depend only on its own
@@ -1862,65 +1946,13 @@ static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop =
   const DevTables& D = c->t->dev;
   const sce_tables* t = c->t;
   hipStream_t s = c->stream;
-  const bool walk_props = !c->timing;
-  std::vector<int> cols;  // the non-empty colour classes, in sweep order
-  for (int col = 0; col < t->n_colors; ++col)
-    if (t->color_ptr[col + 1] > t->color_ptr[col]) cols.push_back(col);
-  const int K = (int)cols.size();
-  int kf = -1;  // the walk launch that carries the pending final
-  if (pending >= 0)
-    for (int k = 0; k < K; ++k)
-      if (cols[k] == t->fin_col) kf = k;
-  // A proposal for this sweep (next = 0) or the next (next = 1) made at launch
-  // j (-1: before the walks, K: the reduction) keys on the completed-sweep
-  // count plus this offset; before the deferred final has run it lags by one.
-  auto off_at = [&](int j, int next) { return 1 + next + ((kf >= 0 && j < kf) ? 1 : 0); };
-  auto range_of = [&](int k) {
-    const int lo = t->color_ptr[cols[k]];
-    return PropRange{lo, t->color_ptr[cols[k] + 1] - lo, 0};
-  };
-  std::vector<std::vector<PropRange>> at((size_t)K + 1);
-  std::vector<PropRange> before, after;
-  auto put = [&](int j, PropRange r, int next) {
-    r.off = off_at(j, next);
-    std::vector<PropRange>& v = at[j];
-    if (!v.empty() && v.back().off == r.off && v.back().lo + v.back().n == r.lo)
-      v.back().n += r.n;  // the classes are contiguous in colour order
-    else
-      v.push_back(r);
-  };
-  if (carry_last && K > 0) {
-    // this sweep's proposals of its last class, made before that class's walk
-    int j = 0;
-    while (j < K - 1 && j == kf) ++j;
-    if (j < K - 1) {
-      put(j, range_of(K - 1), 0);
-    } else {
-      PropRange r = range_of(K - 1);
-      r.off = off_at(-1, 0);
-      before.push_back(r);
-    }
-  }
-  if (next_prop) {
-    for (int k = 0; k < K; ++k) {
-      if (walk_props && k == K - 1) break;  // carried into the next sweep
-      int j = walk_props ? k + 1 : K;
-      while (j < K && j == kf) ++j;
-      put(j, range_of(k), 1);
-    }
-  }
-  for (int j = 0; j < K; ++j)  // a walk launch carries at most two ranges
-    while (at[j].size() > 2) {
-      PropRange r = at[j].back();
-      at[j].pop_back();
-      if (r.off != off_at(j, 1)) return sce_fail(SCE_EINVAL, "internal: proposal placement");
-      r.off = off_at(K, 1);
-      after.push_back(r);
-    }
-  while (at[K].size() > 1) {
-    after.push_back(at[K].back());
-    at[K].pop_back();
-  }
+  ChainPlan P;
+  CHK(plan_chain(t, next_prop, pending, carry_last, !c->timing, &P));
+  const int K = P.K, kf = P.kf;
+  const std::vector<int>& cols = P.cols;
+  const std::vector<std::vector<PropRange>>& at = P.at;
+  const std::vector<PropRange>& before = P.before;
+  const std::vector<PropRange>& after = P.after;
   for (const PropRange& r : before) SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, r, s);
   if (need_prop) SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
   for (int k = 0; k < K; ++k) {
@@ -3639,15 +3671,15 @@ __host__ __device__ inline size_t da_lds_bytes(int walk_stride, int FP) {
 // neighbour rows are staged once: the descriptor reads the spins from them and
 // the stage-1 survivors walk them.
 template <int LMAX>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da(
-    DevTables T, double* config, double* zrows, const double* prop, double* dE, int* acc, int* acc1,
-    const ChainDev* st, const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP) {
+__device__ __forceinline__ void da_body(DevTables T, double* config, double* zrows, const double* prop, double* dE,
+                                        int* acc, int* acc1, const ChainDev* st, const SurNet* nets, int q_lo,
+                                        int q_n, int walk_stride, int FP, int vb, int nvb) {
   constexpr int L = 64, G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   extern __shared__ double lds[];
   // one site per wavefront: its index, site and sublattice are wave-uniform
   // (scalar registers, scalar loads)
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / L), lane = threadIdx.x % L;
-  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
+  const int qi = xcd_swizzle(vb, nvb) * G + g;
   const bool valid = qi < q_n;
   const int q = q_lo + qi;
   double* base = lds + g * da_group_stride(walk_stride, FP);
@@ -3715,6 +3747,34 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     acc[q] = accept;
     acc1[q] = pass;
   }
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da(
+    DevTables T, double* config, double* zrows, const double* prop, double* dE, int* acc, int* acc1,
+    const ChainDev* st, const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP) {
+  da_body<LMAX>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo, q_n, walk_stride, FP, blockIdx.x, gridDim.x);
+}
+
+// A delayed-acceptance colour step of a chained sweep: blocks below nwalk run
+// k_da's step, the rest make proposals of other classes (r0, then r1), as
+// k_metro_prop's extra blocks do.  (The final reduction of the previous sweep
+// runs as its own launch before the sweep's walks: carried in this kernel it
+// raised the kernel to 136 VGPRs with spills.)
+template <int LMAX>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da_chain(
+    DevTables T, double* config, double* zrows, double* prop, double* dE, int* acc, int* acc1, const ChainDev* st,
+    const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP, int nwalk, PropRange r0, PropRange r1) {
+  const int b = (int)blockIdx.x;
+  if (b < nwalk) {
+    da_body<LMAX>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo, q_n, walk_stride, FP, b, nwalk);
+    return;
+  }
+  const int pb = b - nwalk, nb0 = prop_blocks(r0.n);
+  if (pb < nb0)
+    propose_body<LMAX>(T, config, prop, st, r0.lo, r0.n, pb * TPB + (int)threadIdx.x, r0.off);
+  else
+    propose_body<LMAX>(T, config, prop, st, r1.lo, r1.n, (pb - nb0) * TPB + (int)threadIdx.x, r1.off);
 }
 
 // Descriptor rows of a colour slice (for harvesting and tests): one wavefront
@@ -4417,20 +4477,59 @@ static void launch_da(const DevTables& D, sce_chain* c, const SurNet* nets, int 
                                                                c->att, c->st, nets, lo, n, ws, FP);
 }
 
-static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas) {
+template <int LMAX>
+static void launch_da_chain(const DevTables& D, sce_chain* c, const SurNet* nets, int lo, int n, PropRange r0,
+                            PropRange r1, hipStream_t s) {
+  constexpr int G = TPB / 64, NLM = (LMAX + 1) * (LMAX + 1);
+  const int ws = D.nb_max * NLM + 1 + NLM;
+  const int FP = sur_fpad(D.nfeat_max);
+  const int nwalk = (n + G - 1) / G;
+  const int nb = nwalk + prop_blocks(r0.n) + prop_blocks(r1.n);
+  k_da_chain<LMAX><<<nb, TPB, da_lds_bytes(ws, FP), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc, c->att, c->st,
+                                                          nets, lo, n, ws, FP, nwalk, r0, r1);
+}
+
+// One delayed-acceptance sweep, chained within a call as enqueue_metro chains
+// plain sweeps (the plan of plan_chain): the next sweep's proposals ride in the
+// walk launches; the previous sweep's final (k_final<MODE_DA>, pending >= 0)
+// is its own launch before the walks, so no launch lags the sweep count.
+static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas, bool need_prop = true, bool next_prop = false,
+                          int pending = -1, bool carry_last = false) {
   const DevTables& D = c->t->dev;
+  const sce_tables* t = c->t;
   const int ws = D.nb_max * D.nlm + 1 + D.nlm;
   if (da_lds_bytes(ws, sur_fpad(D.nfeat_max)) > 65536)
     return sce_fail(SCE_EINVAL, "delayed-acceptance stencil too large for one block");
   if (c->sweeps_done >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
-  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, c->stream);
-  for (int col = 0; col < c->t->n_colors; ++col) {
-    const int lo = c->t->color_ptr[col], n = c->t->color_ptr[col + 1] - lo;
-    if (n == 0) continue;
-    SCE_DISPATCH_LMAX(D.lmax, launch_da, D, c, s->d_nets, lo, n, c->stream);
+  hipStream_t st = c->stream;
+  if (pending >= 0) {
+    const FinalArgs f = final_args(c, pending);
+    k_final<MODE_DA><<<1, 256, 0, st>>>(f.part, f.n, f.ipart, f.n_sites, f.meas, c->st, f.out, f.n_att, f.n_div,
+                                        f.scratch, f.bin);
   }
-  launch_reduce<MODE_DA>(c->dE, D.n_active, c->acc, c->att, c->partial, c->ip64, c->config, D.active, D.n_sites,
-                         meas, c->st, c->obs, D.n_active, (double)D.n_active, c->stream, c->bin);
+  ChainPlan P;
+  CHK(plan_chain(t, next_prop, -1, carry_last, true, &P));
+  const PropRange none{0, 0, 0};
+  for (const PropRange& r : P.before)
+    SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, r, st);
+  if (need_prop) SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, st);
+  for (int k = 0; k < P.K; ++k) {
+    const int lo = t->color_ptr[P.cols[k]], n = t->color_ptr[P.cols[k] + 1] - lo;
+    if (!P.at[k].empty()) {
+      SCE_DISPATCH_LMAX(D.lmax, launch_da_chain, D, c, s->d_nets, lo, n, P.at[k][0],
+                        P.at[k].size() > 1 ? P.at[k][1] : none, st);
+    } else {
+      SCE_DISPATCH_LMAX(D.lmax, launch_da, D, c, s->d_nets, lo, n, st);
+    }
+  }
+  if (next_prop) {
+    SCE_DISPATCH_LMAX(D.lmax, launch_reduce_propose_da, c, meas, st, true, P.at[P.K].empty() ? none : P.at[P.K][0]);
+  } else {
+    launch_reduce<MODE_DA>(c->dE, D.n_active, c->acc, c->att, c->partial, c->ip64, c->config, D.active, D.n_sites,
+                           meas, c->st, c->obs, D.n_active, (double)D.n_active, st, c->bin);
+  }
+  for (const PropRange& r : P.after)
+    SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, r, st);
   c->sweeps_done += 1;
   return SCE_OK;
 }
@@ -4462,53 +4561,59 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
     HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
-  if (!c->use_graph) {
-    // direct launches (the chain's graph replay switched off)
-    for (int64_t i = 1; i <= nsweeps; ++i) {
-      const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
-      const bool meas = measure_interval > 0 && i % measure_interval == 0;
-      const bool fused = meas && !renorm;
-      CHK(enqueue_da(c, s, fused ? 1 : 0));
-      if (renorm) enqueue_renorm(c);
-      if (meas && !fused) enqueue_measure(c);
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (want > 0) HIPCHK(hipMemcpy(obs, c->obs, 4 * (size_t)want * 8, hipMemcpyDeviceToHost));
-    if (n_obs) *n_obs = want;
-    return SCE_OK;
-  }
-  // capture one DA sweep per measurement flavour
-  hipGraphExec_t ge[2] = {nullptr, nullptr};
-  for (int k = 0; k < 2; ++k) {
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const long long before = c->sweeps_done;
-    const int32_t r = enqueue_da(c, s, k);
-    c->sweeps_done = before;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (r != SCE_OK || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      for (auto& x : ge)
-        if (x) (void)hipGraphExecDestroy(x);
-      return r != SCE_OK ? r : sce_fail(SCE_EHIP, "DA capture failed: %s", hipGetErrorString(e));
-    }
-    e = hipGraphInstantiate(&ge[k], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-  }
+  // Chained within the call as plain sweeps are (enqueue_da); with graph
+  // replay each variant [fused measurement + 2 need_prop + 4 next_prop + 8
+  // measurement flag of the pending final] is captured on first use.
+  hipGraphExec_t ge[16] = {};
+  auto drop = [&] {
+    for (auto& x : ge)
+      if (x) (void)hipGraphExecDestroy(x);
+  };
   int32_t rc = SCE_OK;
+  bool have_prop = false;
+  int pending = -1;
   for (int64_t i = 1; i <= nsweeps && rc == SCE_OK; ++i) {
     const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
     const bool meas = measure_interval > 0 && i % measure_interval == 0;
     const bool fused = meas && !renorm;
-    if (hipGraphLaunch(ge[fused ? 1 : 0], c->stream) != hipSuccess) rc = sce_fail(SCE_EHIP, "hipGraphLaunch failed");
-    c->sweeps_done += 1;
+    const bool next_prop = !renorm && i < nsweeps;
+    if (c->use_graph) {
+      const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0) + (pending == 1 ? 8 : 0);
+      if (!ge[k]) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+          rc = sce_fail(SCE_EHIP, "DA capture failed to begin");
+          break;
+        }
+        const long long before = c->sweeps_done;
+        const bool chained = (k & 2) == 0;
+        const int32_t r = enqueue_da(c, s, k & 1, !chained, (k & 4) != 0, chained ? (k >> 3) & 1 : -1, chained);
+        c->sweeps_done = before;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r != SCE_OK || e != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          rc = r != SCE_OK ? r : sce_fail(SCE_EHIP, "DA capture failed: %s", hipGetErrorString(e));
+          break;
+        }
+        e = hipGraphInstantiate(&ge[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          rc = sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+          break;
+        }
+      }
+      if (hipGraphLaunch(ge[k], c->stream) != hipSuccess) rc = sce_fail(SCE_EHIP, "hipGraphLaunch failed");
+      c->sweeps_done += 1;
+    } else {
+      rc = enqueue_da(c, s, fused ? 1 : 0, !have_prop, next_prop, pending, have_prop);
+    }
+    have_prop = next_prop;
+    pending = next_prop ? (fused ? 1 : 0) : -1;
     if (renorm) enqueue_renorm(c);
     if (meas && !fused) enqueue_measure(c);
   }
-  for (auto& x : ge)
-    if (x) (void)hipGraphExecDestroy(x);
+  if (rc != SCE_OK) (void)hipStreamSynchronize(c->stream);
+  drop();
   CHK(rc);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
