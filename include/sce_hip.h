@@ -317,6 +317,14 @@ int32_t sce_chain_time_gradient(sce_chain* c, int32_t lanes, int32_t reps, doubl
  * for bit-exact parity tests ------------------------------------------------- */
 int32_t sce_test_philox(int64_t n, const uint64_t* seeds, const uint32_t* ctr4,
                         uint32_t* out4);
+/* The placement of a chained sweep's proposals (host only, no device): for
+ * classes with 0-based offsets color_ptr[0..n_colors], records (launch, lo, n,
+ * sweep offset) of the proposal ranges; launch -1 = before the walks, 0..K-1 =
+ * the K non-empty classes' walk launches, K = the reduction launch, K + 1 =
+ * after it.  kf: the walk launch carrying a pending final (-1 if none). */
+int32_t sce_test_chain_plan(int32_t n_colors, const int32_t* color_ptr, int32_t next_prop, int32_t pending,
+                            int32_t carry_last, int32_t walk_props, int32_t* out, int32_t cap, int32_t* n_out,
+                            int32_t* kf);
 /* op: 0 exp, 1 log, 2 sin, 3 cos, 4 the surrogate's FP32 tanh (of x rounded to float) */
 int32_t sce_test_detmath(int64_t n, int32_t op, const double* x, double* out);
 int32_t sce_test_zlm(int64_t n, int32_t lmax, const double* dirs3, double* out);

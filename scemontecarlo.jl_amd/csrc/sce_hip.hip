@@ -1865,23 +1865,25 @@ struct ChainPlan {
   std::vector<PropRange> before, after;
 };
 
-static int32_t plan_chain(const sce_tables* t, bool next_prop, int pending, bool carry_last, bool walk_props,
-                          ChainPlan* P) {
+// color_ptr: the n_colors + 1 class offsets (0-based); fin_col: the class whose
+// walk carries a pending final.
+static int32_t plan_chain(int n_colors, const int32_t* color_ptr, int fin_col, bool next_prop, int pending,
+                          bool carry_last, bool walk_props, ChainPlan* P) {
   ChainPlan& c = *P;
-  for (int col = 0; col < t->n_colors; ++col)
-    if (t->color_ptr[col + 1] > t->color_ptr[col]) c.cols.push_back(col);
+  for (int col = 0; col < n_colors; ++col)
+    if (color_ptr[col + 1] > color_ptr[col]) c.cols.push_back(col);
   const int K = c.K = (int)c.cols.size();
   if (pending >= 0)
     for (int k = 0; k < K; ++k)
-      if (c.cols[k] == t->fin_col) c.kf = k;
+      if (c.cols[k] == fin_col) c.kf = k;
   const int kf = c.kf;
   // A proposal for this sweep (next = 0) or the next (next = 1) made at launch
   // j (-1: before the walks, K: the reduction) keys on the completed-sweep
   // count plus this offset; before the deferred final has run it lags by one.
   auto off_at = [&](int j, int next) { return 1 + next + ((kf >= 0 && j < kf) ? 1 : 0); };
   auto range_of = [&](int k) {
-    const int lo = t->color_ptr[c.cols[k]];
-    return PropRange{lo, t->color_ptr[c.cols[k] + 1] - lo, 0};
+    const int lo = color_ptr[c.cols[k]];
+    return PropRange{lo, color_ptr[c.cols[k] + 1] - lo, 0};
   };
   c.at.assign((size_t)K + 1, {});
   auto put = [&](int j, PropRange r, int next) {
@@ -1947,7 +1949,7 @@ static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop =
   const sce_tables* t = c->t;
   hipStream_t s = c->stream;
   ChainPlan P;
-  CHK(plan_chain(t, next_prop, pending, carry_last, !c->timing, &P));
+  CHK(plan_chain(t->n_colors, t->color_ptr.data(), t->fin_col, next_prop, pending, carry_last, !c->timing, &P));
   const int K = P.K, kf = P.kf;
   const std::vector<int>& cols = P.cols;
   const std::vector<std::vector<PropRange>>& at = P.at;
@@ -2709,6 +2711,38 @@ extern "C" int32_t sce_test_philox(int64_t n, const uint64_t* seeds, const uint3
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   return o.down(out4, 4 * n);
+}
+
+extern "C" int32_t sce_test_chain_plan(int32_t n_colors, const int32_t* color_ptr, int32_t next_prop,
+                                       int32_t pending, int32_t carry_last, int32_t walk_props, int32_t* out,
+                                       int32_t cap, int32_t* n_out, int32_t* kf) {
+  if (n_colors < 0 || !color_ptr || !out || !n_out || !kf) return sce_fail(SCE_EINVAL, "NULL argument");
+  for (int c = 0; c < n_colors; ++c)
+    if (color_ptr[c + 1] < color_ptr[c]) return sce_fail(SCE_EINVAL, "class offsets must not decrease");
+  int fin_col = -1;  // the smallest non-empty class, as sce_tables_create picks it
+  for (int c = 0; c < n_colors; ++c) {
+    const int n = color_ptr[c + 1] - color_ptr[c];
+    if (n > 0 && (fin_col < 0 || n < color_ptr[fin_col + 1] - color_ptr[fin_col])) fin_col = c;
+  }
+  ChainPlan P;
+  CHK(plan_chain(n_colors, color_ptr, fin_col, next_prop != 0, pending, carry_last != 0, walk_props != 0, &P));
+  int m = 0;
+  auto emit = [&](int launch, const PropRange& r) {
+    if (m < cap) {
+      out[4 * m] = launch;
+      out[4 * m + 1] = r.lo;
+      out[4 * m + 2] = r.n;
+      out[4 * m + 3] = r.off;
+    }
+    ++m;
+  };
+  for (const PropRange& r : P.before) emit(-1, r);
+  for (int j = 0; j <= P.K; ++j)
+    for (const PropRange& r : P.at[j]) emit(j, r);
+  for (const PropRange& r : P.after) emit(P.K + 1, r);
+  *n_out = m;
+  *kf = P.kf;
+  return m <= cap ? SCE_OK : sce_fail(SCE_EDIM, "%d placements, room for %d", m, cap);
 }
 
 extern "C" int32_t sce_test_detmath(int64_t n, int32_t op, const double* x, double* out) {
@@ -4508,7 +4542,7 @@ static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas, bool need_pr
                                         f.scratch, f.bin);
   }
   ChainPlan P;
-  CHK(plan_chain(t, next_prop, -1, carry_last, true, &P));
+  CHK(plan_chain(t->n_colors, t->color_ptr.data(), t->fin_col, next_prop, -1, carry_last, true, &P));
   const PropRange none{0, 0, 0};
   for (const PropRange& r : P.before)
     SCE_DISPATCH_LMAX(D.lmax, launch_propose_range, D, c->config, c->prop, c->st, r, st);

@@ -305,6 +305,19 @@ def dev_philox(seeds, ctrs):
     return out
 
 
+def chain_plan(color_ptr, next_prop: bool, pending: int, carry_last: bool, walk_props: bool = True):
+    """Where a chained sweep makes its proposals (host only): (records, kf) with
+    records rows (launch, lo, n, sweep offset), launch -1 before the walks,
+    0..K-1 the non-empty classes' walks, K the reduction, K + 1 after it."""
+    cp = np.ascontiguousarray(color_ptr, np.int32)
+    out = np.zeros((4 * len(cp) + 8, 4), np.int32)
+    n = C.c_int32()
+    kf = C.c_int32()
+    L.check(L.lib().sce_test_chain_plan(len(cp) - 1, L.ptr(cp), int(next_prop), int(pending), int(carry_last),
+                                        int(walk_props), L.ptr(out), len(out), C.byref(n), C.byref(kf)))
+    return out[: n.value], kf.value
+
+
 def dev_detmath(op: int, x):
     x = np.ascontiguousarray(x, np.float64)
     out = np.zeros_like(x)
