@@ -416,7 +416,7 @@ def main():
     grad_ms = S.gpu_time_gradient(gst, S.GRADIENT_LANES, 20)
     adj_bytes = 16 * int(H.site_prog.size)
     line["energy_gradient"] = {
-        "kernel": f"k_grad (all sites, {S.GRADIENT_LANES} strided shares per site)",
+        "kernel": f"k_grad_walk (all sites, {S.GRADIENT_LANES} strided shares per site)",
         "ms_per_call": grad_ms, "site_gradients_per_s": H.n_sites / (grad_ms * 1e-3),
         "adjacency_bytes_per_call": adj_bytes,
         "adjacency_GBs": adj_bytes / (grad_ms * 1e-3) / 1e9,

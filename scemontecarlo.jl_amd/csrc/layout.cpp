@@ -59,6 +59,8 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
   L->na = na;
   L->walk_ptr.assign(na + 1, 0);
   L->nb_ptr.assign(na + 1, 0);
+  L->prog_ptr.assign(na + 1, 0);
+  L->prog_off.clear();
   L->walk.clear();
   L->nb.clear();
   L->feat_ptr.assign(na + 1, 0);
@@ -80,6 +82,7 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
     L->walk_ptr[a] = (int32_t)L->walk.size();
     L->nb_ptr[a] = (int32_t)(L->nb.size() / 4);
     L->feat_ptr[a] = (int32_t)(L->feat.size() / 4);
+    L->prog_ptr[a] = (int32_t)L->prog_off.size();
     if (!d->site_active[a]) continue;
     std::map<std::tuple<int, int, int, int>, int> nbmap;
     std::vector<std::array<int, 4>> nbl;
@@ -115,6 +118,7 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
       ++pos;
       const int pid = d->site_prog[j - 1];
       const int col = d->site_col[j - 1];
+      L->prog_off.push_back((int32_t)(L->walk.size() - L->walk_ptr[a]));
       for (int e = d->sprog_ptr[pid - 1]; e < d->sprog_ptr[pid]; ++e) {
         WalkEntry we;
         we.w = d->sent_w[e - 1];
@@ -138,6 +142,7 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
         L->walk.push_back(we);
       }
     }
+    L->prog_off.push_back((int32_t)(L->walk.size() - L->walk_ptr[a]));
     for (auto& v : nbl) L->nb.insert(L->nb.end(), v.begin(), v.end());
     L->nb_max = std::max(L->nb_max, NB);
     // descriptor cluster specs, in walk order
@@ -164,6 +169,7 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
   }
   L->walk_ptr[na] = (int32_t)L->walk.size();
   L->nb_ptr[na] = (int32_t)(L->nb.size() / 4);
+  L->prog_ptr[na] = (int32_t)L->prog_off.size();
   L->feat_ptr[na] = (int32_t)(L->feat.size() / 4);
   for (int a = 0; a < na; ++a) L->w_max = std::max(L->w_max, L->walk_ptr[a + 1] - L->walk_ptr[a]);
 

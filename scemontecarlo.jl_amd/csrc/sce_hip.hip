@@ -99,6 +99,10 @@ struct DevTables {
   const int32_t* nb_ptr;
   const int4* nb;
   const int32_t* nbq;  // per colour position: nb_max neighbour site ids (padded with the site's own)
+  const int32_t* prog_ptr;  // na + 1: per sublattice, into prog_off
+  const int32_t* prog_off;  // walk offsets of the pid-sorted adjacency's programs (+ end)
+  const uint16_t* adj_perm; // per colour position: raw adjacency index -> pid-sorted index
+  int adj_max;              // adjacency entries per site, at most (the adj_perm stride)
   const int32_t* inst_term;  // 0-based
   const int32_t* inst_ptr;   // 0-based offsets
   const int32_t* inst_sites; // 0-based
@@ -1256,6 +1260,74 @@ __global__ __launch_bounds__(TPB) void k_grad(GradTables T, const double* __rest
   }
 }
 
+// The same gradient on the walk layout: one wavefront per active site (colour
+// position q), the neighbour rows staged in LDS as the walk stages them, and
+// each raw adjacency entry's program read from the sublattice's pid-sorted
+// walk (adj_perm maps raw to sorted index; the walk entry's product
+// (Z[f0] Z[f1]) Z[f2] with unused factors 1.0 is the reference's p of every
+// path bit for bit).  Shares, entry order, skips and fold as k_grad.  Rows of
+// inactive sites are not written (the caller zeroes G).
+template <int LMAX, int VL>
+__global__ __launch_bounds__(TPB) void k_grad_walk(DevTables T, const double* __restrict__ config,
+                                                   const double* __restrict__ zrows, double* __restrict__ G) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), W = TPB / 64, K = VL / 64;
+  extern __shared__ double lds[];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), h = threadIdx.x % 64;
+  const int zs = T.nb_max * NLM + 1;
+  double* Z = lds + w * (zs + 3 * NLM + 3 * VL);
+  double* grow = Z + zs;
+  double* part = grow + 3 * NLM;
+  const int q = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + w);
+  const bool valid = q < T.n_active;
+  int s = 0, a = 0;
+  if (valid) {
+    s = __builtin_amdgcn_readfirstlane(T.color_sites[q]);
+    a = stage_neighbours_q<64, NLM>(T, zrows, q, s, h, Z, [] {});
+    if (h < NLM) {
+      int l, m;
+      sce::lm_of(h, &l, &m);
+      double g[3];
+      sce::grad_zlm_one<LMAX>(l, m, config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2], g);
+      grow[3 * h] = g[0];
+      grow[3 * h + 1] = g[1];
+      grow[3 * h + 2] = g[2];
+    }
+  }
+  __syncthreads();
+  if (valid) {
+    const int pp = T.prog_ptr[a], nadj = T.prog_ptr[a + 1] - pp - 1;
+    const WalkEntry* __restrict__ wl = T.walk + T.walk_ptr[a];
+    const uint16_t* __restrict__ perm = T.adj_perm + (size_t)q * T.adj_max;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int v = h + 64 * k;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      for (int r = v; r < nadj; r += VL) {
+        const int kk = perm[r];
+        const int e1 = T.prog_off[pp + kk + 1];
+        for (int e = T.prog_off[pp + kk]; e < e1; ++e) {
+          const WalkEntry en = wl[e];
+          const double p = (Z[en.f0] * Z[en.f1]) * Z[en.f2];
+          if (p == 0.0) continue;
+          const double wp = en.w * p;
+          a0 = a0 + wp * grow[3 * en.tgt];
+          a1 = a1 + wp * grow[3 * en.tgt + 1];
+          a2 = a2 + wp * grow[3 * en.tgt + 2];
+        }
+      }
+      part[3 * v] = a0;
+      part[3 * v + 1] = a1;
+      part[3 * v + 2] = a2;
+    }
+  }
+  __syncthreads();
+  if (valid && h < 3) {
+    double g = 0.0;
+    for (int v = 0; v < VL; ++v) g += part[3 * v + h];
+    G[3 * (size_t)s + h] = g;
+  }
+}
+
 __global__ void k_inst_energy(DevTables T, const double* __restrict__ zrows,
                               double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1552,6 +1624,31 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
     }
     if ((r = dupload(t->mem, &p32, nq.data(), nq.size())) != SCE_OK) return fail(r);
     D.nbq = p32;
+    // the gradient deals a site's adjacency programs in their raw order; the
+    // walk holds them pid-sorted (translation invariant): the permutation
+    // between the two, per colour position
+    int amax = 0;
+    for (int a = 0; a < L.na; ++a) amax = std::max(amax, L.prog_ptr[a + 1] - L.prog_ptr[a] - 1);
+    if (amax > 65535) return fail(sce_fail(SCE_EINVAL, "%d adjacency entries per site exceed the gradient's index", amax));
+    std::vector<uint16_t> perm((size_t)d->n_active * std::max(amax, 1));
+    std::vector<int> js;
+    for (int q = 0; q < d->n_active; ++q) {
+      const int s0 = cs[q], j0 = d->site_ptr[s0], n = d->site_ptr[s0 + 1] - j0;
+      js.resize(n);
+      for (int k = 0; k < n; ++k) js[k] = k;
+      std::stable_sort(js.begin(), js.end(), [&](int x, int y) {
+        return d->site_prog[j0 + x - 1] < d->site_prog[j0 + y - 1];
+      });
+      for (int k = 0; k < n; ++k) perm[(size_t)q * std::max(amax, 1) + js[k]] = (uint16_t)k;
+    }
+    D.adj_max = std::max(amax, 1);
+    uint16_t* p16;
+    if ((r = dupload1(t->mem, &p16, perm.data(), perm.size())) != SCE_OK) return fail(r);
+    D.adj_perm = p16;
+    if ((r = dupload(t->mem, &p32, L.prog_ptr.data(), L.prog_ptr.size())) != SCE_OK) return fail(r);
+    D.prog_ptr = p32;
+    if ((r = dupload1(t->mem, &p32, L.prog_off.data(), L.prog_off.size())) != SCE_OK) return fail(r);
+    D.prog_off = p32;
   }
   if ((r = dupload(t->mem, &p32, L.feat_ptr.data(), L.feat_ptr.size())) != SCE_OK) return fail(r);
   D.feat_ptr = p32;
@@ -2582,13 +2679,27 @@ static void launch_rows_of(int n, const double* config, double* zrows, hipStream
 }
 
 template <int LMAX>
-static void launch_grad(const GradTables& T, const double* config, const double* zrows, double* G, int lanes,
-                        hipStream_t s) {
-  const int nb = (T.n_sites + TPB / 64 - 1) / (TPB / 64);
+// The walk-layout kernel where its LDS (neighbour rows, gradient rows and
+// share partials of four sites) fits one block, else the adjacency-table one;
+// the two are bit-identical.
+static void launch_grad(const GradTables& T, const DevTables& D, const double* config, const double* zrows, double* G,
+                        int lanes, hipStream_t s) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
+  const size_t lb = (size_t)(TPB / 64) * (D.nb_max * NLM + 1 + 3 * NLM + 3 * lanes) * sizeof(double);
+  if (lb > 65536) {
+    const int nb = (T.n_sites + TPB / 64 - 1) / (TPB / 64);
+    if (nb == 0) return;
+    if (lanes == 64) k_grad<LMAX, 64><<<nb, TPB, 0, s>>>(T, config, zrows, G);
+    else if (lanes == 128) k_grad<LMAX, 128><<<nb, TPB, 0, s>>>(T, config, zrows, G);
+    else k_grad<LMAX, 256><<<nb, TPB, 0, s>>>(T, config, zrows, G);
+    return;
+  }
+  (void)hipMemsetAsync(G, 0, 3 * (size_t)D.n_sites * sizeof(double), s);  // inactive sites
+  const int nb = (D.n_active + TPB / 64 - 1) / (TPB / 64);
   if (nb == 0) return;
-  if (lanes == 64) k_grad<LMAX, 64><<<nb, TPB, 0, s>>>(T, config, zrows, G);
-  else if (lanes == 128) k_grad<LMAX, 128><<<nb, TPB, 0, s>>>(T, config, zrows, G);
-  else k_grad<LMAX, 256><<<nb, TPB, 0, s>>>(T, config, zrows, G);
+  if (lanes == 64) k_grad_walk<LMAX, 64><<<nb, TPB, lb, s>>>(D, config, zrows, G);
+  else if (lanes == 128) k_grad_walk<LMAX, 128><<<nb, TPB, lb, s>>>(D, config, zrows, G);
+  else k_grad_walk<LMAX, 256><<<nb, TPB, lb, s>>>(D, config, zrows, G);
 }
 
 static int32_t check_grad_lanes(int lanes) {
@@ -2607,7 +2718,7 @@ extern "C" int32_t sce_energy_gradient(sce_tables* t, const double* config, doub
   CHK(z.alloc((size_t)D.nlm * n));
   CHK(g.alloc(3 * n));
   SCE_DISPATCH_LMAX(D.lmax, launch_rows_of, D.n_sites, cf.p, z.p, (hipStream_t)0);
-  SCE_DISPATCH_LMAX(D.lmax, launch_grad, t->grad, cf.p, z.p, g.p, lanes, (hipStream_t)0);
+  SCE_DISPATCH_LMAX(D.lmax, launch_grad, t->grad, D, cf.p, z.p, g.p, lanes, (hipStream_t)0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   return g.down(G, 3 * n);
@@ -2620,7 +2731,7 @@ extern "C" int32_t sce_chain_energy_gradient(sce_chain* c, double* G, int32_t la
   const DevTables& D = c->t->dev;
   Scratch<double> g;
   CHK(g.alloc(3 * (size_t)D.n_sites));
-  SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, c->config, c->zrows, g.p, lanes, c->stream);
+  SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, D, c->config, c->zrows, g.p, lanes, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return g.down(G, 3 * (size_t)D.n_sites);
@@ -2635,12 +2746,12 @@ extern "C" int32_t sce_chain_time_gradient(sce_chain* c, int32_t lanes, int32_t 
   const DevTables& D = c->t->dev;
   Scratch<double> g;
   CHK(g.alloc(3 * (size_t)D.n_sites));
-  SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, c->config, c->zrows, g.p, lanes, c->stream);
+  SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, D, c->config, c->zrows, g.p, lanes, c->stream);
   hipEvent_t e0 = next_event(c), e1 = next_event(c);
   c->ev_used = 0;
   HIPCHK(hipEventRecord(e0, c->stream));
   for (int r = 0; r < reps; ++r)
-    SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, c->config, c->zrows, g.p, lanes, c->stream);
+    SCE_DISPATCH_LMAX(D.lmax, launch_grad, c->t->grad, D, c->config, c->zrows, g.p, lanes, c->stream);
   HIPCHK(hipEventRecord(e1, c->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));

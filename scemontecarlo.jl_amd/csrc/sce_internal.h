@@ -36,6 +36,10 @@ struct HostLayout {
   std::vector<int32_t> walk_ptr;  // na + 1
   std::vector<WalkEntry> walk;
   std::vector<int32_t> nb_ptr;    // na + 1
+  // per sublattice, the walk offset (relative to walk_ptr[a]) where each of its
+  // pid-sorted adjacency entries' program entries start, plus the end
+  std::vector<int32_t> prog_ptr;  // na + 1, into prog_off
+  std::vector<int32_t> prog_off;
   std::vector<int32_t> nb;        // 4 ints per neighbour: atom (0-based), dc1, dc2, dc3
   // rotation-invariant surrogate descriptor of a site (per sublattice):
   // features [e.e_n for the NB stencil neighbours, (e.e_n)^2 for them, then one
