@@ -64,3 +64,24 @@ def test_gradient_rejects_bad_arguments():
         S.gpu_energy_gradient(gH, cfg, 32)
     with pytest.raises(ValueError):
         S.gpu_energy_gradient(gH, cfg[:-1], 128)
+
+
+def test_gradient_both_kernels_at_lmax_6():
+    """A body-1 l = 6 term raises the rows to 49 entries: with the 26-site bcc
+    stencil the walk-layout kernel's LDS (rows, gradient rows and share
+    partials of four sites) fits for 64 and 128 shares but not for 256, where
+    the adjacency-table kernel runs; both equal the oracle bit for bit."""
+    from sce_amd import models as M
+    na, terms = M.lattice_model("bcc", pair_shells=3, pair_lmax=1, seed=8)
+    rng = np.random.default_rng(8)
+    for a in range(1, na + 1):
+        terms.append(M.MultipoleTerm(1e-3 * rng.normal(), [a], [(0, 0, 0)], [6], rng.normal(size=13)))
+    H, T = build_pair(na, terms, (3, 3, 3))
+    assert H.lmax == 6
+    gH = S.GPUTiledHamiltonian(H)
+    assert gH.nb_max == 26
+    for lanes in (64, 128, 256):
+        lds = 4 * (26 * 49 + 1 + 3 * 49 + 3 * lanes) * 8
+        assert (lds <= 65536) == (lanes < 256)
+        cfg = rand_config(H.n_sites, 40 + lanes)
+        assert np.array_equal(S.gpu_energy_gradient(gH, cfg, lanes), T.energy_gradient(cfg, lanes=lanes)), lanes
