@@ -1265,8 +1265,10 @@ __global__ __launch_bounds__(TPB) void k_grad(GradTables T, const double* __rest
 // each raw adjacency entry's program read from the sublattice's pid-sorted
 // walk (adj_perm maps raw to sorted index; the walk entry's product
 // (Z[f0] Z[f1]) Z[f2] with unused factors 1.0 is the reference's p of every
-// path bit for bit).  Shares, entry order, skips and fold as k_grad.  Rows of
-// inactive sites are not written (the caller zeroes G).
+// path bit for bit).  A program's entries are loaded four at a time (clamped
+// indices, the extra terms dropped by selects) and added in entry order.
+// Shares, entry order, skips and fold as k_grad.  Rows of inactive sites are
+// not written (the caller zeroes G).
 template <int LMAX, int VL>
 __global__ __launch_bounds__(TPB) void k_grad_walk(DevTables T, const double* __restrict__ config,
                                                    const double* __restrict__ zrows, double* __restrict__ G) {
@@ -1305,14 +1307,22 @@ __global__ __launch_bounds__(TPB) void k_grad_walk(DevTables T, const double* __
       for (int r = v; r < nadj; r += VL) {
         const int kk = perm[r];
         const int e1 = T.prog_off[pp + kk + 1];
-        for (int e = T.prog_off[pp + kk]; e < e1; ++e) {
-          const WalkEntry en = wl[e];
-          const double p = (Z[en.f0] * Z[en.f1]) * Z[en.f2];
-          if (p == 0.0) continue;
-          const double wp = en.w * p;
-          a0 = a0 + wp * grow[3 * en.tgt];
-          a1 = a1 + wp * grow[3 * en.tgt + 1];
-          a2 = a2 + wp * grow[3 * en.tgt + 2];
+        for (int base = T.prog_off[pp + kk]; base < e1; base += 4) {
+          WalkEntry en[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) en[u] = wl[base + u < e1 ? base + u : e1 - 1];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double p = (Z[en[u].f0] * Z[en[u].f1]) * Z[en[u].f2];
+            const bool use = base + u < e1 && p != 0.0;
+            const double wp = en[u].w * p;
+            const double b0 = a0 + wp * grow[3 * en[u].tgt];
+            const double b1 = a1 + wp * grow[3 * en[u].tgt + 1];
+            const double b2 = a2 + wp * grow[3 * en[u].tgt + 2];
+            a0 = use ? b0 : a0;
+            a1 = use ? b1 : a1;
+            a2 = use ? b2 : a2;
+          }
         }
       }
       part[3 * v] = a0;
