@@ -221,6 +221,8 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
     return {"workload": f"64 replicas x fcc 12^3 = {R * H.n_sites} spins, ladder {kts[0]:.2e}-{kts[-1]:.2e} eV "
                         f"(J1 = {J1:.3e} eV), exchange every sweep on the device",
             "spin_updates_per_s": R * H.n_active * sweeps_measure / wall,
+            "bytes_per_update": algorithmic_bytes_per_update(H),
+            "algorithmic_GBs": R * H.n_active * sweeps_measure / wall * algorithmic_bytes_per_update(H) / 1e9,
             "ms_per_sweep_all_replicas": 1e3 * wall / sweeps_measure, "sweeps": sweeps_measure,
             "lanes": lanes, "measurements": int(obs.shape[1]),
             "swap_acceptance_mean": float(np.nanmean(sw)), "swap_acceptance_min": float(np.nanmin(sw)),
