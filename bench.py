@@ -364,6 +364,9 @@ def main():
     sur = train_info = None
     if not args.no_da:   # harvesting is plain sweeps: it continues the equilibration
         sur, train_info = train_surrogate(args, S, H, gH, gst, beta)
+    # untimed: a few measured sweeps, so that the graph variants the timed
+    # region replays (measured, chained) are captured before it starts
+    S.gpu_run_sweeps(gst, gH, None, beta, 8, renorm_interval=1000, lanes=args.lanes, measure_interval=1)
     snapshot = gst.download()[0]
     # kernel timing: the colour walks of 20 passes launched back to back between
     # one pair of HIP events on the chain's stream (state restored afterwards)
