@@ -3583,11 +3583,13 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // Inference reads the hidden layers' weights in quad-packed form:
 // WQ[(q * 64 + j) * 4 + r] = W[j][4q + r], so lane j's four weights of input quad
 // q are one 16-byte load and a wavefront's load of quad q is 1 KiB contiguous.
-// Layer 1 is zero padded to SUR_QB quads (64 inputs) per batch pair; the
-// descriptor rows in LDS are zero padded alike.
-constexpr int SUR_BQ = 4;              // quads per weight batch (16 floats per lane)
+// Layer 1 is zero padded to SUR_QB quads (16 inputs) per batch pair; the
+// descriptor rows in LDS are zero padded alike.  (Batches of 4 quads, padding
+// to 32 inputs, ran 3.5% slower per DA sweep: F = 112 then carried 16 zero
+// inputs.)
+constexpr int SUR_BQ = 2;              // quads per weight batch (8 floats per lane)
 constexpr int SUR_QB = 2 * SUR_BQ;     // quads per double batch
-constexpr int SUR_FQ = 4 * SUR_QB;     // inputs per double batch (64)
+constexpr int SUR_FQ = 4 * SUR_QB;     // inputs per double batch (16)
 
 __host__ __device__ inline int sur_fpad(int F) { return (F + SUR_FQ - 1) / SUR_FQ * SUR_FQ; }
 
@@ -3812,7 +3814,7 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
 }
 
 // LDS of one DA group (doubles): neighbour rows + dz (walk_stride, rounded up to
-// even) | phiA, phiB (FP floats each, FP a multiple of 64) | h1A, h1B (64 floats
+// even) | phiA, phiB (FP floats each, FP a multiple of SUR_FQ) | h1A, h1B (64 floats
 // each).  Every float block starts 16-byte aligned.
 __host__ __device__ inline int da_phi_offset(int walk_stride) { return (walk_stride + 1) & ~1; }
 __host__ __device__ inline int da_group_stride(int walk_stride, int FP) {
