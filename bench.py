@@ -151,6 +151,7 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta):
     ess_s = ess / wall if math.isfinite(ess) else None
     ess_s_p = ess_p / wall_plain if math.isfinite(ess_p) else None
     n = args.ess_sweeps
+    tw, tw_p = S.tau_int_windowed(obs[:, 0]), S.tau_int_windowed(obs_plain[:, 0])
     return {
         "sweeps": n,
         "spin_updates_per_s": H.n_active * n / wall,
@@ -163,7 +164,12 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta):
         "ess_per_s_ratio_bound": wall_plain / wall,
         "tau_int_E": (n / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
         "plain_same_length": {"ms_per_step": 1e3 * wall_plain / n, "effective_samples_per_s_E": ess_s_p,
-                              "tau_int_E": (n / ess_p - 1) / 2 if math.isfinite(ess_p) and ess_p > 0 else None},
+                              "tau_int_E": (n / ess_p - 1) / 2 if math.isfinite(ess_p) and ess_p > 0 else None,
+                              "tau_int_E_windowed": tw_p},
+        # cross-check with the autocorrelation summed to Sokal's window, whose
+        # estimate of a short tau scatters far less than the plateau's
+        "tau_int_E_windowed": tw,
+        "ess_per_s_ratio_windowed": (wall_plain * (1 + 2 * tw_p)) / (wall * (1 + 2 * tw)),
         "acceptance": inf.acc_metro / max(inf.att_metro, 1),
         "stage1_pass": inf.acc_stage1 / max(inf.att_metro, 1),
         "stage2_acceptance": inf.acc_metro / max(inf.acc_stage1, 1),

@@ -12,7 +12,7 @@ from .gpu import (GPUTiledHamiltonian, GPUChainState, gpu_metropolis_sweep,  # n
                   from_host, swap_payload, DEFAULT_LANES, gpu_energy_gradient,
                   gpu_chain_energy_gradient, gpu_time_gradient, GRADIENT_LANES)
 from .observables import (LogBinner, BinStore, jackknife, standard_stats,  # noqa: F401
-                          effective_samples)
+                          effective_samples, tau_int_windowed)
 from .batch import GPUBatch, gpu_run_pt, resume_pt, autotune_lanes  # noqa: F401
 from .surrogate import Surrogate, gpu_da_run_sweeps  # noqa: F401
 
@@ -21,7 +21,7 @@ __all__ = [
     "ChainState", "resolve_kt", "KB_EV", "MU_B_EV_T", "GPUTiledHamiltonian", "GPUChainState",
     "gpu_metropolis_sweep", "gpu_overrelaxation_sweep", "gpu_run_sweeps", "gpu_renormalize",
     "to_host", "from_host", "swap_payload", "DEFAULT_LANES", "LogBinner", "BinStore",
-    "jackknife", "standard_stats", "effective_samples", "GPUBatch", "gpu_run_pt", "resume_pt",
+    "jackknife", "standard_stats", "effective_samples", "tau_int_windowed", "GPUBatch", "gpu_run_pt", "resume_pt",
     "autotune_lanes", "Surrogate", "gpu_da_run_sweeps", "gpu_energy_gradient",
     "gpu_chain_energy_gradient", "gpu_time_gradient", "GRADIENT_LANES",
 ]

@@ -203,6 +203,30 @@ def standard_stats(obs: np.ndarray, kT: float, n_active: int, nbins: int = 32) -
     return stats_from_binners(binners, means, kT, n_active)
 
 
+def tau_int_windowed(series: np.ndarray, c: float = 6.0) -> float:
+    """tau_int by summing the normalised autocorrelation up to Sokal's automatic
+    window (the first M with M >= c (1/2 + sum_{t<=M} rho(t))), in the
+    log-binning convention (1 + 2 tau = variance ratio, so tau = 0 for iid
+    data).  A cross-check of the plateau estimate, which scatters widely when
+    tau is a few samples; not the reference's definition."""
+    x = np.asarray(series, np.float64)
+    n = len(x)
+    if n < 2:
+        return math.nan
+    x = x - x.mean()
+    f = np.fft.rfft(x, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n]
+    if acf[0] <= 0.0:
+        return math.nan
+    acf = acf / acf[0]
+    t = 0.5
+    for m in range(1, n):
+        t += acf[m]
+        if m >= c * t:
+            break
+    return t - 0.5
+
+
 def effective_samples(series: np.ndarray) -> float:
     """n / (1 + 2 tau_int), tau_int from the log-binning plateau (binning.jl:120-128)."""
     b = LogBinner(1).push_series(series)

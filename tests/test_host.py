@@ -158,6 +158,21 @@ def test_logbinner_iid_and_ar1():
     assert b.tau_int()[0] > 5
 
 
+def test_windowed_tau_on_ar1_and_iid():
+    """The windowed estimate against the AR(1) closed form tau = rho / (1 - rho)
+    (log-binning convention) for short and long tau, and 0 for iid data."""
+    for rho in (0.5, 0.8, 0.9):
+        n = 2 ** 17
+        e = np.random.default_rng(int(rho * 10)).standard_normal(n)
+        x = np.empty(n)
+        v = 0.0
+        for i in range(n):
+            v = rho * v + e[i]
+            x[i] = v
+        assert S.tau_int_windowed(x) == pytest.approx(rho / (1 - rho), rel=0.1, abs=0.05)
+    assert abs(S.tau_int_windowed(np.random.default_rng(3).standard_normal(2 ** 16))) < 0.05
+
+
 def test_binstore_and_jackknife():
     s = S.BinStore(1, 3, 4)
     for v in range(1, 15):
