@@ -65,7 +65,7 @@ double ora_uniform(uint32_t hi, uint32_t lo) {
 /* ======================================================================
  * Deterministic elementary functions.  The reference's Box-Muller, rotation
  * and accept test call the backend libm (philox.jl:96-98, gpu_sweep.jl:136),
- * which makes host and device trajectories differ in ULPs.  This build instead
+ * which makes host and device trajectories differ in ULPs.  This library instead
  * defines exp/log/sincos by named published methods — Cody & Waite argument
  * reduction (Software Manual for the Elementary Functions, 1980) followed by
  * the Taylor / atanh series truncated below 1e-17 relative — evaluated with
@@ -161,7 +161,7 @@ void ora_sincos(double x, double* so, double* co) {
   }
 }
 
-/* Box-Muller from one block — philox.jl:100-105, with this build's log/sincos. */
+/* Box-Muller from one block — philox.jl:100-105, with the project log/sincos. */
 void ora_normal2(const uint32_t blk[4], double out[2]) {
   double u1 = ora_uniform(blk[0], blk[1]);
   double u2 = ora_uniform(blk[2], blk[3]);
@@ -835,7 +835,7 @@ void ora_keyed_proposal(uint64_t seed, int32_t site, int32_t sweep, const double
 /* ======================================================================
  * Device walk contract.  The reference kernel strides ws lanes over the CSR
  * adjacency entries and folds the lane partials serially
- * (gpu_sweep.jl:48-85,132-135).  This build's kernel instead walks, per site,
+ * (gpu_sweep.jl:48-85,132-135).  The device kernel here instead walks, per site,
  * the adjacency entries in ascending program id (a per-sublattice invariant
  * order of a tiled Hamiltonian), flattens (entry, program entry) pairs and deals
  * them round-robin over `lanes` lanes; each lane accumulates
@@ -1245,7 +1245,7 @@ void ora_cpu_counters(const ora_cpu_chain* ch, int64_t c[4]) {
 }
 
 /* ======================================================================
- * Learned delayed acceptance (this build's extension; no reference code).
+ * Learned delayed acceptance (an extension of the library; no reference code).
  * Descriptor of site s: its neighbour sites in order of first appearance in
  * the pid-ordered walk (factor order as in site_coeffs!), features
  * [e.e_n ..., (e.e_n)^2 ..., one per triplet: e.(e_x x e_y), one per body-4

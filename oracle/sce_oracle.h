@@ -85,7 +85,7 @@ int64_t ora_keyed_or_sweep(const ora_tiled* t, double* config, double* zrows,
                            int64_t* natt);
 double ora_renormalize(const ora_tiled* t, double* config, double* zrows, double* energy);
 
-/* ---- learned delayed acceptance (this build's extension) ---- */
+/* ---- learned delayed acceptance (an extension of the library) ---- */
 int ora_sur_features(const ora_tiled* t, const double* config, int s, const double* e, float* out);
 double ora_sur_delta(int F, const float* params, double scale, const float* phiA, const float* phiB);
 int64_t ora_da_sweep(const ora_tiled* t, double* config, double* zrows, double* energy, double beta,

@@ -384,7 +384,7 @@ SCE_HD void keyed_proposal(uint64_t seed, int32_t site, int32_t sweep, const dou
   rotate3(e, ax, step * n4, e2);
 }
 
-// Philox slots of this build: 0..2 are the reference's Metropolis slots
+// Philox slots of the library: 0..2 are the reference's Metropolis slots
 // (philox.jl:65-67); 3 is the delayed-acceptance second-stage uniform; 16+r is
 // the accept uniform of overrelaxation pass r of a compound sweep.
 constexpr uint32_t SLOT_DA2 = 3u;

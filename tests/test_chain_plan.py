@@ -37,7 +37,7 @@ def _cases():
 @pytest.mark.parametrize("carry_last", [True, False])
 def test_proposal_placement(carry_last, next_prop, pending, walk_props):
     if carry_last and not walk_props:
-        pytest.skip("not a state of the drivers: a sweep carries its last class only when walks carry proposals")
+        pytest.skip("not a state the sweep loops produce: a sweep carries its last class only when walks carry proposals")
     for cp in _cases():
         cp = np.asarray(cp)
         sizes = np.diff(cp)
