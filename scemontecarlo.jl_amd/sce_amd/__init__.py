@@ -4,7 +4,8 @@ Host-side mirror of the reference's GPU API over the C ABI of libsce_hip.so
 (include/sce_hip.h).  Import puts nothing on the GPU; the first device call
 loads the library and fails loudly if it has not been built.
 """
-from .models import MultipoleTerm, lattice_model, config_model, random_config  # noqa: F401
+from .models import (MultipoleTerm, Setup, config_model, lattice_model, random_config,  # noqa: F401
+                     read_setup, setup_magmoms, setup_model)
 from .hamiltonian import (TiledHamiltonian, ChainState, resolve_kt, KB_EV,  # noqa: F401
                           MU_B_EV_T)
 from .gpu import (GPUTiledHamiltonian, GPUChainState, gpu_metropolis_sweep,  # noqa: F401

@@ -323,3 +323,243 @@ def random_config(n_sites: int, rng) -> np.ndarray:
     """Uniform unit spins (normalised Gaussian triples, state.jl:80-86), n x 3."""
     v = rng.standard_normal((n_sites, 3))
     return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+# ---------------------------------------------------------------------------
+# setup files and general cells (the model-input side, src/hamiltonian.jl:396-400)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Setup:
+    """A cell and its interaction defaults, read from a setup file in the layout the
+    reference's bench fixtures read (bench/fixtures.jl:68, bench/assets/nd2fe14b.toml):
+    ``[structure]`` lattice vectors as rows (Å), fractional ``positions``, 1-based
+    ``species`` per atom and ``species_labels``; ``[interaction]`` ``nbody``,
+    ``cutoff`` (Å, ``inf`` allowed), per-species ``lmax`` and ``isotropy``;
+    ``[symmetry]`` ``tol``."""
+
+    lattice: np.ndarray
+    positions: np.ndarray
+    species: List[int]
+    species_labels: List[str]
+    nbody: int = 2
+    cutoff: float = math.inf
+    lmax: List[int] = field(default_factory=list)
+    isotropy: bool = True
+    tol: float = 1e-5
+
+    @property
+    def n_atoms(self) -> int:
+        return len(self.species)
+
+    def cartesian(self) -> np.ndarray:
+        return self.positions @ self.lattice
+
+
+def read_setup(path: str) -> Setup:
+    """Parse a setup file (TOML) into a :class:`Setup`; raises ``ValueError`` on a
+    malformed cell or interaction section."""
+    import tomli
+    with open(path, "rb") as f:
+        d = tomli.load(f)
+    if "structure" not in d:
+        raise ValueError(f"{path}: no [structure] section")
+    s, it = d["structure"], d.get("interaction", {})
+    lat = np.asarray(s["lattice"], np.float64)
+    pos = np.asarray(s["positions"], np.float64)
+    sp = [int(x) for x in s["species"]]
+    labels = [str(x) for x in s.get("species_labels", [str(k) for k in range(1, max(sp, default=0) + 1)])]
+    if lat.shape != (3, 3) or abs(np.linalg.det(lat)) < 1e-12:
+        raise ValueError(f"{path}: lattice must be three independent 3-vectors")
+    if pos.ndim != 2 or pos.shape[1] != 3 or len(sp) != pos.shape[0] or not sp:
+        raise ValueError(f"{path}: positions ({pos.shape}) and species ({len(sp)}) disagree")
+    if min(sp) < 1 or max(sp) > len(labels):
+        raise ValueError(f"{path}: species must index species_labels (1..{len(labels)})")
+    lmax = [int(x) for x in it.get("lmax", [1] * len(labels))]
+    if len(lmax) != len(labels) or min(lmax) < 0:
+        raise ValueError(f"{path}: lmax needs one entry >= 0 per species ({len(labels)})")
+    st = Setup(lat, pos, sp, labels, int(it.get("nbody", 2)), float(it.get("cutoff", math.inf)), lmax,
+               bool(it.get("isotropy", True)), float(d.get("symmetry", {}).get("tol", 1e-5)))
+    _check_interaction(st.nbody, st.cutoff)
+    return st
+
+
+def _check_interaction(nbody, cutoff):
+    if nbody not in (1, 2, 3):
+        raise ValueError(f"nbody must be 1, 2 or 3; got {nbody}")
+    if not cutoff > 0:
+        raise ValueError(f"cutoff must be > 0; got {cutoff}")
+    if nbody == 3 and math.isinf(cutoff):
+        raise ValueError("triplet clusters need a finite cutoff")
+
+
+def setup_magmoms(setup: Setup, m: float = 2.0) -> List[float]:
+    """Per-atom moments (μ_B): ``m`` on atoms of species with ``lmax >= 1``, 0 on
+    the others (bench/fixtures.jl:80-88 puts 0 on boron, the Nd₂Fe₁₄B cell's only
+    ``lmax = 0`` species)."""
+    return [float(m) if setup.lmax[s - 1] >= 1 else 0.0 for s in setup.species]
+
+
+def _image_range(setup: Setup, r: float):
+    """Integer shifts per axis that cover every image within ``r`` of the cell."""
+    recip = np.linalg.inv(setup.lattice).T
+    n = [int(math.ceil(r * float(np.linalg.norm(recip[i])))) + 1 for i in range(3)]
+    return itertools.product(*(range(-k, k + 1) for k in n))
+
+
+def _neighbours(setup: Setup, a0: int, r: float, magnetic):
+    """(atom, shift, position) of every magnetic site within ``r`` of home atom ``a0``
+    (1-based), the site itself excluded."""
+    P = setup.cartesian()
+    out = []
+    for c in _image_range(setup, r):
+        off = np.asarray(c, np.float64) @ setup.lattice
+        for a in range(setup.n_atoms):
+            if not magnetic[a]:
+                continue
+            p = P[a] + off
+            d = float(np.linalg.norm(p - P[a0 - 1]))
+            if 1e-9 < d <= r + setup.tol:
+                out.append((a + 1, tuple(int(x) for x in c), p))
+    return out
+
+
+def setup_pairs(setup: Setup, cutoff: float, magnetic):
+    """Translation-unique pair clusters between magnetic atoms: every pair within
+    ``cutoff`` (Å), or, for ``cutoff = inf``, one pair per two distinct atoms of the
+    cell at its minimum image (ties resolved to the smallest shift tuple)."""
+    seen, out = set(), []
+    if math.isinf(cutoff):
+        P = setup.cartesian()
+        shifts = sorted(itertools.product((-1, 0, 1), repeat=3))
+        for a0 in range(1, setup.n_atoms + 1):
+            for a in range(a0 + 1, setup.n_atoms + 1):
+                if not (magnetic[a0 - 1] and magnetic[a - 1]):
+                    continue
+                d = [float(np.linalg.norm(P[a - 1] + np.asarray(c, np.float64) @ setup.lattice - P[a0 - 1]))
+                     for c in shifts]
+                best = min(d)
+                c = next(s for s, x in zip(shifts, d) if x <= best + setup.tol)
+                key = _canon([(a0, (0, 0, 0)), (a, c)])
+                if key not in seen:
+                    seen.add(key)
+                    out.append(key)
+        return sorted(out)
+    for a0 in range(1, setup.n_atoms + 1):
+        if not magnetic[a0 - 1]:
+            continue
+        for (a, c, _) in _neighbours(setup, a0, cutoff, magnetic):
+            key = _canon([(a0, (0, 0, 0)), (a, c)])
+            if key not in seen:
+                seen.add(key)
+                out.append(key)
+    return sorted(out)
+
+
+def setup_triplets(setup: Setup, cutoff: float, magnetic):
+    """Translation-unique triplet clusters of magnetic atoms whose three pair
+    distances are all within ``cutoff`` (Å)."""
+    seen, out = set(), []
+    for a0 in range(1, setup.n_atoms + 1):
+        if not magnetic[a0 - 1]:
+            continue
+        nb = _neighbours(setup, a0, cutoff, magnetic)
+        for x, y in itertools.combinations(nb, 2):
+            if float(np.linalg.norm(x[2] - y[2])) > cutoff + setup.tol:
+                continue
+            key = _canon([(a0, (0, 0, 0)), (x[0], x[1]), (y[0], y[1])])
+            if key not in seen:
+                seen.add(key)
+                out.append(key)
+    return sorted(out)
+
+
+_GAUNT = {}
+
+
+def real_gaunt(ls: Sequence[int]) -> np.ndarray:
+    """Rotation-invariant coupling of real harmonics: the tensor
+    ∫ Z_{l1 m1} Z_{l2 m2} Z_{l3 m3} dΩ over the sphere, scaled to unit Frobenius
+    norm (zeros when the l's admit no invariant).  The integral is exact: a
+    Gauss–Legendre rule in cos θ and a uniform rule in φ, both of degree above
+    l1 + l2 + l3, over the library's own Z_lm rows."""
+    key = tuple(int(l) for l in ls)
+    if key in _GAUNT:
+        return _GAUNT[key]
+    from . import _lib as L
+    lmax = max(key)
+    x, w = np.polynomial.legendre.leggauss(lmax * 3 // 2 + 4)
+    nphi = 3 * lmax + 4
+    phi = 2.0 * math.pi * np.arange(nphi) / nphi
+    st = np.sqrt(1.0 - x * x)
+    dirs = np.stack([np.outer(st, np.cos(phi)), np.outer(st, np.sin(phi)),
+                     np.repeat(x[:, None], nphi, axis=1)], axis=-1).reshape(-1, 3)
+    wt = np.repeat(w, nphi) * (2.0 * math.pi / nphi)
+    z = np.empty((len(dirs), (lmax + 1) ** 2))
+    L.check(L.lib().sce_host_zrows(lmax, len(dirs), L.ptr(np.ascontiguousarray(dirs)), L.ptr(z)))
+    blk = [z[:, l * l:(l + 1) * (l + 1)] for l in key]
+    t = np.einsum("p,pa,pb,pc->abc", wt, *blk)
+    nrm = float(np.linalg.norm(t))
+    t = t / nrm if nrm > 1e-12 else np.zeros_like(t)
+    _GAUNT[key] = t
+    return t
+
+
+def _pair_templates(la, lb, isotropic):
+    if isotropic:
+        return [(l, l) for l in range(1, min(la, lb) + 1)]
+    return [(l1, l2) for l1 in range(1, la + 1) for l2 in range(1, lb + 1) if (l1 + l2) % 2 == 0]
+
+
+def _triplet_templates(la, lb, lc):
+    return [(l1, l2, l3) for l1 in range(1, la + 1) for l2 in range(1, lb + 1) for l3 in range(1, lc + 1)
+            if (l1 + l2 + l3) % 2 == 0 and abs(l1 - l2) <= l3 <= l1 + l2]
+
+
+def setup_model(setup: Setup, *, nbody: int = None, cutoff: float = None, lmax: Sequence[int] = None,
+                isotropy: bool = None, seed: int = 13, scale: float = 0.01) -> Tuple[int, List[MultipoleTerm]]:
+    """Seeded synthetic model on a setup file's cell: ``(n_cell_atoms, terms)``, the
+    stand-in for ``multipole_terms`` of an SCEFitting model built on the same cell
+    (src/hamiltonian.jl:396-400; bench/fixtures.jl:66-74 draws its coefficients as
+    ``0.01·randn``).  Keywords left ``None`` take the file's ``[interaction]`` values.
+
+    Atoms of species with ``lmax = 0`` carry no term.  Term order: on-site
+    (anisotropic models only: even l = 2, 4, … ≤ the species' lmax, per atom),
+    then pairs in :func:`setup_pairs` order, then triplets in
+    :func:`setup_triplets` order; each cluster lists its templates in
+    lexicographic ``ls`` order, and every template draws ``coef ~ N(0, scale)``.
+    Templates keep the total l even (time-reversal even).  Isotropic tensors are
+    the identity on (l, l) pairs and :func:`real_gaunt` on triplets (the
+    triangle-rule l's); anisotropic ones are dense N(0, 1)/sqrt(size).  This is
+    not SCEFitting's symmetry-adapted basis (the site-symmetry SALCs need the
+    space group): term counts and coefficients differ from a fitted model's,
+    the tiling, walk and contraction shapes do not."""
+    nbody = setup.nbody if nbody is None else int(nbody)
+    cutoff = setup.cutoff if cutoff is None else float(cutoff)
+    lm = list(setup.lmax if lmax is None else [int(x) for x in lmax])
+    iso = setup.isotropy if isotropy is None else bool(isotropy)
+    _check_interaction(nbody, cutoff)
+    if len(lm) != len(setup.species_labels):
+        raise ValueError(f"lmax needs one entry per species ({len(setup.species_labels)})")
+    la = [lm[s - 1] for s in setup.species]
+    magnetic = [l >= 1 for l in la]
+    rng = np.random.default_rng(seed)
+    terms: List[MultipoleTerm] = []
+    if not iso:
+        for a in range(1, setup.n_atoms + 1):
+            for l in range(2, la[a - 1] + 1, 2):
+                f = rng.standard_normal(2 * l + 1) / math.sqrt(2 * l + 1)
+                terms.append(MultipoleTerm(float(rng.normal(0, scale)), [a], [(0, 0, 0)], [l], f))
+    if nbody >= 2:
+        for key in setup_pairs(setup, cutoff, magnetic):
+            for ls in _pair_templates(la[key[0][0] - 1], la[key[1][0] - 1], iso):
+                f = np.eye(2 * ls[0] + 1) if iso else _tensor(rng, ls, False)
+                terms.append(_term(key, rng.normal(0, scale), ls, f))
+    if nbody >= 3:
+        for key in setup_triplets(setup, cutoff, magnetic):
+            for ls in _triplet_templates(*(la[m[0] - 1] for m in key)):
+                f = real_gaunt(ls) if iso else _tensor(rng, ls, False)
+                if iso and not np.any(f):
+                    continue
+                terms.append(_term(key, rng.normal(0, scale), ls, f))
+    return setup.n_atoms, terms
