@@ -280,7 +280,7 @@ def uneven_leg(S, M, sweeps=100, n_chains=256):
             "sweeps": sweeps, "classes": out}
 
 
-def setup_leg(S, sweeps=200):
+def setup_leg(S, sweeps=200, graph=True):
     """The Nd2Fe14B cell read from its setup file (tests/golden/nd2fe14b.toml, the
     data of bench/assets/nd2fe14b.toml), modelled as the reference's triplet-heavy
     fixture asks (bench/fixtures.jl:90-100: nbody 3, 3.5 A cutoff, l <= 2 on Nd and
@@ -303,6 +303,7 @@ def setup_leg(S, sweeps=200):
         best = None
         for lanes in (16, 32, 64):
             g = S.GPUChainState(gH, S.ChainState(H, cfg, 0.6), seed=5)
+            g.set_graph(graph)
             S.gpu_run_sweeps(g, gH, None, 1 / 0.025, 20, renorm_interval=0, lanes=lanes)
             t0 = time.perf_counter()
             S.gpu_run_sweeps(g, gH, None, 1 / 0.025, sweeps, renorm_interval=1000, lanes=lanes)
@@ -357,6 +358,8 @@ def main():
                     help="skip the batched legs (64-replica PT, 256-chain uneven batch)")
     ap.add_argument("--no-setup-model", action="store_true",
                     help="skip the Nd2Fe14B setup-file leg (nbody 3 at 4^3 and 8^3 cells)")
+    ap.add_argument("--setup-model-only", action="store_true",
+                    help="run only the Nd2Fe14B setup-file leg and print its JSON")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every sweep's kernels directly instead of replaying hipGraphs")
     ap.add_argument("--da-rows", type=float, default=1e7, help="harvested training tuples")
@@ -387,6 +390,9 @@ def main():
     import sce_amd as S
     from sce_amd import models as M
 
+    if args.setup_model_only:
+        print(json.dumps({"setup_model": setup_leg(S, graph=not args.no_graph)}), flush=True)
+        return
     na, terms, dims = M.config_model(args.workload, seed=0)
     H = S.TiledHamiltonian(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
@@ -480,7 +486,7 @@ def main():
     if not args.no_batch:
         line["batched"] = {"parallel_tempering": pt_leg(S, M), "uneven_batch": uneven_leg(S, M)}
     if not args.no_setup_model:
-        line["setup_model"] = setup_leg(S)
+        line["setup_model"] = setup_leg(S, graph=not args.no_graph)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(na, terms, dims, cfg, args.kT)
     print(json.dumps(line), flush=True)
