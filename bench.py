@@ -54,13 +54,14 @@ def fp64_flops_per_update(H) -> float:
 def cpu_baseline(na, terms, dims, cfg, kT, budget_s=4.0):
     """The reference CPU path (oracle restatement of metropolis_sweep!: per-site
     xoshiro256++ streams, colour classes split over threads with a barrier) on the
-    same model, timed at 1, 4 and up to 16 threads for about `budget_s` each."""
+    same model, timed at 1, 4, 16 and nproc threads (the CPUs this process may run
+    on) for about `budget_s` each; `value` is the fastest of them."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     T = O.Tiled(na, terms, dims)
-    ncpu = os.cpu_count() or 1
-    counts = sorted({1, min(4, ncpu), min(16, ncpu)})
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    counts = sorted({1, min(4, ncpu), min(16, ncpu), ncpu})
     by = {}
     detail = []
     for nthreads in counts:
@@ -76,9 +77,9 @@ def cpu_baseline(na, terms, dims, cfg, kT, budget_s=4.0):
         dt = time.perf_counter() - t0
         by[str(nthreads)] = T.n_active * n / dt
         detail.append(f"{n} sweeps/{dt:.1f} s at {nthreads} thread(s)")
-    best = max(counts)
+    best = max(counts, key=lambda k: by[str(k)])
     return {"value": by[str(best)], "unit": "spin updates/s", "cores": best, "kind": "port",
-            "by_threads": by,
+            "by_threads": by, "nproc": ncpu,
             "sample": f"Metropolis sweeps of the reference CPU path (site_coeffs! + delta_energy, "
                       f"colour classes over the threads) on the same {T.n_sites}-spin model: "
                       + "; ".join(detail)}
@@ -117,7 +118,29 @@ def train_surrogate(args, S, H, gH, gst, beta):
                  "train_loss_last_epoch": {str(a): float(l[-1]) for a, l in losses.items()}}
 
 
-def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta):
+def ess_leg(args, S, H, gH, snapshot, step, beta):
+    """`ess_sweeps` plain sweeps (graph-replayed, lanes as the timed region) from
+    the equilibrated snapshot, E and m recorded every sweep: effective samples of
+    E per second with tau_int(E) from the log-binning plateau, and Sokal's
+    windowed estimate beside it."""
+    pst = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=778)
+    pst.set_graph(not args.no_graph)
+    S.gpu_run_sweeps(pst, gH, None, beta, 8, renorm_interval=1000, lanes=args.lanes, measure_interval=1)
+    t0 = time.perf_counter()
+    obs = S.gpu_run_sweeps(pst, gH, None, beta, args.ess_sweeps, renorm_interval=1000,
+                           lanes=args.lanes, measure_interval=1)
+    wall = time.perf_counter() - t0
+    n = args.ess_sweeps
+    ess = S.effective_samples(obs[:, 0])
+    return {"sweeps": n, "wall_s": wall, "ms_per_step": 1e3 * wall / n,
+            "spin_updates_per_s": H.n_active * n / wall,
+            "effective_samples_per_s_E": ess / wall if math.isfinite(ess) else None,
+            "tau_int_E": (n / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
+            "tau_int_E_windowed": S.tau_int_windowed(obs[:, 0]),
+            "obs": obs}
+
+
+def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, plain):
     """Delayed acceptance against the plain sweep at the same kT: `ess_sweeps`
     DA sweeps and as many plain sweeps (graph-replayed, lanes as the timed
     region), both from the equilibrated snapshot the plain timed region started
@@ -126,19 +149,16 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta):
     this comparison runs longer than the timed region."""
     from sce_amd import surrogate as SG
 
-    pst = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=778)
-    pst.set_graph(not args.no_graph)
-    t0 = time.perf_counter()
-    obs_plain = S.gpu_run_sweeps(pst, gH, None, beta, args.ess_sweeps, renorm_interval=1000,
-                                 lanes=args.lanes, measure_interval=1)
-    wall_plain = time.perf_counter() - t0
+    obs_plain, wall_plain = plain["obs"], plain["wall_s"]
     dst = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=777)
     dst.set_graph(not args.no_graph)
+    SG.gpu_da_run_sweeps(dst, sur, None, beta, 8, renorm_interval=1000, measure_interval=1)
+    dst.reset_counters()
     t0 = time.perf_counter()
     obs = SG.gpu_da_run_sweeps(dst, sur, None, beta, args.ess_sweeps, renorm_interval=1000, measure_interval=1)
     wall = time.perf_counter() - t0
     inf = dst.info()
-    ess = S.effective_samples(obs[:, 0])
+    ess = S.effective_samples(obs[:, 0])   # NaN when the timed series is too short for a plateau
     ess_p = S.effective_samples(obs_plain[:, 0])
     kT = 1.0 / beta
     sd = S.standard_stats(obs, kT, H.n_active)
@@ -344,7 +364,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=500,
+                    help="untimed sweeps at the frozen step, after the equilibration")
+    ap.add_argument("--equil", type=int, default=4000,
+                    help="fixed equilibration with run_mc's step adaptation (every 50 sweeps toward "
+                         "acceptance 0.5), run whatever --warmup is, before every measured leg")
     ap.add_argument("--workload", default="bcc32")
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--kT", type=float, default=0.04,
@@ -403,12 +427,13 @@ def main():
     beta = 1.0 / args.kT
     gst.set_persistent(args.persistent)
     gst.set_graph(not args.no_graph)
-    # warm-up with run_mc's step adaptation (src/run.jl:172: every 50 sweeps toward
-    # acceptance 0.5), then the step stays frozen for everything measured
+    # fixed equilibration with run_mc's step adaptation (src/run.jl:172: every 50
+    # sweeps toward acceptance 0.5), independent of --warmup; then --warmup sweeps
+    # at the frozen step, which stays frozen for everything measured
     from sce_amd.batch import _adapt
     done = 0
-    while done < args.warmup:
-        n = min(50, args.warmup - done)
+    while done < args.equil:
+        n = min(50, args.equil - done)
         gst.reset_counters()
         S.gpu_run_sweeps(gst, gH, None, beta, n, renorm_interval=0, lanes=args.lanes)
         done += n
@@ -416,6 +441,8 @@ def main():
             inf = gst.info()
             gst.step = _adapt(gst.step, inf.acc_metro, inf.att_metro, 0.5)
     S.gpu_renormalize(gst)
+    if args.warmup > 0:
+        S.gpu_run_sweeps(gst, gH, None, beta, args.warmup, renorm_interval=1000, lanes=args.lanes)
     sur = train_info = None
     if not args.no_da:   # harvesting is plain sweeps: it continues the equilibration
         sur, train_info = train_surrogate(args, S, H, gH, gst, beta)
@@ -434,7 +461,7 @@ def main():
     wall = time.perf_counter() - t0
     inf = gst.info()
     ups = H.n_active * args.steps / wall
-    ess = S.effective_samples(obs[:, 0])
+    ess = S.effective_samples(obs[:, 0])   # NaN when the timed series is too short for a plateau
     b_site = algorithmic_bytes_per_update(H)
     b_launch = b_site * H.n_active / H.n_colors
     f_site = fp64_flops_per_update(H)
@@ -452,13 +479,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random spins, seeded N(0, 1 meV) isotropic SALC couplings)",
+        "data": "synthetic (seeded random spins; seeded N(0, 1 meV) coefficients on isotropic tensors: "
+                "identity (l, l) pair blocks, Levi-Civita triangles, three-pairing tetrahedra)",
         "config": {"workload": f"{args.workload}: bcc {dims[0]}^3 x 2 atoms = {H.n_sites} spins, "
                                "body-2 l<=2 to 3rd shell + l=1 triangles + l=1 tetrahedra",
                    "kT_eV": args.kT, "lanes_per_site": args.lanes, "colours": H.n_colors,
                    "step": gst.step},
-        "effective_samples_per_s_E": ess / wall if math.isfinite(ess) else None,
-        "tau_int_E": (args.steps / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
+        "timed_region_effective_samples_per_s_E": ess / wall if math.isfinite(ess) else None,
+        "timed_region_tau_int_E": (args.steps / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
         "acceptance": inf.acc_metro / max(inf.att_metro, 1),
         "launch_mode": ("persistent" if args.persistent else
                         "direct launches" if args.no_graph else "hipGraph per sweep"),
@@ -481,8 +509,16 @@ def main():
         "adjacency_bytes_per_call": adj_bytes,
         "adjacency_GBs": adj_bytes / (grad_ms * 1e-3) / 1e9,
         "duration_source": "HIP events around 20 back-to-back launches on the chain's stream"}
+    # effective samples of E per second: from a plain leg long enough for a
+    # log-binning plateau (the timed region's own estimate is reported above,
+    # or null when it is too short to have one)
+    ess_leg_out = ess_leg(args, S, H, gH, snapshot, gst.step, beta)
+    line["effective_samples_per_s_E"] = ess_leg_out["effective_samples_per_s_E"]
+    line["tau_int_E"] = ess_leg_out["tau_int_E"]
+    line["ess_leg"] = {k: v for k, v in ess_leg_out.items() if k != "obs"}
     if not args.no_da:
-        line["delayed_acceptance"] = da_leg(args, S, H, gH, sur, train_info, snapshot, gst.step, beta)
+        line["delayed_acceptance"] = da_leg(args, S, H, gH, sur, train_info, snapshot, gst.step, beta,
+                                            plain=ess_leg_out)
     if not args.no_batch:
         line["batched"] = {"parallel_tempering": pt_leg(S, M), "uneven_batch": uneven_leg(S, M)}
     if not args.no_setup_model:

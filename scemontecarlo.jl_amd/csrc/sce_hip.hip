@@ -1436,7 +1436,64 @@ struct DevBuf {
         return;
       }
   }
+  bool owns(const void* p) const {
+    for (void* q : ptrs)
+      if (q == p) return true;
+    return false;
+  }
+  // Hand allocation `mine` to `other` and take `theirs` from it (a payload swap
+  // moves ownership with the pointers, so each owner frees what it holds).
+  static void exchange(DevBuf& a, void* mine, DevBuf& b, void* theirs) {
+    for (void*& q : a.ptrs)
+      if (q == mine) {
+        q = theirs;
+        break;
+      }
+    for (void*& q : b.ptrs)
+      if (q == theirs) {
+        q = mine;
+        break;
+      }
+  }
 };
+
+// A captured graph and its executable.  The source graph lives as long as the
+// executable (tools that inspect a launched executable may walk its source
+// nodes), and both are destroyed only once the stream replaying them has
+// drained.
+struct GraphSlot {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  explicit operator bool() const { return x != nullptr; }
+  void reset() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+  }
+};
+
+// Capture what `enqueue` issues on stream s into slot (empty on entry).
+template <class F>
+static int32_t capture_graph(hipStream_t s, GraphSlot& slot, F&& enqueue) {
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return sce_fail(SCE_EHIP, "hipStreamBeginCapture failed");
+  const int32_t r = enqueue();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (r != SCE_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return r != SCE_OK ? r : sce_fail(SCE_EHIP, "stream capture failed: %s", hipGetErrorString(e));
+  }
+  e = hipGraphInstantiate(&slot.x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    slot.x = nullptr;
+    (void)hipGraphDestroy(g);
+    return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  slot.g = g;
+  return SCE_OK;
+}
 
 template <class T>
 static int32_t dalloc(DevBuf& b, T** p, size_t n) {
@@ -1496,7 +1553,7 @@ struct sce_chain {
   DevBinner* bin_mem = nullptr;
   // captured sweeps: [fused measurement + 2 need_prop + 4 next_prop (with the final
   // deferred) + 8 measurement flag of the previous sweep's deferred final]
-  hipGraphExec_t gexec[16] = {};
+  GraphSlot gexec[16];
   int g_lanes = 0;
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
@@ -1510,15 +1567,11 @@ struct sce_chain {
   double total_ms = 0.0, walk_ms = 0.0;
   long long walk_launches = 0;
   void drop_graphs() {
-    for (auto& g : gexec)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& g : gexec) g.reset();
     g_lanes = 0;
   }
   ~sce_chain() {
-    if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -2148,23 +2201,13 @@ static int32_t ensure_graph(sce_chain* c, int L, int k) {
     c->g_lanes = L;
   }
   if (c->gexec[k]) return SCE_OK;
-  hipGraph_t g = nullptr;
-  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const long long before = c->sweeps_done;
   const bool chained = (k & 2) == 0;  // the previous sweep made proposals and deferred its final
-  const int32_t r = enqueue_metro(c, L, k & 1, !chained, (k & 4) != 0, (k & 4) != 0, chained ? (k >> 3) & 1 : -1,
-                                  chained);
+  const int32_t r = capture_graph(c->stream, c->gexec[k], [&] {
+    return enqueue_metro(c, L, k & 1, !chained, (k & 4) != 0, (k & 4) != 0, chained ? (k >> 3) & 1 : -1, chained);
+  });
   c->sweeps_done = before;
-  hipError_t e = hipStreamEndCapture(c->stream, &g);
-  if (r != SCE_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return r;
-  }
-  if (e != hipSuccess) return sce_fail(SCE_EHIP, "sweep capture failed: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&c->gexec[k], g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-  return SCE_OK;
+  return r;
 }
 
 static int32_t read_state(sce_chain* c, ChainDev* h) {
@@ -2326,6 +2369,9 @@ extern "C" int32_t sce_swap_payload(sce_chain* a, sce_chain* b) {
   ChainDev ha, hb;
   CHK(read_state(a, &ha));
   CHK(read_state(b, &hb));
+  // ownership moves with the buffers: each chain frees what it now sweeps on
+  DevBuf::exchange(a->mem, a->config, b->mem, b->config);
+  DevBuf::exchange(a->mem, a->zrows, b->mem, b->zrows);
   std::swap(a->config, b->config);
   std::swap(a->zrows, b->zrows);
   a->drop_graphs();
@@ -2584,7 +2630,7 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
       if (graphs) {
         const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0) + (pending == 1 ? 8 : 0);
         CHK(ensure_graph(c, lanes, k));
-        HIPCHK(hipGraphLaunch(c->gexec[k], c->stream));
+        HIPCHK(hipGraphLaunch(c->gexec[k].x, c->stream));
         c->sweeps_done += 1;
       } else {
         CHK(enqueue_metro(c, lanes, fused ? 1 : 0, !have_prop, next_prop, defer, pending, carry));
@@ -3143,21 +3189,18 @@ struct sce_batch {
   double* obs = nullptr;
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
-  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // [meas*2 + exch]
+  GraphSlot gexec[4];  // [meas*2 + exch]
   int g_lanes = 0;
   long long sweeps_done = 0;
   bool use_graph = true;
   void drop_graphs() {
-    for (auto& g : gexec)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& g : gexec) g.reset();
     g_lanes = 0;
   }
   ~sce_batch() {
     drop_graphs();
-    if (stream) (void)hipStreamSynchronize(stream), (void)hipStreamDestroy(stream);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -3328,20 +3371,9 @@ static int32_t benqueue_sweep(sce_batch* b, int L, int meas, int exch) {
 
 static int32_t bensure_graph(sce_batch* b, int L, int meas, int exch) {
   if (b->g_lanes != L) b->drop_graphs();
-  hipGraphExec_t& ge = b->gexec[meas * 2 + exch];
+  GraphSlot& ge = b->gexec[meas * 2 + exch];
   if (ge) return SCE_OK;
-  hipGraph_t g = nullptr;
-  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  const int32_t r = benqueue_sweep(b, L, meas, exch);
-  hipError_t e = hipStreamEndCapture(b->stream, &g);
-  if (r != SCE_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return r;
-  }
-  if (e != hipSuccess) return sce_fail(SCE_EHIP, "batch sweep capture failed: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  CHK(capture_graph(b->stream, ge, [&] { return benqueue_sweep(b, L, meas, exch); }));
   b->g_lanes = L;
   return SCE_OK;
 }
@@ -3403,7 +3435,7 @@ extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, i
     const bool fused_meas = meas && !renorm, fused_exch = exch && !renorm;
     if (b->use_graph) {
       CHK(bensure_graph(b, lanes, fused_meas, fused_exch));
-      HIPCHK(hipGraphLaunch(b->gexec[(fused_meas ? 2 : 0) + (fused_exch ? 1 : 0)], b->stream));
+      HIPCHK(hipGraphLaunch(b->gexec[(fused_meas ? 2 : 0) + (fused_exch ? 1 : 0)].x, b->stream));
     } else {
       CHK(benqueue_sweep(b, lanes, fused_meas, fused_exch));
     }
@@ -3494,8 +3526,10 @@ extern "C" int32_t sce_batch_renormalize(sce_batch* b) {
 extern "C" int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc) {
   if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
   HIPCHK(hipStreamSynchronize(b->stream));
-  if (att) HIPCHK(hipMemcpy(att, b->sw_att, (size_t)b->R * 8, hipMemcpyDeviceToHost));
-  if (acc) HIPCHK(hipMemcpy(acc, b->sw_acc, (size_t)b->R * 8, hipMemcpyDeviceToHost));
+  if (b->R < 2) return SCE_OK;  // no pairs
+  // n_chains - 1 entries: pair (k, k+1) at k
+  if (att) HIPCHK(hipMemcpy(att, b->sw_att, (size_t)(b->R - 1) * 8, hipMemcpyDeviceToHost));
+  if (acc) HIPCHK(hipMemcpy(acc, b->sw_acc, (size_t)(b->R - 1) * 8, hipMemcpyDeviceToHost));
   return SCE_OK;
 }
 
@@ -4721,10 +4755,10 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
   // Chained within the call as plain sweeps are (enqueue_da); with graph
   // replay each variant [fused measurement + 2 need_prop + 4 next_prop + 8
   // measurement flag of the pending final] is captured on first use.
-  hipGraphExec_t ge[16] = {};
+  GraphSlot ge[16];
   auto drop = [&] {
-    for (auto& x : ge)
-      if (x) (void)hipGraphExecDestroy(x);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& x : ge) x.reset();
   };
   int32_t rc = SCE_OK;
   bool have_prop = false;
@@ -4737,29 +4771,15 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
     if (c->use_graph) {
       const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0) + (pending == 1 ? 8 : 0);
       if (!ge[k]) {
-        hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-          rc = sce_fail(SCE_EHIP, "DA capture failed to begin");
-          break;
-        }
         const long long before = c->sweeps_done;
         const bool chained = (k & 2) == 0;
-        const int32_t r = enqueue_da(c, s, k & 1, !chained, (k & 4) != 0, chained ? (k >> 3) & 1 : -1, chained);
+        rc = capture_graph(c->stream, ge[k], [&] {
+          return enqueue_da(c, s, k & 1, !chained, (k & 4) != 0, chained ? (k >> 3) & 1 : -1, chained);
+        });
         c->sweeps_done = before;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (r != SCE_OK || e != hipSuccess) {
-          if (g) (void)hipGraphDestroy(g);
-          rc = r != SCE_OK ? r : sce_fail(SCE_EHIP, "DA capture failed: %s", hipGetErrorString(e));
-          break;
-        }
-        e = hipGraphInstantiate(&ge[k], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-          rc = sce_fail(SCE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-          break;
-        }
+        if (rc != SCE_OK) break;
       }
-      if (hipGraphLaunch(ge[k], c->stream) != hipSuccess) rc = sce_fail(SCE_EHIP, "hipGraphLaunch failed");
+      if (hipGraphLaunch(ge[k].x, c->stream) != hipSuccess) rc = sce_fail(SCE_EHIP, "hipGraphLaunch failed");
       c->sweeps_done += 1;
     } else {
       rc = enqueue_da(c, s, fused ? 1 : 0, !have_prop, next_prop, pending, have_prop);
@@ -4769,8 +4789,7 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
     if (renorm) enqueue_renorm(c);
     if (meas && !fused) enqueue_measure(c);
   }
-  if (rc != SCE_OK) (void)hipStreamSynchronize(c->stream);
-  drop();
+  drop();  // drains the stream first
   CHK(rc);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));

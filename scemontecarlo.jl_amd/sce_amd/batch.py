@@ -88,10 +88,11 @@ class GPUBatch:
         L.check(L.lib().sce_batch_renormalize(self._h))
 
     def swap_stats(self):
-        att = np.zeros(self.R, np.int64)
-        acc = np.zeros(self.R, np.int64)
+        # exactly n_chains - 1 entries, as the C ABI documents (pair k, k + 1)
+        att = np.zeros(max(self.R - 1, 0), np.int64)
+        acc = np.zeros(max(self.R - 1, 0), np.int64)
         L.check(L.lib().sce_batch_swap_stats(self._h, L.ptr(att), L.ptr(acc)))
-        return att[: self.R - 1], acc[: self.R - 1]
+        return att, acc
 
     def set_graph(self, on: bool) -> None:
         L.check(L.lib().sce_batch_set_graph(self._h, 1 if on else 0))

@@ -34,7 +34,7 @@ class LogBinner:
     def push(self, xs) -> "LogBinner":
         xs = np.atleast_1d(np.asarray(xs, np.float64))
         if xs.size != self.ncomp:
-            raise ValueError(f"measurement has {xs.size} components; the binner holds {self.ncomp}")
+            raise ValueError(f"LogBinner of width {self.ncomp} was pushed {xs.size} values")
         self.n += 1
         for k in range(MAX_LEVELS):
             self.count[k] += 1
@@ -227,9 +227,16 @@ def tau_int_windowed(series: np.ndarray, c: float = 6.0) -> float:
     return t - 0.5
 
 
-def effective_samples(series: np.ndarray) -> float:
-    """n / (1 + 2 tau_int), tau_int from the log-binning plateau (binning.jl:120-128)."""
+def effective_samples(series: np.ndarray, min_level: int = 1) -> float:
+    """n / (1 + 2 tau_int), tau_int from the log-binning plateau (binning.jl:120-128).
+
+    NaN unless the plateau (the deepest level holding >= 32 bins) is at least
+    level ``min_level``: with fewer than 32 * 2**min_level samples there is no
+    binned level to compare with the raw one, and tau_int would read 0 (every
+    sample counted as independent) whatever the correlation."""
     b = LogBinner(1).push_series(series)
+    if b.count[min_level] < MIN_BINS:
+        return math.nan
     tau = float(b.tau_int()[0])
     if not math.isfinite(tau):
         return math.nan

@@ -146,3 +146,122 @@ def test_split_runs_equal_one_run(renorm, exch):
     assert b1.exchange_state() == b2.exchange_state()
     for k in range(3):
         assert np.array_equal(b1.download(k)[0], b2.download(k)[0])
+
+
+def _ladder_vs_oracle(H, T, gH, R, kts, nsw, lanes, renorm, seed0):
+    """Run an R-rung ladder with exchange after every sweep on the device and
+    through the oracle composition; assert every payload, counter and exchange
+    statistic is bit-identical."""
+    seeds = [seed0 + 7 * r for r in range(R)]
+    states = [S.ChainState(H, rand_config(H.n_sites, seed0 + r), 0.6) for r in range(R)]
+    exch_seed = seed0 * 31 + 5
+    b = GPUBatch([gH] * R, states, seeds, kts, exch_seed)
+    b.run(nsw, lanes, renorm_interval=renorm, measure_interval=0, exchange_interval=1,
+          phase_done=0, phase_total=nsw)
+    payload = [[st.config.copy(), st.zrows.copy(), st.energy] for st in states]
+    att = np.zeros(R - 1, int)
+    acc = np.zeros(R - 1, int)
+    parity, ex = 0, 0
+    for j in range(1, nsw + 1):
+        for r in range(R):
+            c, z, E = payload[r]
+            _, E, _, _ = T.keyed_sweep(c, z, E, 1.0 / kts[r], 0.6, seeds[r], j, lanes)
+            payload[r][2] = E
+        if renorm and j % renorm == 0:
+            for r in range(R):
+                _, payload[r][2] = T.renormalize(payload[r][0], payload[r][1], payload[r][2])
+        if j < nsw:
+            ex += 1
+            a1, a2 = _oracle_exchange(payload, kts, parity, ex, exch_seed)
+            att += a1
+            acc += a2
+            parity = 1 - parity
+    for r in range(R):
+        cfg, z, E = b.download(r)
+        assert np.array_equal(cfg, payload[r][0]), r
+        assert np.array_equal(z, payload[r][1]), r
+        assert E == payload[r][2], r
+    da, dc = b.swap_stats()
+    assert np.array_equal(da, att) and np.array_equal(dc, acc)
+    return acc
+
+
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_c3_fcc_four_shell_ladder_bitwise(lanes):
+    """The C3 model form (fcc 4-atom basis, body-2 l <= 2 to the 4th shell) as a
+    6-rung geometric ladder on a 3^3 cell, exchanging after every sweep."""
+    na, terms = M.lattice_model("fcc", pair_shells=4, pair_lmax=2, seed=12)
+    H, T = build_pair(na, terms, (3, 3, 3))
+    gH = S.GPUTiledHamiltonian(H)
+    R = 6
+    kts = np.geomspace(0.002, 0.03, R)
+    acc = _ladder_vs_oracle(H, T, gH, R, kts, 10, lanes, 4, 300 + lanes)
+    assert acc.sum() > 0
+
+
+def test_c3_full_size_replicas_two_sweeps_bitwise():
+    """Two 6912-site fcc 12^3 replicas of the C3 model: two sweeps with an
+    exchange between them, bit-identical to the oracle."""
+    na, terms, dims = M.config_model("fcc12", seed=0)
+    H, T = build_pair(na, terms, dims)
+    assert H.n_sites == 6912
+    gH = S.GPUTiledHamiltonian(H)
+    _ladder_vs_oracle(H, T, gH, 2, np.array([0.004, 0.006]), 2, 32, 0, 77)
+
+
+def test_uneven_batch_with_zeeman_fields_bitwise():
+    """Uneven batch in the C4 form: per-chain models, moments of 2 mu_B and a
+    0-10 T field along a random axis, sizes up to 10^3 = 1000 sites; every chain
+    bit-identical to the oracle."""
+    rng = np.random.default_rng(404)
+    specs = [("sc", 10, 2), ("bcc", 6, 2), ("sc", 7, 1), ("bcc", 5, 2), ("bcc", 7, 2)]
+    gHs, states, Ts, kts, seeds, steps = [], [], [], [], [], []
+    for k, (lat, N, lmax) in enumerate(specs):
+        na, terms = M.lattice_model(lat, pair_shells=2, pair_lmax=lmax, triangles=(lat == "bcc"), seed=500 + k)
+        B = rng.uniform(0.0, 10.0) * (lambda v: v / np.linalg.norm(v))(rng.standard_normal(3))
+        H, T = build_pair(na, terms, (N, N, N), [2.0] * na, tuple(B))
+        gHs.append(S.GPUTiledHamiltonian(H))
+        steps.append(0.4 + 0.1 * k)
+        states.append(S.ChainState(H, rand_config(H.n_sites, 90 + k), steps[-1]))
+        Ts.append(T)
+        kts.append(float(rng.uniform(0.005, 0.05)))
+        seeds.append(int(rng.integers(1, 2**62)))
+    assert max(g.host.n_sites for g in gHs) >= 1000
+    # lmax differs between chains of one batch only where the library allows it
+    groups = {}
+    for k, g in enumerate(gHs):
+        groups.setdefault(g.host.lmax, []).append(k)
+    for ks in groups.values():
+        b = GPUBatch([gHs[k] for k in ks], [states[k] for k in ks], [seeds[k] for k in ks], [kts[k] for k in ks])
+        b.run(6, 64, renorm_interval=4, measure_interval=0)
+        for i, k in enumerate(ks):
+            c, z, E = states[k].config.copy(), states[k].zrows.copy(), states[k].energy
+            for j in range(1, 7):
+                _, E, _, _ = Ts[k].keyed_sweep(c, z, E, 1.0 / kts[k], steps[k], seeds[k], j, 64)
+                if j % 4 == 0:
+                    _, E = Ts[k].renormalize(c, z, E)
+            cfg, zd, Ed = b.download(i)
+            assert np.array_equal(cfg, c) and np.array_equal(zd, z) and Ed == E, k
+
+
+def test_swap_stats_writes_exactly_n_chains_minus_one():
+    """sce_batch_swap_stats fills n_chains - 1 entries (pair k, k + 1), the length
+    the header documents; nothing past them is written."""
+    import ctypes as C
+    from sce_amd import _lib as L
+    H = S.TiledHamiltonian(*bcc_tet_small(3, dims=(2, 2, 2)))
+    gH = S.GPUTiledHamiltonian(H)
+    R = 4
+    sts = [S.ChainState(H, rand_config(H.n_sites, k), 0.6) for k in range(R)]
+    b = GPUBatch([gH] * R, sts, [1, 2, 3, 4], [0.01, 0.02, 0.03, 0.04], 8)
+    b.run(6, 32, 0, 0, 1, 0, 6)
+    sentinel = -123456789
+    att = np.full(R + 3, sentinel, np.int64)
+    acc = np.full(R + 3, sentinel, np.int64)
+    L.check(L.lib().sce_batch_swap_stats(b._h, L.ptr(att), L.ptr(acc)))
+    assert (att[R - 1:] == sentinel).all() and (acc[R - 1:] == sentinel).all()
+    # 5 exchanges with parity 0, 1, 0, 1, 0: pairs (0,1), (2,3) on even, (1,2) on odd
+    assert att[: R - 1].tolist() == [3, 2, 3]
+    assert (acc[: R - 1] >= 0).all() and (acc[: R - 1] <= att[: R - 1]).all()
+    a2, c2 = b.swap_stats()
+    assert np.array_equal(a2, att[: R - 1]) and np.array_equal(c2, acc[: R - 1])

@@ -370,3 +370,37 @@ def test_checkpoint_resume_is_bitwise(tmp_path):
     other = S.GPUTiledHamiltonian(S.TiledHamiltonian(*bcc_small(10)))
     with pytest.raises(ValueError, match="different model"):
         CK.load_chain(path, other)
+
+
+def test_swap_payload_survives_destroying_the_partner():
+    """_swap_payload!(a, b) (gpu_pt.jl:46-51) moves the buffers with ownership:
+    after the swap, destroying b (and allocating a new chain into the memory it
+    frees) leaves a sweeping b's former payload with a's own keyed stream,
+    bit-identical to the oracle."""
+    import gc
+    na, terms, dims = bcc_small(8)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    beta = 1.0 / 0.01
+    sa = S.ChainState(H, rand_config(H.n_sites, 21), 0.6)
+    sb = S.ChainState(H, rand_config(H.n_sites, 22), 0.6)
+    a = S.GPUChainState(gH, sa, seed=1111)
+    b = S.GPUChainState(gH, sb, seed=2222)
+    S.gpu_metropolis_sweep(a, gH, beta)
+    S.gpu_metropolis_sweep(b, gH, beta)
+    cb, zb, Eb = sb.config.copy(), sb.zrows.copy(), sb.energy
+    _, Eb, _, _ = T.keyed_sweep(cb, zb, Eb, beta, 0.6, 2222, 1, S.DEFAULT_LANES)
+    S.swap_payload(a, b)
+    del b
+    gc.collect()
+    filler = [S.GPUChainState(gH, S.ChainState(H, rand_config(H.n_sites, 30 + k), 0.6), seed=k)
+              for k in range(3)]
+    for f in filler:
+        S.gpu_metropolis_sweep(f, gH, beta)
+    for sw in range(2, 5):   # a's own counter continues: sweeps 2, 3, 4
+        n_dev = S.gpu_metropolis_sweep(a, gH, beta)
+        n_ref, Eb, _, _ = T.keyed_sweep(cb, zb, Eb, beta, 0.6, 1111, sw, S.DEFAULT_LANES)
+        assert n_dev == n_ref
+    ca, za, Ea = a.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    del filler

@@ -158,6 +158,23 @@ def test_logbinner_iid_and_ar1():
     assert b.tau_int()[0] > 5
 
 
+def test_effective_samples_needs_a_plateau():
+    """Too few samples for a binned level with 32 bins: no estimate (NaN), not
+    n.  With one: n / (1 + 2 tau) near the AR(1) closed form tau = rho / (1 - rho)."""
+    rng = np.random.default_rng(9)
+    assert math.isnan(S.effective_samples(rng.standard_normal(20)))
+    assert math.isnan(S.effective_samples(rng.standard_normal(63)))
+    assert math.isfinite(S.effective_samples(rng.standard_normal(64)))
+    rho, n = 0.8, 2 ** 16
+    e = np.random.default_rng(2).standard_normal(n)
+    x = np.empty(n)
+    v = 0.0
+    for i in range(n):
+        v = rho * v + e[i]
+        x[i] = v
+    assert S.effective_samples(x) == pytest.approx(n / (1 + 2 * rho / (1 - rho)), rel=0.35)
+
+
 def test_windowed_tau_on_ar1_and_iid():
     """The windowed estimate against the AR(1) closed form tau = rho / (1 - rho)
     (log-binning convention) for short and long tau, and 0 for iid data."""

@@ -416,3 +416,75 @@ def test_cpu_chain_serial_equals_threaded():
     for (st, cnt) in runs[1:]:
         assert np.array_equal(st[0], runs[0][0][0]) and st[2] == runs[0][0][2]
         assert np.array_equal(cnt, runs[0][1])
+
+
+# ---- contraction order of asymmetric tensors (closed forms) -------------------
+
+def _z_closed(u, l):
+    """Real tesseral row of degree l from its closed form (lm order m = -l..l)."""
+    x, y, z = u
+    if l == 1:
+        return math.sqrt(3 / (4 * math.pi)) * np.array([y, z, x])
+    c2 = math.sqrt(15 / (4 * math.pi))
+    return np.array([c2 * x * y, c2 * y * z, math.sqrt(5 / (16 * math.pi)) * (3 * z * z - 1),
+                     c2 * x * z, c2 / 2 * (x * x - y * y)])
+
+
+def _closed_form_energy(na, terms, dims, cfg):
+    """E = sum over cells and terms of (4 pi)^(body/2) coef sum_idx folded[idx]
+    prod_k Z_{l_k, idx_k}(e of member k), member k at atom a_k, cell c + shift_k
+    (site = (a - 1) + na (c1 + d1 (c2 + d2 c3)), hamiltonian.jl:99-181)."""
+    d1, d2, d3 = dims
+    E = 0.0
+    for c3 in range(d3):
+        for c2 in range(d2):
+            for c1 in range(d1):
+                for t in terms:
+                    rows = []
+                    for a, sh, l in zip(t.atoms, t.shifts, t.ls):
+                        s = (a - 1) + na * ((c1 + sh[0]) % d1 + d1 * ((c2 + sh[1]) % d2 + d2 * ((c3 + sh[2]) % d3)))
+                        rows.append(_z_closed(cfg[s], l))
+                    acc = 0.0
+                    for idx in itertools.product(*[range(2 * l + 1) for l in t.ls]):
+                        f = t.folded[idx]
+                        if f != 0.0:
+                            acc += f * math.prod(rows[k][i] for k, i in enumerate(idx))
+                    E += (4 * math.pi) ** (len(t.atoms) / 2) * t.coef * acc
+    return E
+
+
+def _pair12_terms():
+    """A (1, 2) bond along +y with two off-diagonal entries of folded (3 x 5):
+    member 1 carries l = 1, member 2 l = 2."""
+    f = np.zeros((3, 5))
+    f[0, 3] = 1.3    # Z_{1,-1}(e_0) Z_{2,1}(e_+y)
+    f[2, 0] = -0.5   # Z_{1,1}(e_0) Z_{2,-2}(e_+y)
+    return [M.MultipoleTerm(0.02, [1, 1], [(0, 0, 0), (0, 1, 0)], [1, 2], f)]
+
+
+@pytest.mark.parametrize("name, terms, dims", [
+    ("3body", M.threebody_terms(0.05), (4, 2, 2)),
+    ("4body", M.fourbody_terms(0.03), (4, 2, 2)),
+    ("pair12", _pair12_terms(), (2, 3, 2)),
+], ids=lambda v: v if isinstance(v, str) else "")
+def test_asymmetric_fixture_energies_closed_form(name, terms, dims):
+    """The reference's asymmetric 3- and 4-body fixtures (fixtures.jl:109-131)
+    and an off-diagonal (1,2) pair: the oracle's and the product builder's total
+    energies equal the closed-form contraction, and differ from it with the
+    tensor's member axes reversed (so a transposed folded or swapped member
+    slots fail)."""
+    import sce_amd as S
+    n = int(np.prod(dims))
+    for seed in range(3):
+        cfg = rand_config(n, 100 + seed)
+        ref = _closed_form_energy(1, terms, dims, cfg)
+        T = O.Tiled(1, terms, dims)
+        assert T.total_energy(T.zrows(cfg)) == pytest.approx(ref, rel=1e-13, abs=1e-15)
+        H = S.TiledHamiltonian(1, terms, dims)
+        assert H.total_energy_rows(H.zrows(cfg)) == pytest.approx(ref, rel=1e-13, abs=1e-15)
+        # the plausible slip: member axes of folded reversed (and, for the
+        # mixed-l pair, the members' l swapped with them)
+        rev = [M.MultipoleTerm(t.coef, t.atoms, t.shifts, t.ls[::-1], np.ascontiguousarray(np.transpose(t.folded)))
+               for t in terms]
+        wrong = _closed_form_energy(1, rev, dims, cfg)
+        assert abs(wrong - ref) > 1e-6 * max(abs(ref), 1e-3)
