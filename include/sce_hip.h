@@ -305,6 +305,14 @@ int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta, int64_t n
  * (64, 128 or 256; the reference's default is 128), so the result is the same
  * bits on any backend. */
 int32_t sce_energy_gradient(sce_tables* t, const double* config, double* G, int32_t lanes);
+/* No counterpart in the reference: sce_energy_gradient for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and bits; the device copy of
+ * config, its rebuilt rows and the gradient live in one static device array of
+ * 4 Mi doubles (32 MiB) per GPU, so (3 + nlm + 3) * n_sites must not exceed
+ * 4 Mi (SCE_EDIM otherwise).  It calls no hipMalloc/hipFree.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU race. */
+int32_t sce_energy_gradient_noalloc(sce_tables* t, const double* config, double* G, int32_t lanes);
 /* gpu_energy_gradient!(dG, gst, gH, gsc) — :222-226: the same on the chain's
  * device spins and rows (no rebuild: the rows are Z(config) by the sweep
  * invariant). */

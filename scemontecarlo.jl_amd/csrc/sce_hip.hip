@@ -27,6 +27,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sce_internal.h"
@@ -2778,6 +2779,62 @@ extern "C" int32_t sce_energy_gradient(sce_tables* t, const double* config, doub
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   return g.down(G, 3 * n);
+}
+
+// ---------------------------------------------------------------------------
+// The all-site gradient without allocation: the device copy of the
+// configuration, its rebuilt rows and the gradient live in one file-scope
+// device array (each GPU holds its own copy for the life of the process).
+// Every call on a GPU shares that GPU's array, so calls that overlap on one GPU
+// race; the array keeps the last call's contents.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_DOUBLES = (size_t)4 << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+static int32_t scratch_ptr(double** out) {
+  // resolved once per device, with that device current: the array's name is
+  // a host-side stand-in, not a device address
+  constexpr int MAXDEV = 64;
+  static std::mutex mu;
+  static double* ptrs[MAXDEV] = {};
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAXDEV) return sce_fail(SCE_EINVAL, "device %d beyond the %d the library tracks", dev, MAXDEV);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return sce_fail(SCE_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    ptrs[dev] = (double*)p;
+  }
+  *out = ptrs[dev];
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_energy_gradient_noalloc(sce_tables* t, const double* config, double* G, int32_t lanes) {
+  if (!t || !config || !G) return sce_fail(SCE_EINVAL, "sce_energy_gradient_noalloc: NULL argument");
+  CHK(check_grad_lanes(lanes));
+  const DevTables& D = t->dev;
+  const size_t n = (size_t)D.n_sites;
+  const size_t need = (3 + (size_t)D.nlm + 3) * n;
+  if (need > NOALLOC_DOUBLES)
+    return sce_fail(SCE_EDIM,
+                    "sce_energy_gradient_noalloc: %zu sites at nlm %d need %zu doubles of scratch; the static "
+                    "array holds %zu (use sce_energy_gradient)",
+                    n, D.nlm, need, NOALLOC_DOUBLES);
+  double* base = nullptr;
+  CHK(scratch_ptr(&base));
+  double* cf = base;                 // 3 x n
+  double* z = cf + 3 * n;            // nlm x n
+  double* g = z + (size_t)D.nlm * n; // 3 x n (zeroed for inactive sites by launch_grad)
+  if (n) HIPCHK(hipMemcpy(cf, config, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  SCE_DISPATCH_LMAX(D.lmax, launch_rows_of, D.n_sites, cf, z, (hipStream_t)0);
+  SCE_DISPATCH_LMAX(D.lmax, launch_grad, t->grad, D, cf, z, g, lanes, (hipStream_t)0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  if (n) HIPCHK(hipMemcpy(G, g, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  return SCE_OK;
 }
 
 extern "C" int32_t sce_chain_energy_gradient(sce_chain* c, double* G, int32_t lanes) {

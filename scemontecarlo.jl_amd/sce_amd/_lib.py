@@ -100,7 +100,7 @@ EXPORTS = [
     "sce_sur_set_params", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
     "sce_sur_features", "sce_da_run_sweeps", "sce_chain_set_counters", "sce_chain_time_walk",
     "sce_chain_set_graph", "sce_chain_set_drift", "sce_chain_binner_reset", "sce_chain_binner_off", "sce_chain_binner_get",
-    "sce_energy_gradient", "sce_chain_energy_gradient", "sce_chain_time_gradient",
+    "sce_energy_gradient", "sce_energy_gradient_noalloc", "sce_chain_energy_gradient", "sce_chain_time_gradient",
     "sce_test_mfma", "sce_test_chain_plan",
 ]
 
@@ -159,7 +159,8 @@ def lib():
             "sce_chain_set_drift": [P, d],
             "sce_chain_binner_reset": [P, i32, i32], "sce_chain_binner_off": [P],
             "sce_chain_binner_get": [P, P, P, P, P, P, P, i32],
-            "sce_energy_gradient": [P, P, P, i32], "sce_chain_energy_gradient": [P, P, i32],
+            "sce_energy_gradient": [P, P, P, i32], "sce_energy_gradient_noalloc": [P, P, P, i32],
+            "sce_chain_energy_gradient": [P, P, i32],
             "sce_chain_time_gradient": [P, i32, i32, P],
             "sce_test_mfma": [i64, P, P, P, P],
             "sce_test_chain_plan": [i32, P, i32, i32, i32, i32, P, i32, P, P],
