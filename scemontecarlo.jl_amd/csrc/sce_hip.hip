@@ -100,6 +100,7 @@ struct DevTables {
   const int32_t* nb_ptr;
   const int4* nb;
   const int32_t* nbq;  // per colour position: nb_max neighbour site ids (padded with the site's own)
+  const unsigned long long* nbq_low;  // per position: bit j = slot j is an active site of lower colour (nb_max <= 64)
   const int32_t* prog_ptr;  // na + 1: per sublattice, into prog_off
   const int32_t* prog_off;  // walk offsets of the pid-sorted adjacency's programs (+ end)
   const uint16_t* adj_perm; // per colour position: raw adjacency index -> pid-sorted index
@@ -152,19 +153,15 @@ constexpr int PROPOSE_LANES = 4;
 
 // Thread t of the proposal pass (PROPOSE_LANES per site) for the sweep whose
 // 1-based index is the chain's completed sweeps + sweep_off.
+// The keyed proposal of site s (0-based) for the 1-based sweep `sweep` from
+// spin e, by a group of PROPOSE_LANES lanes (part = lane within the group; all
+// four take part in the exchange): each lane draws its half of a Box-Muller
+// pair, the radius (even part) or the angle.  Valid in part 0 only (returns
+// true there): e2, the accept uniform u and the row z of e2.
 template <int LMAX>
-__device__ __forceinline__ void propose_body(const DevTables& T, const double* __restrict__ config,
-                                             double* __restrict__ prop, const ChainDev* __restrict__ st,
-                                             int q_lo, int q_n, int t, int sweep_off) {
-  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
-  const int qi = t / PROPOSE_LANES, part = t % PROPOSE_LANES;
-  const bool valid = qi < q_n;
-  const int q = q_lo + (valid ? qi : q_n - 1);  // every lane takes part in the exchange
-  const int s = T.color_sites[q];
-  const uint64_t seed = st->seed;
-  const int32_t sweep = (int32_t)(st->sweep_index + sweep_off);
-  const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
-  // this lane's half of a Box-Muller pair: the radius (even part) or the angle
+__device__ __forceinline__ bool propose_core(uint64_t seed, int s, int32_t sweep, const double e[3], double step,
+                                             int part, double e2[3], double* u_out,
+                                             double z[(LMAX + 1) * (LMAX + 1)]) {
   const sce::u32x4 b = sce::philox_block(seed, s + 1, sweep, 1u + (uint32_t)(part >> 1), 0u);
   double x0 = 0.0, x1 = 0.0;
   if ((part & 1) == 0)
@@ -174,11 +171,10 @@ __device__ __forceinline__ void propose_body(const DevTables& T, const double* _
   const int l0 = (int)(threadIdx.x % 64) & ~(PROPOSE_LANES - 1);
   const double r1 = __shfl(x0, l0), s1 = __shfl(x0, l0 + 1), c1 = __shfl(x1, l0 + 1);
   const double r2 = __shfl(x0, l0 + 2), s2 = __shfl(x0, l0 + 3), c2 = __shfl(x1, l0 + 3);
-  if (!valid || part != 0) return;
+  if (part != 0) return false;
   const sce::u32x4 b0 = sce::philox_block(seed, s + 1, sweep, 0u, 0u);
   const double u_flip = sce::uniform01(b0.x, b0.y);
-  const double u = sce::uniform01(b0.z, b0.w);
-  double e2[3], z[NLM];
+  *u_out = sce::uniform01(b0.z, b0.w);
   if (u_flip < 0.2) {
     e2[0] = -e[0];
     e2[1] = -e[1];
@@ -187,9 +183,26 @@ __device__ __forceinline__ void propose_body(const DevTables& T, const double* _
     const double n1 = r1 * c1, n2 = r1 * s1, n3 = r2 * c2, n4 = r2 * s2;
     const double nn = __builtin_sqrt((n1 * n1 + n2 * n2) + n3 * n3);
     const double ax[3] = {n1 / nn, n2 / nn, n3 / nn};
-    sce::rotate3(e, ax, st->step * n4, e2);
+    sce::rotate3(e, ax, step * n4, e2);
   }
   sce::zlm_row<LMAX>(e2[0], e2[1], e2[2], z);
+  return true;
+}
+
+template <int LMAX>
+__device__ __forceinline__ void propose_body(const DevTables& T, const double* __restrict__ config,
+                                             double* __restrict__ prop, const ChainDev* __restrict__ st,
+                                             int q_lo, int q_n, int t, int sweep_off) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  const int qi = t / PROPOSE_LANES, part = t % PROPOSE_LANES;
+  const bool valid = qi < q_n;
+  const int q = q_lo + (valid ? qi : q_n - 1);  // every lane takes part in the exchange
+  const int s = T.color_sites[q];
+  const int32_t sweep = (int32_t)(st->sweep_index + sweep_off);
+  const double e[3] = {config[3 * (size_t)s], config[3 * (size_t)s + 1], config[3 * (size_t)s + 2]};
+  double e2[3], u, z[NLM];
+  const bool mine = propose_core<LMAX>(st->seed, s, sweep, e, st->step, part, e2, &u, z);
+  if (!valid || !mine) return;
   double* o = prop + (size_t)q * PS;
   o[0] = e2[0];
   o[1] = e2[1];
@@ -921,59 +934,6 @@ __global__ __launch_bounds__(TPB) void k_metro_prop(DevTables T, double* __restr
                               T.nb_max * NLM + 1 + NLM);
 }
 
-// ---------------------------------------------------------------------------
-// persistent sweeps: one cooperative launch runs many Metropolis sweeps —
-// proposals, the colour steps and the deterministic reductions — separated by
-// grid barriers instead of kernel boundaries.  The arithmetic is exactly that
-// of k_propose / k_metro / k_chunks + k_final.
-// ---------------------------------------------------------------------------
-constexpr int BAR_GROUP = 64;  // blocks per first-level barrier counter
-constexpr int BAR_PAD = 32;    // u32 words between counters (one 128-byte line)
-constexpr unsigned BAR_SPIN_LIMIT = 1u << 26;
-
-// Two-level grid barrier with a bounded wait: a block that waits longer than
-// the spin limit (or sees another block's error) sets *err and returns false,
-// so every wave leaves the kernel even if co-residency were ever violated.
-// bar[0] = generation, bar[1] = top-level count, bar[2 + BAR_PAD*g] = group g.
-__device__ bool grid_sync(unsigned* bar, int* err, unsigned* gen_sh, int* ok_sh) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    const unsigned g = *gen_sh;
-    const unsigned nb = gridDim.x;
-    const unsigned grp = blockIdx.x / BAR_GROUP;
-    const unsigned ngrp = (nb + BAR_GROUP - 1) / BAR_GROUP;
-    const unsigned gsize = min((unsigned)BAR_GROUP, nb - grp * BAR_GROUP);
-    __threadfence();
-    if (atomicAdd(&bar[2 + BAR_PAD * grp], 1u) == gsize - 1) {
-      atomicExch(&bar[2 + BAR_PAD * grp], 0u);
-      if (atomicAdd(&bar[1], 1u) == ngrp - 1) {
-        atomicExch(&bar[1], 0u);
-        __threadfence();
-        atomicAdd(&bar[0], 1u);
-      }
-    }
-    unsigned spins = 0;
-    while (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > BAR_SPIN_LIMIT) {
-        atomicExch(err, 1);
-        ok = 0;
-        break;
-      }
-      if ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        ok = 0;
-        break;
-      }
-    }
-    __threadfence();
-    *gen_sh = g + 1;
-    *ok_sh = ok;
-  }
-  __syncthreads();
-  return *ok_sh != 0;
-}
-
 // Block-level recursive tree over m partial sums (the levels above the first
 // of ora-style tree: m <= 512 -> one chunk tree, else chunk sums recursively).
 // Valid in thread 0.  `scratch` receives intermediate levels.
@@ -1003,23 +963,6 @@ __device__ double final_tree(const double* buf, long long m, double* scratch, do
   }
 }
 
-struct SweepArgs {
-  DevTables T;
-  double *config, *zrows, *prop, *dE, *partial, *obs;
-  int *acc, *ipart;
-  ChainDev* st;
-  const int* cptr;
-  int n_colors;
-  long long nsweeps, sweep0, i0;  // sweeps before this launch: chain total, this call
-  long long obs_row0;
-  int measure_interval;
-  int defer_last;  // the last sweep's measurement follows a renormalisation
-  unsigned* bar;
-  int* err;
-  double beta, step;
-  unsigned long long seed;
-};
-
 template <int LMAX>
 __device__ __forceinline__ void propose_one(const DevTables& T, const double* __restrict__ config,
                                             double* __restrict__ prop, int q, uint64_t seed,
@@ -1039,107 +982,229 @@ __device__ __forceinline__ void propose_one(const DevTables& T, const double* __
   for (int k = 0; k < NLM; ++k) o[4 + k] = z[k];
 }
 
-template <int L, int LMAX>
-__global__ __launch_bounds__(TPB) void k_sweeps(SweepArgs a) {
-  constexpr int G = TPB / L;
+// ---------------------------------------------------------------------------
+// Dependency-driven persistent sweep: one cooperative launch runs `nsweeps`
+// Metropolis sweeps with no grid barrier.  Wave w owns the colour positions
+// q = w, w + W, ... and updates them for sweep 1, 2, ... in (sweep, position)
+// order.  The colour-serial schedule fixes what each update reads: a
+// neighbour of lower colour after its update of the same sweep, one of higher
+// colour after its update of the previous sweep (and before its update of this
+// one, which waits on this site in turn).  So item (t, q) waits until every
+// neighbour n has done[n] >= t (lower colour) or >= t - 1 (higher colour;
+// padding slots hold the site itself), and publishes done[s] = t once its
+// writes are complete: every update reads and writes exactly what it does in
+// the per-colour launches, so the results are the same bits.  Every
+// dependency of an item comes earlier in (sweep, position) order and every
+// wave walks its items in that order, so with all waves resident the earliest
+// unfinished item can always proceed: no deadlock.  Waits are bounded anyway
+// (spin limit -> *err, every wave leaves).
+//
+// Data one wave writes and another reads (rows, spins, proposals, done flags)
+// moves through agent-scope atomic loads and stores: coherent across the XCDs'
+// separate L2 caches without cache-wide write-backs or invalidations.  A
+// wave-scope release (wait for its stores) orders an update's writes before
+// its done flag; a wave-scope acquire orders a waiting wave's reads after the
+// flags it saw.  Per-sweep outputs go to segment buffers (dE and accept flags
+// by colour position, the spins of measured sweeps by site), reduced after
+// the launch by the same chunk and final trees as the per-colour path.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ld_ag(const double* p) {
+  return __longlong_as_double(
+      __hip_atomic_load(reinterpret_cast<const long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st_ag(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<long long*>(p), __double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_ag(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_ag(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr unsigned FLOW_SPIN_LIMIT = 1u << 22;
+
+struct FlowArgs {
+  DevTables T;
+  double *config, *zrows, *prop;
+  double* dE_seg;    // nsweeps x n_active, colour-position order
+  int* acc_seg;      // nsweeps x n_active
+  double* cfg_seg;   // measured sweeps x 3 x n_sites (spins after the sweep)
+  int* done;         // per site: the last sweep (1-based, absolute) it completed
+  const unsigned long long* low;  // per position: bit j = neighbour slot j has a lower colour
+  int* err;
+  long long t0;      // completed sweeps before the launch
+  int nsweeps;
+  long long i0;      // sweeps of this call before the launch (the measurement schedule)
+  int measure_interval;
+  int defer_last;    // the last sweep's measurement follows a renormalisation
+  double beta, step;
+  unsigned long long seed;
+  unsigned spin_limit;
+};
+
+// measurement slot of segment sweep i (1-based) or -1
+__host__ __device__ inline int flow_meas_slot(long long i0, int i, int n, int mi, int defer_last) {
+  if (mi <= 0 || ((i0 + i) % mi) != 0 || (defer_last && i == n)) return -1;
+  return (int)((i0 + i) / mi - i0 / mi - 1);
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(64) void k_flow(FlowArgs a) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   extern __shared__ double lds[];
-  __shared__ double sh[256];
-  __shared__ unsigned gen_sh;
-  __shared__ int ok_sh;
   const DevTables& T = a.T;
-  const int t = threadIdx.x;
-  if (t == 0) gen_sh = __hip_atomic_load(&a.bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const int nthreads = gridDim.x * TPB;
-  const int gtid = blockIdx.x * TPB + t;
-  const int vblk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const long long n = T.n_active;
-  const long long nch = (n + TREE_B - 1) / TREE_B;
-  const long long nchm = ((long long)T.n_sites + TREE_B - 1) / TREE_B;
-  double* mpart = a.partial + nch;              // 3 x nchm
-  double* scratch = mpart + 3 * nchm;
-  long long row = a.obs_row0;
-  for (long long i = 1; i <= a.nsweeps; ++i) {
-    const int32_t sweep = (int32_t)(a.sweep0 + i);
-    for (int q = gtid; q < (int)n; q += nthreads)
-      propose_one<LMAX>(T, a.config, a.prop, q, a.seed, sweep, a.step);
-    if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
-    for (int c = 0; c < a.n_colors; ++c) {
-      const int lo = a.cptr[c], nc = a.cptr[c + 1] - lo;
-      if (nc == 0) continue;
-      for (int base = vblk * G; base < nc; base += gridDim.x * G) {
-        const int qi = base + t / L;
-        metro_group<L, LMAX, false>(T, a.config, a.zrows, a.prop, a.dE, a.acc, a.beta, lo + qi,
-                                    qi < nc, nullptr, lds,
-                                    T.nb_max * (LMAX + 1) * (LMAX + 1) + 1 + (LMAX + 1) * (LMAX + 1));
+  const int lane = threadIdx.x;
+  double* Z = lds;
+  double* dz = Z + T.nb_max * NLM + 1;
+  const int n = T.n_active, W = gridDim.x;
+  const int nbm = T.nb_max;
+  const int k = lane < NLM ? lane : NLM - 1;
+  if ((int)blockIdx.x >= n) return;
+  // the first item's position tables (each item prefetches the next one's)
+  int q = blockIdx.x;
+  int s_nx = T.color_sites[q];
+  unsigned long long low_nx = a.low[q];
+  int nb_nx = lane < nbm ? T.nbq[(size_t)q * nbm + lane] : -1;
+  for (int i = 1; i <= a.nsweeps; ++i) {
+    const int t = (int)(a.t0 + i);
+    const int slot = flow_meas_slot(a.i0, i, a.nsweeps, a.measure_interval, a.defer_last);
+    for (; q < n; q += W) {
+      const int s = __builtin_amdgcn_readfirstlane(s_nx);
+      const unsigned long long low = low_nx;
+      const int nb = nb_nx >= 0 ? nb_nx : s;
+      // this site's proposal (made by this wave), current row and spin: the
+      // wave's own earlier writes, independent of the neighbours' progress
+      __builtin_amdgcn_s_waitcnt(0);
+      const double* pq = a.prop + (size_t)q * PS;
+      const double u_acc = ld_ag(pq + 3);
+      const double p_new = ld_ag(pq + 4 + k), p_cur = ld_ag(a.zrows + (size_t)s * NLM + k);
+      const double e_old = lane < 3 ? ld_ag(a.config + 3 * (size_t)s + lane) : 0.0;
+      const double e_new = lane < 3 ? ld_ag(pq + lane) : 0.0;
+      // wait for the neighbours (lane j: slot j)
+      {
+        const int need = (lane < nbm && ((low >> lane) & 1ull)) ? t : t - 1;
+        unsigned spins = 0;
+        while (!__all(ld_ag(a.done + nb) >= need)) {
+          if (++spins > a.spin_limit) {
+            if (lane == 0) st_ag(a.err, 1);
+            return;
+          }
+          if ((spins & 63u) == 0 && ld_ag(a.err) != 0) return;
+          if (spins > 4) __builtin_amdgcn_s_sleep(1);
+        }
       }
-      if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
-    }
-    // first level of the deterministic reductions: 512-element chunk trees
-    const bool meas = a.measure_interval > 0 && ((a.i0 + i) % a.measure_interval) == 0 &&
-                      !(a.defer_last && i == a.nsweeps);
-    for (long long ch = blockIdx.x; ch < nch; ch += gridDim.x) {
-      const long long i0 = ch * TREE_B + t, i1 = i0 + 256;
-      const double v = block_tree(i0 < n ? a.dE[i0] : 0.0, i1 < n ? a.dE[i1] : 0.0, sh);
-      long long cnt = (i0 < n ? a.acc[i0] : 0) + (i1 < n ? a.acc[i1] : 0);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // stage the neighbour rows (slot-major, nb_max rows) and the 1.0 slot
+      const int32_t* nq = T.nbq + (size_t)q * nbm;
+      const int total = nbm * NLM;
+      for (int b = 0; b < total; b += 4 * 64) {
+        int at[4];
+        double v[4];
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_down(cnt, off, 64);
-      __shared__ long long csum[4];
-      if ((t & 63) == 0) csum[t >> 6] = cnt;
-      __syncthreads();
-      if (t == 0) {
-        a.partial[ch] = v;
-        a.ipart[ch] = (int)(csum[0] + csum[1] + csum[2] + csum[3]);
+        for (int u = 0; u < 4; ++u) {
+          const int x = b + u * 64 + lane;
+          at[u] = x < total ? x : total - 1;
+          const int nbx = nq[at[u] / NLM];
+          v[u] = ld_ag(a.zrows + (size_t)nbx * NLM + (at[u] - (at[u] / NLM) * NLM));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Z[at[u]] = v[u];
+      }
+      const int atom = s % T.na;
+      if (lane == 0) Z[(T.nb_ptr[atom + 1] - T.nb_ptr[atom]) * NLM] = 1.0;
+      if (lane < NLM) dz[lane] = p_new - p_cur;
+      // the next item's position tables, loaded while this one walks
+      {
+        int qn = q + W;
+        if (qn >= n) qn = blockIdx.x;  // the next sweep's first item
+        s_nx = T.color_sites[qn];
+        low_nx = a.low[qn];
+        nb_nx = lane < nbm ? T.nbq[(size_t)qn * nbm + lane] : -1;
       }
       __syncthreads();
+      const int w0 = T.walk_ptr[atom];
+      double dsum = walk_delta<64, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
+      dsum = fold_lanes<64>(dsum);
+      int accept = 0;
+      if (lane == 0) accept = (dsum <= 0.0 || u_acc < sce::det_exp(-a.beta * dsum)) ? 1 : 0;
+      accept = __shfl(accept, 0, 64);
+      if (accept) {
+        if (lane < NLM) st_ag(a.zrows + (size_t)s * NLM + lane, p_new);
+        if (lane < 3) st_ag(a.config + 3 * (size_t)s + lane, e_new);
+      }
+      const double e_now = accept ? e_new : e_old;
+      if (slot >= 0 && lane < 3) st_ag(a.cfg_seg + ((size_t)slot * T.n_sites + s) * 3 + lane, e_now);
+      if (lane == 0) {
+        st_ag(a.dE_seg + (size_t)(i - 1) * n + q, accept ? dsum : 0.0);
+        st_ag(a.acc_seg + (size_t)(i - 1) * n + q, accept);
+      }
+      // publish: this update's writes complete before its flag
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_s_waitcnt(0);
+      if (lane == 0) st_ag(a.done + s, t);
+      __syncthreads();  // LDS reused by the next item
     }
-    if (meas) {
-      const long long N = T.n_sites;
-      for (long long ch = blockIdx.x; ch < nchm; ch += gridDim.x) {
-        const long long i0 = ch * TREE_B + t, i1 = i0 + 256;
+    q = blockIdx.x;
+    // the next sweep's proposals of this wave's positions, from their spins
+    // after this sweep (nothing else writes them): sixteen positions at a
+    // time, four lanes each
+    if (i < a.nsweeps) {
+      const int g = lane / PROPOSE_LANES;
+      for (int q0 = (int)blockIdx.x; q0 < n; q0 += 16 * W) {
+        const int qg = q0 + g * W;
+        const int qv = qg < n ? qg : q0;  // every lane takes part in the exchange
+        const int sv = T.color_sites[qv];
+        const double e[3] = {ld_ag(a.config + 3 * (size_t)sv), ld_ag(a.config + 3 * (size_t)sv + 1),
+                             ld_ag(a.config + 3 * (size_t)sv + 2)};
+        double e2[3], u = 0.0, z[NLM];
+        if (propose_core<LMAX>(a.seed, sv, t + 1, e, a.step, lane % PROPOSE_LANES, e2, &u, z) && qg < n) {
+          double* o = a.prop + (size_t)qg * PS;
+          st_ag(o, e2[0]);
+          st_ag(o + 1, e2[1]);
+          st_ag(o + 2, e2[2]);
+          st_ag(o + 3, u);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double x0 = (i0 < N && T.active[i0]) ? a.config[3 * i0 + c] : 0.0;
-          const double x1 = (i1 < N && T.active[i1]) ? a.config[3 * i1 + c] : 0.0;
-          const double v = block_tree(x0, x1, sh);
-          if (t == 0) mpart[c * nchm + ch] = v;
+          for (int kk = 0; kk < NLM; ++kk) st_ag(o + 4 + kk, z[kk]);
         }
       }
     }
-    if (!grid_sync(a.bar, a.err, &gen_sh, &ok_sh)) return;
-    if (blockIdx.x == 0) {
-      const double dsum = final_tree(a.partial, nch, scratch, sh);
-      double m3[3] = {0.0, 0.0, 0.0};
-      if (meas) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double v = final_tree(mpart + c * nchm, nchm, scratch, sh);
-          __syncthreads();
-          m3[c] = v;
-        }
-      }
-      if (t == 0) {
-        long long nacc = 0;
-        volatile const int* vi = a.ipart;
-        for (long long ch = 0; ch < nch; ++ch) nacc += vi[ch];
-        ChainDev* st = a.st;
-        st->energy = st->energy + dsum;
-        st->acc_metro += nacc;
-        st->att_metro += n;
-        st->nacc_last = nacc;
-        st->sweep_index += 1;
-        if (meas) {
-          const double nd = (double)n;
-          a.obs[4 * row] = st->energy;
-          a.obs[4 * row + 1] = m3[0] / nd;
-          a.obs[4 * row + 2] = m3[1] / nd;
-          a.obs[4 * row + 3] = m3[2] / nd;
-          st->n_obs = (int)(row + 1);
-        }
-      }
-      __syncthreads();
-    }
-    if (meas) ++row;
+  }
+}
+
+__global__ void k_flow_init(int* done, int n, int t0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) done[i] = t0;
+}
+__global__ void k_flow_inactive(int* done, const uint8_t* __restrict__ active, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !active[i]) done[i] = 2147483647;
+}
+
+// first reduction level of every sweep of a segment: block (sweep, chunk)
+__global__ __launch_bounds__(256) void k_flow_chunks(const double* __restrict__ dE_seg, const int* __restrict__ acc_seg,
+                                                     const double* __restrict__ cfg_seg, const uint8_t* __restrict__ active,
+                                                     int n, int n_sites, double* part, long long* ipart, long long pstride,
+                                                     long long istride, int per_sweep, int nsweeps, long long i0, int mi,
+                                                     int defer_last) {
+  const int i = blockIdx.x / per_sweep, b = blockIdx.x % per_sweep;
+  const int slot = flow_meas_slot(i0, i + 1, nsweeps, mi, defer_last);
+  chunk_body<MODE_METRO>(b, dE_seg + (size_t)i * n, n, acc_seg + (size_t)i * n, nullptr, part + i * pstride,
+                         ipart + i * istride, slot >= 0 ? cfg_seg + (size_t)slot * 3 * n_sites : nullptr, active,
+                         n_sites, slot >= 0 ? 1 : 0);
+}
+
+// the segment's final reductions, in sweep order (energy, counters, sweep
+// index, measurement rows and the device binner advance as per sweep)
+__global__ __launch_bounds__(256) void k_flow_final(double* part, long long* ipart, long long pstride, long long istride,
+                                                    int n, int n_sites, ChainDev* st, double* obs, double* scratch,
+                                                    DevBinner* bin, int nsweeps, long long i0, int mi, int defer_last) {
+  for (int i = 0; i < nsweeps; ++i) {
+    const int slot = flow_meas_slot(i0, i + 1, nsweeps, mi, defer_last);
+    final_body<MODE_METRO>(part + i * pstride, n, ipart + i * istride, n_sites, slot >= 0 ? 1 : 0, st, obs, n,
+                           (double)n, scratch, slot >= 0 ? bin : nullptr);
+    __syncthreads();
   }
 }
 
@@ -1544,9 +1609,14 @@ struct sce_chain {
   double *config = nullptr, *zrows = nullptr, *prop = nullptr, *dE = nullptr, *partial = nullptr,
          *inst_e = nullptr, *obs = nullptr;
   int *acc = nullptr, *att = nullptr, *ipart = nullptr, *err = nullptr;
-  unsigned* bar = nullptr;
-  size_t bar_words = 0;
   ChainDev* st = nullptr;
+  // dependency-driven persistent sweeps: segment buffers, sized on first use
+  struct Flow {
+    int cap = 0, mcap = 0;  // sweeps and measured sweeps the buffers hold
+    double *dE = nullptr, *cfg = nullptr, *part = nullptr, *scratch = nullptr;
+    int *acc = nullptr, *done = nullptr;
+    long long* ipart = nullptr;
+  } flow;
   long long* ip64 = nullptr;  // integer partials of the reductions
   bool persistent = false;  // run sweeps in one cooperative launch (opt-in)
   bool use_graph = true;    // replay captured sweeps as hipGraphs
@@ -1688,6 +1758,24 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
     }
     if ((r = dupload(t->mem, &p32, nq.data(), nq.size())) != SCE_OK) return fail(r);
     D.nbq = p32;
+    // which neighbour slots of a position hold lower colours (the waits of the
+    // dependency-driven persistent sweep)
+    if (L.nb_max <= 64) {
+      std::vector<int> col_of((size_t)d->n_sites, -1);
+      for (int c = 0; c < d->n_colors; ++c)
+        for (int q = d->color_ptr[c] - 1; q < d->color_ptr[c + 1] - 1; ++q) col_of[cs[q]] = c;
+      std::vector<unsigned long long> low((size_t)std::max(d->n_active, 1), 0ull);
+      for (int q = 0; q < d->n_active; ++q) {
+        const int cq = col_of[cs[q]];
+        for (int j = 0; j < L.nb_max; ++j) {
+          const int cn = col_of[nq[(size_t)q * L.nb_max + j]];
+          if (cn >= 0 && cn < cq) low[q] |= 1ull << j;
+        }
+      }
+      unsigned long long* pl;
+      if ((r = dupload(t->mem, &pl, low.data(), low.size())) != SCE_OK) return fail(r);
+      D.nbq_low = pl;
+    }
     // the gradient deals a site's adjacency programs in their raw order; the
     // walk holds them pid-sorted (translation invariant): the permutation
     // between the two, per colour position
@@ -1816,6 +1904,17 @@ static int32_t check_lds(const DevTables& D, int L) {
                     "lanes = %d; use more lanes", D.nb_max, walk_lds_bytes(D, L), L);
   return SCE_OK;
 }
+
+#define SCE_DISPATCH_LMAX_R(r, lmax, F, ...)             \
+  switch (lmax) {                                        \
+    case 0: r = F<0>(__VA_ARGS__); break;                \
+    case 1: r = F<1>(__VA_ARGS__); break;                \
+    case 2: r = F<2>(__VA_ARGS__); break;                \
+    case 3: r = F<3>(__VA_ARGS__); break;                \
+    case 4: r = F<4>(__VA_ARGS__); break;                \
+    case 5: r = F<5>(__VA_ARGS__); break;                \
+    default: r = F<6>(__VA_ARGS__); break;               \
+  }
 
 #define SCE_DISPATCH_LMAX(lmax, F, ...)                  \
   switch (lmax) {                                        \
@@ -2244,10 +2343,8 @@ extern "C" int32_t sce_chain_create(sce_tables* t, const double* config, const d
   if ((r = dalloc(c->mem, &c->ipart, (size_t)nmax / TREE_B + 4)) != SCE_OK) return fail(r);
   if ((r = dalloc(c->mem, &c->ip64, 2 * ((size_t)nmax / TREE_B + 4))) != SCE_OK) return fail(r);
   if ((r = dalloc(c->mem, &c->err, 4)) != SCE_OK) return fail(r);
-  c->bar_words = 2 + (size_t)BAR_PAD * (65536 / BAR_GROUP + 2);
-  if ((r = dalloc(c->mem, &c->bar, c->bar_words)) != SCE_OK) return fail(r);
-  if (hipMemset(c->err, 0, 4 * sizeof(int)) != hipSuccess || hipMemset(c->bar, 0, c->bar_words * 4) != hipSuccess)
-    return fail(sce_fail(SCE_EHIP, "hipMemset of the barrier state failed"));
+  if (hipMemset(c->err, 0, 4 * sizeof(int)) != hipSuccess)
+    return fail(sce_fail(SCE_EHIP, "hipMemset of the error flag failed"));
   if ((r = dalloc(c->mem, &c->st, 1)) != SCE_OK) return fail(r);
   ChainDev h{};
   h.energy = energy;
@@ -2410,75 +2507,116 @@ extern "C" int32_t sce_overrelaxation_sweep(sce_chain* c, double beta, int32_t o
   return SCE_OK;
 }
 
-template <int L, int LMAX>
-static int32_t launch_sweeps_l(sce_chain* c, SweepArgs& sa, hipStream_t s) {
+// Segment buffers of the dependency-driven sweep for `cap` sweeps of which at
+// most `mcap` are measured.
+static constexpr int FLOW_SEG = 128;
+static int32_t flow_reserve(sce_chain* c, int cap, int mcap) {
+  sce_chain::Flow& F = c->flow;
   const DevTables& D = c->t->dev;
-  const size_t shmem = walk_lds_bytes(D, L);
-  const void* f = reinterpret_cast<const void*>(&k_sweeps<L, LMAX>);
-  int per_cu = 0, cus = 0, dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, TPB, shmem));
-  constexpr int G = TPB / L;
-  long long want = std::max<long long>((c->t->max_class + G - 1) / G, 1);
-  long long grid = std::min<long long>((long long)per_cu * cus, want);
-  grid = std::min<long long>(grid, 65536);
-  if (grid < 1) return sce_fail(SCE_EHIP, "persistent sweep kernel cannot be resident");
-  HIPCHK(hipMemsetAsync(c->bar, 0, c->bar_words * sizeof(unsigned), s));
-  HIPCHK(hipMemsetAsync(c->err, 0, sizeof(int), s));
-  void* args[] = {&sa};
-  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)grid), dim3(TPB), args, (unsigned)shmem, s));
+  const size_t n = (size_t)std::max(D.n_active, 1), N = (size_t)std::max(D.n_sites, 1);
+  const size_t nbx = std::max<size_t>((n + TREE_B - 1) / TREE_B, 1), nchm = (N + TREE_B - 1) / TREE_B;
+  if (!F.done) {
+    CHK(dalloc(c->mem, &F.done, N));
+    CHK(dalloc(c->mem, &F.scratch, partial_capacity((long long)std::max(n, N))));
+  }
+  if (cap > F.cap) {
+    c->mem.release(F.dE);
+    c->mem.release(F.acc);
+    c->mem.release(F.part);
+    c->mem.release(F.ipart);
+    F.dE = nullptr, F.acc = nullptr, F.part = nullptr, F.ipart = nullptr;
+    F.cap = 0;
+    CHK(dalloc(c->mem, &F.dE, (size_t)cap * n));
+    CHK(dalloc(c->mem, &F.acc, (size_t)cap * n));
+    CHK(dalloc(c->mem, &F.part, (size_t)cap * (nbx + 3 * nchm)));
+    CHK(dalloc(c->mem, &F.ipart, (size_t)cap * 2 * nbx));
+    F.cap = cap;
+  }
+  if (mcap > F.mcap) {
+    c->mem.release(F.cfg);
+    F.cfg = nullptr;
+    F.mcap = 0;
+    CHK(dalloc(c->mem, &F.cfg, (size_t)mcap * 3 * N));
+    F.mcap = mcap;
+  }
   return SCE_OK;
 }
 
 template <int LMAX>
-static int32_t launch_sweeps(int L, sce_chain* c, SweepArgs& sa, hipStream_t s) {
-  if (L == 16) return launch_sweeps_l<16, LMAX>(c, sa, s);
-  if (L == 32) return launch_sweeps_l<32, LMAX>(c, sa, s);
-  return launch_sweeps_l<64, LMAX>(c, sa, s);
+static int32_t launch_flow(sce_chain* c, FlowArgs& fa, hipStream_t s) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
+  const DevTables& D = c->t->dev;
+  const size_t shmem = ((size_t)D.nb_max * NLM + 1 + NLM) * sizeof(double);
+  const void* f = reinterpret_cast<const void*>(&k_flow<LMAX>);
+  int per_cu = 0, cus = 0, dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, shmem));
+  const long long grid = std::min<long long>((long long)per_cu * cus, std::max(D.n_active, 1));
+  if (grid < 1) return sce_fail(SCE_EHIP, "the persistent sweep kernel cannot be resident");
+  void* args[] = {&fa};
+  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)grid), dim3(64), args, (unsigned)shmem, s));
+  return SCE_OK;
 }
 
-// Enqueue `n` Metropolis sweeps as one persistent launch.
+// n sweeps (sweeps i0 + 1 .. i0 + n of this call) as one dependency-driven
+// launch plus the segment's reductions.
 static int32_t enqueue_persistent(sce_chain* c, int L, double beta, long long n, long long i0,
-                                  long long row0, int64_t measure_interval, int defer_last) {
-  CHK(check_lds(c->t->dev, L));
+                                  int64_t measure_interval, int defer_last) {
+  const DevTables& D = c->t->dev;
+  if (L != 64) return sce_fail(SCE_EINVAL, "persistent sweeps walk with 64 lanes per site; got %d", L);
+  if (!D.nbq_low) return sce_fail(SCE_EINVAL, "persistent sweeps need at most 64 neighbour slots; the stencil has %d",
+                                  D.nb_max);
+  if (n <= 0) return SCE_OK;
+  if (c->sweeps_done + n >= 2147483647LL - 1)
+    return sce_fail(SCE_ERANGE_COUNTER, "sweep_index would exhaust the 32-bit RNG counter word");
+  const int mi = (int)std::max<int64_t>(measure_interval, 0);
+  int nmeas = 0;
+  for (int i = 1; i <= n; ++i) nmeas += flow_meas_slot(i0, i, (int)n, mi, defer_last) >= 0 ? 1 : 0;
+  CHK(flow_reserve(c, (int)n, nmeas));
   ChainDev h;
   CHK(read_state(c, &h));
-  SweepArgs sa{};
-  sa.T = c->t->dev;
-  sa.config = c->config;
-  sa.zrows = c->zrows;
-  sa.prop = c->prop;
-  sa.dE = c->dE;
-  sa.partial = c->partial;
-  sa.obs = c->obs;
-  sa.acc = c->acc;
-  sa.ipart = c->ipart;
-  sa.st = c->st;
-  sa.cptr = c->t->d_color_ptr;
-  sa.n_colors = c->t->n_colors;
-  sa.nsweeps = n;
-  sa.sweep0 = c->sweeps_done;
-  sa.i0 = i0;
-  sa.obs_row0 = row0;
-  sa.measure_interval = (int)measure_interval;
-  sa.defer_last = defer_last;
-  sa.bar = c->bar;
-  sa.err = c->err;
-  sa.beta = beta;
-  sa.step = h.step;
-  sa.seed = h.seed;
+  const sce_chain::Flow& F = c->flow;
+  hipStream_t s = c->stream;
+  const int na = D.n_active, N = D.n_sites;
+  // first sweep's proposals; every site's done flag = the completed sweeps
+  // (inactive sites never update: always done)
+  SCE_DISPATCH_LMAX(D.lmax, launch_propose, D, c->config, c->prop, c->st, s);
+  k_flow_init<<<(N + 255) / 256, 256, 0, s>>>(F.done, N, (int)c->sweeps_done);
+  if (D.n_active < D.n_sites) k_flow_inactive<<<(N + 255) / 256, 256, 0, s>>>(F.done, D.active, N);
+  HIPCHK(hipMemsetAsync(c->err, 0, sizeof(int), s));
+  FlowArgs fa{};
+  fa.T = D;
+  fa.config = c->config;
+  fa.zrows = c->zrows;
+  fa.prop = c->prop;
+  fa.dE_seg = F.dE;
+  fa.acc_seg = F.acc;
+  fa.cfg_seg = F.cfg;
+  fa.done = F.done;
+  fa.low = D.nbq_low;
+  fa.err = c->err;
+  fa.t0 = c->sweeps_done;
+  fa.nsweeps = (int)n;
+  fa.i0 = i0;
+  fa.measure_interval = mi;
+  fa.defer_last = defer_last;
+  fa.beta = beta;
+  fa.step = h.step;
+  fa.seed = h.seed;
+  fa.spin_limit = FLOW_SPIN_LIMIT;
   int32_t r = SCE_OK;
-  switch (c->t->dev.lmax) {
-    case 0: r = launch_sweeps<0>(L, c, sa, c->stream); break;
-    case 1: r = launch_sweeps<1>(L, c, sa, c->stream); break;
-    case 2: r = launch_sweeps<2>(L, c, sa, c->stream); break;
-    case 3: r = launch_sweeps<3>(L, c, sa, c->stream); break;
-    case 4: r = launch_sweeps<4>(L, c, sa, c->stream); break;
-    case 5: r = launch_sweeps<5>(L, c, sa, c->stream); break;
-    default: r = launch_sweeps<6>(L, c, sa, c->stream); break;
-  }
-  if (r != SCE_OK) return r;
+  SCE_DISPATCH_LMAX_R(r, D.lmax, launch_flow, c, fa, s);
+  CHK(r);
+  // the reductions of every sweep of the segment (the per-colour path's trees)
+  const long long nbx = std::max<long long>((na + TREE_B - 1) / TREE_B, 1), nchm = (N + TREE_B - 1) / TREE_B;
+  const long long per = nmeas > 0 ? std::max(nbx, nchm) : nbx;
+  const long long pstride = nbx + 3 * nchm, istride = 2 * nbx;
+  k_flow_chunks<<<(unsigned)(per * n), 256, 0, s>>>(F.dE, F.acc, F.cfg, D.active, na, N, F.part, F.ipart, pstride,
+                                                     istride, (int)per, (int)n, i0, mi, defer_last);
+  k_flow_final<<<1, 256, 0, s>>>(F.part, F.ipart, pstride, istride, na, N, c->st, c->obs, F.scratch, c->bin, (int)n,
+                                 i0, mi, defer_last);
+  HIPCHK(hipGetLastError());
   c->sweeps_done += n;
   return SCE_OK;
 }
@@ -2558,8 +2696,6 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   CHK(check_lanes(lanes));
   if (nsweeps < 0) return sce_fail(SCE_EINVAL, "nsweeps must be >= 0");
   if (or_per_metropolis < 0) return sce_fail(SCE_EINVAL, "or_per_metropolis must be >= 0");
-  if (c->bin && c->persistent && or_per_metropolis == 0 && measure_interval > 0)
-    return sce_fail(SCE_EINVAL, "the device binner is fed by per-colour launches; disable persistent sweeps");
   const int64_t want_obs = measure_interval > 0 ? nsweeps / measure_interval : 0;
   if (want_obs > 0 && (!obs || obs_cap < want_obs))
     return sce_fail(SCE_EDIM, "obs buffer holds %lld rows; %lld needed", (long long)obs_cap, (long long)want_obs);
@@ -2586,10 +2722,11 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   hipEvent_t t0 = next_event(c);
   HIPCHK(hipEventRecord(t0, c->stream));
   if (c->persistent && or_per_metropolis == 0) {
-    // segments end at renormalisation points; each is one cooperative launch
-    int64_t i = 0, row = 0;
+    // segments end at renormalisation points (and hold at most FLOW_SEG
+    // sweeps); each is one dependency-driven cooperative launch
+    int64_t i = 0;
     while (i < nsweeps) {
-      int64_t seg = nsweeps - i;
+      int64_t seg = std::min<int64_t>(nsweeps - i, FLOW_SEG);
       if (renorm_interval > 0) seg = std::min<int64_t>(seg, renorm_interval - (i % renorm_interval));
       hipEvent_t e0 = nullptr;
       if (c->timing) {
@@ -2598,9 +2735,8 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
       }
       const bool renorm_after = renorm_interval > 0 && (i + seg) % renorm_interval == 0;
       const bool meas_end = measure_interval > 0 && (i + seg) % measure_interval == 0;
-      CHK(enqueue_persistent(c, lanes, beta, seg, i, row, measure_interval, renorm_after && meas_end));
+      CHK(enqueue_persistent(c, lanes, beta, seg, i, measure_interval, renorm_after && meas_end));
       if (c->timing) HIPCHK(hipEventRecord(next_event(c), c->stream));
-      if (measure_interval > 0) row += (i + seg) / measure_interval - i / measure_interval;
       i += seg;
       if (renorm_after) {
         enqueue_renorm(c);
@@ -2652,7 +2788,7 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   if (c->persistent && or_per_metropolis == 0 && nsweeps > 0) {
     int e = 0;
     HIPCHK(hipMemcpy(&e, c->err, sizeof e, hipMemcpyDeviceToHost));
-    if (e) return sce_fail(SCE_EHIP, "persistent sweep: grid barrier wait exceeded its bound");
+    if (e) return sce_fail(SCE_EHIP, "persistent sweep: a wait on neighbour updates exceeded its bound");
   }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, t0, t1));

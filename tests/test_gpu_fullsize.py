@@ -69,3 +69,23 @@ def test_benchmark_size_delayed_acceptance_sweep_bitwise():
     assert E_d == E
     inf = g.info()
     assert (inf.acc_metro, inf.acc_stage1) == (nacc, n1)
+
+
+def test_benchmark_size_persistent_sweeps_equal_graph_path():
+    """The dependency-driven persistent sweep at bcc 32^3: 40 sweeps measured
+    every sweep, with a renormalisation at 25, bit-identical to per-colour
+    graph launches (rows, spins, energy, counters, measurement rows)."""
+    _, H, T, gH = _model("bcc32")
+    st = S.ChainState(H, rand_config(H.n_sites, 99), 1.3)
+    a = S.GPUChainState(gH, st, seed=31)
+    b = S.GPUChainState(gH, st, seed=31)
+    a.set_persistent(True)
+    oa = S.gpu_run_sweeps(a, gH, None, 1 / 0.04, 40, renorm_interval=25, measure_interval=1)
+    ob = S.gpu_run_sweeps(b, gH, None, 1 / 0.04, 40, renorm_interval=25, measure_interval=1)
+    assert np.array_equal(oa.view(np.uint64), ob.view(np.uint64))
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca.view(np.uint64), cb.view(np.uint64))
+    assert np.array_equal(za.view(np.uint64), zb.view(np.uint64)) and Ea == Eb
+    ia, ib = a.info(), b.info()
+    assert (ia.acc_metro, ia.att_metro, ia.sweep_index) == (ib.acc_metro, ib.att_metro, ib.sweep_index)

@@ -255,10 +255,13 @@ def test_chained_sweeps_equal_single_sweeps(case, lanes):
         assert obs.shape == (3, 4) and list(obs[:, 0]) == [energies[1], energies[3], energies[5]]
 
 
-@pytest.mark.parametrize("lanes", LANES)
-@pytest.mark.parametrize("case", [c for c in fixture_cases() if c[0] in ("dimer", "dimer+field", "bcc-tet", "fcc-l2", "sc-onsite")],
-                         ids=lambda c: c[0])
-def test_persistent_equals_graph_path_and_oracle(case, lanes):
+@pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
+def test_persistent_equals_graph_path_and_oracle(case):
+    """The dependency-driven persistent sweep (each site waits only on its
+    neighbours' sweep counters) against per-colour graph launches and the
+    oracle: the same bits for config, rows, energy, counters and every
+    measurement row, across renormalisations and measurement schedules."""
+    lanes = 64
     name, na, terms, dims, mm, fl = case
     H, T = build_pair(na, terms, dims, mm, fl)
     gH = S.GPUTiledHamiltonian(H)
@@ -272,7 +275,7 @@ def test_persistent_equals_graph_path_and_oracle(case, lanes):
     ca, za, Ea = a.download()
     cb, zb, Eb = b.download()
     assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
-    assert np.array_equal(oa[:, 0], ob[:, 0]) and np.allclose(oa, ob, rtol=0, atol=1e-15)
+    assert np.array_equal(oa.view(np.uint64), ob.view(np.uint64))
     # oracle: the same sweeps with device-contract renormalisations
     c2, z2, E = st.config.copy(), st.zrows.copy(), st.energy
     for sw in range(1, 24):
@@ -282,6 +285,28 @@ def test_persistent_equals_graph_path_and_oracle(case, lanes):
     assert np.array_equal(ca, c2) and np.array_equal(za, z2) and Ea == E
     ia, ib = a.info(), b.info()
     assert (ia.acc_metro, ia.att_metro, ia.sweep_index) == (ib.acc_metro, ib.att_metro, ib.sweep_index)
+
+
+def test_persistent_long_segments_and_lane_guard():
+    """Calls longer than one segment buffer (128 sweeps), every sweep measured,
+    no renormalisation inside: persistent == graph path, row by row; the
+    dependency-driven walk takes 64 lanes per site and says so otherwise."""
+    na, terms, dims = bcc_small(13)
+    H = S.TiledHamiltonian(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 5), 0.6)
+    a = S.GPUChainState(gH, st, seed=7)
+    b = S.GPUChainState(gH, st, seed=7)
+    a.set_persistent(True)
+    for n in (300, 17):
+        oa = S.gpu_run_sweeps(a, gH, None, 30.0, n, renorm_interval=0, measure_interval=1)
+        ob = S.gpu_run_sweeps(b, gH, None, 30.0, n, renorm_interval=0, measure_interval=1)
+        assert np.array_equal(oa.view(np.uint64), ob.view(np.uint64)), n
+    ca, za, Ea = a.download()
+    cb, zb, Eb = b.download()
+    assert np.array_equal(ca, cb) and np.array_equal(za, zb) and Ea == Eb
+    with pytest.raises(SceError, match="64 lanes"):
+        S.gpu_run_sweeps(a, gH, None, 30.0, 3, renorm_interval=0, lanes=32)
 
 
 def test_resume_from_downloaded_state_and_counters_is_bitwise():
