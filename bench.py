@@ -300,6 +300,59 @@ def uneven_leg(S, M, sweeps=100, n_chains=256):
             "sweeps": sweeps, "classes": out}
 
 
+def compound_leg(S, M, sweeps=10000, equil=1000, kts=(0.01, 0.02, 0.04, 0.08)):
+    """Plain form on the compound sweep of run_mc (src/run.jl:143-150): bcc 16^3
+    (8192 spins, body-2 l<=2 to the 3rd shell + l=1 triangles), each sweep one
+    Metropolis sweep then one overrelaxation pass, `sweeps` measured sweeps at
+    each of four fixed kT after `equil` sweeps with run_mc's step adaptation.
+    Every compound sweep is one replayed hipGraph (measurement fused into the
+    overrelaxation reduction, the next sweep's proposals made there too);
+    Metropolis and overrelaxation attempts per second are reported apart."""
+    from sce_amd.batch import _adapt
+    na, terms, dims = M.config_model("bcc16", seed=0)
+    H = S.TiledHamiltonian(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    out = []
+    for k, kT in enumerate(kts):
+        st = S.ChainState(H, S.random_config(H.n_sites, np.random.default_rng(100 + k)), 0.6)
+        g = S.GPUChainState(gH, st, seed=9000 + k)
+        beta = 1.0 / kT
+        done = 0
+        while done < equil:
+            n = min(50, equil - done)
+            g.reset_counters()
+            S.gpu_run_sweeps(g, gH, None, beta, n, renorm_interval=0, or_per_metropolis=1)
+            done += n
+            if n == 50:
+                inf = g.info()
+                g.step = _adapt(g.step, inf.acc_metro, inf.att_metro, 0.5)
+        S.gpu_renormalize(g)
+        S.gpu_run_sweeps(g, gH, None, beta, 8, renorm_interval=1000, or_per_metropolis=1, measure_interval=1)
+        g.reset_counters()
+        t0 = time.perf_counter()
+        obs = S.gpu_run_sweeps(g, gH, None, beta, sweeps, renorm_interval=1000, or_per_metropolis=1,
+                               measure_interval=1)
+        wall = time.perf_counter() - t0
+        inf = g.info()
+        stats = S.standard_stats(obs, kT, H.n_active)
+        ess = S.effective_samples(obs[:, 0])
+        row = {"kT_eV": kT, "ms_per_compound_sweep": 1e3 * wall / sweeps,
+               "metropolis_attempts_per_s": inf.att_metro / wall, "or_attempts_per_s": inf.att_or / wall,
+               "metropolis_acceptance": inf.acc_metro / max(inf.att_metro, 1),
+               "or_acceptance": inf.acc_or / max(inf.att_or, 1), "step": g.step,
+               "effective_samples_per_s_E": ess / wall if math.isfinite(ess) else None,
+               "energy_per_site": stats["energy"].mean / H.n_active, "absm": stats["absm"].mean}
+        if k == 0:   # the same sweeps launched directly (the form before graph capture)
+            g.set_graph(False)
+            t0 = time.perf_counter()
+            S.gpu_run_sweeps(g, gH, None, beta, 1000, renorm_interval=1000, or_per_metropolis=1, measure_interval=1)
+            row["ms_per_compound_sweep_direct_launches"] = 1e3 * (time.perf_counter() - t0) / 1000
+        out.append(row)
+    return {"workload": f"bcc16: bcc 16^3 x 2 atoms = {H.n_sites} spins, body-2 l<=2 to 3rd shell + l=1 "
+                        "triangles; compound sweep = 1 Metropolis + 1 overrelaxation pass",
+            "sweeps_per_kT": sweeps, "equilibration_sweeps": equil, "colours": H.n_colors, "by_kT": out}
+
+
 def setup_leg(S, sweeps=200, graph=True):
     """The Nd2Fe14B cell read from its setup file (tests/golden/nd2fe14b.toml, the
     data of bench/assets/nd2fe14b.toml), modelled as the reference's triplet-heavy
@@ -380,6 +433,10 @@ def main():
     ap.add_argument("--no-da", action="store_true", help="skip the delayed-acceptance leg")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched legs (64-replica PT, 256-chain uneven batch)")
+    ap.add_argument("--no-compound", action="store_true",
+                    help="skip the bcc 16^3 Metropolis + overrelaxation leg (4 kT x 10k sweeps)")
+    ap.add_argument("--compound-only", action="store_true",
+                    help="run only the bcc 16^3 Metropolis + overrelaxation leg and print its JSON")
     ap.add_argument("--no-setup-model", action="store_true",
                     help="skip the Nd2Fe14B setup-file leg (nbody 3 at 4^3 and 8^3 cells)")
     ap.add_argument("--setup-model-only", action="store_true",
@@ -414,6 +471,9 @@ def main():
     import sce_amd as S
     from sce_amd import models as M
 
+    if args.compound_only:
+        print(json.dumps({"compound": compound_leg(S, M)}), flush=True)
+        return
     if args.setup_model_only:
         print(json.dumps({"setup_model": setup_leg(S, graph=not args.no_graph)}), flush=True)
         return
@@ -521,6 +581,8 @@ def main():
                                             plain=ess_leg_out)
     if not args.no_batch:
         line["batched"] = {"parallel_tempering": pt_leg(S, M), "uneven_batch": uneven_leg(S, M)}
+    if not args.no_compound:
+        line["compound"] = compound_leg(S, M)
     if not args.no_setup_model:
         line["setup_model"] = setup_leg(S, graph=not args.no_graph)
     if not args.no_cpu:

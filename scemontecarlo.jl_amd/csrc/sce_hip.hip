@@ -1626,6 +1626,10 @@ struct sce_chain {
   // deferred) + 8 measurement flag of the previous sweep's deferred final]
   GraphSlot gexec[16];
   int g_lanes = 0;
+  // captured compound sweeps (Metropolis + overrelaxation passes):
+  // [fused measurement + 2 need_prop + 4 next_prop], for cg_or passes
+  GraphSlot cgexec[8];
+  int cg_or = 0;
   size_t obs_cap = 0;
   hipStream_t stream = nullptr;
   long long sweeps_done = 0;  // host mirror of sweep_index
@@ -1640,7 +1644,9 @@ struct sce_chain {
   void drop_graphs() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& g : gexec) g.reset();
+    for (auto& g : cgexec) g.reset();
     g_lanes = 0;
+    cg_or = 0;
   }
   ~sce_chain() {
     drop_graphs();
@@ -2050,7 +2056,8 @@ static void launch_reduce_propose(sce_chain* c, int meas, hipStream_t s, bool de
   const long long nprop = prop_blocks(r.n);
   double* scratch = c->partial + nbx + 3 * std::max<long long>(nchm, 1);
   k_chunks_propose<LMAX, MODE><<<(unsigned)(nchunk + nprop), 256, 0, s>>>(c->dE, n, c->acc,
-                                                                         MODE == MODE_DA ? c->att : nullptr,
+                                                                         (MODE == MODE_DA || MODE == MODE_OR) ? c->att
+                                                                                                             : nullptr,
                                                                          c->partial, c->ip64,
                                                                    c->config, D.active, D.n_sites, meas, nchunk, D,
                                                                    c->prop, c->st, r);
@@ -2254,7 +2261,18 @@ static int32_t enqueue_metro(sce_chain* c, int L, int meas = 0, bool need_prop =
   return SCE_OK;
 }
 
-static int32_t enqueue_or(sce_chain* c, int L, int pass) {
+template <int LMAX>
+static void launch_reduce_propose_or(sce_chain* c, int meas, hipStream_t s, bool defer_final, PropRange r) {
+  launch_reduce_propose<LMAX, MODE_OR>(c, meas, s, defer_final, r);
+}
+
+// One overrelaxation pass (every colour, then the reduction).  meas: the
+// compound sweep's measurement rides on this pass's reduction (the reference
+// measures after the whole compound sweep, run.jl:143-150, 191-200);
+// next_prop: the next Metropolis sweep's proposals ride in its first level
+// (every spin is final for the sweep, and the Metropolis final of this sweep
+// has run, so they key on the completed count + 1).
+static int32_t enqueue_or(sce_chain* c, int L, int pass, int meas = 0, bool next_prop = false) {
   const DevTables& D = c->t->dev;
   CHK(check_lds(D, L));
   hipStream_t s = c->stream;
@@ -2263,8 +2281,25 @@ static int32_t enqueue_or(sce_chain* c, int L, int pass) {
     if (n == 0) continue;
     SCE_DISPATCH_LMAX(D.lmax, launch_or_run, L, D, c->config, c->zrows, c->dE, c->acc, c->att, c->st, pass, lo, n, s);
   }
-  launch_reduce<MODE_OR>(c->dE, D.n_active, c->acc, c->att, c->partial, c->ip64, c->config, D.active,
-                         D.n_sites, 0, c->st, nullptr, 0, 1.0, s);
+  if (next_prop) {
+    SCE_DISPATCH_LMAX(D.lmax, launch_reduce_propose_or, c, meas, s, false, PropRange{0, D.n_active, 1});
+  } else {
+    launch_reduce<MODE_OR>(c->dE, D.n_active, c->acc, c->att, c->partial, c->ip64, c->config, D.active,
+                           D.n_sites, meas, c->st, meas ? c->obs : nullptr, D.n_active, (double)D.n_active, s,
+                           meas ? c->bin : nullptr);
+  }
+  return SCE_OK;
+}
+
+// One compound sweep (run.jl:143-150): a Metropolis sweep (its proposals made
+// unless the previous compound sweep's last reduction made them), then
+// `npass` overrelaxation passes; meas and next_prop ride on the last pass.
+static int32_t enqueue_compound(sce_chain* c, int L, int npass, int meas, bool need_prop, bool next_prop) {
+  CHK(enqueue_metro(c, L, 0, need_prop, false, false, -1, false));
+  for (int p = 0; p < npass; ++p) {
+    const bool last = p + 1 == npass;
+    CHK(enqueue_or(c, L, p, last ? meas : 0, last && next_prop));
+  }
   return SCE_OK;
 }
 
@@ -2306,6 +2341,21 @@ static int32_t ensure_graph(sce_chain* c, int L, int k) {
   const int32_t r = capture_graph(c->stream, c->gexec[k], [&] {
     return enqueue_metro(c, L, k & 1, !chained, (k & 4) != 0, (k & 4) != 0, chained ? (k >> 3) & 1 : -1, chained);
   });
+  c->sweeps_done = before;
+  return r;
+}
+
+// Capture one compound sweep as a hipGraph (variant k of cgexec).
+static int32_t ensure_compound_graph(sce_chain* c, int L, int npass, int k) {
+  if (c->g_lanes != L || c->cg_or != npass) {
+    c->drop_graphs();
+    c->g_lanes = L;
+    c->cg_or = npass;
+  }
+  if (c->cgexec[k]) return SCE_OK;
+  const long long before = c->sweeps_done;
+  const int32_t r = capture_graph(c->stream, c->cgexec[k],
+                                  [&] { return enqueue_compound(c, L, npass, k & 1, (k & 2) != 0, (k & 4) != 0); });
   c->sweeps_done = before;
   return r;
 }
@@ -2744,8 +2794,30 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
           enqueue_measure(c);
       }
     }
+  } else if (or_per_metropolis > 0) {
+    // compound sweeps: each is one captured graph (Metropolis, then the
+    // passes); the last pass's reduction carries the measurement and, unless a
+    // renormalisation or the call's end comes next, the next sweep's proposals
+    bool have_prop = false;
+    for (int64_t i = 1; i <= nsweeps; ++i) {
+      const bool renorm = renorm_interval > 0 && i % renorm_interval == 0;
+      const bool meas = measure_interval > 0 && i % measure_interval == 0;
+      const bool fused = meas && !renorm;
+      const bool next_prop = !renorm && i < nsweeps;
+      if (c->use_graph && !c->timing) {
+        const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0);
+        CHK(ensure_compound_graph(c, lanes, or_per_metropolis, k));
+        HIPCHK(hipGraphLaunch(c->cgexec[k].x, c->stream));
+        c->sweeps_done += 1;
+      } else {
+        CHK(enqueue_compound(c, lanes, or_per_metropolis, fused ? 1 : 0, !have_prop, next_prop));
+      }
+      have_prop = next_prop;
+      if (renorm) enqueue_renorm(c);
+      if (meas && !fused) enqueue_measure(c);
+    }
   } else {
-    const bool graphs = c->use_graph && !c->timing && or_per_metropolis == 0;
+    const bool graphs = c->use_graph && !c->timing;
     // Proposals depend only on each site's own spin at the sweep's start, so a
     // sweep's reduction may make the next sweep's (fused launch) when nothing
     // changes spins in between: not across renormalisations or overrelaxation,
@@ -2761,8 +2833,8 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
       const bool meas = measure_interval > 0 && i % measure_interval == 0;
       // a measurement rides on the sweep's own reduction unless something
       // (overrelaxation, renormalisation) comes between them (run.jl:191-200)
-      const bool fused = meas && !renorm && or_per_metropolis == 0;
-      const bool next_prop = or_per_metropolis == 0 && !renorm && i < nsweeps;
+      const bool fused = meas && !renorm;
+      const bool next_prop = !renorm && i < nsweeps;
       const bool defer = next_prop && !c->timing;
       if (graphs) {
         const int k = (fused ? 1 : 0) + (have_prop ? 0 : 2) + (next_prop ? 4 : 0) + (pending == 1 ? 8 : 0);
@@ -2775,7 +2847,6 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
       have_prop = next_prop;
       pending = defer ? (fused ? 1 : 0) : -1;
       carry = next_prop && !c->timing;
-      for (int p = 0; p < or_per_metropolis; ++p) CHK(enqueue_or(c, lanes, p));
       if (renorm) enqueue_renorm(c);
       if (meas && !fused) enqueue_measure(c);
     }

@@ -184,6 +184,51 @@ def test_overrelaxation_bitwise(case):
     assert (inf.acc_or, inf.att_or) == (acc_or, att_or)
 
 
+@pytest.mark.parametrize("case", [c for c in fixture_cases() if c[0] in ("dimer", "3body", "bcc-l2-tri", "bcc-tet", "dimer+field", "sc-onsite")],
+                         ids=lambda c: c[0])
+def test_compound_sweeps_graph_and_chained_bitwise(case):
+    """Compound sweeps (run.jl:143-150: one Metropolis sweep then two
+    overrelaxation passes) through sce_run_sweeps — graph-replayed and direct,
+    the next sweep's proposals made in the last pass's reduction, the
+    measurement riding on it — equal the oracle's composition sweep by sweep,
+    measurement rows included, across renormalisations."""
+    name, na, terms, dims, mm, fl = case
+    H, T = build_pair(na, terms, dims, mm, fl)
+    gH = S.GPUTiledHamiltonian(H)
+    st = S.ChainState(H, rand_config(H.n_sites, 19), 0.6)
+    seed, beta, lanes, nsw = 505, 1 / 0.05, 64, 11
+    c2, z2, E = st.config.copy(), st.zrows.copy(), st.energy
+    rows = []
+    acc_or = att_or = 0
+    act = H.site_active.astype(bool)
+    for sw in range(1, nsw + 1):
+        _, E, _, _ = T.keyed_sweep(c2, z2, E, beta, 0.6, seed, sw, lanes)
+        for p in range(2):
+            a, n, E = T.keyed_or_sweep(c2, z2, E, beta, seed, sw, p, lanes)
+            acc_or += a
+            att_or += n
+        if sw % 5 == 0:
+            _, E = T.renormalize(c2, z2, E)
+        if sw % 2 == 0:
+            rows.append([E, *np.mean(c2[act], axis=0)])
+    rows = np.array(rows)
+    outs = []
+    for graph in (True, False):
+        g = S.GPUChainState(gH, st, seed=seed)
+        g.set_graph(graph)
+        obs = S.gpu_run_sweeps(g, gH, None, beta, nsw, renorm_interval=5, lanes=lanes, or_per_metropolis=2,
+                               measure_interval=2)
+        cfg_d, z_d, E_d = g.download()
+        assert np.array_equal(cfg_d.view(np.uint64), c2.view(np.uint64)), graph
+        assert np.array_equal(z_d.view(np.uint64), z2.view(np.uint64)), graph
+        assert E_d == E, graph
+        assert np.array_equal(obs[:, 0], rows[:, 0]) and np.allclose(obs[:, 1:], rows[:, 1:], rtol=0, atol=1e-15)
+        inf = g.info()
+        assert (inf.acc_or, inf.att_or, inf.sweep_index) == (acc_or, att_or, nsw)
+        outs.append(obs)
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
+
+
 def test_renormalize_and_total_energy_bitwise():
     na, terms, dims = bcc_small(3)
     H, T = build_pair(na, terms, dims)
