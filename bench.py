@@ -510,6 +510,9 @@ def main():
     # region replays (measured, chained) are captured before it starts
     S.gpu_run_sweeps(gst, gH, None, beta, 8, renorm_interval=1000, lanes=args.lanes, measure_interval=1)
     snapshot = gst.download()[0]
+    # the autotuner's view of this chain (reported; the timed region keeps the
+    # pinned --lanes and launch mode, which are part of the determinism scope)
+    tl, tp, tt = S.autotune_chain(gH, S.ChainState(H, snapshot, gst.step), beta, sweeps=200)
     # kernel timing: the colour walks of 20 passes launched back to back between
     # one pair of HIP events on the chain's stream (state restored afterwards)
     walk_ms_avg, walk_n = gst.time_walk(args.lanes, 20)
@@ -548,6 +551,9 @@ def main():
         "timed_region_effective_samples_per_s_E": ess / wall if math.isfinite(ess) else None,
         "timed_region_tau_int_E": (args.steps / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
         "acceptance": inf.acc_metro / max(inf.att_metro, 1),
+        "autotune": {"seconds_per_sweep": tt, "fastest": {"lanes": tl, "persistent": tp},
+                     "note": "200 measured sweeps of a throwaway copy of the chain per candidate "
+                             "(lanes per site; p = dependency-driven persistent sweep)"},
         "launch_mode": ("persistent" if args.persistent else
                         "direct launches" if args.no_graph else "hipGraph per sweep"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -111,6 +111,17 @@ int32_t sce_layout_check(const sce_tables_desc* d, int64_t out[5]);
 /* Layout facts: [n_sublattices, max_neighbours, max_walk_entries,
  * total_walk_entries, fmax = widest surrogate descriptor row] */
 int32_t sce_tables_layout_info(const sce_tables* t, int64_t out[5]);
+/* The per-position tables sce_tables_create builds on the device (no reference
+ * counterpart; diagnostics): for every colour position q (n_active of them),
+ * the nb_max neighbour sites (0-based, padded with the site itself), the
+ * lower-colour mask of those slots (when nb_max <= 64) and the raw adjacency
+ * index -> pid-sorted index permutation (adj_max per position).  sizes =
+ * [nb_max, adj_max, has_mask]; any array may be NULL (sizes only).
+ * sce_position_tables_host makes the same tables on the host (no device). */
+int32_t sce_tables_position_tables(sce_tables* t, int32_t* nbq, uint64_t* low, uint16_t* perm,
+                                   int64_t sizes[3]);
+int32_t sce_position_tables_host(const sce_tables_desc* d, int32_t* nbq, uint64_t* low, uint16_t* perm,
+                                 int64_t sizes[3]);
 
 /* ---- device chain: GPUChainState(gH, st; seed) ------------------------------ */
 int32_t sce_chain_create(sce_tables* t, const double* config, const double* zrows,

@@ -51,7 +51,7 @@ int factor_sites(const sce_tables_desc* d, int j, int sites[8]) {
 
 }  // namespace
 
-int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
+int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L, bool verify) {
   const Geo g{d->n_cell_atoms, {d->dims[0], d->dims[1], d->dims[2]}};
   const int na = g.na, nlm = d->nlm, N = d->n_sites;
   if (na * g.d[0] * g.d[1] * g.d[2] != N)
@@ -68,6 +68,10 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
   L->nfeat_max = 0;
   L->nb_max = 0;
   L->w_max = 0;
+  L->rep_ptr.assign(na + 1, 0);
+  L->rep_pid.clear();
+  L->rep_cls.clear();
+  L->rep_nb.clear();
   // representative site of sublattice a: atom a in cell 0 (0-based site id a)
   std::vector<std::vector<int>> rep_js(na);      // pid-sorted adjacency of the rep
   std::vector<std::vector<int>> rep_nbidx(na);   // per rep adjacency entry: nb index per factor
@@ -83,6 +87,7 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
     L->nb_ptr[a] = (int32_t)(L->nb.size() / 4);
     L->feat_ptr[a] = (int32_t)(L->feat.size() / 4);
     L->prog_ptr[a] = (int32_t)L->prog_off.size();
+    L->rep_ptr[a] = (int32_t)L->rep_pid.size();
     if (!d->site_active[a]) continue;
     std::map<std::tuple<int, int, int, int>, int> nbmap;
     std::vector<std::array<int, 4>> nbl;
@@ -143,6 +148,17 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
       }
     }
     L->prog_off.push_back((int32_t)(L->walk.size() - L->walk_ptr[a]));
+    pos = 0;
+    for (int j : rep_js[a]) {
+      int nbi[3] = {-1, -1, -1};
+      int nf = 0;
+      while (rep_nbidx[a][pos] >= 0) nbi[nf++] = rep_nbidx[a][pos++];
+      ++pos;
+      const int col = d->site_col[j - 1];
+      L->rep_pid.push_back(d->site_prog[j - 1]);
+      L->rep_cls.push_back((int8_t)(col > 0 ? 1 : col < 0 ? 2 : 0));
+      L->rep_nb.insert(L->rep_nb.end(), {nf, nbi[0], nbi[1], nbi[2]});
+    }
     for (auto& v : nbl) L->nb.insert(L->nb.end(), v.begin(), v.end());
     L->nb_max = std::max(L->nb_max, NB);
     // descriptor cluster specs, in walk order
@@ -171,7 +187,9 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
   L->nb_ptr[na] = (int32_t)(L->nb.size() / 4);
   L->prog_ptr[na] = (int32_t)L->prog_off.size();
   L->feat_ptr[na] = (int32_t)(L->feat.size() / 4);
+  L->rep_ptr[na] = (int32_t)L->rep_pid.size();
   for (int a = 0; a < na; ++a) L->w_max = std::max(L->w_max, L->walk_ptr[a + 1] - L->walk_ptr[a]);
+  if (!verify) return SCE_OK;
 
   // verification: every active site is its sublattice stencil, translated
   for (int s0 = 0; s0 < N; ++s0) {
@@ -210,5 +228,59 @@ int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* L) {
       ++pos;
     }
   }
+  return SCE_OK;
+}
+
+int32_t sce_position_tables_host(const sce_tables_desc* d, const HostLayout& L, std::vector<int32_t>& nq,
+                                 std::vector<unsigned long long>& low, std::vector<uint16_t>& perm, int* adj_max) {
+  const int na = L.na, nbm = L.nb_max, nact = d->n_active;
+  std::vector<int32_t> cs(nact);
+  for (int q = 0; q < nact; ++q) cs[q] = d->color_sites[q] - 1;
+  // the neighbour sites of every colour position
+  nq.assign((size_t)nact * nbm, 0);
+  for (int q = 0; q < nact; ++q) {
+    const int s = cs[q], a = s % na, cell = s / na;
+    const int c1 = cell % d->dims[0], c2 = (cell / d->dims[0]) % d->dims[1], c3 = cell / (d->dims[0] * d->dims[1]);
+    for (int j = 0; j < nbm; ++j) {
+      int n = s;
+      if (j < L.nb_ptr[a + 1] - L.nb_ptr[a]) {
+        const int32_t* o = &L.nb[4 * (size_t)(L.nb_ptr[a] + j)];
+        n = o[0] + na * ((c1 + o[1]) % d->dims[0] +
+                         d->dims[0] * ((c2 + o[2]) % d->dims[1] + d->dims[1] * ((c3 + o[3]) % d->dims[2])));
+      }
+      nq[(size_t)q * nbm + j] = n;
+    }
+  }
+  // which neighbour slots of a position hold lower colours
+  low.clear();
+  if (nbm <= 64) {
+    std::vector<int> col_of((size_t)d->n_sites, -1);
+    for (int c = 0; c < d->n_colors; ++c)
+      for (int q = d->color_ptr[c] - 1; q < d->color_ptr[c + 1] - 1; ++q) col_of[cs[q]] = c;
+    low.assign((size_t)std::max(nact, 1), 0ull);
+    for (int q = 0; q < nact; ++q) {
+      const int cq = col_of[cs[q]];
+      for (int j = 0; j < nbm; ++j) {
+        const int cn = col_of[nq[(size_t)q * nbm + j]];
+        if (cn >= 0 && cn < cq) low[q] |= 1ull << j;
+      }
+    }
+  }
+  // raw adjacency index -> pid-sorted index, per colour position
+  int amax = 0;
+  for (int a = 0; a < na; ++a) amax = std::max(amax, L.prog_ptr[a + 1] - L.prog_ptr[a] - 1);
+  if (amax > 65535) return sce_fail(SCE_EINVAL, "%d adjacency entries per site exceed the gradient's index", amax);
+  const int am = std::max(amax, 1);
+  perm.assign((size_t)nact * am, 0);
+  std::vector<int> js;
+  for (int q = 0; q < nact; ++q) {
+    const int s0 = cs[q], j0 = d->site_ptr[s0], n = d->site_ptr[s0 + 1] - j0;
+    js.resize(n);
+    for (int k = 0; k < n; ++k) js[k] = k;
+    std::stable_sort(js.begin(), js.end(),
+                     [&](int x, int y) { return d->site_prog[j0 + x - 1] < d->site_prog[j0 + y - 1]; });
+    for (int k = 0; k < n; ++k) perm[(size_t)q * am + js[k]] = (uint16_t)k;
+  }
+  *adj_max = am;
   return SCE_OK;
 }

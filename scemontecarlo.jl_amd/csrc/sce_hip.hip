@@ -1667,6 +1667,137 @@ static size_t partial_capacity(long long n) {
 
 // The reference's adjacency programs for the energy gradient, uploaded as the
 // descriptor holds them (1-based).
+// ---------------------------------------------------------------------------
+// Device-side table build: the per-position tables of the walk (neighbour
+// sites, lower-colour masks, raw -> pid-sorted adjacency permutation) made on
+// the device from the uploaded reference tables, each site verified against
+// its sublattice's representative adjacency translated to its cell (the
+// host's sce_derive_layout check, one wavefront per position).
+// ---------------------------------------------------------------------------
+struct LayoutBuild {
+  int n_sites, n_active, na, d0, d1, d2, nb_max, adj_max, n_colors;
+  const int32_t* color_sites;  // 0-based
+  const int32_t* color_ptr;    // 0-based offsets
+  const uint8_t* active;
+  const int32_t* nb_ptr;
+  const int4* nb;
+  // the reference tables, 1-based as uploaded
+  const int32_t *site_ptr, *site_prog, *site_col, *site_col2, *site_inst, *inst_ptr, *inst_sites;
+  const int8_t* site_slot;
+  // the representative adjacency per sublattice
+  const int32_t* rep_ptr;
+  const int32_t* rep_pid;
+  const int8_t* rep_cls;
+  const int4* rep_nb;
+  int32_t* nbq;
+  unsigned long long* low;
+  uint16_t* perm;
+  int* col_of;
+  unsigned long long* err;  // min over failures of (site + 1) * 8 + reason
+};
+
+__device__ __forceinline__ int lay_translate(const LayoutBuild& B, int a, int k, int c1, int c2, int c3) {
+  const int4 o = B.nb[B.nb_ptr[a] + k];
+  return o.x + B.na * ((c1 + o.y) % B.d0 + B.d0 * ((c2 + o.z) % B.d1 + B.d1 * ((c3 + o.w) % B.d2)));
+}
+__device__ __forceinline__ void lay_fail(const LayoutBuild& B, int s, int reason) {
+  atomicMin(B.err, (unsigned long long)(s + 1) * 8ull + (unsigned long long)reason);
+}
+
+__global__ void k_lay_sites(LayoutBuild B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.n_sites) return;
+  if ((B.active[s] != 0) != (B.active[s % B.na] != 0)) lay_fail(B, s, 1);  // activity differs
+}
+
+__global__ void k_lay_colour(LayoutBuild B) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B.n_active) return;
+  int c = 0;
+  while (c + 1 < B.n_colors && B.color_ptr[c + 1] <= q) ++c;
+  B.col_of[B.color_sites[q]] = c;
+}
+
+__global__ void k_lay_nbq(LayoutBuild B) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B.n_active) return;
+  const int s = B.color_sites[q], a = s % B.na, cell = s / B.na;
+  const int c1 = cell % B.d0, c2 = (cell / B.d0) % B.d1, c3 = cell / (B.d0 * B.d1);
+  const int cnt = B.nb_ptr[a + 1] - B.nb_ptr[a];
+  for (int j = 0; j < B.nb_max; ++j) B.nbq[(size_t)q * B.nb_max + j] = j < cnt ? lay_translate(B, a, j, c1, c2, c3) : s;
+}
+
+__global__ void k_lay_low(LayoutBuild B) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B.n_active) return;
+  const int cq = B.col_of[B.color_sites[q]];
+  unsigned long long m = 0ull;
+  for (int j = 0; j < B.nb_max; ++j) {
+    const int cn = B.col_of[B.nbq[(size_t)q * B.nb_max + j]];
+    if (cn >= 0 && cn < cq) m |= 1ull << j;
+  }
+  B.low[q] = m;
+}
+
+// one wavefront per colour position
+__global__ __launch_bounds__(64) void k_lay_verify(LayoutBuild B) {
+  const int q = blockIdx.x;
+  if (q >= B.n_active) return;
+  const int lane = threadIdx.x;
+  const int s = B.color_sites[q], a = s % B.na, cell = s / B.na;
+  const int c1 = cell % B.d0, c2 = (cell / B.d0) % B.d1, c3 = cell / (B.d0 * B.d1);
+  const int j0 = B.site_ptr[s], n = B.site_ptr[s + 1] - j0;
+  const int r0 = B.rep_ptr[a];
+  if (n != B.rep_ptr[a + 1] - r0) {
+    if (lane == 0) lay_fail(B, s, 2);  // adjacency length
+    return;
+  }
+  for (int k = lane; k < n; k += 64) {
+    const int j = j0 + k;  // 1-based adjacency entry
+    const int pid = B.site_prog[j - 1];
+    // stable rank by program id
+    int rank = 0;
+    for (int m = 0; m < n; ++m) {
+      const int pm = B.site_prog[j0 + m - 1];
+      rank += (pm < pid || (pm == pid && m < k)) ? 1 : 0;
+    }
+    B.perm[(size_t)q * B.adj_max + k] = (uint16_t)rank;
+    const int r = r0 + rank;
+    const int col = B.site_col[j - 1];
+    const int cls = col > 0 ? 1 : col < 0 ? 2 : 0;
+    if (B.rep_pid[r] != pid || B.rep_cls[r] != cls) {
+      lay_fail(B, s, 3);  // program sequence
+      continue;
+    }
+    int fs[3], nf = 0;
+    if (col > 0) {
+      fs[nf++] = col;
+    } else if (col < 0) {
+      fs[nf++] = -col;
+      fs[nf++] = B.site_col2[j - 1];
+    } else {
+      const int inst = B.site_inst[j - 1];
+      const int off = B.inst_ptr[inst - 1] - 1;
+      const int body = B.inst_ptr[inst] - B.inst_ptr[inst - 1];
+      const int own = B.site_slot[j - 1];
+      if (body - 1 > 3) {
+        lay_fail(B, s, 4);
+        continue;
+      }
+      for (int x = 1; x <= body; ++x)
+        if (x != own) fs[nf++] = B.inst_sites[off + x - 1];
+    }
+    const int4 rn = B.rep_nb[r];
+    const int want[3] = {rn.y, rn.z, rn.w};
+    if (nf != rn.x) {
+      lay_fail(B, s, 4);  // neighbour / factor count
+      continue;
+    }
+    for (int f = 0; f < nf; ++f)
+      if (lay_translate(B, a, want[f], c1, c2, c3) + 1 != fs[f]) lay_fail(B, s, 4);
+  }
+}
+
 static int32_t upload_grad_tables(sce_tables* t, const sce_tables_desc* d) {
   GradTables& g = t->grad;
   g.n_sites = d->n_sites;
@@ -1695,6 +1826,129 @@ static int32_t upload_grad_tables(sce_tables* t, const sce_tables_desc* d) {
   return SCE_OK;
 }
 
+// The per-position tables of the walk, made on the device (k_lay_*), with every
+// site verified against its sublattice's representative adjacency; the
+// temporaries are freed again.
+static int32_t build_position_tables(sce_tables* t, const sce_tables_desc* d, const HostLayout& L) {
+  DevTables& D = t->dev;
+  const GradTables& G = t->grad;
+  LayoutBuild B{};
+  B.n_sites = d->n_sites;
+  B.n_active = d->n_active;
+  B.na = L.na;
+  B.d0 = d->dims[0];
+  B.d1 = d->dims[1];
+  B.d2 = d->dims[2];
+  B.nb_max = L.nb_max;
+  B.adj_max = D.adj_max;
+  B.n_colors = d->n_colors;
+  B.color_sites = D.color_sites;
+  B.color_ptr = t->d_color_ptr;
+  B.active = D.active;
+  B.nb_ptr = D.nb_ptr;
+  B.nb = D.nb;
+  B.site_ptr = G.site_ptr;
+  B.site_prog = G.site_prog;
+  B.site_col = G.site_col;
+  B.site_col2 = G.site_col2;
+  B.site_inst = G.site_inst;
+  B.inst_ptr = G.inst_ptr;
+  B.inst_sites = G.inst_sites;
+  const size_t nact = (size_t)std::max(d->n_active, 1);
+  std::vector<void*> tmp;
+  auto release = [&] {
+    for (void* p : tmp) t->mem.release(p);
+  };
+  auto up_tmp = [&](auto** p, const auto* src, size_t n) -> int32_t {
+    int32_t r = dupload1(t->mem, p, src, n);
+    if (r == SCE_OK) tmp.push_back((void*)*p);
+    return r;
+  };
+  int32_t r = SCE_OK;
+  int8_t* slot;
+  int32_t *rp, *rpid;
+  int8_t* rcls;
+  int* col_of;
+  unsigned long long* err;
+  if ((r = up_tmp(&slot, d->site_slot, (size_t)d->n_inst_sites)) != SCE_OK ||
+      (r = up_tmp(&rp, L.rep_ptr.data(), L.rep_ptr.size())) != SCE_OK ||
+      (r = up_tmp(&rpid, L.rep_pid.data(), L.rep_pid.size())) != SCE_OK ||
+      (r = up_tmp(&rcls, L.rep_cls.data(), L.rep_cls.size())) != SCE_OK) {
+    release();
+    return r;
+  }
+  int4* rnb;
+  if ((r = up_tmp(&rnb, reinterpret_cast<const int4*>(L.rep_nb.data()), L.rep_nb.size() / 4)) != SCE_OK ||
+      (r = dalloc(t->mem, &col_of, (size_t)std::max(d->n_sites, 1))) != SCE_OK) {
+    release();
+    return r;
+  }
+  tmp.push_back(col_of);
+  if ((r = dalloc(t->mem, &err, 1)) != SCE_OK) {
+    release();
+    return r;
+  }
+  tmp.push_back(err);
+  B.site_slot = slot;
+  B.rep_ptr = rp;
+  B.rep_pid = rpid;
+  B.rep_cls = rcls;
+  B.rep_nb = rnb;
+  B.col_of = col_of;
+  B.err = err;
+  // the tables that stay: neighbour sites, lower-colour masks, permutation
+  int32_t* nbq;
+  uint16_t* perm;
+  if ((r = dalloc(t->mem, &nbq, nact * std::max(L.nb_max, 1))) != SCE_OK ||
+      (r = dalloc(t->mem, &perm, nact * D.adj_max)) != SCE_OK) {
+    release();
+    return r;
+  }
+  B.nbq = nbq;
+  B.perm = perm;
+  if (L.nb_max <= 64) {
+    unsigned long long* low;
+    if ((r = dalloc(t->mem, &low, nact)) != SCE_OK) {
+      release();
+      return r;
+    }
+    B.low = low;
+    D.nbq_low = low;
+  }
+  D.nbq = nbq;
+  D.adj_perm = perm;
+  const unsigned long long none = ~0ull;
+  hipError_t e = hipMemcpy(err, &none, sizeof none, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(col_of, 0xff, (size_t)std::max(d->n_sites, 1) * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(perm, 0, nact * D.adj_max * sizeof(uint16_t));
+  if (e != hipSuccess) {
+    release();
+    return sce_fail(SCE_EHIP, "layout build: %s", hipGetErrorString(e));
+  }
+  const int N = d->n_sites, n = d->n_active;
+  if (N > 0) k_lay_sites<<<(N + 255) / 256, 256>>>(B);
+  if (n > 0) {
+    k_lay_colour<<<(n + 255) / 256, 256>>>(B);
+    k_lay_nbq<<<(n + 255) / 256, 256>>>(B);
+    if (B.low) k_lay_low<<<(n + 255) / 256, 256>>>(B);
+    k_lay_verify<<<n, 64>>>(B);
+  }
+  unsigned long long code = none;
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(&code, err, sizeof code, hipMemcpyDeviceToHost);
+  release();
+  if (e != hipSuccess) return sce_fail(SCE_EHIP, "layout build: %s", hipGetErrorString(e));
+  if (code != none) {
+    static const char* why[] = {"", "activity differs from its sublattice", "adjacency length differs from its sublattice",
+                                "program sequence differs from its sublattice",
+                                "a neighbour is not the translated stencil"};
+    const int site = (int)(code / 8), reason = (int)(code % 8);
+    return sce_fail(SCE_ELAYOUT, "site %d: %s", site, why[reason < 5 ? reason : 0]);
+  }
+  return SCE_OK;
+}
+
 extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out) {
   if (!d || !out) return sce_fail(SCE_EINVAL, "sce_tables_create: NULL argument");
   *out = nullptr;
@@ -1705,7 +1959,7 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return sce_fail(SCE_ENODEV, "no HIP device available");
   HostLayout L;
-  CHK(sce_derive_layout(d, &L));
+  CHK(sce_derive_layout(d, &L, false));  // every site is verified by the device build below
   auto* t = new sce_tables();
   auto fail = [&](int32_t r) {
     delete t;
@@ -1744,65 +1998,21 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   if ((r = dupload(t->mem, &pn, reinterpret_cast<const int4*>(L.nb.data()), L.nb.size() / 4)) != SCE_OK)
     return fail(r);
   D.nb = pn;
+  t->n_colors = d->n_colors;
+  t->color_ptr.resize((size_t)d->n_colors + 1);
+  for (int c = 0; c <= d->n_colors; ++c) t->color_ptr[c] = d->color_ptr[c] - 1;
+  if ((r = dupload(t->mem, &t->d_color_ptr, t->color_ptr.data(), t->color_ptr.size())) != SCE_OK)
+    return fail(r);
+  if ((r = upload_grad_tables(t, d)) != SCE_OK) return fail(r);
   {
-    // the neighbour sites of every colour position, resolved once here so that
-    // a walk's row loads depend on one load whose address is its position
-    std::vector<int32_t> nq((size_t)d->n_active * L.nb_max);
-    for (int q = 0; q < d->n_active; ++q) {
-      const int s = cs[q], a = s % L.na, cell = s / L.na;
-      const int c1 = cell % d->dims[0], c2 = (cell / d->dims[0]) % d->dims[1],
-                c3 = cell / (d->dims[0] * d->dims[1]);
-      for (int j = 0; j < L.nb_max; ++j) {
-        int n = s;
-        if (j < L.nb_ptr[a + 1] - L.nb_ptr[a]) {
-          const int32_t* o = &L.nb[4 * (size_t)(L.nb_ptr[a] + j)];
-          n = o[0] + L.na * ((c1 + o[1]) % d->dims[0] +
-                             d->dims[0] * ((c2 + o[2]) % d->dims[1] + d->dims[1] * ((c3 + o[3]) % d->dims[2])));
-        }
-        nq[(size_t)q * L.nb_max + j] = n;
-      }
-    }
-    if ((r = dupload(t->mem, &p32, nq.data(), nq.size())) != SCE_OK) return fail(r);
-    D.nbq = p32;
-    // which neighbour slots of a position hold lower colours (the waits of the
-    // dependency-driven persistent sweep)
-    if (L.nb_max <= 64) {
-      std::vector<int> col_of((size_t)d->n_sites, -1);
-      for (int c = 0; c < d->n_colors; ++c)
-        for (int q = d->color_ptr[c] - 1; q < d->color_ptr[c + 1] - 1; ++q) col_of[cs[q]] = c;
-      std::vector<unsigned long long> low((size_t)std::max(d->n_active, 1), 0ull);
-      for (int q = 0; q < d->n_active; ++q) {
-        const int cq = col_of[cs[q]];
-        for (int j = 0; j < L.nb_max; ++j) {
-          const int cn = col_of[nq[(size_t)q * L.nb_max + j]];
-          if (cn >= 0 && cn < cq) low[q] |= 1ull << j;
-        }
-      }
-      unsigned long long* pl;
-      if ((r = dupload(t->mem, &pl, low.data(), low.size())) != SCE_OK) return fail(r);
-      D.nbq_low = pl;
-    }
-    // the gradient deals a site's adjacency programs in their raw order; the
-    // walk holds them pid-sorted (translation invariant): the permutation
-    // between the two, per colour position
+    // the per-position tables, built and verified on the device
     int amax = 0;
     for (int a = 0; a < L.na; ++a) amax = std::max(amax, L.prog_ptr[a + 1] - L.prog_ptr[a] - 1);
     if (amax > 65535) return fail(sce_fail(SCE_EINVAL, "%d adjacency entries per site exceed the gradient's index", amax));
-    std::vector<uint16_t> perm((size_t)d->n_active * std::max(amax, 1));
-    std::vector<int> js;
-    for (int q = 0; q < d->n_active; ++q) {
-      const int s0 = cs[q], j0 = d->site_ptr[s0], n = d->site_ptr[s0 + 1] - j0;
-      js.resize(n);
-      for (int k = 0; k < n; ++k) js[k] = k;
-      std::stable_sort(js.begin(), js.end(), [&](int x, int y) {
-        return d->site_prog[j0 + x - 1] < d->site_prog[j0 + y - 1];
-      });
-      for (int k = 0; k < n; ++k) perm[(size_t)q * std::max(amax, 1) + js[k]] = (uint16_t)k;
-    }
     D.adj_max = std::max(amax, 1);
-    uint16_t* p16;
-    if ((r = dupload1(t->mem, &p16, perm.data(), perm.size())) != SCE_OK) return fail(r);
-    D.adj_perm = p16;
+    if ((r = build_position_tables(t, d, L)) != SCE_OK) return fail(r);
+  }
+  {
     if ((r = dupload(t->mem, &p32, L.prog_ptr.data(), L.prog_ptr.size())) != SCE_OK) return fail(r);
     D.prog_ptr = p32;
     if ((r = dupload1(t->mem, &p32, L.prog_off.data(), L.prog_off.size())) != SCE_OK) return fail(r);
@@ -1845,23 +2055,17 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   D.efac_row = p32;
   if ((r = dupload(t->mem, &pd, d->term_coef, (size_t)d->n_terms)) != SCE_OK) return fail(r);
   D.term_coef = pd;
-  t->n_colors = d->n_colors;
-  t->color_ptr.resize((size_t)d->n_colors + 1);
-  for (int c = 0; c <= d->n_colors; ++c) t->color_ptr[c] = d->color_ptr[c] - 1;
   for (int c = 0; c < d->n_colors; ++c) {
     const int n = t->color_ptr[c + 1] - t->color_ptr[c];
     t->max_class = std::max(t->max_class, n);
     if (n > 0 && (t->fin_col < 0 || n < t->color_ptr[t->fin_col + 1] - t->color_ptr[t->fin_col])) t->fin_col = c;
   }
-  if ((r = dupload(t->mem, &t->d_color_ptr, t->color_ptr.data(), t->color_ptr.size())) != SCE_OK)
-    return fail(r);
   t->total_walk = (int64_t)L.walk.size();
   t->sub_F.assign(L.na, 0);
   for (int a = 0; a < L.na; ++a)
     if (d->site_active[a])
       t->sub_F[a] = 2 * (L.nb_ptr[a + 1] - L.nb_ptr[a]) + (L.feat_ptr[a + 1] - L.feat_ptr[a]);
   t->w_max = L.w_max;
-  if ((r = upload_grad_tables(t, d)) != SCE_OK) return fail(r);
   *out = t;
   return SCE_OK;
 }
@@ -1880,6 +2084,39 @@ extern "C" int32_t sce_layout_check(const sce_tables_desc* d, int64_t o[5]) {
 
 extern "C" int32_t sce_tables_destroy(sce_tables* t) {
   delete t;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_tables_position_tables(sce_tables* t, int32_t* nbq, uint64_t* low, uint16_t* perm,
+                                              int64_t sizes[3]) {
+  if (!t || !sizes) return sce_fail(SCE_EINVAL, "NULL argument");
+  const DevTables& D = t->dev;
+  sizes[0] = D.nb_max;
+  sizes[1] = D.adj_max;
+  sizes[2] = D.nbq_low ? 1 : 0;
+  const size_t n = (size_t)D.n_active;
+  if (nbq && n && D.nb_max) HIPCHK(hipMemcpy(nbq, D.nbq, n * D.nb_max * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (low && n && D.nbq_low) HIPCHK(hipMemcpy(low, D.nbq_low, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (perm && n) HIPCHK(hipMemcpy(perm, D.adj_perm, n * D.adj_max * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_position_tables_host(const sce_tables_desc* d, int32_t* nbq, uint64_t* low, uint16_t* perm,
+                                            int64_t sizes[3]) {
+  if (!d || !sizes) return sce_fail(SCE_EINVAL, "NULL argument");
+  HostLayout L;
+  CHK(sce_derive_layout(d, &L, true));
+  std::vector<int32_t> nq;
+  std::vector<unsigned long long> lw;
+  std::vector<uint16_t> pm;
+  int amax = 0;
+  CHK(sce_position_tables_host(d, L, nq, lw, pm, &amax));
+  sizes[0] = L.nb_max;
+  sizes[1] = amax;
+  sizes[2] = lw.empty() ? 0 : 1;
+  if (nbq) std::copy(nq.begin(), nq.end(), nbq);
+  if (low && !lw.empty()) std::copy(lw.begin(), lw.begin() + std::max(d->n_active, 0), low);
+  if (perm) std::copy(pm.begin(), pm.end(), perm);
   return SCE_OK;
 }
 

@@ -49,9 +49,24 @@ struct HostLayout {
   std::vector<int32_t> feat_ptr;  // na + 1, into feat (units of specs)
   std::vector<int32_t> feat;      // 4 ints per spec
   int nfeat_max = 0;              // largest 2*NB + specs
+  // the representative site's pid-sorted adjacency, per sublattice (what every
+  // site of the sublattice must repeat, translated): program id, path class
+  // (1 pair, 2 triplet, 0 general) and stencil index of each factor
+  std::vector<int32_t> rep_ptr;   // na + 1, into the rep_* arrays
+  std::vector<int32_t> rep_pid;
+  std::vector<int8_t> rep_cls;
+  std::vector<int32_t> rep_nb;    // 4 ints per entry: factor count, stencil indices (-1 unused)
 };
 
-// Derives the layout from the reference tables and verifies, for every active
-// site, that its adjacency is the sublattice stencil translated to its cell.
-// Returns SCE_ELAYOUT (with a message) for tables that are not such a tiling.
-int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* out);
+// Derives the layout from the reference tables (the representative site of each
+// sublattice) and, when `verify`, checks on the host for every active site that
+// its adjacency is the sublattice stencil translated to its cell.  Returns
+// SCE_ELAYOUT (with a message) for tables that are not such a tiling.
+int32_t sce_derive_layout(const sce_tables_desc* d, HostLayout* out, bool verify = true);
+
+// The per-position tables a device build makes (the host restatement): for
+// every colour position q, the nb_max neighbour sites (0-based, padded with the
+// site itself); when nb_max <= 64 the lower-colour mask of its slots; the raw
+// adjacency index -> pid-sorted index permutation (adj_max per position).
+int32_t sce_position_tables_host(const sce_tables_desc* d, const HostLayout& L, std::vector<int32_t>& nbq,
+                                 std::vector<unsigned long long>& low, std::vector<uint16_t>& perm, int* adj_max);

@@ -86,7 +86,8 @@ _lib = None
 EXPORTS = [
     "sce_version", "sce_last_error", "sce_device_count", "sce_tiled_build", "sce_tiled_desc",
     "sce_tiled_free", "sce_host_zrows", "sce_host_total_energy", "sce_tables_create",
-    "sce_tables_destroy", "sce_tables_layout_info", "sce_layout_check", "sce_chain_create", "sce_chain_destroy",
+    "sce_tables_destroy", "sce_tables_layout_info", "sce_layout_check", "sce_tables_position_tables",
+    "sce_position_tables_host", "sce_chain_create", "sce_chain_destroy",
     "sce_chain_info_get", "sce_chain_set_step", "sce_chain_reset_counters", "sce_download",
     "sce_upload", "sce_swap_payload", "sce_metropolis_sweep", "sce_overrelaxation_sweep",
     "sce_run_sweeps", "sce_renormalize", "sce_chain_total_energy", "sce_chain_set_timing", "sce_chain_set_persistent",
@@ -122,6 +123,7 @@ def lib():
             "sce_host_zrows": [i32, i64, P, P], "sce_host_total_energy": [P, P, P],
             "sce_tables_create": [P, P], "sce_tables_destroy": [P],
             "sce_tables_layout_info": [P, P], "sce_layout_check": [P, P],
+            "sce_tables_position_tables": [P, P, P, P, P], "sce_position_tables_host": [P, P, P, P, P],
             "sce_chain_create": [P, P, P, d, u64, d, P], "sce_chain_destroy": [P],
             "sce_chain_info_get": [P, P], "sce_chain_set_step": [P, d],
             "sce_chain_reset_counters": [P], "sce_download": [P, P, P, P],

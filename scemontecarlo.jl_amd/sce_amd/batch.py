@@ -152,6 +152,7 @@ class PTResult:
     final_configs: List[np.ndarray]
     seed: int
     wall_s: float = 0.0
+    lanes: int = 0     # lanes per site the run swept with (part of the determinism scope)
 
 
 def _adapt(step, acc, att, target):
@@ -235,7 +236,11 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
     _validate_pt(kts, exchange_interval, sweeps_measure, measure_interval)
     plan = PTPlan(kts, int(exchange_interval), int(sweeps_therm), int(sweeps_measure),
                   int(measure_interval), float(step), float(adapt_target), int(adapt_interval),
-                  int(renorm_interval), int(nbins), int(seed), int(lanes))
+                  int(renorm_interval), int(nbins), int(seed), 0 if lanes == "auto" else int(lanes))
+    if lanes == "auto":
+        # tuned on a throwaway copy of the ladder, then pinned in the plan (and
+        # so in every checkpoint and the result) like the reference's workgroupsize
+        plan.lanes = autotune_lanes(lambda: _pt_batch(gH, plan, init), sweeps=20)
     ck = _Checkpointer(checkpoint, checkpoint_interval) if checkpoint is not None else None
     b = _pt_batch(gH, plan, init)
     return _pt_run(b, gH, plan, "therm", 0, None, ck)
@@ -318,7 +323,7 @@ def _pt_run(b: GPUBatch, gH: GPUTiledHamiltonian, plan: PTPlan, phase: str, done
     with np.errstate(invalid="ignore", divide="ignore"):
         swaps = np.where(att > 0, acc / np.maximum(att, 1), np.nan)
     finals = [b.download(r)[0] for r in range(R)]
-    return PTResult(points, swaps, finals, plan.seed, wall)
+    return PTResult(points, swaps, finals, plan.seed, wall, plan.lanes)
 
 
 def resume_pt(path: str, gH: GPUTiledHamiltonian, checkpoint: Optional[str] = "same",
@@ -338,12 +343,14 @@ def resume_pt(path: str, gH: GPUTiledHamiltonian, checkpoint: Optional[str] = "s
 
 
 def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache: Optional[dict] = None,
-                   key=None) -> int:
+                   key=None, timings: Optional[dict] = None) -> int:
     """The project autotuner: times `sweeps` graph-replayed sweeps of a scratch
-    batch for each lanes-per-site candidate and returns the fastest.  `lanes` is
-    part of the determinism scope, so tuning runs on a throwaway batch built by
-    `make_batch()` before the production run.  Results are memoised in `cache`
-    under `key` (e.g. a size-class signature)."""
+    batch for each lanes-per-site candidate (sites per 256-thread block = 256 /
+    lanes) and returns the fastest.  `lanes` is part of the determinism scope, so
+    tuning runs on a throwaway batch built by `make_batch()` before the production
+    run, and the caller records the choice with the run.  Results are memoised in
+    `cache` under `key` (e.g. a size-class signature); `timings` receives the
+    seconds per sweep of every candidate that ran."""
     if cache is not None and key is not None and key in cache:
         return cache[key]
     best, best_t = None, math.inf
@@ -358,6 +365,8 @@ def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache:
             continue
         finally:
             del b
+        if timings is not None:
+            timings[lanes] = dt / sweeps
         if dt < best_t:
             best, best_t = lanes, dt
     if best is None:

@@ -239,6 +239,37 @@ def gpu_run_sweeps(gst: GPUChainState, gH: GPUTiledHamiltonian, st: Optional[Cha
     return obs[: n.value]
 
 
+def autotune_chain(gH: GPUTiledHamiltonian, st: ChainState, beta: float, sweeps: int = 200,
+                   candidates=((16, False), (32, False), (64, False), (64, True))):
+    """The project autotuner for single chains: times `sweeps` sweeps (measured
+    every sweep, as production runs are) of a throwaway copy of the chain for
+    every (lanes per site, persistent) candidate — per-colour graph launches at
+    16/32/64 lanes, or the dependency-driven persistent sweep at 64 — and returns
+    ``(lanes, persistent, timings)`` of the fastest.  Lanes are part of the
+    determinism scope: a caller pins the choice for its run and records it."""
+    import time
+    timings = {}
+    best, best_t = None, float("inf")
+    for lanes, pers in candidates:
+        g = GPUChainState(gH, st, seed=1)
+        g.set_persistent(pers)
+        try:
+            gpu_run_sweeps(g, gH, None, beta, 10, renorm_interval=0, lanes=lanes, measure_interval=1)
+            t0 = time.perf_counter()
+            gpu_run_sweeps(g, gH, None, beta, sweeps, renorm_interval=0, lanes=lanes, measure_interval=1)
+            dt = (time.perf_counter() - t0) / sweeps
+        except L.SceError:
+            continue
+        finally:
+            del g
+        timings[f"{lanes}{'p' if pers else ''}"] = dt
+        if dt < best_t:
+            best, best_t = (lanes, pers), dt
+    if best is None:
+        raise RuntimeError("no candidate could run this chain")
+    return best[0], best[1], timings
+
+
 GRADIENT_LANES = 128  # the reference's default workgroup size for the gradient
 
 

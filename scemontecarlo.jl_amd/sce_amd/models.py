@@ -426,12 +426,17 @@ def _neighbours(setup: Setup, a0: int, r: float, magnetic):
 
 def setup_pairs(setup: Setup, cutoff: float, magnetic):
     """Translation-unique pair clusters between magnetic atoms: every pair within
-    ``cutoff`` (Å), or, for ``cutoff = inf``, one pair per two distinct atoms of the
-    cell at its minimum image (ties resolved to the smallest shift tuple)."""
+    ``cutoff`` (Å) — an atom with its own periodic images included — or, for
+    ``cutoff = inf``, one pair per two distinct atoms of the cell at its minimum
+    image (ties resolved to the smallest shift tuple; an atom is not paired with
+    its own images, which have no single minimum).  The minimum image is searched
+    over every shift within the cell's diameter, so skewed, unreduced cells
+    resolve it as reduced ones do."""
     seen, out = set(), []
     if math.isinf(cutoff):
         P = setup.cartesian()
-        shifts = sorted(itertools.product((-1, 0, 1), repeat=3))
+        diameter = float(sum(np.linalg.norm(v) for v in setup.lattice))
+        shifts = sorted(_image_range(setup, diameter))
         for a0 in range(1, setup.n_atoms + 1):
             for a in range(a0 + 1, setup.n_atoms + 1):
                 if not (magnetic[a0 - 1] and magnetic[a - 1]):

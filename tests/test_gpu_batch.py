@@ -265,3 +265,16 @@ def test_swap_stats_writes_exactly_n_chains_minus_one():
     assert (acc[: R - 1] >= 0).all() and (acc[: R - 1] <= att[: R - 1]).all()
     a2, c2 = b.swap_stats()
     assert np.array_equal(a2, att[: R - 1]) and np.array_equal(c2, acc[: R - 1])
+
+
+def test_pt_autotuned_lanes_are_recorded():
+    """gpu_run_pt(lanes="auto"): the tuner's choice is pinned in the plan, returned
+    with the result, and the run equals the same run with those lanes given."""
+    H = S.TiledHamiltonian(*bcc_tet_small(4, dims=(2, 2, 2)))
+    gH = S.GPUTiledHamiltonian(H)
+    kw = dict(kT=[0.01, 0.02, 0.04], exchange_interval=1, sweeps_therm=20, sweeps_measure=30, seed=17)
+    r = S.gpu_run_pt(gH, lanes="auto", **kw)
+    assert r.lanes in (16, 32, 64)
+    r2 = S.gpu_run_pt(gH, lanes=r.lanes, **kw)
+    for a, b in zip(r.final_configs, r2.final_configs):
+        assert np.array_equal(a, b)

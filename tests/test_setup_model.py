@@ -108,6 +108,27 @@ def test_minimum_image_pairs(setup):
         assert abs(_dist(setup, k)[0] - best) < 1e-9
 
 
+def test_minimum_image_pairs_on_a_skewed_cell():
+    """An unreduced, strongly sheared cell (b = (3, 1, 0) in units of a): its
+    minimum images lie beyond the {-1, 0, 1}^3 shifts; every pair's distance
+    equals brute force over +-6 cells, and the chosen shift attains it."""
+    lat = np.array([[1.0, 0.0, 0.0], [3.0, 1.0, 0.0], [0.7, 0.4, 1.1]])
+    pos = np.array([[0.0, 0.0, 0.0], [0.43, 0.61, 0.27], [0.9, 0.15, 0.8]])
+    st = M.Setup(lattice=lat, positions=pos, species=[1, 1, 1], species_labels=["X"], lmax=[1])
+    pairs = M.setup_pairs(st, math.inf, [True] * 3)
+    assert len(pairs) == 3
+    P = st.cartesian()
+    far = False
+    for (a, ca), (b, cb) in pairs:
+        sh = np.asarray(cb, float) - np.asarray(ca, float)
+        got = float(np.linalg.norm(P[b - 1] + sh @ lat - P[a - 1]))
+        best = min(float(np.linalg.norm(P[b - 1] - P[a - 1] + np.asarray(c, float) @ lat))
+                   for c in itertools.product(range(-6, 7), repeat=3))
+        assert abs(got - best) < 1e-9, ((a, ca), (b, cb))
+        far |= max(abs(x) for x in sh) > 1
+    assert far   # at least one minimum image lies outside {-1, 0, 1}^3
+
+
 def test_triplets_against_brute_force(setup):
     cutoff = 3.0
     pts, lab, mag = _supercell(setup, 1)
