@@ -200,7 +200,64 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, plain):
     }
 
 
-def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
+def da_kt_scan(args, S, H, gH, sur, snapshot, step, kts=(0.02, 0.025, 0.03, 0.035, 0.04), equil=2000,
+               sweeps=8000):
+    """Delayed acceptance against the plain sweep across kT at the frozen step
+    adapted at the bench's kT (the surrogate as trained there): at each kT, `equil`
+    plain sweeps from the equilibrated snapshot, then `sweeps` DA and `sweeps`
+    plain sweeps from that same state (graph-replayed, E and m every sweep).
+    Reports the stage-1 pass rate (the share of proposals walked exactly),
+    tau_int(E) (log-binning plateau and Sokal's window), sweep times, ESS/s and
+    the largest observable disagreement in sigma."""
+    from sce_amd import surrogate as SG
+    out = []
+    for k, kT in enumerate(kts):
+        beta = 1.0 / kT
+        g0 = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=3100 + k)
+        S.gpu_run_sweeps(g0, gH, None, beta, equil, renorm_interval=1000, lanes=args.lanes)
+        start = g0.download()[0]
+        row = {"kT_eV": kT}
+        obs = {}
+        for name in ("plain", "da"):
+            g = S.GPUChainState(gH, S.ChainState(H, start, step), seed=3200 + k)
+            if name == "plain":
+                S.gpu_run_sweeps(g, gH, None, beta, 8, renorm_interval=1000, lanes=args.lanes, measure_interval=1)
+                g.reset_counters()
+                t0 = time.perf_counter()
+                o = S.gpu_run_sweeps(g, gH, None, beta, sweeps, renorm_interval=1000, lanes=args.lanes,
+                                     measure_interval=1)
+            else:
+                SG.gpu_da_run_sweeps(g, sur, None, beta, 8, renorm_interval=1000, measure_interval=1)
+                g.reset_counters()
+                t0 = time.perf_counter()
+                o = SG.gpu_da_run_sweeps(g, sur, None, beta, sweeps, renorm_interval=1000, measure_interval=1)
+            wall = time.perf_counter() - t0
+            inf = g.info()
+            ess = S.effective_samples(o[:, 0])
+            tw = S.tau_int_windowed(o[:, 0])
+            obs[name] = o
+            row[name] = {"ms_per_sweep": 1e3 * wall / sweeps, "acceptance": inf.acc_metro / max(inf.att_metro, 1),
+                         "tau_int_E": (sweeps / ess - 1) / 2 if math.isfinite(ess) and ess > 0 else None,
+                         "tau_int_E_windowed": tw,
+                         "ess_per_s_E_windowed": sweeps / (1 + 2 * tw) / wall if math.isfinite(tw) else None}
+            if name == "da":
+                row[name]["stage1_pass"] = inf.acc_stage1 / max(inf.att_metro, 1)
+                row[name]["stage2_acceptance"] = inf.acc_metro / max(inf.acc_stage1, 1)
+        sd = S.standard_stats(obs["da"], kT, H.n_active)
+        sp = S.standard_stats(obs["plain"], kT, H.n_active)
+        row["max_observable_diff_sigma"] = max(
+            abs(sd[k2].mean - sp[k2].mean) / math.hypot(sd[k2].err, sp[k2].err)
+            for k2 in ("energy", "specific_heat", "absm", "susceptibility", "binder")
+            if math.hypot(sd[k2].err, sp[k2].err) > 0)
+        a, b = row["da"]["ess_per_s_E_windowed"], row["plain"]["ess_per_s_E_windowed"]
+        row["ess_per_s_ratio_windowed"] = a / b if a and b else None
+        row["exact_walk_bound"] = 1.0 / max(row["da"]["stage1_pass"], 1e-9)
+        out.append(row)
+    return {"kts_eV": list(kts), "step": step, "equilibration_sweeps": equil, "sweeps": sweeps, "by_kT": out,
+            "note": "exact_walk_bound = 1 / stage-1 pass rate: the most a free surrogate could gain"}
+
+
+def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
     """Batched parallel tempering: 64 replicas of fcc 12^3 (6912 spins, body-2 l<=2 to
     the 4th shell) on a geometric kT ladder spanning 0.2-2.0 J1, replica exchange
     on the device after every sweep, one launch per colour for all replicas, lanes
@@ -223,7 +280,9 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
     def make():
         states = [S.ChainState(H, S.random_config(H.n_sites, np.random.default_rng(init[r])), 0.6)
                   for r in range(R)]
-        return GPUBatch([gH] * R, states, dev, kts, exch)
+        b = GPUBatch([gH] * R, states, dev, kts, exch)
+        b.set_graph(graph)
+        return b
 
     lanes = autotune_lanes(make, sweeps=20)   # lanes per site: a scratch batch, before the run
     b = make()
@@ -255,7 +314,7 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500):
             "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep"}
 
 
-def uneven_leg(S, M, sweeps=100, n_chains=256):
+def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True):
     """Uneven batch: 256 chains, sc or bcc with N^3, N uniform in 6..20, each with
     its own seeded body-2 l<=2 (+ body-3 l=1 triangles on bcc) model, kT and a
     Zeeman term (2 mu_B moments, field 0-10 T along a random axis).  Chains run in
@@ -282,8 +341,10 @@ def uneven_leg(S, M, sweeps=100, n_chains=256):
         gHs = [S.GPUTiledHamiltonian(H) for H, _, _, _ in chains]
 
         def make():
-            return GPUBatch(gHs, [st for _, st, _, _ in chains], [sd for _, _, sd, _ in chains],
-                            [kt for _, _, _, kt in chains])
+            b = GPUBatch(gHs, [st for _, st, _, _ in chains], [sd for _, _, sd, _ in chains],
+                         [kt for _, _, _, kt in chains])
+            b.set_graph(graph)
+            return b
         lanes = autotune_lanes(make, sweeps=10)
         b = make()
         b.run(5, lanes, 0)
@@ -418,7 +479,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=500,
-                    help="untimed sweeps at the frozen step, after the equilibration")
+                    help="untimed sweeps at the frozen step, run right before the timed region")
     ap.add_argument("--equil", type=int, default=4000,
                     help="fixed equilibration with run_mc's step adaptation (every 50 sweeps toward "
                          "acceptance 0.5), run whatever --warmup is, before every measured leg")
@@ -435,6 +496,10 @@ def main():
                     help="skip the batched legs (64-replica PT, 256-chain uneven batch)")
     ap.add_argument("--no-compound", action="store_true",
                     help="skip the bcc 16^3 Metropolis + overrelaxation leg (4 kT x 10k sweeps)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched legs (64-replica PT, 256-chain uneven batch) and print their JSON")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the single-chain autotuner report (its persistent candidate is a cooperative launch)")
     ap.add_argument("--compound-only", action="store_true",
                     help="run only the bcc 16^3 Metropolis + overrelaxation leg and print its JSON")
     ap.add_argument("--no-setup-model", action="store_true",
@@ -444,6 +509,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every sweep's kernels directly instead of replaying hipGraphs")
     ap.add_argument("--da-rows", type=float, default=1e7, help="harvested training tuples")
+    ap.add_argument("--no-da-scan", action="store_true",
+                    help="skip the delayed-acceptance kT scan (5 kT x (2k + 2 x 8k) sweeps)")
     ap.add_argument("--ess-sweeps", type=int, default=20000,
                     help="sweeps of each chain in the delayed-acceptance vs plain comparison")
     ap.add_argument("--da-epochs", type=int, default=2)
@@ -471,6 +538,11 @@ def main():
     import sce_amd as S
     from sce_amd import models as M
 
+    if args.batch_only:
+        print(json.dumps({"batched": {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph),
+                                      "uneven_batch": uneven_leg(S, M, graph=not args.no_graph)}}),
+              flush=True)
+        return
     if args.compound_only:
         print(json.dumps({"compound": compound_leg(S, M)}), flush=True)
         return
@@ -501,8 +573,6 @@ def main():
             inf = gst.info()
             gst.step = _adapt(gst.step, inf.acc_metro, inf.att_metro, 0.5)
     S.gpu_renormalize(gst)
-    if args.warmup > 0:
-        S.gpu_run_sweeps(gst, gH, None, beta, args.warmup, renorm_interval=1000, lanes=args.lanes)
     sur = train_info = None
     if not args.no_da:   # harvesting is plain sweeps: it continues the equilibration
         sur, train_info = train_surrogate(args, S, H, gH, gst, beta)
@@ -512,10 +582,17 @@ def main():
     snapshot = gst.download()[0]
     # the autotuner's view of this chain (reported; the timed region keeps the
     # pinned --lanes and launch mode, which are part of the determinism scope)
-    tl, tp, tt = S.autotune_chain(gH, S.ChainState(H, snapshot, gst.step), beta, sweeps=200)
+    tl = tp = tt = None
+    if not args.no_autotune:
+        tl, tp, tt = S.autotune_chain(gH, S.ChainState(H, snapshot, gst.step), beta, sweeps=200)
     # kernel timing: the colour walks of 20 passes launched back to back between
     # one pair of HIP events on the chain's stream (state restored afterwards)
     walk_ms_avg, walk_n = gst.time_walk(args.lanes, 20)
+    # --warmup untimed sweeps at the frozen step, as the same kind of call as the
+    # timed region and right before it
+    if args.warmup > 0:
+        S.gpu_run_sweeps(gst, gH, None, beta, args.warmup, renorm_interval=1000, lanes=args.lanes,
+                         measure_interval=1)
     gst.reset_counters()
     # the timed region: `steps` sweeps with E and m recorded on the device every sweep
     t0 = time.perf_counter()
@@ -585,8 +662,11 @@ def main():
     if not args.no_da:
         line["delayed_acceptance"] = da_leg(args, S, H, gH, sur, train_info, snapshot, gst.step, beta,
                                             plain=ess_leg_out)
+        if not args.no_da_scan:
+            line["delayed_acceptance"]["kT_scan"] = da_kt_scan(args, S, H, gH, sur, snapshot, gst.step)
     if not args.no_batch:
-        line["batched"] = {"parallel_tempering": pt_leg(S, M), "uneven_batch": uneven_leg(S, M)}
+        line["batched"] = {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph),
+                           "uneven_batch": uneven_leg(S, M, graph=not args.no_graph)}
     if not args.no_compound:
         line["compound"] = compound_leg(S, M)
     if not args.no_setup_model:

@@ -3,9 +3,10 @@
 Bit-exact against the oracle: descriptor rows, the FP32 inference path, whole
 two-stage sweeps (config, rows, energy, both acceptance counters).  Tolerance:
 the MFMA FP32 training gradient against the FP64 numpy backprop (itself pinned
-by finite differences).  Behaviour: training lowers the loss and a trained
-surrogate screens proposals with far fewer wasted exact walks than an
-untrained one.
+by finite differences).  Behaviour: training lowers the loss, and delayed
+acceptance with the trained surrogate against the plain sweep walks the exact
+dE for a fraction of the proposals at the plain acceptance and samples the
+same distribution.
 """
 import numpy as np
 import pytest
@@ -156,20 +157,27 @@ def test_harvest_is_the_plain_chain():
     assert np.array_equal(cfg_d, c2) and np.array_equal(z_d, z2) and E_d == E
 
 
-def test_trained_surrogate_screens_better_than_untrained():
+def test_trained_delayed_acceptance_against_the_plain_sweep():
+    """What the trained network gains over the method without it: the same
+    chain, from the same state, sampled by the plain sweep and by delayed
+    acceptance with the trained surrogate.  The DA chain walks the exact dE
+    only for the proposals its surrogate passes (a fraction well below one),
+    keeps the plain acceptance (stage 2 rarely overturns stage 1), and samples
+    the same distribution (<E> and <|m|> within 3 sigma).  An untrained network
+    of the same shape, as a control, wastes far more exact walks on moves that
+    stage 2 then rejects."""
     # isotropic tensors: the site energy is a function of the rotation-invariant
     # descriptor, so a trained surrogate can follow the exact dE closely
+    # kT = 0.04 eV: above the random-sign couplings' freezing (tau_int(E) of a few
+    # sweeps), where two chains from one state sample the same equilibrium
     na, terms, dims = bcc_tet_small(2, dims=(6, 6, 6), isotropic=True)
     H, _ = build_pair(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
-    kT = 0.02
+    kT = 0.04
     beta = 1 / kT
     st = S.ChainState(H, rand_config(H.n_sites, 1), 0.5)
     gst = S.GPUChainState(gH, st, seed=5)
-    S.gpu_run_sweeps(gst, gH, None, beta, 300, renorm_interval=100)   # equilibrate
-    gst.reset_counters()
-    S.gpu_run_sweeps(gst, gH, None, beta, 100, renorm_interval=100)
-    plain = gst.info().acc_metro / gst.info().att_metro
+    S.gpu_run_sweeps(gst, gH, None, beta, 2000, renorm_interval=100)   # equilibrate
     cfg0, _, _ = gst.download()
 
     trained = SG.Surrogate(gH, seed=2)
@@ -177,21 +185,34 @@ def test_trained_surrogate_screens_better_than_untrained():
     trained.harvest(gst, beta, 400)
     losses = trained.train_all(epochs=6, batch=1024, lr=1e-3)
     for a, l in losses.items():
-        assert l[-1] < 0.2 * l[0] and l[-1] < 0.01, (a, l)   # loss in units of the rms dE^2
+        assert l[-1] < 0.5 * l[0] and l[-1] < 0.01, (a, l)   # loss in units of the rms dE^2
         untrained.set_params(a, untrained.get_params(a), trained.info(a)["scale"])
 
+    n = 8000
+    p = S.GPUChainState(gH, S.ChainState(H, cfg0, 0.5), seed=77)
+    obs_p = S.gpu_run_sweeps(p, gH, None, beta, n, renorm_interval=1000, measure_interval=1)
+    ip = p.info()
+    plain_acc = ip.acc_metro / ip.att_metro
     res = {}
     for name, sur in (("trained", trained), ("untrained", untrained)):
         g = S.GPUChainState(gH, S.ChainState(H, cfg0, 0.5), seed=77)
-        SG.gpu_da_run_sweeps(g, sur, None, beta, 200, renorm_interval=100)
+        obs = SG.gpu_da_run_sweeps(g, sur, None, beta, n, renorm_interval=1000, measure_interval=1)
         i = g.info()
-        res[name] = (i.acc_metro / i.att_metro, i.acc_stage1 / i.att_metro, i.acc_metro / max(i.acc_stage1, 1))
-    print("plain acceptance", plain, "losses", {a: list(l) for a, l in losses.items()}, "DA", res)
-    acc_t, pass_t, s2_t = res["trained"]
-    acc_u, pass_u, s2_u = res["untrained"]
-    # a trained surrogate keeps the plain acceptance while its stage 2 rarely rejects
-    assert acc_t > 0.9 * plain and s2_t > 0.9
-    assert acc_t > acc_u and s2_t > s2_u
+        res[name] = (i.acc_metro / i.att_metro, i.acc_stage1 / i.att_metro, i.acc_metro / max(i.acc_stage1, 1), obs)
+    acc_t, walks_t, s2_t, obs_t = res["trained"]
+    acc_u, walks_u, s2_u, _ = res["untrained"]
+    print("plain acceptance", plain_acc, "trained (acc, exact-walk fraction, stage 2)", acc_t, walks_t, s2_t,
+          "untrained", acc_u, walks_u, s2_u)
+    # exact walks only for passed proposals, nearly all of which stage 2 keeps
+    assert walks_t < 0.9 and s2_t > 0.9 and acc_t > 0.9 * plain_acc
+    # an untrained surrogate spends its exact walks on moves stage 2 rejects
+    assert s2_u < s2_t and acc_u < acc_t
+    # the same distribution as the plain sweep
+    sd = S.standard_stats(obs_t, kT, H.n_active)
+    sp = S.standard_stats(obs_p, kT, H.n_active)
+    for k in ("energy", "absm"):
+        sig = np.hypot(sd[k].err, sp[k].err)
+        assert abs(sd[k].mean - sp[k].mean) < 3 * sig, (k, sd[k].mean, sp[k].mean, sig)
 
 
 def test_da_direct_launches_equal_graph_replay():
