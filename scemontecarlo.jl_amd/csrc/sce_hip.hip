@@ -505,11 +505,28 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
   }
   const int stride = T.nb_max * NLM + 1 + NLM;
   double* Z = lds + g * stride;
+  // the site's own spin and row and its keyed accept uniform do not depend on
+  // the walk: read and drawn while the neighbour rows stage
+  double e0[3] = {0.0, 0.0, 0.0}, zo0[NLM], u_or = 0.0;
+#pragma unroll
+  for (int k = 0; k < NLM; ++k) zo0[k] = 0.0;
   if (valid) {
-    if constexpr (TEST)
+    if constexpr (TEST) {
       stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
-    else
-      atom = stage_neighbours_q<L, NLM>(T, zrows, q, s, lane, Z, [] {});
+    } else {
+      atom = stage_neighbours_q<L, NLM>(T, zrows, q, s, lane, Z, [&] {
+        if (lane == 0) {
+          e0[0] = config[3 * (size_t)s];
+          e0[1] = config[3 * (size_t)s + 1];
+          e0[2] = config[3 * (size_t)s + 2];
+#pragma unroll
+          for (int k = 0; k < NLM; ++k) zo0[k] = zrows[(size_t)s * NLM + k];
+          const sce::u32x4 b = sce::philox_block(st->seed, s + 1, (int32_t)st->sweep_index,
+                                                 sce::SLOT_OR0 + (uint32_t)or_pass, 0u);
+          u_or = sce::uniform01(b.x, b.y);
+        }
+      });
+    }
   }
   __syncthreads();
   double c[NLM];
@@ -537,11 +554,9 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
       }
     }
   }
+  // per row, the shuffle-down tree into the first lane (DPP for its last steps)
 #pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1) {
-#pragma unroll
-    for (int k = 0; k < NLM; ++k) c[k] = c[k] + __shfl_down(c[k], off, L);
-  }
+  for (int k = 0; k < NLM; ++k) c[k] = fold_lanes<L>(c[k]);
   if (!valid || lane != 0) return;
   if (TEST) {
 #pragma unroll
@@ -556,30 +571,28 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
     if (!(hn < 1e-12)) {
       at = 1;
       const double ax[3] = {h0 / hn, h1 / hn, h2 / hn};
-      double* e = config + 3 * (size_t)s;
+      const double* e = e0;
       const double dd = (e[0] * ax[0] + e[1] * ax[1]) + e[2] * ax[2];
       const double tt = 2.0 * dd;
       const double e2[3] = {tt * ax[0] - e[0], tt * ax[1] - e[1], tt * ax[2] - e[2]};
       double zn[NLM];
       sce::zlm_row<LMAX>(e2[0], e2[1], e2[2], zn);
-      double* zo = zrows + (size_t)s * NLM;
       double dEv = 0.0;
 #pragma unroll
       for (int k = 0; k < NLM; ++k) {
         if (c[k] == 0.0) continue;
-        dEv += c[k] * (zn[k] - zo[k]);
+        dEv += c[k] * (zn[k] - zo0[k]);
       }
-      const sce::u32x4 b = sce::philox_block(st->seed, s + 1, (int32_t)st->sweep_index,
-                                             sce::SLOT_OR0 + (uint32_t)or_pass, 0u);
-      const double u = sce::uniform01(b.x, b.y);
-      if (dEv <= 0.0 || u < sce::det_exp(-st->beta * dEv)) {
+      if (dEv <= 0.0 || u_or < sce::det_exp(-st->beta * dEv)) {
         ac = 1;
         d_e = dEv;
-        e[0] = e2[0];
-        e[1] = e2[1];
-        e[2] = e2[2];
+        double* ew = config + 3 * (size_t)s;
+        ew[0] = e2[0];
+        ew[1] = e2[1];
+        ew[2] = e2[2];
+        double* zw = zrows + (size_t)s * NLM;
 #pragma unroll
-        for (int k = 0; k < NLM; ++k) zo[k] = zn[k];
+        for (int k = 0; k < NLM; ++k) zw[k] = zn[k];
       }
     }
   }

@@ -89,3 +89,26 @@ def test_benchmark_size_persistent_sweeps_equal_graph_path():
     assert np.array_equal(za.view(np.uint64), zb.view(np.uint64)) and Ea == Eb
     ia, ib = a.info(), b.info()
     assert (ia.acc_metro, ia.att_metro, ia.sweep_index) == (ib.acc_metro, ib.att_metro, ib.sweep_index)
+
+
+def test_config2_compound_sweeps_bitwise():
+    """The plain form at its own size: bcc 16^3 (8192 spins, body-2 l<=2 to the
+    3rd shell + l=1 triangles), compound sweeps of one Metropolis sweep and one
+    overrelaxation pass (run.jl:143-150), graph-replayed with the measurement and
+    the next proposals fused into the pass's reduction, bit-identical to the
+    oracle's composition."""
+    _, H, T, gH = _model("bcc16")
+    st = S.ChainState(H, rand_config(H.n_sites, 16), 0.9)
+    seed, beta = 1616, 1 / 0.02
+    g = S.GPUChainState(gH, st, seed=seed)
+    obs = S.gpu_run_sweeps(g, gH, None, beta, 3, renorm_interval=0, or_per_metropolis=1, measure_interval=1)
+    c2, z2, E = st.config.copy(), st.zrows.copy(), st.energy
+    Es = []
+    for sw in range(1, 4):
+        _, E, _, _ = T.keyed_sweep(c2, z2, E, beta, 0.9, seed, sw, 64)
+        _, _, E = T.keyed_or_sweep(c2, z2, E, beta, seed, sw, 0, 64)
+        Es.append(E)
+    cfg_d, z_d, E_d = g.download()
+    assert np.array_equal(cfg_d.view(np.uint64), c2.view(np.uint64))
+    assert np.array_equal(z_d.view(np.uint64), z2.view(np.uint64))
+    assert E_d == E and np.array_equal(obs[:, 0], np.array(Es))
