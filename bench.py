@@ -260,8 +260,9 @@ def da_kt_scan(args, S, H, gH, sur, snapshot, step, kts=(0.02, 0.025, 0.03, 0.03
 def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
     """Batched parallel tempering: 64 replicas of fcc 12^3 (6912 spins, body-2 l<=2 to
     the 4th shell) on a geometric kT ladder spanning 0.2-2.0 J1, replica exchange
-    on the device after every sweep, one launch per colour for all replicas, lanes
-    per site picked by the autotuner.
+    on the device after every sweep.  The autotuner picks lanes per site and the
+    launch form: one launch per colour for all replicas (replayed hipGraphs) or
+    one persistent launch per segment with the exchange decided in-kernel.
     J1 = 3 x rms raw coefficient of the first-shell (1,1) bonds (the bond energy
     of a (1,1) template is 4 pi N1^2 c = 3c times a unit-norm tensor form)."""
     na, terms, dims = M.config_model("fcc12", seed=0)
@@ -274,7 +275,7 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
     kts = J1 * np.geomspace(0.2, 2.0, R)
     H = S.TiledHamiltonian(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
-    from sce_amd.batch import GPUBatch, derive_seeds, _adapt, autotune_lanes
+    from sce_amd.batch import GPUBatch, derive_seeds, _adapt, autotune_batch
     init, exch, dev = derive_seeds(2026, R)
 
     def make():
@@ -284,8 +285,12 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
         b.set_graph(graph)
         return b
 
-    lanes = autotune_lanes(make, sweeps=20)   # lanes per site: a scratch batch, before the run
+    tune = {}
+    # lanes per site and launch form (per-colour graphs or one persistent launch
+    # per segment): scratch batches, before the run
+    lanes, pers = autotune_batch(make, sweeps=20, timings=tune)
     b = make()
+    b.set_persistent(pers)
     steps = [0.6] * R
     done = 0
     while done < sweeps_therm:            # untimed: exchange + step adaptation (gpu_pt.jl:115-140)
@@ -309,9 +314,11 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
             "bytes_per_update": algorithmic_bytes_per_update(H),
             "algorithmic_GBs": R * H.n_active * sweeps_measure / wall * algorithmic_bytes_per_update(H) / 1e9,
             "ms_per_sweep_all_replicas": 1e3 * wall / sweeps_measure, "sweeps": sweeps_measure,
-            "lanes": lanes, "measurements": int(obs.shape[1]),
+            "lanes": lanes, "persistent": pers, "measurements": int(obs.shape[1]),
+            "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()},
             "swap_acceptance_mean": float(np.nanmean(sw)), "swap_acceptance_min": float(np.nanmin(sw)),
-            "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep"}
+            "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep; "
+                    "autotune keys: lanes per site, 'p' = one persistent launch per segment"}
 
 
 def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True):
@@ -319,8 +326,8 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True):
     its own seeded body-2 l<=2 (+ body-3 l=1 triangles on bcc) model, kT and a
     Zeeman term (2 mu_B moments, field 0-10 T along a random axis).  Chains run in
     three size classes (N <= 10, <= 15, <= 20), one batch per class, with lanes
-    per class picked by the autotuner."""
-    from sce_amd.batch import GPUBatch, autotune_lanes
+    per site and launch form per class picked by the autotuner."""
+    from sce_amd.batch import GPUBatch, autotune_batch
 
     rng = np.random.default_rng(20261203)
     classes = {0: [], 1: [], 2: []}
@@ -345,16 +352,19 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True):
                          [kt for _, _, _, kt in chains])
             b.set_graph(graph)
             return b
-        lanes = autotune_lanes(make, sweeps=10)
+        tune = {}
+        lanes, pers = autotune_batch(make, sweeps=10, timings=tune)
         b = make()
+        b.set_persistent(pers)
         b.run(5, lanes, 0)
         t0 = time.perf_counter()
         b.run(sweeps, lanes, 1000)
         dt = time.perf_counter() - t0
         t_all += dt
         n_sites = sum(H.n_active for H, _, _, _ in chains)
-        out[f"class_{cls}"] = {"chains": len(chains), "spins": n_sites, "lanes": lanes,
-                               "ms_per_sweep": 1e3 * dt / sweeps, "spin_updates_per_s": n_sites * sweeps / dt}
+        out[f"class_{cls}"] = {"chains": len(chains), "spins": n_sites, "lanes": lanes, "persistent": pers,
+                               "ms_per_sweep": 1e3 * dt / sweeps, "spin_updates_per_s": n_sites * sweeps / dt,
+                               "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()}}
         del b
     return {"workload": f"{n_chains} chains, sc/bcc N^3 with N in 6..20, {total_sites} spins, per-chain model, "
                         "kT and Zeeman field", "spin_updates_per_s": total_sites * sweeps / t_all,

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <queue>
 #include <vector>
 
 #include "sce_internal.h"
@@ -1063,9 +1064,119 @@ __host__ __device__ inline int flow_meas_slot(long long i0, int i, int n, int mi
   return (int)((i0 + i) / mi - i0 / mi - 1);
 }
 
+// One item of a dependency-driven sweep, by one wavefront: wait until every
+// neighbour has done[nb] >= t (lower colour) or >= t - 1 (otherwise), stage
+// the neighbour rows, walk, decide, write the site, its sweep outputs (dE and
+// the accept flag; the spin when cfg_out) and publish done[s] = t.  `next`
+// runs between the staging and the walk (the caller's prefetch of its next
+// item).  False when a wait exceeded its bound (the wave must leave).
+template <int LMAX, class Next>
+__device__ __forceinline__ bool flow_update(const DevTables& T, double* config, double* zrows, const double* prop,
+                                            int* done, int* err, unsigned spin_limit, int q, int s,
+                                            unsigned long long low, int nb, int t, double beta, double* Z, double* dz,
+                                            double* dE_out, int* acc_out, double* cfg_out, Next&& next) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  const int lane = threadIdx.x;
+  const int nbm = T.nb_max;
+  const int k = lane < NLM ? lane : NLM - 1;
+  // this site's proposal (made by this wave), current row and spin: the
+  // wave's own earlier writes, independent of the neighbours' progress
+  __builtin_amdgcn_s_waitcnt(0);
+  const double* pq = prop + (size_t)q * PS;
+  const double u_acc = ld_ag(pq + 3);
+  const double p_new = ld_ag(pq + 4 + k), p_cur = ld_ag(zrows + (size_t)s * NLM + k);
+  const double e_old = lane < 3 ? ld_ag(config + 3 * (size_t)s + lane) : 0.0;
+  const double e_new = lane < 3 ? ld_ag(pq + lane) : 0.0;
+  // wait for the neighbours (lane j: slot j)
+  {
+    const int need = (lane < nbm && ((low >> lane) & 1ull)) ? t : t - 1;
+    unsigned spins = 0;
+    while (!__all(ld_ag(done + nb) >= need)) {
+      if (++spins > spin_limit) {
+        if (lane == 0) st_ag(err, 1);
+        return false;
+      }
+      if ((spins & 63u) == 0 && ld_ag(err) != 0) return false;
+      if (spins > 4) __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // stage the neighbour rows (slot-major, nb_max rows) and the 1.0 slot
+  const int32_t* nq = T.nbq + (size_t)q * nbm;
+  const int total = nbm * NLM;
+  for (int b = 0; b < total; b += 4 * 64) {
+    int at[4];
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = b + u * 64 + lane;
+      at[u] = x < total ? x : total - 1;
+      const int nbx = nq[at[u] / NLM];
+      v[u] = ld_ag(zrows + (size_t)nbx * NLM + (at[u] - (at[u] / NLM) * NLM));
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Z[at[u]] = v[u];
+  }
+  const int atom = s % T.na;
+  if (lane == 0) Z[(T.nb_ptr[atom + 1] - T.nb_ptr[atom]) * NLM] = 1.0;
+  if (lane < NLM) dz[lane] = p_new - p_cur;
+  next();
+  __syncthreads();
+  const int w0 = T.walk_ptr[atom];
+  double dsum = walk_delta<64, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
+  dsum = fold_lanes<64>(dsum);
+  int accept = 0;
+  if (lane == 0) accept = (dsum <= 0.0 || u_acc < sce::det_exp(-beta * dsum)) ? 1 : 0;
+  accept = __shfl(accept, 0, 64);
+  if (accept) {
+    if (lane < NLM) st_ag(zrows + (size_t)s * NLM + lane, p_new);
+    if (lane < 3) st_ag(config + 3 * (size_t)s + lane, e_new);
+  }
+  const double e_now = accept ? e_new : e_old;
+  if (cfg_out && lane < 3) st_ag(cfg_out + lane, e_now);
+  if (lane == 0) {
+    st_ag(dE_out, accept ? dsum : 0.0);
+    st_ag(acc_out, accept);
+  }
+  // publish: this update's writes complete before its flag
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0);
+  if (lane == 0) st_ag(done + s, t);
+  __syncthreads();  // LDS reused by the next item
+  return true;
+}
+
+// The proposals of sweep `sweep` for the positions q0, q0 + W, ... < n, from
+// their current spins (written by this wave only): sixteen positions at a
+// time, four lanes each.
+template <int LMAX>
+__device__ __forceinline__ void flow_propose(const DevTables& T, const double* config, double* prop, int q0, int W,
+                                             unsigned long long seed, int32_t sweep, double step) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  const int lane = threadIdx.x, n = T.n_active;
+  const int g = lane / PROPOSE_LANES;
+  for (int qb = q0; qb < n; qb += 16 * W) {
+    const int qg = qb + g * W;
+    const int qv = qg < n ? qg : qb;  // every lane takes part in the exchange
+    const int sv = T.color_sites[qv];
+    const double e[3] = {ld_ag(config + 3 * (size_t)sv), ld_ag(config + 3 * (size_t)sv + 1),
+                         ld_ag(config + 3 * (size_t)sv + 2)};
+    double e2[3], u = 0.0, z[NLM];
+    if (propose_core<LMAX>(seed, sv, sweep, e, step, lane % PROPOSE_LANES, e2, &u, z) && qg < n) {
+      double* o = prop + (size_t)qg * PS;
+      st_ag(o, e2[0]);
+      st_ag(o + 1, e2[1]);
+      st_ag(o + 2, e2[2]);
+      st_ag(o + 3, u);
+#pragma unroll
+      for (int kk = 0; kk < NLM; ++kk) st_ag(o + 4 + kk, z[kk]);
+    }
+  }
+}
+
 template <int LMAX>
 __global__ __launch_bounds__(64) void k_flow(FlowArgs a) {
-  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
   extern __shared__ double lds[];
   const DevTables& T = a.T;
   const int lane = threadIdx.x;
@@ -1073,7 +1184,6 @@ __global__ __launch_bounds__(64) void k_flow(FlowArgs a) {
   double* dz = Z + T.nb_max * NLM + 1;
   const int n = T.n_active, W = gridDim.x;
   const int nbm = T.nb_max;
-  const int k = lane < NLM ? lane : NLM - 1;
   if ((int)blockIdx.x >= n) return;
   // the first item's position tables (each item prefetches the next one's)
   int q = blockIdx.x;
@@ -1087,102 +1197,23 @@ __global__ __launch_bounds__(64) void k_flow(FlowArgs a) {
       const int s = __builtin_amdgcn_readfirstlane(s_nx);
       const unsigned long long low = low_nx;
       const int nb = nb_nx >= 0 ? nb_nx : s;
-      // this site's proposal (made by this wave), current row and spin: the
-      // wave's own earlier writes, independent of the neighbours' progress
-      __builtin_amdgcn_s_waitcnt(0);
-      const double* pq = a.prop + (size_t)q * PS;
-      const double u_acc = ld_ag(pq + 3);
-      const double p_new = ld_ag(pq + 4 + k), p_cur = ld_ag(a.zrows + (size_t)s * NLM + k);
-      const double e_old = lane < 3 ? ld_ag(a.config + 3 * (size_t)s + lane) : 0.0;
-      const double e_new = lane < 3 ? ld_ag(pq + lane) : 0.0;
-      // wait for the neighbours (lane j: slot j)
-      {
-        const int need = (lane < nbm && ((low >> lane) & 1ull)) ? t : t - 1;
-        unsigned spins = 0;
-        while (!__all(ld_ag(a.done + nb) >= need)) {
-          if (++spins > a.spin_limit) {
-            if (lane == 0) st_ag(a.err, 1);
-            return;
-          }
-          if ((spins & 63u) == 0 && ld_ag(a.err) != 0) return;
-          if (spins > 4) __builtin_amdgcn_s_sleep(1);
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // stage the neighbour rows (slot-major, nb_max rows) and the 1.0 slot
-      const int32_t* nq = T.nbq + (size_t)q * nbm;
-      const int total = nbm * NLM;
-      for (int b = 0; b < total; b += 4 * 64) {
-        int at[4];
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int x = b + u * 64 + lane;
-          at[u] = x < total ? x : total - 1;
-          const int nbx = nq[at[u] / NLM];
-          v[u] = ld_ag(a.zrows + (size_t)nbx * NLM + (at[u] - (at[u] / NLM) * NLM));
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) Z[at[u]] = v[u];
-      }
-      const int atom = s % T.na;
-      if (lane == 0) Z[(T.nb_ptr[atom + 1] - T.nb_ptr[atom]) * NLM] = 1.0;
-      if (lane < NLM) dz[lane] = p_new - p_cur;
       // the next item's position tables, loaded while this one walks
-      {
+      auto next = [&] {
         int qn = q + W;
         if (qn >= n) qn = blockIdx.x;  // the next sweep's first item
         s_nx = T.color_sites[qn];
         low_nx = a.low[qn];
         nb_nx = lane < nbm ? T.nbq[(size_t)qn * nbm + lane] : -1;
-      }
-      __syncthreads();
-      const int w0 = T.walk_ptr[atom];
-      double dsum = walk_delta<64, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
-      dsum = fold_lanes<64>(dsum);
-      int accept = 0;
-      if (lane == 0) accept = (dsum <= 0.0 || u_acc < sce::det_exp(-a.beta * dsum)) ? 1 : 0;
-      accept = __shfl(accept, 0, 64);
-      if (accept) {
-        if (lane < NLM) st_ag(a.zrows + (size_t)s * NLM + lane, p_new);
-        if (lane < 3) st_ag(a.config + 3 * (size_t)s + lane, e_new);
-      }
-      const double e_now = accept ? e_new : e_old;
-      if (slot >= 0 && lane < 3) st_ag(a.cfg_seg + ((size_t)slot * T.n_sites + s) * 3 + lane, e_now);
-      if (lane == 0) {
-        st_ag(a.dE_seg + (size_t)(i - 1) * n + q, accept ? dsum : 0.0);
-        st_ag(a.acc_seg + (size_t)(i - 1) * n + q, accept);
-      }
-      // publish: this update's writes complete before its flag
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_s_waitcnt(0);
-      if (lane == 0) st_ag(a.done + s, t);
-      __syncthreads();  // LDS reused by the next item
+      };
+      if (!flow_update<LMAX>(T, a.config, a.zrows, a.prop, a.done, a.err, a.spin_limit, q, s, low, nb, t, a.beta, Z,
+                             dz, a.dE_seg + (size_t)(i - 1) * n + q, a.acc_seg + (size_t)(i - 1) * n + q,
+                             slot >= 0 ? a.cfg_seg + ((size_t)slot * T.n_sites + s) * 3 : nullptr, next))
+        return;
     }
     q = blockIdx.x;
     // the next sweep's proposals of this wave's positions, from their spins
-    // after this sweep (nothing else writes them): sixteen positions at a
-    // time, four lanes each
-    if (i < a.nsweeps) {
-      const int g = lane / PROPOSE_LANES;
-      for (int q0 = (int)blockIdx.x; q0 < n; q0 += 16 * W) {
-        const int qg = q0 + g * W;
-        const int qv = qg < n ? qg : q0;  // every lane takes part in the exchange
-        const int sv = T.color_sites[qv];
-        const double e[3] = {ld_ag(a.config + 3 * (size_t)sv), ld_ag(a.config + 3 * (size_t)sv + 1),
-                             ld_ag(a.config + 3 * (size_t)sv + 2)};
-        double e2[3], u = 0.0, z[NLM];
-        if (propose_core<LMAX>(a.seed, sv, t + 1, e, a.step, lane % PROPOSE_LANES, e2, &u, z) && qg < n) {
-          double* o = a.prop + (size_t)qg * PS;
-          st_ag(o, e2[0]);
-          st_ag(o + 1, e2[1]);
-          st_ag(o + 2, e2[2]);
-          st_ag(o + 3, u);
-#pragma unroll
-          for (int kk = 0; kk < NLM; ++kk) st_ag(o + 4 + kk, z[kk]);
-        }
-      }
-    }
+    // after this sweep (nothing else writes them)
+    if (i < a.nsweeps) flow_propose<LMAX>(T, a.config, a.prop, (int)blockIdx.x, W, a.seed, t + 1, a.step);
   }
 }
 
@@ -3683,6 +3714,362 @@ __global__ void k_breset_obs(BChain* ch, int R) {
   if (k < R) ch[k].st->n_obs = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched dependency-driven sweeps: every chain of a batch (replica ladder or
+// uneven chains) in one cooperative launch per segment of sweeps, replica
+// exchange included.  Chain k owns waves [wave_lo, wave_lo + wave_n); within
+// a chain the schedule is the single-chain one (wave w of W owns positions
+// w, w + W, ... and waits on its neighbours' done flags, here counted in
+// segment sweeps from 0).  Across chains the only coupling is the exchange:
+//  - each wave adds its count of completed positions of (chain, sweep) to
+//    cnt; the wave that completes the count reduces the sweep's dE in the
+//    chunk and final trees of the per-colour path (so the energy is the same
+//    bits) after the chain's previous sweep is reduced (red), and records
+//    the energies before and after the sweep;
+//  - with an exchange after the sweep, the second reducer of a pair (pcnt)
+//    draws the keyed uniform, swaps the payload pointers and energies, and
+//    releases both chains (ready); a chain without a partner releases itself;
+//  - a wave proposes the next sweep only once its chain is released, from
+//    the payload the chain then holds.
+// Every wait is on work earlier in (sweep, chain order) and every wave is
+// resident, so the launch always drains; waits are bounded anyway (spin
+// limit -> *err, every wave leaves).  The sweep's counts, measurements and
+// state updates are made after the launch from the segment buffers.
+// ---------------------------------------------------------------------------
+struct BFlowChain {
+  int pos_off, site_off;  // first global colour position and site of the chain
+  int wave_lo, wave_n;    // the chain's waves
+  long long part_off, ipart_off;  // per-sweep reduction partials of the chain
+};
+
+struct BFlowArgs {
+  const BChain* ch;
+  const BFlowChain* fc;
+  int R, G, NS;   // chains, colour positions and sites of the batch
+  int nsweeps;
+  long long i0;   // sweeps of the phase before the launch (measurement schedule)
+  int measure_interval, defer_last;
+  unsigned long long exch_mask[2];  // bit i - 1: an exchange follows segment sweep i
+  const BatchDev* bd;
+  const double* kts;
+  long long *sw_att, *sw_acc;
+  double* dE_seg;    // nsweeps x G, colour-position order
+  int* acc_seg;      // nsweeps x G
+  double* cfg_seg;   // measured sweeps x 3 x NS (spins after the sweep)
+  int* done;         // per site: the last segment sweep it completed
+  int *cnt, *pcnt;   // R x nsweeps: positions completed; reducers of pair (k, k + 1) done
+  double *Ecur, *Estart;  // R: energy now; R x nsweeps: energy before each sweep
+  int *red, *ready;  // R: last sweep reduced; last sweep whose exchange is settled
+  unsigned long long *pcfg, *pzr;  // R: the payload each chain holds (exchanged in-kernel)
+  double* part;      // per (chain, sweep): chunk partials (dE, then the three spin sums)
+  long long* ipart;  // per (chain, sweep): accept counts per chunk
+  int* err;
+  unsigned spin_limit;
+};
+
+__device__ __forceinline__ unsigned long long ld_ag(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_ag(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool exch_after(const unsigned long long m[2], int i) {
+  return i >= 1 && i <= 128 && ((m[(i - 1) >> 6] >> ((i - 1) & 63)) & 1ull);
+}
+// exchanges of the segment before sweep i
+__device__ __forceinline__ int exch_before(const unsigned long long m[2], int i) {
+  const int b = i - 1;  // bits 0 .. b-1
+  const unsigned long long lo = b >= 64 ? m[0] : (b > 0 ? m[0] & ((1ull << b) - 1ull) : 0ull);
+  const unsigned long long hi = b > 64 ? (b >= 128 ? m[1] : m[1] & ((1ull << (b - 64)) - 1ull)) : 0ull;
+  return __popcll(lo) + __popcll(hi);
+}
+
+// block_tree (256 threads) by one wavefront: lane l holds threads l + 64 j,
+// x0[j] and x1[j] their two operands.  The same additions in the same order;
+// the result is valid in lane 0.
+__device__ __forceinline__ double wave_block_tree(const double x0[4], const double x1[4]) {
+  const double v0 = x0[0] + x1[0], v1 = x0[1] + x1[1], v2 = x0[2] + x1[2], v3 = x0[3] + x1[3];
+  double v = (v0 + v2) + (v1 + v3);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_down(v, off, 64);
+  return v;
+}
+
+// The sum of x[0 .. n) by the chunk trees of chunk_body and the final tree of
+// final_body (n <= 512 * 512), by one wavefront; valid in every lane.
+__device__ double wave_sweep_sum(const double* x, int n) {
+  const int lane = threadIdx.x;
+  const int nbx = n > 0 ? (n + TREE_B - 1) / TREE_B : 1;
+  double p[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) p[j] = 0.0;
+  for (int b = 0; b < nbx; ++b) {
+    double x0[4], x1[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i0 = b * TREE_B + lane + 64 * j, i1 = i0 + 256;
+      x0[j] = i0 < n ? ld_ag(x + i0) : 0.0;
+      x1[j] = i1 < n ? ld_ag(x + i1) : 0.0;
+    }
+    const double v = __shfl(wave_block_tree(x0, x1), 0, 64);
+    if (nbx == 1) return v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if ((b >> 6) == j && lane == (b & 63)) p[j] = v;
+  }
+  double x0[4], x1[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    x0[j] = p[j];
+    x1[j] = p[j + 4];
+  }
+  return __shfl(wave_block_tree(x0, x1), 0, 64);
+}
+
+// bounded wait until *flag >= need; false (and *err set) past the bound
+__device__ __forceinline__ bool flow_wait(const int* flag, int need, int* err, unsigned spin_limit) {
+  unsigned spins = 0;
+  while (ld_ag(flag) < need) {
+    if (++spins > spin_limit) {
+      if (threadIdx.x == 0) st_ag(err, 1);
+      return false;
+    }
+    if ((spins & 63u) == 0 && ld_ag(err) != 0) return false;
+    if (spins > 4) __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return true;
+}
+
+// Reduction of sweep i of chain k, and the exchange that follows it (by the
+// pair's second reducer).  One wavefront; false when a wait exceeded its bound.
+__device__ bool bflow_reduce(const BFlowArgs& a, int k, int i) {
+  const int lane = threadIdx.x;
+  const BFlowChain f = a.fc[k];
+  const int n = a.ch[k].T.n_active;
+  if (!flow_wait(a.red + k, i - 1, a.err, a.spin_limit)) return false;
+  const double e0 = ld_ag(a.Ecur + k);
+  const double e1 = e0 + wave_sweep_sum(a.dE_seg + (size_t)(i - 1) * a.G + f.pos_off, n);
+  if (lane == 0) {
+    st_ag(a.Estart + (size_t)k * a.nsweeps + (i - 1), e0);
+    st_ag(a.Ecur + k, e1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0);
+  if (lane == 0) st_ag(a.red + k, i);
+  if (!exch_after(a.exch_mask, i)) return true;
+  const int nex = exch_before(a.exch_mask, i);
+  const int parity = a.bd->parity ^ (nex & 1);
+  const long long ex = a.bd->exch_count + nex + 1;
+  int j = -1;
+  if (k < a.R - 1 && ((k - parity) & 1) == 0) j = k;
+  else if (k >= 1 && ((k - 1 - parity) & 1) == 0) j = k - 1;
+  if (j < 0) {
+    __builtin_amdgcn_s_waitcnt(0);
+    if (lane == 0) st_ag(a.ready + k, i);
+    return true;
+  }
+  int second = 0;
+  if (lane == 0)
+    second = __hip_atomic_fetch_add(a.pcnt + (size_t)j * a.nsweeps + (i - 1), 1, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == 1;
+  second = __shfl(second, 0, 64);
+  if (!second) return true;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane == 0) {
+    // the adjacent-pair rule of k_exchange, pair j of exchange ex
+    const sce::u32x4 rb = sce::philox_block(a.bd->exch_seed, j + 1, (int32_t)ex, SLOT_EXCH, 0u);
+    const double u = sce::uniform01(rb.x, rb.y);
+    const double ea = ld_ag(a.Ecur + j), eb = ld_ag(a.Ecur + j + 1);
+    const double logw = (1.0 / a.kts[j] - 1.0 / a.kts[j + 1]) * (ea - eb);
+    const double m = logw < 0.0 ? logw : 0.0;
+    __hip_atomic_fetch_add(a.sw_att + j, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (u < sce::det_exp(m)) {
+      const unsigned long long c0 = ld_ag(a.pcfg + j), c1 = ld_ag(a.pcfg + j + 1);
+      const unsigned long long z0 = ld_ag(a.pzr + j), z1 = ld_ag(a.pzr + j + 1);
+      st_ag(a.pcfg + j, c1);
+      st_ag(a.pcfg + j + 1, c0);
+      st_ag(a.pzr + j, z1);
+      st_ag(a.pzr + j + 1, z0);
+      st_ag(a.Ecur + j, eb);
+      st_ag(a.Ecur + j + 1, ea);
+      __hip_atomic_fetch_add(a.sw_acc + j, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0);
+  if (lane == 0) {
+    st_ag(a.ready + j, i);
+    st_ag(a.ready + j + 1, i);
+  }
+  return true;
+}
+
+// Six waves per SIMD (80 VGPRs, some spilled): batches are throughput-bound,
+// and the extra resident waves outweigh the spills (64 x fcc 12^3 ladder:
+// 1.79 vs 2.00 ms per sweep at the natural four; 256 uneven chains: 3.24 vs
+// 3.43 ms).
+template <int LMAX>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_bflow(BFlowArgs a) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x;
+  // this wave's chain: the last with wave_lo <= blockIdx.x
+  int k = 0;
+  {
+    int lo = 0, hi = a.R - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.fc[mid].wave_lo <= (int)blockIdx.x) lo = mid;
+      else hi = mid - 1;
+    }
+    k = lo;
+  }
+  const BFlowChain f = a.fc[k];
+  const int wl = (int)blockIdx.x - f.wave_lo, W = f.wave_n;
+  const BChain* C = a.ch + k;
+  const DevTables& T = C->T;
+  const int n = T.n_active, nbm = T.nb_max;
+  // a chain without active sites has one wave, which only reduces (to 0)
+  if (wl >= W || (n > 0 && wl >= n)) return;
+  double* Z = lds;
+  double* dz = Z + nbm * NLM + 1;
+  const ChainDev* st = C->st;
+  const double beta = st->beta, step = st->step;
+  const unsigned long long seed = st->seed;
+  const long long base = st->sweep_index;  // sweeps before the launch
+  int* done = a.done + f.site_off;
+  double* config = reinterpret_cast<double*>(ld_ag(a.pcfg + k));
+  double* zrows = reinterpret_cast<double*>(ld_ag(a.pzr + k));
+  int q = wl;
+  int s_nx = 0, nb_nx = -1;
+  unsigned long long low_nx = 0;
+  if (n > 0) {
+    s_nx = T.color_sites[q];
+    low_nx = T.nbq_low[q];
+    nb_nx = lane < nbm ? T.nbq[(size_t)q * nbm + lane] : -1;
+  }
+  for (int i = 1; i <= a.nsweeps; ++i) {
+    const int slot = flow_meas_slot(a.i0, i, a.nsweeps, a.measure_interval, a.defer_last);
+    int mine = 0;
+    for (; q < n; q += W) {
+      const int s = __builtin_amdgcn_readfirstlane(s_nx);
+      const unsigned long long low = low_nx;
+      const int nb = nb_nx >= 0 ? nb_nx : s;
+      auto next = [&] {
+        int qn = q + W;
+        if (qn >= n) qn = wl;  // the next sweep's first item
+        s_nx = T.color_sites[qn];
+        low_nx = T.nbq_low[qn];
+        nb_nx = lane < nbm ? T.nbq[(size_t)qn * nbm + lane] : -1;
+      };
+      const size_t g = (size_t)(i - 1) * a.G + f.pos_off + q;
+      if (!flow_update<LMAX>(T, config, zrows, C->prop, done, a.err, a.spin_limit, q, s, low, nb, i, beta, Z, dz,
+                             a.dE_seg + g, a.acc_seg + g,
+                             slot >= 0 ? a.cfg_seg + ((size_t)slot * a.NS + f.site_off + s) * 3 : nullptr, next))
+        return;
+      ++mine;
+    }
+    q = wl;
+    // this wave's share of the sweep is published (flow_update waits for its
+    // stores before each flag); the wave completing the count reduces
+    int last = 0;
+    if (lane == 0)
+      last = __hip_atomic_fetch_add(a.cnt + (size_t)k * a.nsweeps + (i - 1), mine, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) + mine == n;
+    last = __shfl(last, 0, 64);
+    if (last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (!bflow_reduce(a, k, i)) return;
+    }
+    if (i < a.nsweeps) {
+      if (exch_after(a.exch_mask, i)) {
+        // the payload this chain holds for the next sweep
+        if (!flow_wait(a.ready + k, i, a.err, a.spin_limit)) return;
+        config = reinterpret_cast<double*>(ld_ag(a.pcfg + k));
+        zrows = reinterpret_cast<double*>(ld_ag(a.pzr + k));
+      }
+      flow_propose<LMAX>(T, config, C->prop, wl, W, seed, (int32_t)(base + i + 1), step);
+    }
+  }
+}
+
+// Segment start: done flags (inactive sites never update: always done),
+// counters, energies and the payload each chain holds.
+__global__ void k_bflow_init(BFlowArgs a) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < a.NS) {
+    int lo = 0, hi = a.R - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.fc[mid].site_off <= idx) lo = mid;
+      else hi = mid - 1;
+    }
+    const int s = (int)(idx - a.fc[lo].site_off);
+    a.done[idx] = a.ch[lo].T.active[s] ? 0 : 2147483647;
+  }
+  if (idx < (long long)a.R * a.nsweeps) {
+    a.cnt[idx] = 0;
+    a.pcnt[idx] = 0;
+  }
+  if (idx < a.R) {
+    const BChain& c = a.ch[idx];
+    a.Ecur[idx] = c.st->energy;
+    a.red[idx] = 0;
+    a.ready[idx] = 0;
+    a.pcfg[idx] = reinterpret_cast<unsigned long long>(c.config);
+    a.pzr[idx] = reinterpret_cast<unsigned long long>(c.zrows);
+  }
+}
+
+// first reduction level of every (chain, sweep) of a segment: block (b, i, k)
+// is chunk b of sweep i of chain k (chains with fewer chunks return at once)
+__global__ __launch_bounds__(256) void k_bflow_chunks(BFlowArgs a) {
+  const int b = blockIdx.x, i = blockIdx.y, k = blockIdx.z;
+  const BChain& c = a.ch[k];
+  const BFlowChain f = a.fc[k];
+  const int n = c.T.n_active, N = c.T.n_sites;
+  const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1), nchm = ((long long)N + TREE_B - 1) / TREE_B;
+  const int slot = flow_meas_slot(a.i0, i + 1, a.nsweeps, a.measure_interval, a.defer_last);
+  chunk_body<MODE_METRO>(b, a.dE_seg + (size_t)i * a.G + f.pos_off, n, a.acc_seg + (size_t)i * a.G + f.pos_off,
+                         nullptr, a.part + f.part_off + (size_t)i * (nbx + 3 * nchm),
+                         a.ipart + f.ipart_off + (size_t)i * 2 * nbx,
+                         slot >= 0 ? a.cfg_seg + ((size_t)slot * a.NS + f.site_off) * 3 : nullptr, c.T.active, N,
+                         slot >= 0 ? 1 : 0);
+}
+
+// The segment's final reductions of chain blockIdx.x, in sweep order: each
+// starts from the energy the chain held before the sweep (after the previous
+// exchange), so energy, counters, sweep index and measurement rows advance
+// as in the per-colour path; then the chain takes the energy and payload it
+// holds after the segment, and block 0 advances the exchange schedule.
+__global__ __launch_bounds__(256) void k_bflow_final(BFlowArgs a, BChain* ch, BatchDev* bd, int nex) {
+  const int k = blockIdx.x;
+  BChain& c = ch[k];
+  const BFlowChain f = a.fc[k];
+  const int n = c.T.n_active, N = c.T.n_sites;
+  const long long nbx = std::max<long long>((n + TREE_B - 1) / TREE_B, 1), nchm = ((long long)N + TREE_B - 1) / TREE_B;
+  double* scratch = c.part + nbx + 3 * std::max<long long>(nchm, 1);
+  for (int i = 0; i < a.nsweeps; ++i) {
+    const int slot = flow_meas_slot(a.i0, i + 1, a.nsweeps, a.measure_interval, a.defer_last);
+    if (threadIdx.x == 0) c.st->energy = a.Estart[(size_t)k * a.nsweeps + i];
+    final_body<MODE_METRO>(a.part + f.part_off + (size_t)i * (nbx + 3 * nchm), n,
+                           a.ipart + f.ipart_off + (size_t)i * 2 * nbx, N, slot >= 0 ? 1 : 0, c.st, c.obs, n,
+                           (double)n, scratch);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    c.st->energy = a.Ecur[k];
+    c.config = reinterpret_cast<double*>(a.pcfg[k]);
+    c.zrows = reinterpret_cast<double*>(a.pzr[k]);
+    if (k == 0) {
+      bd->exch_count += nex;
+      bd->parity ^= (nex & 1);
+    }
+  }
+}
+
 struct sce_batch {
   int R = 0, lmax = 0, max_colors = 0, stride = 0;
   bool homogeneous = false;
@@ -3707,6 +4094,18 @@ struct sce_batch {
   int g_lanes = 0;
   long long sweeps_done = 0;
   bool use_graph = true;
+  bool persistent = false;
+  // buffers of the batched dependency-driven sweep (k_bflow)
+  struct BFlow {
+    int cap = 0, mcap = 0, grid = 0, G = 0, NS = 0;
+    double *dE = nullptr, *cfg = nullptr, *Ecur = nullptr, *Estart = nullptr, *part = nullptr;
+    int *acc = nullptr, *done = nullptr, *cnt = nullptr, *pcnt = nullptr, *red = nullptr, *ready = nullptr;
+    int* err = nullptr;
+    long long* ipart = nullptr;
+    unsigned long long *pcfg = nullptr, *pzr = nullptr;
+    BFlowChain* fc = nullptr;
+    std::vector<BFlowChain> hfc;
+  } bf;
   void drop_graphs() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& g : gexec) g.reset();
@@ -3913,6 +4312,212 @@ static int32_t brenorm(sce_batch* b) {
   return SCE_OK;
 }
 
+// Chains of the batched persistent sweep: position and site offsets, and
+// waves per chain.  Every chain gets one wave, then waves go one at a time to
+// the chain with the most positions per wave, up to what can be resident.
+template <int LMAX>
+static int32_t bflow_waves(sce_batch* b, int* max_waves) {
+  const size_t shmem = (size_t)b->stride * sizeof(double);
+  const void* f = reinterpret_cast<const void*>(&k_bflow<LMAX>);
+  int per_cu = 0, cus = 0, dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, shmem));
+  *max_waves = per_cu * cus;
+  return SCE_OK;
+}
+
+static int32_t bflow_setup(sce_batch* b) {
+  sce_batch::BFlow& F = b->bf;
+  if (!F.hfc.empty()) return SCE_OK;
+  for (int k = 0; k < b->R; ++k) {
+    const DevTables& D = b->tabs[k]->dev;
+    if (!D.nbq_low)
+      return sce_fail(SCE_EINVAL, "persistent sweeps need at most 64 neighbour slots; chain %d's stencil has %d", k,
+                      D.nb_max);
+    if (D.n_active > TREE_B * TREE_B || D.n_sites > TREE_B * TREE_B)
+      return sce_fail(SCE_EDIM, "persistent batched sweeps take chains of at most %d sites", TREE_B * TREE_B);
+  }
+  int wmax = 0;
+  int32_t r = SCE_OK;
+  SCE_DISPATCH_LMAX_R(r, b->lmax, bflow_waves, b, &wmax);
+  CHK(r);
+  if (wmax < b->R)
+    return sce_fail(SCE_EINVAL, "persistent batched sweeps need one resident wave per chain: %d chains, %d waves", b->R,
+                    wmax);
+  std::vector<int> nw((size_t)b->R, 1);
+  auto load = [&](int k) {
+    const int n = std::max(b->tabs[k]->dev.n_active, 1);
+    return ((long long)n + nw[k] - 1) / nw[k];
+  };
+  std::priority_queue<std::pair<long long, int>> pq;
+  for (int k = 0; k < b->R; ++k) pq.push({load(k), -k});
+  int used = b->R;
+  while (used < wmax && !pq.empty()) {
+    const int k = -pq.top().second;
+    pq.pop();
+    if (nw[k] >= std::max(b->tabs[k]->dev.n_active, 1)) break;  // the busiest chain has a wave per position
+    ++nw[k];
+    ++used;
+    pq.push({load(k), -k});
+  }
+  F.hfc.assign((size_t)b->R, BFlowChain{});
+  int pos = 0, site = 0, wave = 0;
+  for (int k = 0; k < b->R; ++k) {
+    F.hfc[k].pos_off = pos;
+    F.hfc[k].site_off = site;
+    F.hfc[k].wave_lo = wave;
+    F.hfc[k].wave_n = nw[k];
+    pos += b->tabs[k]->dev.n_active;
+    site += b->tabs[k]->dev.n_sites;
+    wave += nw[k];
+  }
+  F.G = pos;
+  F.NS = site;
+  F.grid = wave;
+  CHK(dalloc(b->mem, &F.done, (size_t)site));
+  CHK(dalloc(b->mem, &F.Ecur, (size_t)b->R));
+  CHK(dalloc(b->mem, &F.red, (size_t)b->R));
+  CHK(dalloc(b->mem, &F.ready, (size_t)b->R));
+  CHK(dalloc(b->mem, &F.pcfg, (size_t)b->R));
+  CHK(dalloc(b->mem, &F.pzr, (size_t)b->R));
+  CHK(dalloc(b->mem, &F.err, 4));
+  return SCE_OK;
+}
+
+// segment buffers for `cap` sweeps of which at most `mcap` are measured
+static int32_t bflow_reserve(sce_batch* b, int cap, int mcap) {
+  sce_batch::BFlow& F = b->bf;
+  if (cap > F.cap) {
+    for (void* p : {(void*)F.dE, (void*)F.acc, (void*)F.cnt, (void*)F.pcnt, (void*)F.Estart, (void*)F.part,
+                    (void*)F.ipart, (void*)F.fc})
+      b->mem.release(p);
+    F.dE = nullptr, F.acc = nullptr, F.cnt = nullptr, F.pcnt = nullptr, F.Estart = nullptr, F.part = nullptr;
+    F.ipart = nullptr, F.fc = nullptr;
+    F.cap = 0;
+    long long part = 0, ipart = 0;
+    for (int k = 0; k < b->R; ++k) {
+      const DevTables& D = b->tabs[k]->dev;
+      const long long nbx = std::max<long long>((D.n_active + TREE_B - 1) / TREE_B, 1);
+      const long long nchm = ((long long)D.n_sites + TREE_B - 1) / TREE_B;
+      F.hfc[k].part_off = part;
+      F.hfc[k].ipart_off = ipart;
+      part += (long long)cap * (nbx + 3 * nchm);
+      ipart += (long long)cap * 2 * nbx;
+    }
+    CHK(dalloc(b->mem, &F.dE, (size_t)cap * (size_t)std::max(F.G, 1)));
+    CHK(dalloc(b->mem, &F.acc, (size_t)cap * (size_t)std::max(F.G, 1)));
+    CHK(dalloc(b->mem, &F.cnt, (size_t)cap * (size_t)b->R));
+    CHK(dalloc(b->mem, &F.pcnt, (size_t)cap * (size_t)b->R));
+    CHK(dalloc(b->mem, &F.Estart, (size_t)cap * (size_t)b->R));
+    CHK(dalloc(b->mem, &F.part, (size_t)part));
+    CHK(dalloc(b->mem, &F.ipart, (size_t)ipart));
+    CHK(dupload(b->mem, &F.fc, F.hfc.data(), F.hfc.size()));
+    F.cap = cap;
+  }
+  if (mcap > F.mcap) {
+    b->mem.release(F.cfg);
+    F.cfg = nullptr;
+    F.mcap = 0;
+    CHK(dalloc(b->mem, &F.cfg, (size_t)mcap * 3 * (size_t)std::max(F.NS, 1)));
+    F.mcap = mcap;
+  }
+  return SCE_OK;
+}
+
+template <int LMAX>
+static int32_t launch_bflow(sce_batch* b, BFlowArgs& fa) {
+  const void* f = reinterpret_cast<const void*>(&k_bflow<LMAX>);
+  void* args[] = {&fa};
+  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)b->bf.grid), dim3(64), args,
+                                    (unsigned)((size_t)b->stride * sizeof(double)), b->stream));
+  return SCE_OK;
+}
+
+// n sweeps of every chain (sweeps i0 + 1 .. i0 + n of the phase) as one
+// batched dependency-driven launch plus the segment's reductions.  mask: the
+// segment sweeps followed by an exchange (nex of them).
+static int32_t benqueue_flow(sce_batch* b, int n, long long i0, int64_t measure_interval, int defer_last,
+                             const unsigned long long mask[2], int nex) {
+  if (n <= 0) return SCE_OK;
+  const int mi = (int)std::max<int64_t>(measure_interval, 0);
+  int nmeas = 0;
+  for (int i = 1; i <= n; ++i) nmeas += flow_meas_slot(i0, i, n, mi, defer_last) >= 0 ? 1 : 0;
+  CHK(bflow_reserve(b, n, nmeas));
+  const sce_batch::BFlow& F = b->bf;
+  hipStream_t s = b->stream;
+  // first sweep's proposals (the per-colour path's launch)
+  SCE_DISPATCH_LMAX(b->lmax, launch_bpropose, b, s);
+  BFlowArgs fa{};
+  fa.ch = b->d_ch;
+  fa.fc = F.fc;
+  fa.R = b->R;
+  fa.G = F.G;
+  fa.NS = F.NS;
+  fa.nsweeps = n;
+  fa.i0 = i0;
+  fa.measure_interval = mi;
+  fa.defer_last = defer_last;
+  fa.exch_mask[0] = mask[0];
+  fa.exch_mask[1] = mask[1];
+  fa.bd = b->bd;
+  fa.kts = b->kts;
+  fa.sw_att = b->sw_att;
+  fa.sw_acc = b->sw_acc;
+  fa.dE_seg = F.dE;
+  fa.acc_seg = F.acc;
+  fa.cfg_seg = F.cfg;
+  fa.done = F.done;
+  fa.cnt = F.cnt;
+  fa.pcnt = F.pcnt;
+  fa.Ecur = F.Ecur;
+  fa.Estart = F.Estart;
+  fa.red = F.red;
+  fa.ready = F.ready;
+  fa.pcfg = F.pcfg;
+  fa.pzr = F.pzr;
+  fa.part = F.part;
+  fa.ipart = F.ipart;
+  fa.err = F.err;
+  fa.spin_limit = FLOW_SPIN_LIMIT;
+  const long long init = std::max<long long>({(long long)F.NS, (long long)b->R * n, 1});
+  k_bflow_init<<<(unsigned)((init + 255) / 256), 256, 0, s>>>(fa);
+  int32_t r = SCE_OK;
+  SCE_DISPATCH_LMAX_R(r, b->lmax, launch_bflow, b, fa);
+  CHK(r);
+  // the reductions of every (chain, sweep): per-chain chunk counts, padded to
+  // the largest chain (blocks past a chain's chunks return at once)
+  long long per = 1;
+  for (int k = 0; k < b->R; ++k) {
+    const DevTables& D = b->tabs[k]->dev;
+    const long long nbx = std::max<long long>((D.n_active + TREE_B - 1) / TREE_B, 1);
+    const long long nchm = ((long long)D.n_sites + TREE_B - 1) / TREE_B;
+    per = std::max(per, nmeas > 0 ? std::max(nbx, nchm) : nbx);
+  }
+  k_bflow_chunks<<<dim3((unsigned)per, (unsigned)n, (unsigned)b->R), 256, 0, s>>>(fa);
+  k_bflow_final<<<b->R, 256, 0, s>>>(fa, b->d_ch, b->bd, nex);
+  HIPCHK(hipGetLastError());
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_set_persistent(sce_batch* b, int32_t on) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  b->persistent = on != 0;
+  return SCE_OK;
+}
+
+// A renormalisation, then the measurement and the exchange of its sweep
+// (the reference measures after renormalising, run.jl:191-200).
+static int32_t brenorm_then(sce_batch* b, bool meas, bool exch) {
+  CHK(brenorm(b));
+  if (meas) {
+    k_bchunks<MODE_MEASURE><<<b->total_meas, 256, 0, b->stream>>>(b->d_ch, b->d_seg_meas, b->nseg_meas, 1);
+    k_bfinal<MODE_MEASURE><<<b->R, 256, 0, b->stream>>>(b->d_ch, 1);
+  }
+  if (exch) k_exchange<<<1, 1, 0, b->stream>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
+  return SCE_OK;
+}
+
 extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, int64_t renorm_interval,
                                  int64_t measure_interval, int64_t exchange_interval,
                                  int64_t phase_done, int64_t phase_total, double* obs,
@@ -3941,30 +4546,60 @@ extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, i
     HIPCHK(hipMemcpy(b->d_ch, b->h_ch.data(), b->h_ch.size() * sizeof(BChain), hipMemcpyHostToDevice));
   }
   k_breset_obs<<<(b->R + 255) / 256, 256, 0, b->stream>>>(b->d_ch, b->R);
-  for (int64_t i = 1; i <= nsweeps; ++i) {
-    const int64_t j = phase_done + i;  // sweep number within the phase
-    const bool renorm = renorm_interval > 0 && j % renorm_interval == 0;
-    const bool meas = measure_interval > 0 && j % measure_interval == 0;
-    const bool exch = exchange_interval > 0 && j % exchange_interval == 0 && j < phase_total;
-    const bool fused_meas = meas && !renorm, fused_exch = exch && !renorm;
-    if (b->use_graph) {
-      CHK(bensure_graph(b, lanes, fused_meas, fused_exch));
-      HIPCHK(hipGraphLaunch(b->gexec[(fused_meas ? 2 : 0) + (fused_exch ? 1 : 0)].x, b->stream));
-    } else {
-      CHK(benqueue_sweep(b, lanes, fused_meas, fused_exch));
-    }
-    b->sweeps_done += 1;
-    if (renorm) {
-      CHK(brenorm(b));
-      if (meas) {
-        k_bchunks<MODE_MEASURE><<<b->total_meas, 256, 0, b->stream>>>(b->d_ch, b->d_seg_meas, b->nseg_meas, 1);
-        k_bfinal<MODE_MEASURE><<<b->R, 256, 0, b->stream>>>(b->d_ch, 1);
+  const bool flow = b->persistent && nsweeps > 0;
+  if (flow) {
+    if (lanes != 64) return sce_fail(SCE_EINVAL, "persistent sweeps walk with 64 lanes per site; got %d", lanes);
+    CHK(bflow_setup(b));
+    HIPCHK(hipMemsetAsync(b->bf.err, 0, sizeof(int), b->stream));
+    // segments end at renormalisation points and hold at most FLOW_SEG sweeps;
+    // a renormalisation's sweep has its measurement and exchange after it
+    int64_t i = 0;
+    while (i < nsweeps) {
+      int64_t seg = std::min<int64_t>(nsweeps - i, FLOW_SEG);
+      if (renorm_interval > 0) seg = std::min<int64_t>(seg, renorm_interval - ((phase_done + i) % renorm_interval));
+      const int64_t jend = phase_done + i + seg;
+      const bool renorm_after = renorm_interval > 0 && jend % renorm_interval == 0;
+      unsigned long long mask[2] = {0ull, 0ull};
+      int nex = 0;
+      for (int64_t u = 1; u <= seg; ++u) {
+        const int64_t j = phase_done + i + u;
+        const bool ex = exchange_interval > 0 && j % exchange_interval == 0 && j < phase_total;
+        if (ex && !(renorm_after && u == seg)) {
+          mask[(u - 1) >> 6] |= 1ull << ((u - 1) & 63);
+          ++nex;
+        }
       }
-      if (exch) k_exchange<<<1, 1, 0, b->stream>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
+      CHK(benqueue_flow(b, (int)seg, phase_done + i, measure_interval, renorm_after ? 1 : 0, mask, nex));
+      b->sweeps_done += seg;
+      i += seg;
+      if (renorm_after)
+        CHK(brenorm_then(b, measure_interval > 0 && jend % measure_interval == 0,
+                         exchange_interval > 0 && jend % exchange_interval == 0 && jend < phase_total));
+    }
+  } else {
+    for (int64_t i = 1; i <= nsweeps; ++i) {
+      const int64_t j = phase_done + i;  // sweep number within the phase
+      const bool renorm = renorm_interval > 0 && j % renorm_interval == 0;
+      const bool meas = measure_interval > 0 && j % measure_interval == 0;
+      const bool exch = exchange_interval > 0 && j % exchange_interval == 0 && j < phase_total;
+      const bool fused_meas = meas && !renorm, fused_exch = exch && !renorm;
+      if (b->use_graph) {
+        CHK(bensure_graph(b, lanes, fused_meas, fused_exch));
+        HIPCHK(hipGraphLaunch(b->gexec[(fused_meas ? 2 : 0) + (fused_exch ? 1 : 0)].x, b->stream));
+      } else {
+        CHK(benqueue_sweep(b, lanes, fused_meas, fused_exch));
+      }
+      b->sweeps_done += 1;
+      if (renorm) CHK(brenorm_then(b, meas, exch));
     }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (flow) {
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, b->bf.err, sizeof e, hipMemcpyDeviceToHost));
+    if (e) return sce_fail(SCE_EHIP, "persistent batched sweep: a wait exceeded its bound");
+  }
   // device rows: b->obs_cap per chain; the caller's: obs_cap per chain
   if (want > 0)
     HIPCHK(hipMemcpy2D(obs, 4 * (size_t)obs_cap * 8, b->obs, 4 * b->obs_cap * 8, 4 * (size_t)want * 8,

@@ -97,6 +97,12 @@ class GPUBatch:
     def set_graph(self, on: bool) -> None:
         L.check(L.lib().sce_batch_set_graph(self._h, 1 if on else 0))
 
+    def set_persistent(self, on: bool) -> None:
+        """One persistent launch per renormalisation segment for every chain,
+        exchange included (64 lanes per site); bit-identical to the per-colour
+        launches."""
+        L.check(L.lib().sce_batch_set_persistent(self._h, 1 if on else 0))
+
     def set_counters(self, k: int, sweep_index: int, acc_metro: int, att_metro: int,
                      acc_or: int = 0, att_or: int = 0, max_drift: float = 0.0) -> None:
         """Restore chain k's keyed-RNG sweep index, counters and drift record."""
@@ -153,6 +159,7 @@ class PTResult:
     seed: int
     wall_s: float = 0.0
     lanes: int = 0     # lanes per site the run swept with (part of the determinism scope)
+    persistent: bool = False  # one persistent launch per segment (results identical either way)
 
 
 def _adapt(step, acc, att, target):
@@ -225,13 +232,16 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
                step: float = 0.6, adapt_target: float = 0.5, adapt_interval: int = 50,
                renorm_interval: int = 1000, nbins: int = 32, init=None, seed: int = 0,
                lanes: int = DEFAULT_LANES, checkpoint: Optional[str] = None,
-               checkpoint_interval: int = 0) -> PTResult:
+               checkpoint_interval: int = 0, persistent: Optional[bool] = None) -> PTResult:
     """Replica exchange on the device — gpu_run_pt (src/gpu/gpu_pt.jl:261-319).
 
     `checkpoint` / `checkpoint_interval` write restartable checkpoints (kind
     "gpu_pt", interval in sweeps, 0: only at the thermalisation -> measurement
     boundary); `resume_pt(path, gH)` continues one bit-identically.  As in the
-    reference there is no end-of-run write."""
+    reference there is no end-of-run write.  `lanes="auto"` tunes lanes per
+    site (pinned in the plan) and, unless `persistent` is given, whether the
+    ladder runs as one persistent launch per segment (64 lanes) or as
+    per-colour launches; the launch form never changes the results."""
     kts = resolve_kt(temperature, kT)
     _validate_pt(kts, exchange_interval, sweeps_measure, measure_interval)
     plan = PTPlan(kts, int(exchange_interval), int(sweeps_therm), int(sweeps_measure),
@@ -240,10 +250,15 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
     if lanes == "auto":
         # tuned on a throwaway copy of the ladder, then pinned in the plan (and
         # so in every checkpoint and the result) like the reference's workgroupsize
-        plan.lanes = autotune_lanes(lambda: _pt_batch(gH, plan, init), sweeps=20)
+        cands = _BATCH_CANDIDATES if persistent is None else tuple(
+            (n, bool(persistent)) for n in ((64,) if persistent else (16, 32, 64)))
+        plan.lanes, persistent = autotune_batch(lambda: _pt_batch(gH, plan, init), cands, sweeps=20)
     ck = _Checkpointer(checkpoint, checkpoint_interval) if checkpoint is not None else None
     b = _pt_batch(gH, plan, init)
-    return _pt_run(b, gH, plan, "therm", 0, None, ck)
+    b.set_persistent(bool(persistent))
+    res = _pt_run(b, gH, plan, "therm", 0, None, ck)
+    res.persistent = bool(persistent)
+    return res
 
 
 def _pt_batch(gH: GPUTiledHamiltonian, plan: PTPlan, init=None) -> GPUBatch:
@@ -342,21 +357,26 @@ def resume_pt(path: str, gH: GPUTiledHamiltonian, checkpoint: Optional[str] = "s
     return _pt_run(st["batch"], gH, st["plan"], st["phase"], st["done"], st["rows"], ck)
 
 
-def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache: Optional[dict] = None,
-                   key=None, timings: Optional[dict] = None) -> int:
-    """The project autotuner: times `sweeps` graph-replayed sweeps of a scratch
-    batch for each lanes-per-site candidate (sites per 256-thread block = 256 /
-    lanes) and returns the fastest.  `lanes` is part of the determinism scope, so
-    tuning runs on a throwaway batch built by `make_batch()` before the production
-    run, and the caller records the choice with the run.  Results are memoised in
-    `cache` under `key` (e.g. a size-class signature); `timings` receives the
-    seconds per sweep of every candidate that ran."""
-    if cache is not None and key is not None and key in cache:
-        return cache[key]
+# (lanes per site, persistent launch) forms the batch autotuner tries
+_BATCH_CANDIDATES = ((16, False), (32, False), (64, False), (64, True))
+
+
+def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
+                   timings: Optional[dict] = None):
+    """The project autotuner for batches: times `sweeps` sweeps of a scratch
+    batch built by `make_batch()` in each (lanes per site, persistent) form —
+    per-colour launches replayed as hipGraphs at 16/32/64 lanes (sites per
+    256-thread block = 256 / lanes), or one persistent launch per segment at 64
+    lanes — and returns the fastest as (lanes, persistent).  `lanes` is part of
+    the determinism scope, so tuning runs on a throwaway batch before the
+    production run and the caller records the choice with the run; the launch
+    form is not.  `timings` receives the seconds per sweep of every candidate
+    that ran, keyed "16", "32", "64" and "64p"."""
     best, best_t = None, math.inf
-    for lanes in candidates:
+    for lanes, pers in candidates:
         b = make_batch()
         try:
+            b.set_persistent(pers)
             b.run(3, lanes, 0)          # capture + warm
             t0 = time.perf_counter()
             b.run(sweeps, lanes, 0)
@@ -366,11 +386,26 @@ def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache:
         finally:
             del b
         if timings is not None:
-            timings[lanes] = dt / sweeps
+            timings[f"{lanes}{'p' if pers else ''}"] = dt / sweeps
         if dt < best_t:
-            best, best_t = lanes, dt
+            best, best_t = (int(lanes), bool(pers)), dt
     if best is None:
-        raise RuntimeError("no lanes candidate could run this batch")
-    if cache is not None and key is not None:
-        cache[key] = best
+        raise RuntimeError("no candidate form could run this batch")
     return best
+
+
+def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache: Optional[dict] = None,
+                   key=None, timings: Optional[dict] = None) -> int:
+    """Lanes per site only (per-colour launches): autotune_batch over the
+    candidate lane counts.  Results are memoised in `cache` under `key` (e.g. a
+    size-class signature); `timings` receives the seconds per sweep of every
+    candidate that ran, keyed by lanes."""
+    if cache is not None and key is not None and key in cache:
+        return cache[key]
+    t = {} if timings is not None else None
+    lanes, _ = autotune_batch(make_batch, tuple((n, False) for n in candidates), sweeps, t)
+    if timings is not None:
+        timings.update({int(k): v for k, v in t.items()})
+    if cache is not None and key is not None:
+        cache[key] = lanes
+    return lanes

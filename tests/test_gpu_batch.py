@@ -275,6 +275,7 @@ def test_pt_autotuned_lanes_are_recorded():
     kw = dict(kT=[0.01, 0.02, 0.04], exchange_interval=1, sweeps_therm=20, sweeps_measure=30, seed=17)
     r = S.gpu_run_pt(gH, lanes="auto", **kw)
     assert r.lanes in (16, 32, 64)
+    assert not r.persistent or r.lanes == 64
     r2 = S.gpu_run_pt(gH, lanes=r.lanes, **kw)
     for a, b in zip(r.final_configs, r2.final_configs):
         assert np.array_equal(a, b)
