@@ -396,7 +396,7 @@ function gpu_run_pt(gH::GPUTiledHamiltonian; temperature = nothing, kT = nothing
                     evaluables::Vector{Evaluable} = standard_evaluables(),
                     init = nothing, seed::Integer = rand(UInt64), workgroupsize::Integer = 128,
                     checkpoint::Union{Nothing,AbstractString} = nothing,
-                    checkpoint_interval::Integer = 0)::PTResult
+                    checkpoint_interval::Integer = 0, persistent::Bool = false)::PTResult
     H = gH.host
     kts = resolve_kt(temperature, kT)
     R = length(kts)
@@ -408,6 +408,8 @@ function gpu_run_pt(gH::GPUTiledHamiltonian; temperature = nothing, kT = nothing
     checkpoint === nothing || throw(ArgumentError(
         "checkpointed ladder runs of the HIP path are written by sce_amd.gpu_run_pt(checkpoint = …)"))
     lanes = hip_lanes(workgroupsize)
+    (persistent && lanes != 64) && throw(ArgumentError(
+        "persistent = true runs every rung in one launch with 64 lanes per site; got workgroupsize = $workgroupsize"))
     _check_observables(observables)
     _check_evaluables(observables, evaluables)
     # the reference's master-seed derivation order (gpu_pt.jl:17-29)
@@ -418,6 +420,9 @@ function gpu_run_pt(gH::GPUTiledHamiltonian; temperature = nothing, kT = nothing
     dev_seeds = [rand(master, UInt64) for _ = 1:R]
     mirrors = [ChainState(H, _initial_config(H, init, lane_rngs[r]), lane_rngs[r], Float64(step)) for r = 1:R]
     b = _HIPBatch(gH, mirrors, dev_seeds, Vector{Float64}(kts), exch_words[1])
+    # an addition to the reference's keywords: each run of sweeps as one
+    # persistent launch of every rung (bit-identical to per-colour launches)
+    _hip_check(ccall((:sce_batch_set_persistent, SCE_HIP_LIB), Int32, (Ptr{Cvoid}, Int32), b.ptr, Int32(persistent)))
     ex = Int(exchange_interval)
     # thermalisation: exchanges, renormalisation, and step adaptation every
     # adapt_interval sweeps on the rung's counter window
