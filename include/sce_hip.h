@@ -241,9 +241,9 @@ int32_t sce_batch_swap_stats(sce_batch* b, int64_t* att, int64_t* acc);
 int32_t sce_batch_set_graph(sce_batch* b, int32_t on);
 /* Run the sweeps of sce_batch_run, replica exchange included, as one
  * persistent cooperative launch for every chain per renormalisation segment
- * (on != 0; 64 lanes per site, stencils of at most 64 neighbour slots) or as
- * per-colour launches of every chain (0, the default).  Both give
- * bit-identical results.  The reference runs rungs round-robin on one queue
+ * (on != 0; stencils of at most 64 neighbour slots, (lmax + 1)^2 <= lanes) or
+ * as per-colour launches of every chain (0, the default).  Both give
+ * bit-identical results at the same lanes per site.  The reference runs rungs round-robin on one queue
  * (src/gpu/gpu_pt.jl:4-15). */
 int32_t sce_batch_set_persistent(sce_batch* b, int32_t on);
 /* Checkpoint restore of a "gpu_pt" run (resume, src/checkpoint.jl:555-635).

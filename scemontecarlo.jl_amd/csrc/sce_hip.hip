@@ -1064,19 +1064,25 @@ __host__ __device__ inline int flow_meas_slot(long long i0, int i, int n, int mi
   return (int)((i0 + i) / mi - i0 / mi - 1);
 }
 
-// One item of a dependency-driven sweep, by one wavefront: wait until every
-// neighbour has done[nb] >= t (lower colour) or >= t - 1 (otherwise), stage
-// the neighbour rows, walk, decide, write the site, its sweep outputs (dE and
-// the accept flag; the spin when cfg_out) and publish done[s] = t.  `next`
-// runs between the staging and the walk (the caller's prefetch of its next
-// item).  False when a wait exceeded its bound (the wave must leave).
-template <int LMAX, class Next>
+// One item of a dependency-driven sweep per group of L lanes (64 / L sites of
+// one colour class per wavefront, never neighbours of each other): wait until
+// every neighbour has done[nb] >= t (lower colour) or >= t - 1 (otherwise),
+// stage the neighbour rows, walk, decide, write the site, its sweep outputs
+// (dE and the accept flag; the spin when cfg_out) and publish done[s] = t.
+// nb[m]: this lane's neighbour slot lane + m L (the site itself past nb_max).
+// A group with `valid` false mirrors a valid site of its wave and writes
+// nothing.  `next` runs between the staging and the walk (the caller's
+// prefetch of its next item).  Called by every lane of the wave; false when a
+// wait exceeded its bound (the wave must leave).
+template <int LMAX, int L, class Next>
 __device__ __forceinline__ bool flow_update(const DevTables& T, double* config, double* zrows, const double* prop,
                                             int* done, int* err, unsigned spin_limit, int q, int s,
-                                            unsigned long long low, int nb, int t, double beta, double* Z, double* dz,
-                                            double* dE_out, int* acc_out, double* cfg_out, Next&& next) {
-  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
-  const int lane = threadIdx.x;
+                                            unsigned long long low, const int (&nb)[64 / L], bool valid, int t,
+                                            double beta, double* Z, double* dz, double* dE_out, int* acc_out,
+                                            double* cfg_out, Next&& next) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4, NSL = 64 / L;
+  static_assert(NLM <= L, "one lane per row element");
+  const int lane = threadIdx.x % L;
   const int nbm = T.nb_max;
   const int k = lane < NLM ? lane : NLM - 1;
   // this site's proposal (made by this wave), current row and spin: the
@@ -1087,13 +1093,22 @@ __device__ __forceinline__ bool flow_update(const DevTables& T, double* config, 
   const double p_new = ld_ag(pq + 4 + k), p_cur = ld_ag(zrows + (size_t)s * NLM + k);
   const double e_old = lane < 3 ? ld_ag(config + 3 * (size_t)s + lane) : 0.0;
   const double e_new = lane < 3 ? ld_ag(pq + lane) : 0.0;
-  // wait for the neighbours (lane j: slot j)
+  // wait for the neighbours (lane j: slots j, j + L, ...)
   {
-    const int need = (lane < nbm && ((low >> lane) & 1ull)) ? t : t - 1;
+    int need[NSL];
+#pragma unroll
+    for (int m = 0; m < NSL; ++m) {
+      const int j = lane + m * L;
+      need[m] = (j < nbm && ((low >> j) & 1ull)) ? t : t - 1;
+    }
     unsigned spins = 0;
-    while (!__all(ld_ag(done + nb) >= need)) {
+    while (true) {
+      bool ok = true;
+#pragma unroll
+      for (int m = 0; m < NSL; ++m) ok = ok && ld_ag(done + nb[m]) >= need[m];
+      if (__all(ok)) break;
       if (++spins > spin_limit) {
-        if (lane == 0) st_ag(err, 1);
+        if (threadIdx.x == 0) st_ag(err, 1);
         return false;
       }
       if ((spins & 63u) == 0 && ld_ag(err) != 0) return false;
@@ -1104,12 +1119,12 @@ __device__ __forceinline__ bool flow_update(const DevTables& T, double* config, 
   // stage the neighbour rows (slot-major, nb_max rows) and the 1.0 slot
   const int32_t* nq = T.nbq + (size_t)q * nbm;
   const int total = nbm * NLM;
-  for (int b = 0; b < total; b += 4 * 64) {
+  for (int b = 0; b < total; b += 4 * L) {
     int at[4];
     double v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int x = b + u * 64 + lane;
+      const int x = b + u * L + lane;
       at[u] = x < total ? x : total - 1;
       const int nbx = nq[at[u] / NLM];
       v[u] = ld_ag(zrows + (size_t)nbx * NLM + (at[u] - (at[u] / NLM) * NLM));
@@ -1123,25 +1138,27 @@ __device__ __forceinline__ bool flow_update(const DevTables& T, double* config, 
   next();
   __syncthreads();
   const int w0 = T.walk_ptr[atom];
-  double dsum = walk_delta<64, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
-  dsum = fold_lanes<64>(dsum);
+  double dsum = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
+  dsum = fold_lanes<L>(dsum);
   int accept = 0;
   if (lane == 0) accept = (dsum <= 0.0 || u_acc < sce::det_exp(-beta * dsum)) ? 1 : 0;
-  accept = __shfl(accept, 0, 64);
-  if (accept) {
-    if (lane < NLM) st_ag(zrows + (size_t)s * NLM + lane, p_new);
-    if (lane < 3) st_ag(config + 3 * (size_t)s + lane, e_new);
-  }
-  const double e_now = accept ? e_new : e_old;
-  if (cfg_out && lane < 3) st_ag(cfg_out + lane, e_now);
-  if (lane == 0) {
-    st_ag(dE_out, accept ? dsum : 0.0);
-    st_ag(acc_out, accept);
+  accept = __shfl(accept, 0, L);
+  if (valid) {
+    if (accept) {
+      if (lane < NLM) st_ag(zrows + (size_t)s * NLM + lane, p_new);
+      if (lane < 3) st_ag(config + 3 * (size_t)s + lane, e_new);
+    }
+    const double e_now = accept ? e_new : e_old;
+    if (cfg_out && lane < 3) st_ag(cfg_out + lane, e_now);
+    if (lane == 0) {
+      st_ag(dE_out, accept ? dsum : 0.0);
+      st_ag(acc_out, accept);
+    }
   }
   // publish: this update's writes complete before its flag
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_s_waitcnt(0);
-  if (lane == 0) st_ag(done + s, t);
+  if (valid && lane == 0) st_ag(done + s, t);
   __syncthreads();  // LDS reused by the next item
   return true;
 }
@@ -1174,6 +1191,38 @@ __device__ __forceinline__ void flow_propose(const DevTables& T, const double* c
   }
 }
 
+// The proposals of sweep `sweep` for the positions of units u0, u0 + W, ...
+// < nu (unit u: positions ulo[u] .. ulo[u + 1] - 1, at most G of them): as
+// flow_propose, sixteen positions at a time.
+template <int LMAX>
+__device__ __forceinline__ void flow_propose_units(const DevTables& T, const double* config, double* prop,
+                                                   const int* ulo, int nu, int u0, int W, int G,
+                                                   unsigned long long seed, int32_t sweep, double step) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  const int lane = threadIdx.x;
+  const int nf = ((nu - u0 + W - 1) / W) * G;  // position slots of this wave's units
+  for (int fb = 0; fb < nf; fb += 16) {
+    const int f = fb + lane / PROPOSE_LANES;
+    const int u = u0 + (f / G) * W;
+    const int qg = u < nu ? ulo[u] + f % G : -1;
+    const bool ok = u < nu && f < nf && qg < ulo[u + 1];
+    const int qv = ok ? qg : ulo[u0];  // every lane takes part in the exchange
+    const int sv = T.color_sites[qv];
+    const double e[3] = {ld_ag(config + 3 * (size_t)sv), ld_ag(config + 3 * (size_t)sv + 1),
+                         ld_ag(config + 3 * (size_t)sv + 2)};
+    double e2[3], uacc = 0.0, z[NLM];
+    if (propose_core<LMAX>(seed, sv, sweep, e, step, lane % PROPOSE_LANES, e2, &uacc, z) && ok) {
+      double* o = prop + (size_t)qg * PS;
+      st_ag(o, e2[0]);
+      st_ag(o + 1, e2[1]);
+      st_ag(o + 2, e2[2]);
+      st_ag(o + 3, uacc);
+#pragma unroll
+      for (int kk = 0; kk < NLM; ++kk) st_ag(o + 4 + kk, z[kk]);
+    }
+  }
+}
+
 template <int LMAX>
 __global__ __launch_bounds__(64) void k_flow(FlowArgs a) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1);
@@ -1196,7 +1245,7 @@ __global__ __launch_bounds__(64) void k_flow(FlowArgs a) {
     for (; q < n; q += W) {
       const int s = __builtin_amdgcn_readfirstlane(s_nx);
       const unsigned long long low = low_nx;
-      const int nb = nb_nx >= 0 ? nb_nx : s;
+      const int nb[1] = {nb_nx >= 0 ? nb_nx : s};
       // the next item's position tables, loaded while this one walks
       auto next = [&] {
         int qn = q + W;
@@ -1205,9 +1254,10 @@ __global__ __launch_bounds__(64) void k_flow(FlowArgs a) {
         low_nx = a.low[qn];
         nb_nx = lane < nbm ? T.nbq[(size_t)qn * nbm + lane] : -1;
       };
-      if (!flow_update<LMAX>(T, a.config, a.zrows, a.prop, a.done, a.err, a.spin_limit, q, s, low, nb, t, a.beta, Z,
-                             dz, a.dE_seg + (size_t)(i - 1) * n + q, a.acc_seg + (size_t)(i - 1) * n + q,
-                             slot >= 0 ? a.cfg_seg + ((size_t)slot * T.n_sites + s) * 3 : nullptr, next))
+      if (!flow_update<LMAX, 64>(T, a.config, a.zrows, a.prop, a.done, a.err, a.spin_limit, q, s, low, nb, true, t,
+                                 a.beta, Z, dz, a.dE_seg + (size_t)(i - 1) * n + q,
+                                 a.acc_seg + (size_t)(i - 1) * n + q,
+                                 slot >= 0 ? a.cfg_seg + ((size_t)slot * T.n_sites + s) * 3 : nullptr, next))
         return;
     }
     q = blockIdx.x;
@@ -2873,16 +2923,32 @@ static int32_t flow_reserve(sce_chain* c, int cap, int mcap) {
   return SCE_OK;
 }
 
+// Workgroups of one wavefront of kernel f with `shmem` bytes of LDS that are
+// resident on a CU at once.  The occupancy query (and the cooperative launch,
+// which checks against it) overstates LDS-limited residency on this MI355X:
+// a probe in which every wave waits, bounded, for all others found e.g. 9
+// resident per CU at 15.5-16 KB where the query says 10, 25 at 5.5-6 KB
+// (query 26-29) and 4 at 31.5-32 KB (query 5); every size from 1 to 64 KB
+// holds at least 136 KB / shmem.  The dependency-driven sweeps need their
+// whole grid resident, so they size it by the smaller bound.
+static int32_t resident_per_cu(const void* f, size_t shmem, int* per_cu, int* cus) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, 64, shmem));
+  constexpr size_t LDS_RESIDENT = 136 * 1024;
+  if (shmem > 0) *per_cu = std::min<int>(*per_cu, (int)(LDS_RESIDENT / shmem));
+  return SCE_OK;
+}
+
 template <int LMAX>
 static int32_t launch_flow(sce_chain* c, FlowArgs& fa, hipStream_t s) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1);
   const DevTables& D = c->t->dev;
   const size_t shmem = ((size_t)D.nb_max * NLM + 1 + NLM) * sizeof(double);
   const void* f = reinterpret_cast<const void*>(&k_flow<LMAX>);
-  int per_cu = 0, cus = 0, dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, shmem));
+  int per_cu = 0, cus = 0;
+  CHK(resident_per_cu(f, shmem, &per_cu, &cus));
   const long long grid = std::min<long long>((long long)per_cu * cus, std::max(D.n_active, 1));
   if (grid < 1) return sce_fail(SCE_EHIP, "the persistent sweep kernel cannot be resident");
   void* args[] = {&fa};
@@ -3739,6 +3805,7 @@ __global__ void k_breset_obs(BChain* ch, int R) {
 struct BFlowChain {
   int pos_off, site_off;  // first global colour position and site of the chain
   int wave_lo, wave_n;    // the chain's waves
+  int unit_off, n_units;  // the chain's units (runs of <= 64 / L positions of one colour)
   long long part_off, ipart_off;  // per-sweep reduction partials of the chain
 };
 
@@ -3763,6 +3830,8 @@ struct BFlowArgs {
   unsigned long long *pcfg, *pzr;  // R: the payload each chain holds (exchanged in-kernel)
   double* part;      // per (chain, sweep): chunk partials (dE, then the three spin sums)
   long long* ipart;  // per (chain, sweep): accept counts per chunk
+  const int* units;  // per chain, n_units + 1 unit starts (colour positions)
+  int stride;        // LDS doubles per site group
   int* err;
   unsigned spin_limit;
 };
@@ -3906,15 +3975,24 @@ __device__ bool bflow_reduce(const BFlowArgs& a, int k, int i) {
   return true;
 }
 
-// Six waves per SIMD (80 VGPRs, some spilled): batches are throughput-bound,
-// and the extra resident waves outweigh the spills (64 x fcc 12^3 ladder:
-// 1.79 vs 2.00 ms per sweep at the natural four; 256 uneven chains: 3.24 vs
-// 3.43 ms).
-template <int LMAX>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_bflow(BFlowArgs a) {
-  constexpr int NLM = (LMAX + 1) * (LMAX + 1);
+// Waves per SIMD of the batched sweep, forced above the natural count (some
+// registers spill; batches are throughput-bound).  64 x fcc 12^3 ladder, ms
+// per sweep: 64 lanes 1.79 at six vs 2.00 at four (256 uneven chains: 3.24
+// vs 3.43); 32 lanes 1.95 at four vs 2.14 at three, 2.04 at five; 16 lanes
+// 2.71 at three, 2.74 at four.
+template <int L>
+struct BFlowOcc {
+  static constexpr int lo = L == 64 ? 6 : L == 32 ? 4 : 1, hi = L == 64 ? 6 : L == 32 ? 4 : 8;
+};
+
+// Wave w of chain k (group wl of W) owns the chain's units wl, wl + W, ...;
+// group g of L lanes takes position g of each.
+template <int LMAX, int L>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BFlowOcc<L>::lo, BFlowOcc<L>::hi)))
+void k_bflow(BFlowArgs a) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), G = 64 / L, NSL = 64 / L;
   extern __shared__ double lds[];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, g = lane / L, gl = lane % L;
   // this wave's chain: the last with wave_lo <= blockIdx.x
   int k = 0;
   {
@@ -3931,9 +4009,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
   const BChain* C = a.ch + k;
   const DevTables& T = C->T;
   const int n = T.n_active, nbm = T.nb_max;
+  const int* ulo = a.units + f.unit_off;
+  const int nu = f.n_units;
   // a chain without active sites has one wave, which only reduces (to 0)
-  if (wl >= W || (n > 0 && wl >= n)) return;
-  double* Z = lds;
+  if (wl >= W || (nu > 0 && wl >= nu)) return;
+  double* Z = lds + (size_t)g * a.stride;
   double* dz = Z + nbm * NLM + 1;
   const ChainDev* st = C->st;
   const double beta = st->beta, step = st->step;
@@ -3942,36 +4022,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
   int* done = a.done + f.site_off;
   double* config = reinterpret_cast<double*>(ld_ag(a.pcfg + k));
   double* zrows = reinterpret_cast<double*>(ld_ag(a.pzr + k));
-  int q = wl;
-  int s_nx = 0, nb_nx = -1;
+  // the position tables of unit u (group g), prefetched one item ahead
+  int q_nx = 0, s_nx = 0, nb_nx[NSL];
+  bool v_nx = false;
   unsigned long long low_nx = 0;
-  if (n > 0) {
-    s_nx = T.color_sites[q];
-    low_nx = T.nbq_low[q];
-    nb_nx = lane < nbm ? T.nbq[(size_t)q * nbm + lane] : -1;
-  }
+  auto tables = [&](int u) {
+    const int q0 = ulo[u];
+    q_nx = q0 + g;
+    v_nx = q_nx < ulo[u + 1];
+    if (!v_nx) q_nx = q0;
+    s_nx = T.color_sites[q_nx];
+    low_nx = T.nbq_low[q_nx];
+#pragma unroll
+    for (int m = 0; m < NSL; ++m) {
+      const int j = gl + m * L;
+      nb_nx[m] = j < nbm ? T.nbq[(size_t)q_nx * nbm + j] : -1;
+    }
+  };
+  if (nu > 0) tables(wl);
   for (int i = 1; i <= a.nsweeps; ++i) {
     const int slot = flow_meas_slot(a.i0, i, a.nsweeps, a.measure_interval, a.defer_last);
     int mine = 0;
-    for (; q < n; q += W) {
-      const int s = __builtin_amdgcn_readfirstlane(s_nx);
+    for (int u = wl; u < nu; u += W) {
+      const int q = q_nx;
+      const int s = L == 64 ? __builtin_amdgcn_readfirstlane(s_nx) : s_nx;
       const unsigned long long low = low_nx;
-      const int nb = nb_nx >= 0 ? nb_nx : s;
-      auto next = [&] {
-        int qn = q + W;
-        if (qn >= n) qn = wl;  // the next sweep's first item
-        s_nx = T.color_sites[qn];
-        low_nx = T.nbq_low[qn];
-        nb_nx = lane < nbm ? T.nbq[(size_t)qn * nbm + lane] : -1;
-      };
-      const size_t g = (size_t)(i - 1) * a.G + f.pos_off + q;
-      if (!flow_update<LMAX>(T, config, zrows, C->prop, done, a.err, a.spin_limit, q, s, low, nb, i, beta, Z, dz,
-                             a.dE_seg + g, a.acc_seg + g,
-                             slot >= 0 ? a.cfg_seg + ((size_t)slot * a.NS + f.site_off + s) * 3 : nullptr, next))
+      const bool valid = v_nx;
+      int nb[NSL];
+#pragma unroll
+      for (int m = 0; m < NSL; ++m) nb[m] = nb_nx[m] >= 0 ? nb_nx[m] : s;
+      mine += ulo[u + 1] - ulo[u];
+      auto next = [&] { tables(u + W < nu ? u + W : wl); };  // wraps to the next sweep's first unit
+      const size_t gq = (size_t)(i - 1) * a.G + f.pos_off + q;
+      if (!flow_update<LMAX, L>(T, config, zrows, C->prop, done, a.err, a.spin_limit, q, s, low, nb, valid, i, beta,
+                                Z, dz, a.dE_seg + gq, a.acc_seg + gq,
+                                slot >= 0 ? a.cfg_seg + ((size_t)slot * a.NS + f.site_off + s) * 3 : nullptr,
+                                next))
         return;
-      ++mine;
     }
-    q = wl;
     // this wave's share of the sweep is published (flow_update waits for its
     // stores before each flag); the wave completing the count reduces
     int last = 0;
@@ -3990,7 +4078,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         config = reinterpret_cast<double*>(ld_ag(a.pcfg + k));
         zrows = reinterpret_cast<double*>(ld_ag(a.pzr + k));
       }
-      flow_propose<LMAX>(T, config, C->prop, wl, W, seed, (int32_t)(base + i + 1), step);
+      if (nu > 0) flow_propose_units<LMAX>(T, config, C->prop, ulo, nu, wl, W, G, seed, (int32_t)(base + i + 1), step);
     }
   }
 }
@@ -4105,6 +4193,9 @@ struct sce_batch {
     unsigned long long *pcfg = nullptr, *pzr = nullptr;
     BFlowChain* fc = nullptr;
     std::vector<BFlowChain> hfc;
+    int L = 0;              // lanes per site the unit tables and wave groups are for
+    int* units = nullptr;   // per chain, its unit starts (n_units + 1)
+    bool dirty = false;     // hfc changed since its upload
   } bf;
   void drop_graphs() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -4312,76 +4403,109 @@ static int32_t brenorm(sce_batch* b) {
   return SCE_OK;
 }
 
-// Chains of the batched persistent sweep: position and site offsets, and
-// waves per chain.  Every chain gets one wave, then waves go one at a time to
-// the chain with the most positions per wave, up to what can be resident.
-template <int LMAX>
-static int32_t bflow_waves(sce_batch* b, int* max_waves) {
-  const size_t shmem = (size_t)b->stride * sizeof(double);
-  const void* f = reinterpret_cast<const void*>(&k_bflow<LMAX>);
-  int per_cu = 0, cus = 0, dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, shmem));
+// Chains of the batched persistent sweep: position and site offsets, units
+// (runs of at most 64 / L consecutive positions of one colour class, one per
+// wave and item) and waves per chain.  Every chain gets one wave, then waves
+// go one at a time to the chain with the most units per wave, up to what can
+// be resident.
+template <int LMAX, int L>
+static int32_t bflow_resident(sce_batch* b, int* max_waves) {
+  const size_t shmem = (size_t)(64 / L) * (size_t)b->stride * sizeof(double);
+  const void* f = reinterpret_cast<const void*>(&k_bflow<LMAX, L>);
+  int per_cu = 0, cus = 0;
+  CHK(resident_per_cu(f, shmem, &per_cu, &cus));
   *max_waves = per_cu * cus;
   return SCE_OK;
 }
 
-static int32_t bflow_setup(sce_batch* b) {
+template <int LMAX>
+static int32_t bflow_waves(sce_batch* b, int L, int* max_waves) {
+  if (L == 64) return bflow_resident<LMAX, 64>(b, max_waves);
+  if constexpr ((LMAX + 1) * (LMAX + 1) <= 32)
+    if (L == 32) return bflow_resident<LMAX, 32>(b, max_waves);
+  if constexpr ((LMAX + 1) * (LMAX + 1) <= 16)
+    if (L == 16) return bflow_resident<LMAX, 16>(b, max_waves);
+  return sce_fail(SCE_EINVAL, "persistent batched sweeps at %d lanes per site need (lmax + 1)^2 <= %d; lmax is %d",
+                  L, L, LMAX);
+}
+
+static int32_t bflow_setup(sce_batch* b, int L) {
   sce_batch::BFlow& F = b->bf;
-  if (!F.hfc.empty()) return SCE_OK;
-  for (int k = 0; k < b->R; ++k) {
-    const DevTables& D = b->tabs[k]->dev;
-    if (!D.nbq_low)
-      return sce_fail(SCE_EINVAL, "persistent sweeps need at most 64 neighbour slots; chain %d's stencil has %d", k,
-                      D.nb_max);
-    if (D.n_active > TREE_B * TREE_B || D.n_sites > TREE_B * TREE_B)
-      return sce_fail(SCE_EDIM, "persistent batched sweeps take chains of at most %d sites", TREE_B * TREE_B);
+  if (F.hfc.empty()) {
+    for (int k = 0; k < b->R; ++k) {
+      const DevTables& D = b->tabs[k]->dev;
+      if (!D.nbq_low)
+        return sce_fail(SCE_EINVAL, "persistent sweeps need at most 64 neighbour slots; chain %d's stencil has %d", k,
+                        D.nb_max);
+      if (D.n_active > TREE_B * TREE_B || D.n_sites > TREE_B * TREE_B)
+        return sce_fail(SCE_EDIM, "persistent batched sweeps take chains of at most %d sites", TREE_B * TREE_B);
+    }
+    F.hfc.assign((size_t)b->R, BFlowChain{});
+    int pos = 0, site = 0;
+    for (int k = 0; k < b->R; ++k) {
+      F.hfc[k].pos_off = pos;
+      F.hfc[k].site_off = site;
+      pos += b->tabs[k]->dev.n_active;
+      site += b->tabs[k]->dev.n_sites;
+    }
+    F.G = pos;
+    F.NS = site;
+    CHK(dalloc(b->mem, &F.done, (size_t)std::max(site, 1)));
+    CHK(dalloc(b->mem, &F.Ecur, (size_t)b->R));
+    CHK(dalloc(b->mem, &F.red, (size_t)b->R));
+    CHK(dalloc(b->mem, &F.ready, (size_t)b->R));
+    CHK(dalloc(b->mem, &F.pcfg, (size_t)b->R));
+    CHK(dalloc(b->mem, &F.pzr, (size_t)b->R));
+    CHK(dalloc(b->mem, &F.err, 4));
+    CHK(dalloc(b->mem, &F.fc, (size_t)b->R));
+    F.dirty = true;
   }
+  if (F.L == L) return SCE_OK;
   int wmax = 0;
   int32_t r = SCE_OK;
-  SCE_DISPATCH_LMAX_R(r, b->lmax, bflow_waves, b, &wmax);
+  SCE_DISPATCH_LMAX_R(r, b->lmax, bflow_waves, b, L, &wmax);
   CHK(r);
   if (wmax < b->R)
     return sce_fail(SCE_EINVAL, "persistent batched sweeps need one resident wave per chain: %d chains, %d waves", b->R,
                     wmax);
+  const int G = 64 / L;
+  std::vector<int> units;
+  std::vector<int> nu((size_t)b->R, 0);
+  for (int k = 0; k < b->R; ++k) {
+    const sce_tables* tb = b->tabs[k];
+    F.hfc[k].unit_off = (int)units.size();
+    for (int c = 0; c < tb->n_colors; ++c)
+      for (int q = tb->color_ptr[c]; q < tb->color_ptr[c + 1]; q += G) units.push_back(q);
+    nu[k] = (int)units.size() - F.hfc[k].unit_off;
+    units.push_back(tb->dev.n_active);
+    F.hfc[k].n_units = nu[k];
+  }
   std::vector<int> nw((size_t)b->R, 1);
-  auto load = [&](int k) {
-    const int n = std::max(b->tabs[k]->dev.n_active, 1);
-    return ((long long)n + nw[k] - 1) / nw[k];
-  };
+  auto load = [&](int k) { return ((long long)std::max(nu[k], 1) + nw[k] - 1) / nw[k]; };
   std::priority_queue<std::pair<long long, int>> pq;
   for (int k = 0; k < b->R; ++k) pq.push({load(k), -k});
   int used = b->R;
   while (used < wmax && !pq.empty()) {
     const int k = -pq.top().second;
     pq.pop();
-    if (nw[k] >= std::max(b->tabs[k]->dev.n_active, 1)) break;  // the busiest chain has a wave per position
+    if (nw[k] >= std::max(nu[k], 1)) break;  // the busiest chain has a wave per unit
     ++nw[k];
     ++used;
     pq.push({load(k), -k});
   }
-  F.hfc.assign((size_t)b->R, BFlowChain{});
-  int pos = 0, site = 0, wave = 0;
+  int wave = 0;
   for (int k = 0; k < b->R; ++k) {
-    F.hfc[k].pos_off = pos;
-    F.hfc[k].site_off = site;
     F.hfc[k].wave_lo = wave;
     F.hfc[k].wave_n = nw[k];
-    pos += b->tabs[k]->dev.n_active;
-    site += b->tabs[k]->dev.n_sites;
     wave += nw[k];
   }
-  F.G = pos;
-  F.NS = site;
   F.grid = wave;
-  CHK(dalloc(b->mem, &F.done, (size_t)site));
-  CHK(dalloc(b->mem, &F.Ecur, (size_t)b->R));
-  CHK(dalloc(b->mem, &F.red, (size_t)b->R));
-  CHK(dalloc(b->mem, &F.ready, (size_t)b->R));
-  CHK(dalloc(b->mem, &F.pcfg, (size_t)b->R));
-  CHK(dalloc(b->mem, &F.pzr, (size_t)b->R));
-  CHK(dalloc(b->mem, &F.err, 4));
+  HIPCHK(hipStreamSynchronize(b->stream));  // earlier launches may still read the tables
+  b->mem.release(F.units);
+  F.units = nullptr;
+  CHK(dupload(b->mem, &F.units, units.data(), units.size()));
+  F.L = L;
+  F.dirty = true;
   return SCE_OK;
 }
 
@@ -4389,11 +4513,12 @@ static int32_t bflow_setup(sce_batch* b) {
 static int32_t bflow_reserve(sce_batch* b, int cap, int mcap) {
   sce_batch::BFlow& F = b->bf;
   if (cap > F.cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
     for (void* p : {(void*)F.dE, (void*)F.acc, (void*)F.cnt, (void*)F.pcnt, (void*)F.Estart, (void*)F.part,
-                    (void*)F.ipart, (void*)F.fc})
+                    (void*)F.ipart})
       b->mem.release(p);
     F.dE = nullptr, F.acc = nullptr, F.cnt = nullptr, F.pcnt = nullptr, F.Estart = nullptr, F.part = nullptr;
-    F.ipart = nullptr, F.fc = nullptr;
+    F.ipart = nullptr;
     F.cap = 0;
     long long part = 0, ipart = 0;
     for (int k = 0; k < b->R; ++k) {
@@ -4412,26 +4537,43 @@ static int32_t bflow_reserve(sce_batch* b, int cap, int mcap) {
     CHK(dalloc(b->mem, &F.Estart, (size_t)cap * (size_t)b->R));
     CHK(dalloc(b->mem, &F.part, (size_t)part));
     CHK(dalloc(b->mem, &F.ipart, (size_t)ipart));
-    CHK(dupload(b->mem, &F.fc, F.hfc.data(), F.hfc.size()));
     F.cap = cap;
+    F.dirty = true;
   }
   if (mcap > F.mcap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
     b->mem.release(F.cfg);
     F.cfg = nullptr;
     F.mcap = 0;
     CHK(dalloc(b->mem, &F.cfg, (size_t)mcap * 3 * (size_t)std::max(F.NS, 1)));
     F.mcap = mcap;
   }
+  if (F.dirty) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(F.fc, F.hfc.data(), F.hfc.size() * sizeof(BFlowChain), hipMemcpyHostToDevice));
+    F.dirty = false;
+  }
+  return SCE_OK;
+}
+
+template <int LMAX, int L>
+static int32_t launch_bflow_l(sce_batch* b, BFlowArgs& fa) {
+  const void* f = reinterpret_cast<const void*>(&k_bflow<LMAX, L>);
+  void* args[] = {&fa};
+  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)b->bf.grid), dim3(64), args,
+                                    (unsigned)((size_t)(64 / L) * b->stride * sizeof(double)), b->stream));
   return SCE_OK;
 }
 
 template <int LMAX>
 static int32_t launch_bflow(sce_batch* b, BFlowArgs& fa) {
-  const void* f = reinterpret_cast<const void*>(&k_bflow<LMAX>);
-  void* args[] = {&fa};
-  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)b->bf.grid), dim3(64), args,
-                                    (unsigned)((size_t)b->stride * sizeof(double)), b->stream));
-  return SCE_OK;
+  const int L = b->bf.L;
+  if (L == 64) return launch_bflow_l<LMAX, 64>(b, fa);
+  if constexpr ((LMAX + 1) * (LMAX + 1) <= 32)
+    if (L == 32) return launch_bflow_l<LMAX, 32>(b, fa);
+  if constexpr ((LMAX + 1) * (LMAX + 1) <= 16)
+    if (L == 16) return launch_bflow_l<LMAX, 16>(b, fa);
+  return sce_fail(SCE_EINVAL, "persistent batched sweeps: no kernel for %d lanes at lmax %d", L, LMAX);
 }
 
 // n sweeps of every chain (sweeps i0 + 1 .. i0 + n of the phase) as one
@@ -4478,6 +4620,8 @@ static int32_t benqueue_flow(sce_batch* b, int n, long long i0, int64_t measure_
   fa.pzr = F.pzr;
   fa.part = F.part;
   fa.ipart = F.ipart;
+  fa.units = F.units;
+  fa.stride = b->stride;
   fa.err = F.err;
   fa.spin_limit = FLOW_SPIN_LIMIT;
   const long long init = std::max<long long>({(long long)F.NS, (long long)b->R * n, 1});
@@ -4548,8 +4692,7 @@ extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, i
   k_breset_obs<<<(b->R + 255) / 256, 256, 0, b->stream>>>(b->d_ch, b->R);
   const bool flow = b->persistent && nsweeps > 0;
   if (flow) {
-    if (lanes != 64) return sce_fail(SCE_EINVAL, "persistent sweeps walk with 64 lanes per site; got %d", lanes);
-    CHK(bflow_setup(b));
+    CHK(bflow_setup(b, lanes));
     HIPCHK(hipMemsetAsync(b->bf.err, 0, sizeof(int), b->stream));
     // segments end at renormalisation points and hold at most FLOW_SEG sweeps;
     // a renormalisation's sweep has its measurement and exchange after it

@@ -408,8 +408,6 @@ function gpu_run_pt(gH::GPUTiledHamiltonian; temperature = nothing, kT = nothing
     checkpoint === nothing || throw(ArgumentError(
         "checkpointed ladder runs of the HIP path are written by sce_amd.gpu_run_pt(checkpoint = …)"))
     lanes = hip_lanes(workgroupsize)
-    (persistent && lanes != 64) && throw(ArgumentError(
-        "persistent = true runs every rung in one launch with 64 lanes per site; got workgroupsize = $workgroupsize"))
     _check_observables(observables)
     _check_evaluables(observables, evaluables)
     # the reference's master-seed derivation order (gpu_pt.jl:17-29)

@@ -99,8 +99,8 @@ class GPUBatch:
 
     def set_persistent(self, on: bool) -> None:
         """One persistent launch per renormalisation segment for every chain,
-        exchange included (64 lanes per site); bit-identical to the per-colour
-        launches."""
+        exchange included; bit-identical to the per-colour launches at the same
+        lanes per site."""
         L.check(L.lib().sce_batch_set_persistent(self._h, 1 if on else 0))
 
     def set_counters(self, k: int, sweep_index: int, acc_metro: int, att_metro: int,
@@ -240,8 +240,8 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
     boundary); `resume_pt(path, gH)` continues one bit-identically.  As in the
     reference there is no end-of-run write.  `lanes="auto"` tunes lanes per
     site (pinned in the plan) and, unless `persistent` is given, whether the
-    ladder runs as one persistent launch per segment (64 lanes) or as
-    per-colour launches; the launch form never changes the results."""
+    ladder runs as one persistent launch per segment or as per-colour
+    launches; the launch form never changes the results."""
     kts = resolve_kt(temperature, kT)
     _validate_pt(kts, exchange_interval, sweeps_measure, measure_interval)
     plan = PTPlan(kts, int(exchange_interval), int(sweeps_therm), int(sweeps_measure),
@@ -250,8 +250,7 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
     if lanes == "auto":
         # tuned on a throwaway copy of the ladder, then pinned in the plan (and
         # so in every checkpoint and the result) like the reference's workgroupsize
-        cands = _BATCH_CANDIDATES if persistent is None else tuple(
-            (n, bool(persistent)) for n in ((64,) if persistent else (16, 32, 64)))
+        cands = _BATCH_CANDIDATES if persistent is None else tuple((n, bool(persistent)) for n in (16, 32, 64))
         plan.lanes, persistent = autotune_batch(lambda: _pt_batch(gH, plan, init), cands, sweeps=20)
     ck = _Checkpointer(checkpoint, checkpoint_interval) if checkpoint is not None else None
     b = _pt_batch(gH, plan, init)
@@ -358,16 +357,16 @@ def resume_pt(path: str, gH: GPUTiledHamiltonian, checkpoint: Optional[str] = "s
 
 
 # (lanes per site, persistent launch) forms the batch autotuner tries
-_BATCH_CANDIDATES = ((16, False), (32, False), (64, False), (64, True))
+_BATCH_CANDIDATES = ((16, False), (32, False), (64, False), (16, True), (32, True), (64, True))
 
 
 def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
                    timings: Optional[dict] = None):
     """The project autotuner for batches: times `sweeps` sweeps of a scratch
     batch built by `make_batch()` in each (lanes per site, persistent) form —
-    per-colour launches replayed as hipGraphs at 16/32/64 lanes (sites per
-    256-thread block = 256 / lanes), or one persistent launch per segment at 64
-    lanes — and returns the fastest as (lanes, persistent).  `lanes` is part of
+    per-colour launches replayed as hipGraphs, or one persistent launch per
+    segment, each at 16/32/64 lanes (sites per wavefront = 64 / lanes) — and
+    returns the fastest as (lanes, persistent).  `lanes` is part of
     the determinism scope, so tuning runs on a throwaway batch before the
     production run and the caller records the choice with the run; the launch
     form is not.  `timings` receives the seconds per sweep of every candidate

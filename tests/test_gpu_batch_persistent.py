@@ -12,7 +12,7 @@ import sce_amd as S
 from sce_amd import models as M
 from sce_amd.batch import GPUBatch
 from sce_amd._lib import SceError
-from fixtures import bcc_tet_small, rand_config
+from fixtures import bcc_tet_small, fixture_cases, rand_config
 
 pytestmark = pytest.mark.gpu
 
@@ -59,17 +59,20 @@ def _ladder(H, R, kts, step=0.6, seed0=11):
     return make
 
 
-@pytest.mark.parametrize("nsw, renorm, meas, exch", [
-    (12, 5, 1, 1),     # segments 5, 5, 2; the renormalised sweeps exchange after renormalising
-    (20, 7, 3, 2),
-    (140, 0, 2, 3),    # a 128-sweep segment (both words of the exchange mask), then 12
-    (9, 0, 0, 0),      # independent chains
+@pytest.mark.parametrize("nsw, renorm, meas, exch, lanes", [
+    (12, 5, 1, 1, 64),     # segments 5, 5, 2; the renormalised sweeps exchange after renormalising
+    (12, 5, 1, 1, 32),
+    (12, 5, 1, 1, 16),
+    (20, 7, 3, 2, 64),
+    (140, 0, 2, 3, 64),    # a 128-sweep segment (both words of the exchange mask), then 12
+    (140, 0, 2, 3, 16),
+    (9, 0, 0, 0, 32),      # independent chains
 ])
-def test_ladder_persistent_equals_per_colour(nsw, renorm, meas, exch):
+def test_ladder_persistent_equals_per_colour(nsw, renorm, meas, exch, lanes):
     H = S.TiledHamiltonian(*bcc_tet_small(6))
     R = 5
     make = _ladder(H, R, np.geomspace(0.001, 0.02, R))
-    kw = dict(nsweeps=nsw, lanes=64, renorm_interval=renorm, measure_interval=meas, exchange_interval=exch,
+    kw = dict(nsweeps=nsw, lanes=lanes, renorm_interval=renorm, measure_interval=meas, exchange_interval=exch,
               phase_done=0, phase_total=nsw)
     (r0, s0), (r1, s1) = _both(make, [kw])
     assert np.array_equal(r0[0], r1[0])
@@ -97,21 +100,24 @@ def test_split_calls_and_switching_paths():
     _assert_same(_snapshot(ref), _snapshot(b))
 
 
-def test_c3_fcc_four_shell_ladder_persistent():
+@pytest.mark.parametrize("lanes", [16, 32, 64])
+def test_c3_fcc_four_shell_ladder_persistent(lanes):
     """The C3 model form (fcc, body-2 l <= 2 to the 4th shell, 54 neighbour
-    slots) as a 6-rung ladder, exchanging after every sweep."""
+    slots: two or four per lane below 64 lanes) as a 6-rung ladder, exchanging
+    after every sweep."""
     na, terms = M.lattice_model("fcc", pair_shells=4, pair_lmax=2, seed=12)
     H = S.TiledHamiltonian(na, terms, (3, 3, 3))
     R = 6
     make = _ladder(H, R, np.geomspace(0.002, 0.03, R), seed0=41)
-    kw = dict(nsweeps=10, lanes=64, renorm_interval=4, measure_interval=1, exchange_interval=1, phase_done=0,
+    kw = dict(nsweeps=10, lanes=lanes, renorm_interval=4, measure_interval=1, exchange_interval=1, phase_done=0,
               phase_total=10)
     (r0, s0), (r1, s1) = _both(make, [kw])
     assert np.array_equal(r0[0], r1[0])
     _assert_same(s0, s1)
 
 
-def test_c3_full_size_ladder_persistent():
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_c3_full_size_ladder_persistent(lanes):
     """Four 6912-site fcc 12^3 replicas of the C3 model (14 reduction chunks
     per sweep, so the in-kernel energy runs both tree levels), three sweeps
     with exchanges, every sweep measured."""
@@ -119,14 +125,36 @@ def test_c3_full_size_ladder_persistent():
     H = S.TiledHamiltonian(na, terms, dims)
     assert H.n_sites == 6912
     make = _ladder(H, 4, np.array([0.004, 0.005, 0.006, 0.008]), seed0=77)
-    kw = dict(nsweeps=3, lanes=64, renorm_interval=0, measure_interval=1, exchange_interval=1, phase_done=0,
+    kw = dict(nsweeps=3, lanes=lanes, renorm_interval=0, measure_interval=1, exchange_interval=1, phase_done=0,
               phase_total=3)
     (r0, s0), (r1, s1) = _both(make, [kw])
     assert np.array_equal(r0[0], r1[0])
     _assert_same(s0, s1)
 
 
-def test_uneven_zeeman_batch_persistent():
+@pytest.mark.parametrize("lanes", [16, 32, 64])
+def test_fixture_models_as_batches_persistent(lanes):
+    """The ten fixture models (pairs, triplets, bodies of 4, Zeeman-only,
+    inactive sites, lattices) as independent chains, one batch per lmax."""
+    groups = {}
+    for k, (name, na, terms, dims, mm, fl) in enumerate(fixture_cases()):
+        H = S.TiledHamiltonian(na, terms, dims, magmoms=mm, field=fl)
+        groups.setdefault(H.lmax, []).append((k, H))
+    for chains in groups.values():
+        gHs = [S.GPUTiledHamiltonian(H) for _, H in chains]
+        sts = [S.ChainState(H, rand_config(H.n_sites, 7 + k), 0.5) for k, H in chains]
+
+        def make():
+            return GPUBatch(gHs, sts, [100 + k for k, _ in chains], [0.01 + 0.002 * k for k, _ in chains])
+        kw = dict(nsweeps=7, lanes=lanes, renorm_interval=3, measure_interval=2, exchange_interval=0,
+                  phase_done=0, phase_total=7)
+        (r0, s0), (r1, s1) = _both(make, [kw])
+        assert np.array_equal(r0[0], r1[0])
+        _assert_same(s0, s1)
+
+
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_uneven_zeeman_batch_persistent(lanes):
     """Uneven chains (sizes 125-1000 sites, per-chain fields and moments), no
     exchange: each chain advances on its own waves."""
     rng = np.random.default_rng(404)
@@ -143,19 +171,19 @@ def test_uneven_zeeman_batch_persistent():
 
     def make():
         return GPUBatch(gHs, states, seeds, kts)
-    kw = dict(nsweeps=14, lanes=64, renorm_interval=6, measure_interval=3, exchange_interval=0, phase_done=0,
+    kw = dict(nsweeps=14, lanes=lanes, renorm_interval=6, measure_interval=3, exchange_interval=0, phase_done=0,
               phase_total=14)
     (r0, s0), (r1, s1) = _both(make, [kw])
     assert np.array_equal(r0[0], r1[0])
     _assert_same(s0, s1)
 
 
-def test_persistent_batch_needs_64_lanes():
+def test_persistent_batch_lane_guard():
     H = S.TiledHamiltonian(*bcc_tet_small(2, dims=(2, 2, 2)))
     b = _ladder(H, 2, [0.01, 0.02])()
     b.set_persistent(True)
-    with pytest.raises(SceError, match="64 lanes"):
-        b.run(2, 32, 0, 0, 1, 0, 2)
+    with pytest.raises(SceError, match="lanes"):
+        b.run(2, 8, 0, 0, 1, 0, 2)
 
 
 def test_gpu_run_pt_persistent_equals_per_colour():
