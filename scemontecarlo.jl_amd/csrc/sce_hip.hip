@@ -3746,11 +3746,13 @@ __global__ __launch_bounds__(256) void k_bfinal(const BChain* __restrict__ ch, i
 
 // Adjacent-pair exchange on the device (gpu_pt.jl:115-140 schedule: pairs
 // parity, parity+2, ... each with one keyed uniform, then the parity flips).
+// The pairs of one parity share no chain, so thread j of the block takes
+// pairs parity + 2 j, parity + 2 (j + blockDim), ...; one block.
 __global__ void k_exchange(BChain* ch, int R, BatchDev* bd, const double* __restrict__ kts,
                            long long* sw_att, long long* sw_acc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const long long ex = bd->exch_count + 1;
-  for (int i = bd->parity; i < R - 1; i += 2) {
+  const int parity = bd->parity;
+  for (int i = parity + 2 * (int)threadIdx.x; i < R - 1; i += 2 * (int)blockDim.x) {
     const sce::u32x4 b = sce::philox_block(bd->exch_seed, i + 1, (int32_t)ex, SLOT_EXCH, 0u);
     const double u = sce::uniform01(b.x, b.y);
     ChainDev* sa = ch[i].st;
@@ -3771,8 +3773,11 @@ __global__ void k_exchange(BChain* ch, int R, BatchDev* bd, const double* __rest
       sw_acc[i] += 1;
     }
   }
-  bd->parity = 1 - bd->parity;
-  bd->exch_count = ex;
+  __syncthreads();  // every thread has read the exchange state
+  if (threadIdx.x == 0) {
+    bd->parity = 1 - parity;
+    bd->exch_count = ex;
+  }
 }
 
 __global__ void k_breset_obs(BChain* ch, int R) {
@@ -4369,7 +4374,7 @@ static int32_t benqueue_sweep(sce_batch* b, int L, int meas, int exch) {
   }
   k_bchunks<MODE_METRO><<<b->total_chunk[meas], 256, 0, s>>>(b->d_ch, b->d_seg_chunk[meas], b->nseg_chunk[meas], meas);
   k_bfinal<MODE_METRO><<<b->R, 256, 0, s>>>(b->d_ch, meas);
-  if (exch) k_exchange<<<1, 1, 0, s>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
+  if (exch) k_exchange<<<1, 256, 0, s>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
   return SCE_OK;
 }
 
@@ -4658,7 +4663,7 @@ static int32_t brenorm_then(sce_batch* b, bool meas, bool exch) {
     k_bchunks<MODE_MEASURE><<<b->total_meas, 256, 0, b->stream>>>(b->d_ch, b->d_seg_meas, b->nseg_meas, 1);
     k_bfinal<MODE_MEASURE><<<b->R, 256, 0, b->stream>>>(b->d_ch, 1);
   }
-  if (exch) k_exchange<<<1, 1, 0, b->stream>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
+  if (exch) k_exchange<<<1, 256, 0, b->stream>>>(b->d_ch, b->R, b->bd, b->kts, b->sw_att, b->sw_acc);
   return SCE_OK;
 }
 
