@@ -257,7 +257,14 @@ def da_kt_scan(args, S, H, gH, sur, snapshot, step, kts=(0.02, 0.025, 0.03, 0.03
             "note": "exact_walk_bound = 1 / stage-1 pass rate: the most a free surrogate could gain"}
 
 
-def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
+def _batch_candidates(form):
+    """Autotuner candidates of the batch legs' --batch-form: both launch forms,
+    per-colour launches only, or the persistent launch only."""
+    from sce_amd.batch import _BATCH_CANDIDATES
+    return tuple(c for c in _BATCH_CANDIDATES if form == "auto" or c[1] == (form == "persistent"))
+
+
+def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto"):
     """Batched parallel tempering: 64 replicas of fcc 12^3 (6912 spins, body-2 l<=2 to
     the 4th shell) on a geometric kT ladder spanning 0.2-2.0 J1, replica exchange
     on the device after every sweep.  The autotuner picks lanes per site and the
@@ -288,7 +295,7 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
     tune = {}
     # lanes per site and launch form (per-colour graphs or one persistent launch
     # per segment): scratch batches, before the run
-    lanes, pers = autotune_batch(make, sweeps=20, timings=tune)
+    lanes, pers = autotune_batch(make, _batch_candidates(form), sweeps=20, timings=tune)
     b = make()
     b.set_persistent(pers)
     steps = [0.6] * R
@@ -321,7 +328,7 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True):
                     "autotune keys: lanes per site, 'p' = one persistent launch per segment"}
 
 
-def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True):
+def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
     """Uneven batch: 256 chains, sc or bcc with N^3, N uniform in 6..20, each with
     its own seeded body-2 l<=2 (+ body-3 l=1 triangles on bcc) model, kT and a
     Zeeman term (2 mu_B moments, field 0-10 T along a random axis).  Chains run in
@@ -353,7 +360,7 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True):
             b.set_graph(graph)
             return b
         tune = {}
-        lanes, pers = autotune_batch(make, sweeps=10, timings=tune)
+        lanes, pers = autotune_batch(make, _batch_candidates(form), sweeps=10, timings=tune)
         b = make()
         b.set_persistent(pers)
         b.run(5, lanes, 0)
@@ -516,6 +523,9 @@ def main():
                     help="skip the Nd2Fe14B setup-file leg (nbody 3 at 4^3 and 8^3 cells)")
     ap.add_argument("--setup-model-only", action="store_true",
                     help="run only the Nd2Fe14B setup-file leg and print its JSON")
+    ap.add_argument("--batch-form", choices=("auto", "graph", "persistent"), default="auto",
+                    help="launch forms the batch legs' autotuner may pick: per-colour launches (graph), one "
+                         "persistent launch per segment, or either (auto)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every sweep's kernels directly instead of replaying hipGraphs")
     ap.add_argument("--da-rows", type=float, default=1e7, help="harvested training tuples")
@@ -549,8 +559,9 @@ def main():
     from sce_amd import models as M
 
     if args.batch_only:
-        print(json.dumps({"batched": {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph),
-                                      "uneven_batch": uneven_leg(S, M, graph=not args.no_graph)}}),
+        print(json.dumps({"batched": {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph, form=args.batch_form),
+                                      "uneven_batch": uneven_leg(S, M, graph=not args.no_graph,
+                                                                 form=args.batch_form)}}),
               flush=True)
         return
     if args.compound_only:
@@ -675,8 +686,8 @@ def main():
         if not args.no_da_scan:
             line["delayed_acceptance"]["kT_scan"] = da_kt_scan(args, S, H, gH, sur, snapshot, gst.step)
     if not args.no_batch:
-        line["batched"] = {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph),
-                           "uneven_batch": uneven_leg(S, M, graph=not args.no_graph)}
+        line["batched"] = {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph, form=args.batch_form),
+                           "uneven_batch": uneven_leg(S, M, graph=not args.no_graph, form=args.batch_form)}
     if not args.no_compound:
         line["compound"] = compound_leg(S, M)
     if not args.no_setup_model:
