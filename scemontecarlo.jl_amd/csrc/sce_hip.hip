@@ -98,6 +98,10 @@ struct DevTables {
   const uint8_t* active;
   const int32_t* walk_ptr;
   const WalkEntry* walk;
+  // the same walks, each padded with no-op entries to a multiple of WALK_PAD
+  // (what walk_delta reads: no bounds test or clamp per entry)
+  const int32_t* walkp_ptr;
+  const WalkEntry* walkp;
   const int32_t* nb_ptr;
   const int4* nb;
   const int32_t* nbq;  // per colour position: nb_max neighbour site ids (padded with the site's own)
@@ -358,26 +362,62 @@ __device__ __forceinline__ double fold_lanes(double a) {
 // measured 1.5x slower per sweep at 32^3.)
 // ---------------------------------------------------------------------------
 constexpr int WALK_UNROLL = 8;
+// Every sublattice walk is padded to a multiple of WALK_PAD entries, a whole
+// number of batches at every lane count, with no-op entries (weight 0, every
+// factor on the 1.0 slot).
+constexpr int WALK_PAD = WALK_UNROLL * 64;
 
+// The padded walks hold LDS byte offsets from the group's staged block Z: the
+// three factor slots (slot * 8) and the target row of dz, which every walk
+// kernel places right after the neighbour rows and the 1.0 slot
+// ((nb_max * nlm + 1 + tgt) * 8).  One SDWA add forms each LDS address from
+// the block's base and a 16-bit half of the entry.
+struct alignas(16) WalkWords {
+  double w;
+  unsigned f01, f2t;  // (f0 | f1 << 16), (f2 | tgt << 16), as byte offsets
+};
+typedef __attribute__((address_space(3))) const double lds_cdouble;
+__device__ __forceinline__ unsigned lds_add_lo(unsigned base, unsigned packed) {
+  unsigned r;
+  asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+      : "=v"(r) : "v"(base), "v"(packed));
+  return r;
+}
+__device__ __forceinline__ unsigned lds_add_hi(unsigned base, unsigned packed) {
+  unsigned r;
+  asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1"
+      : "=v"(r) : "v"(base), "v"(packed));
+  return r;
+}
+__device__ __forceinline__ double lds_at(unsigned addr) { return *(lds_cdouble*)(size_t)addr; }
+
+// The walk over a padded sublattice walk `wl` of Wp entries (Wp a multiple of
+// WALK_UNROLL * L).  Each lane's sum is the contract's: fma(w * p, dz[tgt], a)
+// over its entries in ascending order, skipping the terms whose product p is
+// zero.  The skip needs no test here: a lane's sum starts at +0 and never
+// becomes -0 (a sum of values of opposite sign that cancels exactly rounds to
+// +0; only an fma whose exact product is a negative number below half the
+// smallest subnormal could round to -0, which unit spins' rows never reach),
+// so adding the term of a zero product, (+-0) * dz, leaves the sum's bits
+// unchanged for finite rows.  The padding entries add 0 * 1 * dz[0] likewise.
+// Without the bounds test, the select and the index clamp, a batch is its
+// loads, the LDS gathers and four FP64 operations per entry.
 template <int L, int NLM>
-__device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, int W, int lane,
-                                             const double* Z, const double* dz) {
+__device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, int Wp, int lane,
+                                             const double* Z) {
+  const WalkWords* __restrict__ ww = reinterpret_cast<const WalkWords*>(wl) + lane;
+  const unsigned zb = (unsigned)(size_t)(lds_cdouble*)Z;
   double a = 0.0;
-  for (int base = 0; base < W; base += WALK_UNROLL * L) {
-    WalkEntry en[WALK_UNROLL];
-    // every load of the batch is issued before the first use: entries past the
-    // end are clamped to the last one and their terms discarded by a select
+  for (int base = 0; base < Wp; base += WALK_UNROLL * L) {
+    WalkWords en[WALK_UNROLL];
+    // every load of the batch is issued before the first use
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
-      const int i = base + u * L + lane;
-      en[u] = wl[i < W ? i : W - 1];
-    }
-#pragma unroll
-    for (int u = 0; u < WALK_UNROLL; ++u) {
-      const int i = base + u * L + lane;
-      const double p = (Z[en[u].f0] * Z[en[u].f1]) * Z[en[u].f2];
-      const double t = __builtin_fma(en[u].w * p, dz[en[u].tgt], a);
-      a = (i < W && p != 0.0) ? t : a;
+      const double p = (lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01))) *
+                       lds_at(lds_add_lo(zb, en[u].f2t));
+      a = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb, en[u].f2t)), a);
     }
   }
   return a;
@@ -412,9 +452,9 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
     if constexpr (L == 64) s = __builtin_amdgcn_readfirstlane(s);
     // independent of the staging: issued with it rather than after the barrier
     const int a0 = s % T.na;
-    const int w0 = T.walk_ptr[a0];
-    W = T.walk_ptr[a0 + 1] - w0;
-    wl = T.walk + w0;
+    const int w0 = T.walkp_ptr[a0];
+    W = T.walkp_ptr[a0 + 1] - w0;
+    wl = T.walkp + w0;
     RowDelta<L, NLM> rd;
     if constexpr (TEST) {
       rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
@@ -429,7 +469,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   }
   __syncthreads();
   double a = 0.0;
-  if (valid) a = walk_delta<L, NLM>(wl, W, lane, Z, dz);
+  if (valid) a = walk_delta<L, NLM>(wl, W, lane, Z);
   a = fold_lanes<L>(a);
   __syncthreads();  // the block's LDS is reused by the next site group
   if (!valid) return;
@@ -1137,8 +1177,8 @@ __device__ __forceinline__ bool flow_update(const DevTables& T, double* config, 
   if (lane < NLM) dz[lane] = p_new - p_cur;
   next();
   __syncthreads();
-  const int w0 = T.walk_ptr[atom];
-  double dsum = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
+  const int w0 = T.walkp_ptr[atom];
+  double dsum = walk_delta<L, NLM>(T.walkp + w0, T.walkp_ptr[atom + 1] - w0, lane, Z);
   dsum = fold_lanes<L>(dsum);
   int accept = 0;
   if (lane == 0) accept = (dsum <= 0.0 || u_acc < sce::det_exp(-beta * dsum)) ? 1 : 0;
@@ -1683,6 +1723,13 @@ static int32_t dupload1(DevBuf& b, T** p, const T* src, size_t n) {
   return SCE_OK;
 }
 
+// Rows of a measurement buffer that must hold `want`: at least 4096 and at
+// least twice the old capacity, so that captured sweeps (which hold the
+// buffer's address) are rarely dropped for a new one.
+static size_t obs_rows_for(size_t want, size_t old_cap) {
+  return std::max(want, std::max<size_t>(4096, 2 * old_cap));
+}
+
 struct sce_tables {
   DevBuf mem;
   DevTables dev{};
@@ -2086,6 +2133,33 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   WalkEntry* pw;
   if ((r = dupload(t->mem, &pw, L.walk.data(), L.walk.size())) != SCE_OK) return fail(r);
   D.walk = pw;
+  {
+    // the padded copy the walks read, in LDS byte offsets from the staged
+    // block: each sublattice's entries, then no-op entries (weight 0, factors
+    // on its 1.0 slot, target row 0) up to a multiple of WALK_PAD.  A block
+    // beyond 16-bit offsets (64 KB) cannot be staged by any walk launch
+    // (check_lds), which then finds no padded walk.
+    const size_t dz0 = (size_t)L.nb_max * d->nlm + 1;
+    if ((dz0 + d->nlm) * sizeof(double) <= 65535) {
+      std::vector<int32_t> pp((size_t)L.na + 1, 0);
+      std::vector<WalkEntry> wp;
+      auto bytes = [&](const WalkEntry& e) {
+        return WalkEntry{e.w, (uint16_t)(8 * e.f0), (uint16_t)(8 * e.f1), (uint16_t)(8 * e.f2),
+                         (uint16_t)(8 * (dz0 + e.tgt))};
+      };
+      for (int a = 0; a < L.na; ++a) {
+        pp[a] = (int32_t)wp.size();
+        for (int i = L.walk_ptr[a]; i < L.walk_ptr[a + 1]; ++i) wp.push_back(bytes(L.walk[i]));
+        const uint16_t one = (uint16_t)((L.nb_ptr[a + 1] - L.nb_ptr[a]) * d->nlm);
+        while (wp.size() % WALK_PAD) wp.push_back(bytes(WalkEntry{0.0, one, one, one, 0}));
+      }
+      pp[L.na] = (int32_t)wp.size();
+      if ((r = dupload(t->mem, &p32, pp.data(), pp.size())) != SCE_OK) return fail(r);
+      D.walkp_ptr = p32;
+      if ((r = dupload(t->mem, &pw, wp.data(), wp.size())) != SCE_OK) return fail(r);
+      D.walkp = pw;
+    }
+  }
   if ((r = dupload(t->mem, &p32, L.nb_ptr.data(), L.nb_ptr.size())) != SCE_OK) return fail(r);
   D.nb_ptr = p32;
   int4* pn;
@@ -2236,7 +2310,7 @@ static int32_t check_lanes(int L) {
 }
 
 static int32_t check_lds(const DevTables& D, int L) {
-  if (walk_lds_bytes(D, L) > 65536)
+  if (walk_lds_bytes(D, L) > 65536 || !D.walkp)
     return sce_fail(SCE_EINVAL, "neighbour stencil of %d sites needs %zu B of LDS per block at "
                     "lanes = %d; use more lanes", D.nb_max, walk_lds_bytes(D, L), L);
   return SCE_OK;
@@ -3099,12 +3173,15 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   if (c->sweeps_done + nsweeps >= 2147483647LL - 1)
     return sce_fail(SCE_ERANGE_COUNTER, "sweep_index would exhaust the 32-bit RNG counter word");
   if (want_obs > 0 && (size_t)want_obs > c->obs_cap) {
+    // captured sweeps hold the row buffer's address, so a new buffer drops
+    // them: grow geometrically, from a size most calls never exceed
+    const size_t cap = obs_rows_for((size_t)want_obs, c->obs_cap);
     double* p;
-    CHK(dalloc(c->mem, &p, 4 * (size_t)want_obs));
+    CHK(dalloc(c->mem, &p, 4 * cap));
     c->drop_graphs();
     c->mem.release(c->obs);
     c->obs = p;
-    c->obs_cap = (size_t)want_obs;
+    c->obs_cap = cap;
   }
   CHK(set_beta(c, beta));
   // reset the observation counter
@@ -4684,14 +4761,15 @@ extern "C" int32_t sce_batch_run(sce_batch* b, int64_t nsweeps, int32_t lanes, i
   if (want > 0 && (!obs || obs_cap < want))
     return sce_fail(SCE_EDIM, "obs holds %lld rows per chain; %lld needed", (long long)obs_cap, (long long)want);
   if (want > 0 && (size_t)want > b->obs_cap) {
+    const size_t cap = obs_rows_for((size_t)want, b->obs_cap);
     double* p;
-    CHK(dalloc(b->mem, &p, 4 * (size_t)want * (size_t)b->R));
+    CHK(dalloc(b->mem, &p, 4 * cap * (size_t)b->R));
     b->drop_graphs();
     b->mem.release(b->obs);
     b->obs = p;
-    b->obs_cap = (size_t)want;
+    b->obs_cap = cap;
     CHK(bsync_descriptors(b));
-    for (int k = 0; k < b->R; ++k) b->h_ch[k].obs = p + 4 * (size_t)want * (size_t)k;
+    for (int k = 0; k < b->R; ++k) b->h_ch[k].obs = p + 4 * cap * (size_t)k;
     HIPCHK(hipMemcpy(b->d_ch, b->h_ch.data(), b->h_ch.size() * sizeof(BChain), hipMemcpyHostToDevice));
   }
   k_breset_obs<<<(b->R + 255) / 256, 256, 0, b->stream>>>(b->d_ch, b->R);
@@ -5210,8 +5288,8 @@ __device__ __forceinline__ void da_body(DevTables T, double* config, double* zro
   const bool walk = valid && pass;
   double a = 0.0;
   if (walk) {
-    const int w0 = T.walk_ptr[atom];
-    a = walk_delta<L, NLM>(T.walk + w0, T.walk_ptr[atom + 1] - w0, lane, Z, dz);
+    const int w0 = T.walkp_ptr[atom];
+    a = walk_delta<L, NLM>(T.walkp + w0, T.walkp_ptr[atom + 1] - w0, lane, Z);
   }
   a = fold_lanes<L>(a);
   if (!valid) return;
@@ -6034,12 +6112,13 @@ extern "C" int32_t sce_da_run_sweeps(sce_chain* c, sce_surrogate* s, double beta
   if (want > 0 && (!obs || obs_cap < want)) return sce_fail(SCE_EDIM, "obs buffer too small");
   if (c->sweeps_done + nsweeps >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
   if (want > 0 && (size_t)want > c->obs_cap) {
+    const size_t cap = obs_rows_for((size_t)want, c->obs_cap);
     double* p;
-    CHK(dalloc(c->mem, &p, 4 * (size_t)want));
+    CHK(dalloc(c->mem, &p, 4 * cap));
     c->drop_graphs();
     c->mem.release(c->obs);
     c->obs = p;
-    c->obs_cap = (size_t)want;
+    c->obs_cap = cap;
   }
   CHK(set_beta(c, beta));
   {
