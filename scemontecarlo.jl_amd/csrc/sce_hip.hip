@@ -102,6 +102,9 @@ struct DevTables {
   // (what walk_delta reads: no bounds test or clamp per entry)
   const int32_t* walkp_ptr;
   const WalkEntry* walkp;
+  // per sublattice: the leading entries of its padded walk with one factor
+  // (the others on the 1.0 slot), and with at most two
+  const int2* walkp_pre;
   const int32_t* nb_ptr;
   const int4* nb;
   const int32_t* nbq;  // per colour position: nb_max neighbour site ids (padded with the site's own)
@@ -402,15 +405,41 @@ __device__ __forceinline__ double lds_at(unsigned addr) { return *(lds_cdouble*)
 // unchanged for finite rows.  The padding entries add 0 * 1 * dz[0] likewise.
 // Without the bounds test, the select and the index clamp, a batch is its
 // loads, the LDS gathers and four FP64 operations per entry.
+// Batches wholly inside the walk's leading one-factor (pair) entries, then
+// its two-factor entries, skip the gathers of the 1.0 slot and the products
+// with it: x * 1.0 is x exactly, so p and every sum keep their bits.
 template <int L, int NLM>
-__device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, int Wp, int lane,
+__device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, int Wp, int2 pre, int lane,
                                              const double* Z) {
+  constexpr int B = WALK_UNROLL * L;
   const WalkWords* __restrict__ ww = reinterpret_cast<const WalkWords*>(wl) + lane;
   const unsigned zb = (unsigned)(size_t)(lds_cdouble*)Z;
+  const int n1 = pre.x - pre.x % B, n2 = pre.y - pre.y % B;
   double a = 0.0;
-  for (int base = 0; base < Wp; base += WALK_UNROLL * L) {
+  int base = 0;
+  for (; base < n1; base += B) {
     WalkWords en[WALK_UNROLL];
     // every load of the batch is issued before the first use
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) {
+      const double p = lds_at(lds_add_lo(zb, en[u].f01));
+      a = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb, en[u].f2t)), a);
+    }
+  }
+  for (; base < n2; base += B) {
+    WalkWords en[WALK_UNROLL];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) {
+      const double p = lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01));
+      a = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb, en[u].f2t)), a);
+    }
+  }
+  for (; base < Wp; base += B) {
+    WalkWords en[WALK_UNROLL];
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
 #pragma unroll
@@ -446,6 +475,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   double* dz = Z + T.nb_max * NLM + 1;
   int s = 0, atom = 0, W = 0;
   const WalkEntry* wl = nullptr;
+  int2 pre = make_int2(0, 0);
   double u_acc = 0.0;
   if (valid) {
     s = TEST ? test_sites[q] : T.color_sites[q];
@@ -455,6 +485,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
     const int w0 = T.walkp_ptr[a0];
     W = T.walkp_ptr[a0 + 1] - w0;
     wl = T.walkp + w0;
+    pre = T.walkp_pre[a0];
     RowDelta<L, NLM> rd;
     if constexpr (TEST) {
       rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
@@ -469,7 +500,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   }
   __syncthreads();
   double a = 0.0;
-  if (valid) a = walk_delta<L, NLM>(wl, W, lane, Z);
+  if (valid) a = walk_delta<L, NLM>(wl, W, pre, lane, Z);
   a = fold_lanes<L>(a);
   __syncthreads();  // the block's LDS is reused by the next site group
   if (!valid) return;
@@ -1178,7 +1209,7 @@ __device__ __forceinline__ bool flow_update(const DevTables& T, double* config, 
   next();
   __syncthreads();
   const int w0 = T.walkp_ptr[atom];
-  double dsum = walk_delta<L, NLM>(T.walkp + w0, T.walkp_ptr[atom + 1] - w0, lane, Z);
+  double dsum = walk_delta<L, NLM>(T.walkp + w0, T.walkp_ptr[atom + 1] - w0, T.walkp_pre[atom], lane, Z);
   dsum = fold_lanes<L>(dsum);
   int accept = 0;
   if (lane == 0) accept = (dsum <= 0.0 || u_acc < sce::det_exp(-beta * dsum)) ? 1 : 0;
@@ -1560,6 +1591,9 @@ __global__ void k_set_params(ChainDev* st, double beta, double step, int set_ste
   st->beta = beta;
   if (set_step) st->step = step;
 }
+
+// a call's measurement rows start at row 0 (in stream order, no host round trip)
+__global__ void k_reset_nobs(ChainDev* st) { st->n_obs = 0; }
 
 // ---------------------------------------------------------------------------
 // diagnostics kernels
@@ -2142,6 +2176,7 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
     const size_t dz0 = (size_t)L.nb_max * d->nlm + 1;
     if ((dz0 + d->nlm) * sizeof(double) <= 65535) {
       std::vector<int32_t> pp((size_t)L.na + 1, 0);
+      std::vector<int2> pre((size_t)L.na);
       std::vector<WalkEntry> wp;
       auto bytes = [&](const WalkEntry& e) {
         return WalkEntry{e.w, (uint16_t)(8 * e.f0), (uint16_t)(8 * e.f1), (uint16_t)(8 * e.f2),
@@ -2152,8 +2187,18 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
         for (int i = L.walk_ptr[a]; i < L.walk_ptr[a + 1]; ++i) wp.push_back(bytes(L.walk[i]));
         const uint16_t one = (uint16_t)((L.nb_ptr[a + 1] - L.nb_ptr[a]) * d->nlm);
         while (wp.size() % WALK_PAD) wp.push_back(bytes(WalkEntry{0.0, one, one, one, 0}));
+        int n1 = 0, n2 = 0;
+        const int n = (int)wp.size() - pp[a];
+        const uint16_t one8 = (uint16_t)(8 * one);
+        while (n1 < n && wp[pp[a] + n1].f1 == one8 && wp[pp[a] + n1].f2 == one8) ++n1;
+        n2 = n1;
+        while (n2 < n && wp[pp[a] + n2].f2 == one8) ++n2;
+        pre[a] = make_int2(n1, n2);
       }
       pp[L.na] = (int32_t)wp.size();
+      int2* pq;
+      if ((r = dupload(t->mem, &pq, pre.data(), pre.size())) != SCE_OK) return fail(r);
+      D.walkp_pre = pq;
       if ((r = dupload(t->mem, &p32, pp.data(), pp.size())) != SCE_OK) return fail(r);
       D.walkp_ptr = p32;
       if ((r = dupload(t->mem, &pw, wp.data(), wp.size())) != SCE_OK) return fail(r);
@@ -3185,12 +3230,7 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
   }
   CHK(set_beta(c, beta));
   // reset the observation counter
-  {
-    ChainDev h;
-    CHK(read_state(c, &h));
-    h.n_obs = 0;
-    HIPCHK(hipMemcpy(c->st, &h, sizeof h, hipMemcpyHostToDevice));
-  }
+  k_reset_nobs<<<1, 1, 0, c->stream>>>(c->st);
   c->timing = c->timing_enabled;
   c->ev_used = 0;
   hipEvent_t t0 = next_event(c);
@@ -5289,7 +5329,7 @@ __device__ __forceinline__ void da_body(DevTables T, double* config, double* zro
   double a = 0.0;
   if (walk) {
     const int w0 = T.walkp_ptr[atom];
-    a = walk_delta<L, NLM>(T.walkp + w0, T.walkp_ptr[atom + 1] - w0, lane, Z);
+    a = walk_delta<L, NLM>(T.walkp + w0, T.walkp_ptr[atom + 1] - w0, T.walkp_pre[atom], lane, Z);
   }
   a = fold_lanes<L>(a);
   if (!valid) return;
