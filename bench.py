@@ -114,6 +114,7 @@ def train_surrogate(args, S, H, gH, gst, beta):
     t0 = time.perf_counter()
     losses = sur.train_all(epochs=args.da_epochs, batch=args.da_batch, lr=args.da_lr)
     t_train = time.perf_counter() - t0
+    sur.set_int8(args.da_stage1 == "int8")
     return sur, {"harvest_tuples": n_harvest * H.n_active, "harvest_s": t_harvest, "train_s": t_train,
                  "train_loss_last_epoch": {str(a): float(l[-1]) for a, l in losses.items()}}
 
@@ -150,6 +151,22 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, plain):
     from sce_amd import surrogate as SG
 
     obs_plain, wall_plain = plain["obs"], plain["wall_s"]
+    # the other stage-1 precision, timed over a shorter run from the same state
+    other = "fp32" if args.da_stage1 == "int8" else "int8"
+    sur.set_int8(other == "int8")
+    ost = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=777)
+    ost.set_graph(not args.no_graph)
+    SG.gpu_da_run_sweeps(ost, sur, None, beta, 8, renorm_interval=1000, measure_interval=1)
+    ost.reset_counters()
+    n_other = min(args.ess_sweeps, 4000)
+    t0 = time.perf_counter()
+    SG.gpu_da_run_sweeps(ost, sur, None, beta, n_other, renorm_interval=1000, measure_interval=1)
+    w_other = time.perf_counter() - t0
+    oi = ost.info()
+    other_leg = {"sweeps": n_other, "ms_per_step": 1e3 * w_other / n_other,
+                 "stage1_pass": oi.acc_stage1 / max(oi.att_metro, 1),
+                 "stage2_acceptance": oi.acc_metro / max(oi.acc_stage1, 1)}
+    sur.set_int8(args.da_stage1 == "int8")
     dst = S.GPUChainState(gH, S.ChainState(H, snapshot, step), seed=777)
     dst.set_graph(not args.no_graph)
     SG.gpu_da_run_sweeps(dst, sur, None, beta, 8, renorm_interval=1000, measure_interval=1)
@@ -195,8 +212,11 @@ def da_leg(args, S, H, gH, sur, train_info, snapshot, step, beta, plain):
         "stage2_acceptance": inf.acc_metro / max(inf.acc_stage1, 1),
         "observables_vs_plain": agree,
         **train_info,
-        "surrogate": "per-sublattice MLP F->64->64->1 (tanh) + linear path, FP32; "
+        "surrogate": "per-sublattice MLP F->64->64->1 (tanh) + linear path, trained in FP32; "
                      f"F = {[sur.info(a)['F'] for a in sur.active_sublattices()]}",
+        "stage1": args.da_stage1 + (" (int8 descriptor and weights, integer dot products on the matrix cores)"
+                                    if args.da_stage1 == "int8" else " (FP32 fused multiply-add chains)"),
+        f"stage1_{other}": other_leg,
     }
 
 
@@ -533,6 +553,9 @@ def main():
                     help="skip the delayed-acceptance kT scan (5 kT x (2k + 2 x 8k) sweeps)")
     ap.add_argument("--ess-sweeps", type=int, default=20000,
                     help="sweeps of each chain in the delayed-acceptance vs plain comparison")
+    ap.add_argument("--da-stage1", choices=("int8", "fp32"), default="int8",
+                    help="precision of the surrogate stage of the delayed-acceptance legs (the other one "
+                         "is timed beside it)")
     ap.add_argument("--da-epochs", type=int, default=2)
     ap.add_argument("--da-batch", type=int, default=4096)
     ap.add_argument("--da-lr", type=float, default=1e-3)

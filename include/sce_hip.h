@@ -285,6 +285,15 @@ int32_t sce_sur_info(const sce_surrogate* s, int32_t sub, int32_t* F, int64_t* n
                      double* scale, int64_t* n_rows);
 int32_t sce_sur_get_params(sce_surrogate* s, int32_t sub, float* out);
 int32_t sce_sur_set_params(sce_surrogate* s, int32_t sub, const float* in, double scale);
+/* Stage 1 of later delayed-acceptance sweeps on the int8 networks (on != 0:
+ * quantised descriptor and weights, exact integer dot products on the matrix
+ * cores) or on the FP32 networks (0, the default).  Either way the exact FP64
+ * dE decides.  No reference counterpart (the surrogate is new). */
+int32_t sce_sur_set_int8(sce_surrogate* s, int32_t on);
+/* The int8 network of sublattice `sub` as floats: V1 (64 x F), r1 (64), b1 (64),
+ * V2 (64 x 64), r2 (64), b2 (64), w3 (64), v (F), input scales (F); refreshed
+ * whenever the FP32 parameters change (set_params, training). */
+int32_t sce_sur_get_qparams(sce_surrogate* s, int32_t sub, float* out);
 /* nsweeps plain Metropolis sweeps of chain c (the same chain evolution as
  * sce_run_sweeps) that also store (phi(e), phi(e'), exact dE) of every attempt
  * as training rows on the device. */

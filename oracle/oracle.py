@@ -85,6 +85,8 @@ def lib():
             "ora_cpu_counters": (None, [P, P]),
             "ora_sur_features": (C.c_int, [P, P, C.c_int, P, P]),
             "ora_sur_delta": (d, [C.c_int, P, d, P, P]),
+            "ora_sur_delta_q": (d, [C.c_int, P, d, P, P]),
+            "ora_da_sweep_q": (i64, [P, P, P, P, d, d, u64, i32, P, P, P, P, P]),
             "ora_test_tanhf": (C.c_float, [C.c_float]),
             "ora_da_sweep": (i64, [P, P, P, P, d, d, u64, i32, P, P, P, P, P]),
         }
@@ -179,6 +181,14 @@ def sur_delta(F, params, scale, phiA, phiB):
     a = np.ascontiguousarray(phiA, np.float32)
     b = np.ascontiguousarray(phiB, np.float32)
     return lib().ora_sur_delta(F, _p(p), scale, _p(a), _p(b))
+
+
+def sur_delta_q(F, qparams, scale, phiA, phiB):
+    """The int8 surrogate difference (ora_sur_delta_q) of one site."""
+    p = np.ascontiguousarray(qparams, np.float32)
+    a = np.ascontiguousarray(phiA, np.float32)
+    b = np.ascontiguousarray(phiB, np.float32)
+    return lib().ora_sur_delta_q(F, _p(p), scale, _p(a), _p(b))
 
 
 def delta_energy(c, zold, znew):
@@ -297,8 +307,9 @@ class Tiled:
         F = lib().ora_sur_features(self._h, _p(cfg), s, _p(ee), _p(out))
         return out[:F]
 
-    def da_sweep(self, config, zrows, energy, beta, step, seed, sweep, params, scales):
-        """params: list (per sublattice) of float32 parameter vectors."""
+    def da_sweep(self, config, zrows, energy, beta, step, seed, sweep, params, scales, int8=False):
+        """params: list (per sublattice) of float32 parameter vectors (with int8:
+        the quantised network blobs of ``Surrogate.get_qparams``)."""
         e = C.c_double(energy)
         keep = [np.ascontiguousarray(p, np.float32) for p in params]
         ptrs = (C.c_void_p * len(keep))(*[k.ctypes.data for k in keep])
@@ -306,8 +317,9 @@ class Tiled:
         dE = np.zeros(max(self.n_active, 1))
         acc = np.zeros(max(self.n_active, 1), np.int32)
         n1 = C.c_int64()
-        n = lib().ora_da_sweep(self._h, _p(config), _p(zrows), C.byref(e), beta, step, seed, sweep,
-                               ptrs, _p(sc), _p(dE), _p(acc), C.byref(n1))
+        f = lib().ora_da_sweep_q if int8 else lib().ora_da_sweep
+        n = f(self._h, _p(config), _p(zrows), C.byref(e), beta, step, seed, sweep,
+              ptrs, _p(sc), _p(dE), _p(acc), C.byref(n1))
         return int(n), int(n1.value), e.value
 
     def renormalize(self, config, zrows, energy):

@@ -1412,12 +1412,83 @@ double ora_sur_delta(int F, const float* params, double scale, const float* phiA
   return ((double)tB[0] - (double)tA[0]) * scale;
 }
 
+/* The int8 surrogate (qp: the quantised network blob, sce_sur_get_qparams:
+ * V1 (64 x F), r1, b1, V2 (64 x 64), r2, b2, w3 (64 each), v (F), input
+ * scales (F)).  Inputs u_f = clamp(rint(phi_f * s_f), -127, 127); a layer's
+ * pre-activation fmaf((float)acc, r_j, b_j) of the exact integer dot product
+ * acc; layer 1's activations enter layer 2 as rint(127 tanh(z)); the head is
+ * ora_sur_delta's. */
+double ora_sur_delta_q(int F, const float* qp, double scale, const float* phiA, const float* phiB) {
+  const float* V1 = qp;
+  const float* r1 = V1 + SUR_H * F;
+  const float* b1 = r1 + SUR_H;
+  const float* V2 = b1 + SUR_H;
+  const float* r2 = V2 + SUR_H * SUR_H;
+  const float* b2 = r2 + SUR_H;
+  const float* w3 = b2 + SUR_H;
+  const float* v = w3 + SUR_H;
+  const float* qin = v + F;
+  int uA[512], uB[512], hA[SUR_H], hB[SUR_H];
+  for (int f = 0; f < F; ++f) {
+    float xa = rintf(phiA[f] * qin[f]), xb = rintf(phiB[f] * qin[f]);
+    uA[f] = (int)fminf(127.0f, fmaxf(-127.0f, xa));
+    uB[f] = (int)fminf(127.0f, fmaxf(-127.0f, xb));
+  }
+  for (int j = 0; j < SUR_H; ++j) {
+    int64_t aA = 0, aB = 0;
+    for (int f = 0; f < F; ++f) {
+      aA += (int64_t)V1[j * F + f] * uA[f];
+      aB += (int64_t)V1[j * F + f] * uB[f];
+    }
+    hA[j] = (int)rintf(127.0f * ora_tanhf(fmaf((float)aA, r1[j], b1[j])));
+    hB[j] = (int)rintf(127.0f * ora_tanhf(fmaf((float)aB, r1[j], b1[j])));
+  }
+  float tA[SUR_H], tB[SUR_H];
+  for (int j = 0; j < SUR_H; ++j) {
+    int64_t aA = 0, aB = 0;
+    for (int k = 0; k < SUR_H; ++k) {
+      aA += (int64_t)V2[j * SUR_H + k] * hA[k];
+      aB += (int64_t)V2[j * SUR_H + k] * hB[k];
+    }
+    float oA = w3[j] * ora_tanhf(fmaf((float)aA, r2[j], b2[j]));
+    float oB = w3[j] * ora_tanhf(fmaf((float)aB, r2[j], b2[j]));
+    float sA = 0.0f, sB = 0.0f;
+    for (int f = j; f < F; f += SUR_H) {
+      sA = sA + v[f] * phiA[f];
+      sB = sB + v[f] * phiB[f];
+    }
+    tA[j] = oA + sA;
+    tB[j] = oB + sB;
+  }
+  for (int off = SUR_H / 2; off >= 1; off >>= 1)
+    for (int l = 0; l < off; ++l) { tA[l] = tA[l] + tA[l + off]; tB[l] = tB[l] + tB[l + off]; }
+  return ((double)tB[0] - (double)tA[0]) * scale;
+}
+
 /* One keyed delayed-acceptance sweep (device contract, 64 lanes).  params[a],
  * F[a], scale[a] per sublattice a = (site-1) % n_cell_atoms.  Returns nacc;
  * *n_stage1 = proposals that passed the surrogate stage. */
+static int64_t da_sweep(const ora_tiled* t, double* config, double* zrows, double* energy, double beta, double step,
+                        uint64_t seed, int32_t sweep, const float* const* params, const double* scale, double* dE_q,
+                        int32_t* acc_q, int64_t* n_stage1, int quant);
+
 int64_t ora_da_sweep(const ora_tiled* t, double* config, double* zrows, double* energy, double beta,
                      double step, uint64_t seed, int32_t sweep, const float* const* params,
                      const double* scale, double* dE_q, int32_t* acc_q, int64_t* n_stage1) {
+  return da_sweep(t, config, zrows, energy, beta, step, seed, sweep, params, scale, dE_q, acc_q, n_stage1, 0);
+}
+
+/* The same sweep with stage 1 on the int8 networks (qparams[a]: blobs of
+ * ora_sur_delta_q). */
+int64_t ora_da_sweep_q(const ora_tiled* t, double* config, double* zrows, double* energy, double beta,
+                       double step, uint64_t seed, int32_t sweep, const float* const* qparams,
+                       const double* scale, double* dE_q, int32_t* acc_q, int64_t* n_stage1) {
+  return da_sweep(t, config, zrows, energy, beta, step, seed, sweep, qparams, scale, dE_q, acc_q, n_stage1, 1);
+}
+
+static int64_t da_sweep(const ora_tiled* t, double* config, double* zrows, double* energy, double beta, double step,
+                        uint64_t seed, int32_t sweep, const float* const* params, const double* scale, double* dE_q,
+                        int32_t* acc_q, int64_t* n_stage1, int quant) {
   int nlm = t->nlm;
   double znew[49], e2[3], u;
   float phiA[512], phiB[512];
@@ -1436,7 +1507,8 @@ int64_t ora_da_sweep(const ora_tiled* t, double* config, double* zrows, double* 
     int a = (s - 1) % t->n_cell_atoms;
     int F = ora_sur_features(t, config, s, e, phiA);
     ora_sur_features(t, config, s, e2, phiB);
-    double dEs = ora_sur_delta(F, params[a], scale[a], phiA, phiB);
+    double dEs = quant ? ora_sur_delta_q(F, params[a], scale[a], phiA, phiB)
+                       : ora_sur_delta(F, params[a], scale[a], phiA, phiB);
     dE_q[q] = 0.0; acc_q[q] = 0;
     if (!(dEs <= 0.0 || u < ora_exp(-beta * dEs))) continue;
     ++*n_stage1;

@@ -88,6 +88,10 @@ double ora_renormalize(const ora_tiled* t, double* config, double* zrows, double
 /* ---- learned delayed acceptance (an extension of the library) ---- */
 int ora_sur_features(const ora_tiled* t, const double* config, int s, const double* e, float* out);
 double ora_sur_delta(int F, const float* params, double scale, const float* phiA, const float* phiB);
+double ora_sur_delta_q(int F, const float* qparams, double scale, const float* phiA, const float* phiB);
+int64_t ora_da_sweep_q(const ora_tiled* t, double* config, double* zrows, double* energy, double beta,
+                       double step, uint64_t seed, int32_t sweep, const float* const* qparams,
+                       const double* scale, double* dE_q, int32_t* acc_q, int64_t* n_stage1);
 int64_t ora_da_sweep(const ora_tiled* t, double* config, double* zrows, double* energy, double beta,
                      double step, uint64_t seed, int32_t sweep, const float* const* params,
                      const double* scale, double* dE_q, int32_t* acc_q, int64_t* n_stage1);

@@ -139,6 +139,13 @@ struct GradTables {
 };
 
 constexpr int TPB = 256;      // threads per block of the walk kernels
+
+// int8 stage-1 inference: descriptor features are quantised as rint(x * s)
+// with s = 127 for features bounded by 1 and 42 for the four-body pairing sums
+// (bounded by 3); each layer's weights per output unit j as
+// rint(w / s_in * 127 / t_j), t_j the largest |w / s_in| of the unit.
+constexpr float SUR_QIN_UNIT = 127.0f, SUR_QIN_PAIRS = 42.0f;
+__host__ __device__ inline int sur_fpq(int F) { return (F + 63) / 64 * 64; }
 constexpr int TREE_B = 512;   // elements per tree chunk
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
@@ -1772,6 +1779,7 @@ struct sce_tables {
   int max_class = 0;
   int fin_col = -1;  // the smallest non-empty colour class: its launch carries a deferred final
   std::vector<int> sub_F;  // surrogate descriptor width per sublattice (0: inactive)
+  std::vector<std::vector<float>> sub_qin;  // int8 input scale of each descriptor feature
   int n_colors = 0;
   int64_t total_walk = 0;
   int w_max = 0;
@@ -2275,9 +2283,18 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   }
   t->total_walk = (int64_t)L.walk.size();
   t->sub_F.assign(L.na, 0);
+  t->sub_qin.assign(L.na, {});
   for (int a = 0; a < L.na; ++a)
-    if (d->site_active[a])
-      t->sub_F[a] = 2 * (L.nb_ptr[a + 1] - L.nb_ptr[a]) + (L.feat_ptr[a + 1] - L.feat_ptr[a]);
+    if (d->site_active[a]) {
+      const int NB = L.nb_ptr[a + 1] - L.nb_ptr[a], nspec = L.feat_ptr[a + 1] - L.feat_ptr[a];
+      t->sub_F[a] = 2 * NB + nspec;
+      // |e.e_n|, (e.e_n)^2, triple products and spin components are at most 1,
+      // the three-pairing sums of four-body specs at most 3
+      std::vector<float>& qs = t->sub_qin[a];
+      qs.assign((size_t)t->sub_F[a], SUR_QIN_UNIT);
+      for (int k = 0; k < nspec; ++k)
+        if (L.feat[4 * (size_t)(L.feat_ptr[a] + k)] == 2) qs[(size_t)2 * NB + k] = SUR_QIN_PAIRS;
+    }
   t->w_max = L.w_max;
   *out = t;
   return SCE_OK;
@@ -5042,9 +5059,17 @@ constexpr int SUR_FQ = 4 * SUR_QB;     // inputs per double batch (16)
 
 __host__ __device__ inline int sur_fpad(int F) { return (F + SUR_FQ - 1) / SUR_FQ * SUR_FQ; }
 
+typedef int int32x4 __attribute__((ext_vector_type(4)));
+
 struct SurNet {
   const float *W1, *b1, *W2, *b2, *w3, *v;  // W1: H x F row-major, W2: H x H row-major
   const float *W1Q, *W2Q;                    // inference copies, quad-packed (W1Q: sur_fpad(F) inputs)
+  // int8 copies in the operand layout of v_mfma_i32_16x16x64_i8 (B operand):
+  // Q1[(ks * 4 + c) * 64 + l] holds V1[16 c + l % 16][64 ks + 16 (l / 16) + 0..15]
+  // (input index padded with zeros to sur_fpq(F)), Q2[c * 64 + l] likewise for V2;
+  // qr1 / qr2: per-unit dequantisation factors t_j / 127; qin: input scales
+  const int32x4 *Q1, *Q2;
+  const float *qr1, *qr2, *qin;
   int F;
   double scale;
 };
@@ -5262,6 +5287,142 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
   return ((double)tB - (double)tA) * net.scale;  // valid in lane 0
 }
 
+// The surrogate difference of one site on one wavefront with its sublattice's
+// int8 network on the matrix cores.  Rows 0 and 1 of the operand tiles are the
+// site's quantised phi(e) and phi(e') (lanes read row (l % 16) & 1, so rows
+// 2..15 repeat them); four 16-column tiles cover the 64 hidden units.  A
+// layer's pre-activation is fmaf((float)acc, r_j, b_j) of the exact integer
+// dot product acc (v_mfma_i32_16x16x64_i8 sums int8 products exactly, |acc| <
+// 2^24, so no order of accumulation matters); layer 1's activations enter
+// layer 2 as rint(127 tanh(z)).  The integer results pass through LDS so that
+// every lane evaluates two activations.  The head is sur_delta's.  All weight
+// loads of the network are issued together before the first product.  qg: the
+// group's LDS (2 FPQ + 128 bytes of operands, 128 int32 results).  No block
+// barrier: the wavefront's own LDS only.
+// Orders one wavefront's LDS writes before its lanes read what other lanes
+// wrote (the compiler would otherwise move accesses of different addresses
+// across each other).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Lane l's share of a sublattice's int8 network: its operands of both layers
+// and the scales, biases and head weight of hidden unit l, every load issued
+// before the first use.  (Issued before the descriptor is formed, to overlap
+// it, they raised the kernel to 100-114 VGPRs: 0.268 against 0.242 ms per DA
+// sweep at bcc 32^3.)
+struct QNetLane {
+  int32x4 w1[2][4], w2[4];
+  float r1, b1, r2, b2, w3;
+};
+
+__device__ __forceinline__ QNetLane qnet_load(const SurNet& net) {
+  const int l = threadIdx.x % 64;
+  const int nks = sur_fpq(net.F) / 64;
+  const __attribute__((address_space(1))) int32x4* Q1 = (const __attribute__((address_space(1))) int32x4*)net.Q1;
+  const __attribute__((address_space(1))) int32x4* Q2 = (const __attribute__((address_space(1))) int32x4*)net.Q2;
+  QNetLane w;
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) w.w1[ks][c] = Q1[((ks < nks ? ks : 0) * 4 + c) * 64 + l];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) w.w2[c] = Q2[c * 64 + l];
+  w.r1 = ((gfloat*)net.qr1)[l];
+  w.b1 = ((gfloat*)net.b1)[l];
+  w.r2 = ((gfloat*)net.qr2)[l];
+  w.b2 = ((gfloat*)net.b2)[l];
+  w.w3 = ((gfloat*)net.w3)[l];
+  return w;
+}
+
+__device__ double sur_delta_q(const SurNet& net, const QNetLane& W, const float* phiA, const float* phiB,
+                              unsigned char* qg, int FPQ, bool valid) {
+  const int l = threadIdx.x % 64;
+  int8_t* A1 = reinterpret_cast<int8_t*>(qg);
+  int8_t* A2 = A1 + 2 * FPQ;
+  int* D = reinterpret_cast<int*>(A2 + 128);
+  float tA = 0.0f, tB = 0.0f;
+  if (valid) {
+    const int nks = sur_fpq(net.F) / 64;
+    const auto& w1 = W.w1;
+    const auto& w2 = W.w2;
+    const float r1 = W.r1, b1 = W.b1, r2 = W.r2, b2 = W.b2, w3 = W.w3;
+    gfloat* qin = (gfloat*)net.qin;
+    for (int f = l; f < 64 * nks; f += 64) {
+      float xa = 0.0f, xb = 0.0f;
+      if (f < net.F) {
+        const float sc = qin[f];
+        xa = fminf(127.0f, fmaxf(-127.0f, __builtin_rintf(phiA[f] * sc)));
+        xb = fminf(127.0f, fmaxf(-127.0f, __builtin_rintf(phiB[f] * sc)));
+      }
+      A1[f] = (int8_t)(int)xa;
+      A1[FPQ + f] = (int8_t)(int)xb;
+    }
+    wave_lds_sync();
+    int32x4 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = int32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      if (ks < nks) {
+        const int32x4 x = *reinterpret_cast<const int32x4*>(A1 + ((l % 16) & 1) * FPQ + 64 * ks + 16 * (l / 16));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(x, w1[ks][c], acc[c], 0, 0, 0);
+      }
+    }
+    // rows 0 and 1 sit in lanes 0..15 (elements 0 and 1): through LDS, then
+    // lane l evaluates unit l of both rows
+    if (l < 16) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        D[16 * c + l] = acc[c][0];
+        D[64 + 16 * c + l] = acc[c][1];
+      }
+    }
+    wave_lds_sync();
+    A2[l] = (int8_t)(int)__builtin_rintf(127.0f * sce::det_tanhf(__builtin_fmaf((float)D[l], r1, b1)));
+    A2[64 + l] = (int8_t)(int)__builtin_rintf(127.0f * sce::det_tanhf(__builtin_fmaf((float)D[64 + l], r1, b1)));
+    wave_lds_sync();
+    const int32x4 x2 = *reinterpret_cast<const int32x4*>(A2 + ((l % 16) & 1) * 64 + 16 * (l / 16));
+    int32x4 acc2[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      acc2[c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(x2, w2[c], int32x4{0, 0, 0, 0}, 0, 0, 0);
+    if (l < 16) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        D[16 * c + l] = acc2[c][0];
+        D[64 + 16 * c + l] = acc2[c][1];
+      }
+    }
+    wave_lds_sync();
+    const float oA = w3 * sce::det_tanhf(__builtin_fmaf((float)D[l], r2, b2));
+    const float oB = w3 * sce::det_tanhf(__builtin_fmaf((float)D[64 + l], r2, b2));
+    float sA = 0.0f, sB = 0.0f;
+    for (int f = l; f < net.F; f += 64) {
+      const float v = ((gfloat*)net.v)[f];
+      sA = sA + v * phiA[f];
+      sB = sB + v * phiB[f];
+    }
+    tA = oA + sA;
+    tB = oB + sB;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    tA = tA + __shfl_down(tA, off, 64);
+    tB = tB + __shfl_down(tB, off, 64);
+  }
+  return ((double)tB - (double)tA) * net.scale;  // valid in lane 0
+}
+
+// Bytes of a group's LDS for sur_delta_q with descriptor rows of FPQ (16-byte
+// multiple).
+__host__ __device__ inline size_t da_q_group_bytes(int FPQ) { return (size_t)2 * FPQ + 128 + 512; }
+__host__ __device__ inline size_t da_q_bytes(int FPQ) { return (size_t)(TPB / 64) * da_q_group_bytes(FPQ); }
+
 // LDS of one DA group (doubles): neighbour rows + dz (walk_stride, rounded up to
 // even) | phiA, phiB (FP floats each, FP a multiple of SUR_FQ) | h1A, h1B (64 floats
 // each).  Every float block starts 16-byte aligned.
@@ -5276,7 +5437,7 @@ __host__ __device__ inline size_t da_lds_bytes(int walk_stride, int FP) {
 // Two-stage delayed-acceptance update: one site per wavefront (64 lanes).  The
 // neighbour rows are staged once: the descriptor reads the spins from them and
 // the stage-1 survivors walk them.
-template <int LMAX>
+template <int LMAX, bool Q>
 __device__ __forceinline__ void da_body(DevTables T, double* config, double* zrows, const double* prop, double* dE,
                                         int* acc, int* acc1, const ChainDev* st, const SurNet* nets, int q_lo,
                                         int q_n, int walk_stride, int FP, int vb, int nvb) {
@@ -5321,7 +5482,14 @@ __device__ __forceinline__ void da_body(DevTables T, double* config, double* zro
                 phiA, phiB);
   }
   __syncthreads();
-  const double dEs = sur_delta(net, phiA, phiB, h1A, h1B, valid);
+  double dEs;
+  if constexpr (Q)
+    dEs = sur_delta_q(net, qnet_load(net), phiA, phiB,
+                      reinterpret_cast<unsigned char*>(lds) + da_lds_bytes(walk_stride, FP) +
+                          g * da_q_group_bytes(sur_fpq(T.nfeat_max)),
+                      sur_fpq(T.nfeat_max), valid);
+  else
+    dEs = sur_delta(net, phiA, phiB, h1A, h1B, valid);
   int pass = 0;
   if (valid && lane == 0) pass = (dEs <= 0.0 || pq[3] < sce::det_exp(-st->beta * dEs)) ? 1 : 0;
   pass = __shfl(pass, 0, L);
@@ -5355,11 +5523,12 @@ __device__ __forceinline__ void da_body(DevTables T, double* config, double* zro
   }
 }
 
-template <int LMAX>
+template <int LMAX, bool Q>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da(
     DevTables T, double* config, double* zrows, const double* prop, double* dE, int* acc, int* acc1,
     const ChainDev* st, const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP) {
-  da_body<LMAX>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo, q_n, walk_stride, FP, blockIdx.x, gridDim.x);
+  da_body<LMAX, Q>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo, q_n, walk_stride, FP, blockIdx.x,
+                   gridDim.x);
 }
 
 // A delayed-acceptance colour step of a chained sweep: blocks below nwalk run
@@ -5367,13 +5536,13 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
 // k_metro_prop's extra blocks do.  (The final reduction of the previous sweep
 // runs as its own launch before the sweep's walks: carried in this kernel it
 // raised the kernel to 136 VGPRs with spills.)
-template <int LMAX>
+template <int LMAX, bool Q>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_da_chain(
     DevTables T, double* config, double* zrows, double* prop, double* dE, int* acc, int* acc1, const ChainDev* st,
     const SurNet* nets, int q_lo, int q_n, int walk_stride, int FP, int nwalk, PropRange r0, PropRange r1) {
   const int b = (int)blockIdx.x;
   if (b < nwalk) {
-    da_body<LMAX>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo, q_n, walk_stride, FP, b, nwalk);
+    da_body<LMAX, Q>(T, config, zrows, prop, dE, acc, acc1, st, nets, q_lo, q_n, walk_stride, FP, b, nwalk);
     return;
   }
   const int pb = b - nwalk, nb0 = prop_blocks(r0.n);
@@ -5417,6 +5586,7 @@ __global__ __launch_bounds__(TPB) void k_features(DevTables T, const double* __r
 }
 
 // Surrogate dE of given descriptor rows through the inference path (tests).
+template <bool Q>
 __global__ __launch_bounds__(TPB) void k_sur_eval(const SurNet* nets, int subl, int n, int fmax, int FP,
                                                   const float* A, const float* B, double* out) {
   extern __shared__ double lds[];
@@ -5433,7 +5603,14 @@ __global__ __launch_bounds__(TPB) void k_sur_eval(const SurNet* nets, int subl, 
   }
   __syncthreads();
   const SurNet net = nets[subl];
-  const double d = sur_delta(net, phiA, phiB, h1A, h1B, valid);
+  double d;
+  if constexpr (Q)
+    d = sur_delta_q(net, qnet_load(net), phiA, phiB,
+                    reinterpret_cast<unsigned char*>(lds) + (size_t)(TPB / 64) * (2 * FP + 2 * SUR_H) * 4 +
+                        g * da_q_group_bytes(sur_fpq(fmax)),
+                    sur_fpq(fmax), valid);
+  else
+    d = sur_delta(net, phiA, phiB, h1A, h1B, valid);
   if (valid && lane == 0) out[i] = d;
 }
 
@@ -5656,6 +5833,9 @@ struct SurSub {
   size_t np = 0;
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v2 = nullptr;
   float* wQ = nullptr;  // W1Q (H x sur_fpad(F)) then W2Q (H x H), quad-packed
+  int8_t* q = nullptr;  // Q1 (64 sur_fpq(F) bytes) then Q2 (4096 bytes)
+  float* qf = nullptr;  // qr1 (H), qr2 (H), qin (sur_fpq(F), zero padded)
+  std::vector<float> qhost;  // the quantised network as the oracle reads it (sur_qblob)
   double scale = 1.0;
   long long adam_t = 0;
 };
@@ -5665,6 +5845,7 @@ struct sce_surrogate {
   int na = 0, fmax = 0;
   std::vector<SurSub> subs;
   SurNet* d_nets = nullptr;
+  bool quant = false;  // stage 1 on the int8 networks (sur_delta_q)
   DevBuf mem;
   // harvested dataset: rows of phi(e), phi(e') (fmax floats each), exact dE, sublattice
   float *dA = nullptr, *dB = nullptr;
@@ -5703,8 +5884,89 @@ static int32_t sur_upload_nets(sce_surrogate* s) {
     n.v = n.w3 + SUR_H;
     n.W1Q = u.wQ;
     n.W2Q = u.wQ + (size_t)SUR_H * sur_fpad(u.F);
+    n.Q1 = reinterpret_cast<const int32x4*>(u.q);
+    n.Q2 = reinterpret_cast<const int32x4*>(u.q + (size_t)SUR_H * sur_fpq(u.F));
+    n.qr1 = u.qf;
+    n.qr2 = u.qf + SUR_H;
+    n.qin = u.qf + 2 * SUR_H;
   }
   HIPCHK(hipMemcpy(s->d_nets, h.data(), h.size() * sizeof(SurNet), hipMemcpyHostToDevice));
+  return SCE_OK;
+}
+
+// The int8 network of one sublattice from its FP32 parameters (host, FP32
+// arithmetic in this order, restated by the tests): for each output unit j of
+// a layer with input scales s_k, w'_jk = W[j][k] / s_k, t_j = max_k |w'_jk|,
+// V[j][k] = rint(w'_jk * (127 / t_j)) and r_j = t_j / 127 (a unit with t_j = 0
+// gets V = 0, r = 0).  Layer 2's inputs are rint(127 h): s_k = 127.
+// The blob the oracle reads: V1 (H x F), r1 (H), b1 (H), V2 (H x H), r2 (H),
+// b2 (H), w3 (H), v (F), qin (F), as floats.
+static void sur_quantise_layer(const float* W, int nout, int nin, const float* sin, std::vector<float>& V,
+                               std::vector<float>& r) {
+  V.assign((size_t)nout * nin, 0.0f);
+  r.assign(nout, 0.0f);
+  for (int j = 0; j < nout; ++j) {
+    float t = 0.0f;
+    for (int k = 0; k < nin; ++k) t = std::max(t, std::fabs(W[(size_t)j * nin + k] / sin[k]));
+    if (t == 0.0f) continue;
+    const float m = 127.0f / t;
+    for (int k = 0; k < nin; ++k) {
+      float q = std::rint((W[(size_t)j * nin + k] / sin[k]) * m);
+      V[(size_t)j * nin + k] = std::min(127.0f, std::max(-127.0f, q));
+    }
+    r[j] = t / 127.0f;
+  }
+}
+
+static size_t sur_qblob_len(int F) { return (size_t)SUR_H * F + 5 * SUR_H + (size_t)SUR_H * SUR_H + 2 * (size_t)F; }
+
+static int32_t sur_quantise(sce_surrogate* s, int a) {
+  SurSub& u = s->subs[a];
+  const int F = u.F, FPQ = sur_fpq(F), H = SUR_H;
+  std::vector<float> p(u.np);
+  HIPCHK(hipMemcpy(p.data(), u.params, u.np * sizeof(float), hipMemcpyDeviceToHost));
+  const float* W1 = p.data();
+  const float* b1 = W1 + (size_t)H * F;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + (size_t)H * H;
+  const float* w3 = b2 + H;
+  const float* v = w3 + H;
+  const std::vector<float>& qin = s->t->sub_qin[a];
+  std::vector<float> V1, r1, V2, r2, s2(H, 127.0f);
+  sur_quantise_layer(W1, H, F, qin.data(), V1, r1);
+  sur_quantise_layer(W2, H, H, s2.data(), V2, r2);
+  u.qhost.clear();
+  u.qhost.insert(u.qhost.end(), V1.begin(), V1.end());
+  u.qhost.insert(u.qhost.end(), r1.begin(), r1.end());
+  u.qhost.insert(u.qhost.end(), b1, b1 + H);
+  u.qhost.insert(u.qhost.end(), V2.begin(), V2.end());
+  u.qhost.insert(u.qhost.end(), r2.begin(), r2.end());
+  u.qhost.insert(u.qhost.end(), b2, b2 + H);
+  u.qhost.insert(u.qhost.end(), w3, w3 + H);
+  u.qhost.insert(u.qhost.end(), v, v + F);
+  u.qhost.insert(u.qhost.end(), qin.begin(), qin.end());
+  // operand layout (SurNet)
+  std::vector<int8_t> q((size_t)H * FPQ + (size_t)H * H, 0);
+  for (int ks = 0; ks < FPQ / 64; ++ks)
+    for (int c = 0; c < 4; ++c)
+      for (int l = 0; l < 64; ++l)
+        for (int b = 0; b < 16; ++b) {
+          const int j = 16 * c + l % 16, k = 64 * ks + 16 * (l / 16) + b;
+          q[(((size_t)ks * 4 + c) * 64 + l) * 16 + b] = k < F ? (int8_t)V1[(size_t)j * F + k] : 0;
+        }
+  int8_t* q2 = q.data() + (size_t)H * FPQ;
+  for (int c = 0; c < 4; ++c)
+    for (int l = 0; l < 64; ++l)
+      for (int b = 0; b < 16; ++b) {
+        const int j = 16 * c + l % 16, k = 16 * (l / 16) + b;
+        q2[((size_t)c * 64 + l) * 16 + b] = (int8_t)V2[(size_t)j * H + k];
+      }
+  std::vector<float> qf((size_t)2 * H + FPQ, 0.0f);
+  std::copy(r1.begin(), r1.end(), qf.begin());
+  std::copy(r2.begin(), r2.end(), qf.begin() + H);
+  std::copy(qin.begin(), qin.end(), qf.begin() + 2 * H);
+  HIPCHK(hipMemcpy(u.q, q.data(), q.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(u.qf, qf.data(), qf.size() * sizeof(float), hipMemcpyHostToDevice));
   return SCE_OK;
 }
 
@@ -5718,7 +5980,7 @@ static int32_t sur_refresh(sce_surrogate* s, int a) {
                                                     u.wQ, u.wQ + (size_t)SUR_H * FQ);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
-  return SCE_OK;
+  return sur_quantise(s, a);
 }
 
 extern "C" int32_t sce_sur_create(sce_tables* t, uint64_t seed, sce_surrogate** out) {
@@ -5762,6 +6024,8 @@ extern "C" int32_t sce_sur_create(sce_tables* t, uint64_t seed, sce_surrogate** 
     if ((r = dupload(s->mem, &u.m, z.data(), z.size())) != SCE_OK) return fail(r);
     if ((r = dupload(s->mem, &u.v2, z.data(), z.size())) != SCE_OK) return fail(r);
     if ((r = dalloc(s->mem, &u.wQ, (size_t)SUR_H * sur_fpad(u.F) + SUR_H * SUR_H)) != SCE_OK) return fail(r);
+    if ((r = dalloc(s->mem, &u.q, (size_t)SUR_H * sur_fpq(u.F) + SUR_H * SUR_H)) != SCE_OK) return fail(r);
+    if ((r = dalloc(s->mem, &u.qf, (size_t)2 * SUR_H + sur_fpq(u.F))) != SCE_OK) return fail(r);
   }
   if ((r = dalloc(s->mem, &s->d_nets, (size_t)s->na)) != SCE_OK) return fail(r);
   if ((r = sur_upload_nets(s)) != SCE_OK) return fail(r);
@@ -5803,6 +6067,20 @@ extern "C" int32_t sce_sur_set_params(sce_surrogate* s, int32_t sub, const float
   s->subs[sub].scale = scale;
   CHK(sur_refresh(s, sub));
   return sur_upload_nets(s);
+}
+
+extern "C" int32_t sce_sur_set_int8(sce_surrogate* s, int32_t on) {
+  if (!s) return sce_fail(SCE_EINVAL, "NULL surrogate");
+  s->quant = on != 0;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_sur_get_qparams(sce_surrogate* s, int32_t sub, float* out) {
+  if (!s || !out || sub < 0 || sub >= s->na || !s->subs[sub].params) return sce_fail(SCE_EINVAL, "bad sublattice");
+  const SurSub& u = s->subs[sub];
+  if (u.qhost.size() != sur_qblob_len(u.F)) return sce_fail(SCE_EINVAL, "no quantised network");
+  std::copy(u.qhost.begin(), u.qhost.end(), out);
+  return SCE_OK;
 }
 
 static int32_t sur_reserve(sce_surrogate* s, long long rows) {
@@ -6035,9 +6313,14 @@ extern "C" int32_t sce_sur_eval(sce_surrogate* s, int32_t sub, int32_t n, const 
   CHK(o.alloc(n));
   const int nb = (n + 3) / 4;
   const int FP = sur_fpad(s->fmax);
-  if (nb)
-    k_sur_eval<<<nb, TPB, (size_t)(TPB / 64) * (2 * FP + 2 * SUR_H) * 4, s->stream>>>(s->d_nets, sub, n, s->fmax,
-                                                                                      FP, a.p, b.p, o.p);
+  const size_t lb = (size_t)(TPB / 64) * (2 * FP + 2 * SUR_H) * 4;
+  if (s->quant && sur_fpq(s->fmax) > 128)
+    return sce_fail(SCE_EINVAL, "int8 stage 1 takes descriptors of at most 128 features (%d here)", s->fmax);
+  if (nb && s->quant)
+    k_sur_eval<true><<<nb, TPB, lb + da_q_bytes(sur_fpq(s->fmax)), s->stream>>>(s->d_nets, sub, n, s->fmax, FP, a.p,
+                                                                                 b.p, o.p);
+  else if (nb)
+    k_sur_eval<false><<<nb, TPB, lb, s->stream>>>(s->d_nets, sub, n, s->fmax, FP, a.p, b.p, o.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return o.down(out, n);
@@ -6072,27 +6355,42 @@ extern "C" int32_t sce_sur_features(sce_tables* t, const double* config, int32_t
 }
 
 // ---- the delayed-acceptance sweep ----------------------------------------------
+// LDS of a delayed-acceptance block: the four groups, then (int8 inference)
+// each group's operands of sur_delta_q.
+static size_t da_block_lds(const DevTables& D, bool q) {
+  const int ws = D.nb_max * D.nlm + 1 + D.nlm;
+  return da_lds_bytes(ws, sur_fpad(D.nfeat_max)) + (q ? da_q_bytes(sur_fpq(D.nfeat_max)) : 0);
+}
+
 template <int LMAX>
-static void launch_da(const DevTables& D, sce_chain* c, const SurNet* nets, int lo, int n, hipStream_t s) {
+static void launch_da(const DevTables& D, sce_chain* c, const SurNet* nets, int lo, int n, bool q, hipStream_t s) {
   constexpr int G = TPB / 64, NLM = (LMAX + 1) * (LMAX + 1);
   const int ws = D.nb_max * NLM + 1 + NLM;
   const int FP = sur_fpad(D.nfeat_max);
   const int nb = (n + G - 1) / G;
-  if (nb)
-    k_da<LMAX><<<nb, TPB, da_lds_bytes(ws, FP), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc,
+  if (!nb) return;
+  if (q)
+    k_da<LMAX, true><<<nb, TPB, da_block_lds(D, true), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc, c->att,
+                                                             c->st, nets, lo, n, ws, FP);
+  else
+    k_da<LMAX, false><<<nb, TPB, da_block_lds(D, false), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc,
                                                                c->att, c->st, nets, lo, n, ws, FP);
 }
 
 template <int LMAX>
 static void launch_da_chain(const DevTables& D, sce_chain* c, const SurNet* nets, int lo, int n, PropRange r0,
-                            PropRange r1, hipStream_t s) {
+                            PropRange r1, bool q, hipStream_t s) {
   constexpr int G = TPB / 64, NLM = (LMAX + 1) * (LMAX + 1);
   const int ws = D.nb_max * NLM + 1 + NLM;
   const int FP = sur_fpad(D.nfeat_max);
   const int nwalk = (n + G - 1) / G;
   const int nb = nwalk + prop_blocks(r0.n) + prop_blocks(r1.n);
-  k_da_chain<LMAX><<<nb, TPB, da_lds_bytes(ws, FP), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc, c->att, c->st,
-                                                          nets, lo, n, ws, FP, nwalk, r0, r1);
+  if (q)
+    k_da_chain<LMAX, true><<<nb, TPB, da_block_lds(D, true), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc,
+                                                                   c->att, c->st, nets, lo, n, ws, FP, nwalk, r0, r1);
+  else
+    k_da_chain<LMAX, false><<<nb, TPB, da_block_lds(D, false), s>>>(D, c->config, c->zrows, c->prop, c->dE, c->acc,
+                                                                     c->att, c->st, nets, lo, n, ws, FP, nwalk, r0, r1);
 }
 
 // One delayed-acceptance sweep, chained within a call as enqueue_metro chains
@@ -6103,9 +6401,11 @@ static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas, bool need_pr
                           int pending = -1, bool carry_last = false) {
   const DevTables& D = c->t->dev;
   const sce_tables* t = c->t;
-  const int ws = D.nb_max * D.nlm + 1 + D.nlm;
-  if (da_lds_bytes(ws, sur_fpad(D.nfeat_max)) > 65536)
+  if (da_block_lds(D, s->quant) > 65536)
     return sce_fail(SCE_EINVAL, "delayed-acceptance stencil too large for one block");
+  if (s->quant && sur_fpq(D.nfeat_max) > 128)
+    return sce_fail(SCE_EINVAL, "int8 stage 1 takes descriptors of at most 128 features (%d here); use FP32",
+                    D.nfeat_max);
   if (c->sweeps_done >= 2147483647LL - 1) return sce_fail(SCE_ERANGE_COUNTER, "sweep counter exhausted");
   hipStream_t st = c->stream;
   if (pending >= 0) {
@@ -6123,9 +6423,9 @@ static int32_t enqueue_da(sce_chain* c, sce_surrogate* s, int meas, bool need_pr
     const int lo = t->color_ptr[P.cols[k]], n = t->color_ptr[P.cols[k] + 1] - lo;
     if (!P.at[k].empty()) {
       SCE_DISPATCH_LMAX(D.lmax, launch_da_chain, D, c, s->d_nets, lo, n, P.at[k][0],
-                        P.at[k].size() > 1 ? P.at[k][1] : none, st);
+                        P.at[k].size() > 1 ? P.at[k][1] : none, s->quant, st);
     } else {
-      SCE_DISPATCH_LMAX(D.lmax, launch_da, D, c, s->d_nets, lo, n, st);
+      SCE_DISPATCH_LMAX(D.lmax, launch_da, D, c, s->d_nets, lo, n, s->quant, st);
     }
   }
   if (next_prop) {

@@ -98,7 +98,7 @@ EXPORTS = [
     "sce_batch_set_graph", "sce_batch_set_persistent", "sce_batch_renormalize", "sce_batch_set_counters",
     "sce_batch_get_exchange", "sce_batch_set_exchange",
     "sce_sur_create", "sce_sur_destroy", "sce_sur_info", "sce_sur_get_params",
-    "sce_sur_set_params", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
+    "sce_sur_set_params", "sce_sur_set_int8", "sce_sur_get_qparams", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
     "sce_sur_features", "sce_da_run_sweeps", "sce_chain_set_counters", "sce_chain_time_walk",
     "sce_chain_set_graph", "sce_chain_set_drift", "sce_chain_binner_reset", "sce_chain_binner_off", "sce_chain_binner_get",
     "sce_energy_gradient", "sce_energy_gradient_noalloc", "sce_chain_energy_gradient", "sce_chain_time_gradient",
@@ -150,7 +150,7 @@ def lib():
             "sce_batch_get_exchange": [P, P, P], "sce_batch_set_exchange": [P, i64, i32, P, P],
             "sce_sur_create": [P, u64, P], "sce_sur_destroy": [P],
             "sce_sur_info": [P, i32, P, P, P, P], "sce_sur_get_params": [P, i32, P],
-            "sce_sur_set_params": [P, i32, P, d], "sce_sur_harvest": [P, P, d, i64],
+            "sce_sur_set_params": [P, i32, P, d], "sce_sur_set_int8": [P, i32], "sce_sur_get_qparams": [P, i32, P], "sce_sur_harvest": [P, P, d, i64],
             "sce_sur_train": [P, i32, i32, i32, d, u64, P],
             "sce_sur_grad": [P, i32, i32, P, P, P, P, P],
             "sce_sur_eval": [P, i32, i32, P, P, P],

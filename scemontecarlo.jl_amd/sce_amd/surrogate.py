@@ -75,6 +75,19 @@ class Surrogate:
             raise L.DimensionMismatch(-2, "parameter vector length")
         L.check(L.lib().sce_sur_set_params(self._h, int(sub), L.ptr(p), float(scale)))
 
+    def set_int8(self, on: bool) -> None:
+        """Stage 1 on the int8 networks (matrix-core integer dot products of the
+        quantised descriptor and weights) or on the FP32 networks (default)."""
+        L.check(L.lib().sce_sur_set_int8(self._h, 1 if on else 0))
+
+    def get_qparams(self, sub: int) -> np.ndarray:
+        """The int8 network of sublattice `sub` as a float32 blob: V1 (64 x F),
+        r1, b1, V2 (64 x 64), r2, b2, w3 (64 each), v (F), input scales (F)."""
+        F = self.info(sub)["F"]
+        out = np.zeros(H * F + 5 * H + H * H + 2 * F, np.float32)
+        L.check(L.lib().sce_sur_get_qparams(self._h, int(sub), L.ptr(out)))
+        return out
+
     def harvest(self, gst: GPUChainState, beta: float, nsweeps: int) -> int:
         """Plain Metropolis sweeps of gst that record every attempt as a training row."""
         L.check(L.lib().sce_sur_harvest(self._h, gst.handle, float(beta), int(nsweeps)))

@@ -279,3 +279,150 @@ def test_da_chain_samples_the_exact_single_spin_law():
         err = blocks.std(ddof=1) / np.sqrt(len(blocks))
         want = _langevin(beta * hs[a])
         assert abs(x.mean() - want) < 4.5 * err + 1e-4, (a, x.mean(), want, err)
+
+
+# ---- int8 stage-1 inference -------------------------------------------------
+
+def _f32(q):
+    """A rational rounded to the nearest float32 (ties to even), exactly."""
+    from fractions import Fraction
+    q = Fraction(q)
+    if q == 0:
+        return np.float32(0.0)
+    e = math.floor(math.log2(abs(q)))
+    while Fraction(2) ** e > abs(q):
+        e -= 1
+    while Fraction(2) ** (e + 1) <= abs(q):
+        e += 1
+    ulp = Fraction(2) ** (max(e, -126) - 23)
+    m = q / ulp
+    n = math.floor(m)
+    rem = m - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return np.float32(float(n * ulp))
+
+
+def _fmaf(x, y, z):
+    from fractions import Fraction
+    return _f32(Fraction(float(x)) * Fraction(float(y)) + Fraction(float(z)))
+
+
+def _delta_q_restated(F, qp, scale, phiA, phiB):
+    """ora_sur_delta_q from its definition, with exact rationals for every
+    rounding step and Python integers for the dot products."""
+    H = R.H
+    o = np.cumsum([0, H * F, H, H, H * H, H, H, H, F, F])
+    V1 = qp[o[0]:o[1]].reshape(H, F).astype(np.int64)
+    r1, b1 = qp[o[1]:o[2]], qp[o[2]:o[3]]
+    V2 = qp[o[3]:o[4]].reshape(H, H).astype(np.int64)
+    r2, b2, w3, v, qin = qp[o[4]:o[5]], qp[o[5]:o[6]], qp[o[6]:o[7]], qp[o[7]:o[8]], qp[o[8]:o[9]]
+
+    def quant(phi):
+        x = np.rint(np.float32(phi) * qin)          # one float32 product, then RNE
+        return np.clip(x, -127, 127).astype(np.int64)
+
+    def head(phi, u):
+        acc1 = V1 @ u
+        h1 = np.array([int(np.rint(np.float32(127.0) * np.float32(O.tanhf(_fmaf(float(a), r1[j], b1[j])))))
+                       for j, a in enumerate(acc1)], np.int64)
+        acc2 = V2 @ h1
+        t = np.zeros(H, np.float32)
+        for j in range(H):
+            o2 = np.float32(w3[j] * np.float32(O.tanhf(_fmaf(float(acc2[j]), r2[j], b2[j]))))
+            s = np.float32(0.0)
+            for f in range(j, F, H):
+                s = np.float32(s + np.float32(v[f] * np.float32(phi[f])))
+            t[j] = np.float32(o2 + s)
+        off = H // 2
+        while off >= 1:
+            t[:off] = (t[:off] + t[off:2 * off]).astype(np.float32)
+            off //= 2
+        return t[0]
+
+    tA = head(phiA, quant(phiA))
+    tB = head(phiB, quant(phiB))
+    return (float(tB) - float(tA)) * scale
+
+
+def quantise_np(params, F, qin):
+    """The library's quantisation of an FP32 network (float32 steps in its order)."""
+    H = R.H
+    p = np.asarray(params, np.float32)
+    W1 = p[:H * F].reshape(H, F)
+    W2 = p[H * F + H:H * F + H + H * H].reshape(H, H)
+
+    def layer(W, sin):
+        Wd = (W / sin[None, :]).astype(np.float32)
+        t = np.abs(Wd).max(axis=1)
+        V = np.zeros_like(Wd)
+        r = np.zeros(W.shape[0], np.float32)
+        for j in range(W.shape[0]):
+            if t[j] == 0:
+                continue
+            m = np.float32(127.0) / t[j]
+            V[j] = np.clip(np.rint((Wd[j] * m).astype(np.float32)), -127, 127)
+            r[j] = t[j] / np.float32(127.0)
+        return V, r
+
+    V1, r1 = layer(W1, np.asarray(qin, np.float32))
+    V2, r2 = layer(W2, np.full(H, 127.0, np.float32))
+    return V1, r1, V2, r2
+
+
+def _rand_qblob(F, rng, qin):
+    p = R.glorot(F, rng).astype(np.float32)
+    H = R.H
+    V1, r1, V2, r2 = quantise_np(p, F, qin)
+    lay = np.cumsum([0, H * F, H, H * H, H, H, F])
+    b1 = rng.uniform(-0.2, 0.2, H).astype(np.float32)
+    b2 = rng.uniform(-0.2, 0.2, H).astype(np.float32)
+    w3, v = p[lay[4]:lay[5]], rng.uniform(-0.3, 0.3, F).astype(np.float32)
+    return p, np.concatenate([V1.ravel(), r1, b1, V2.ravel(), r2, b2, w3, v, qin]).astype(np.float32)
+
+
+@pytest.mark.parametrize("F", [7, 20, 112])
+def test_int8_surrogate_matches_exact_restatement(F):
+    """ora_sur_delta_q against its definition computed with exact integers and
+    correctly rounded rationals, on descriptor-like inputs (|phi| <= 1, a few
+    four-body pairing sums up to 3 in magnitude with input scale 42)."""
+    rng = np.random.default_rng(F)
+    qin = np.full(F, 127.0, np.float32)
+    qin[-(F // 5):] = 42.0
+    _, qp = _rand_qblob(F, rng, qin)
+    for k in range(3):
+        a = rng.uniform(-1, 1, F).astype(np.float32)
+        b = rng.uniform(-1, 1, F).astype(np.float32)
+        a[-(F // 5):] *= 3
+        b[-(F // 5):] *= 3
+        want = _delta_q_restated(F, qp, 0.37, a, b)
+        got = O.sur_delta_q(F, qp, 0.37, a, b)
+        assert got == want, (k, got, want)
+
+
+def test_int8_quantisation_and_network_agree_with_fp32():
+    """The quantised weights are int8 with per-unit factors that reproduce the
+    FP32 weights to half a step, and the int8 surrogate differences follow the
+    FP32 network's (a screen, not an exact value)."""
+    F = 40
+    rng = np.random.default_rng(5)
+    qin = np.full(F, 127.0, np.float32)
+    p, qp = _rand_qblob(F, rng, qin)
+    H = R.H
+    V1, r1, V2, r2 = quantise_np(p, F, qin)
+    assert np.abs(V1).max() <= 127 and np.abs(V2).max() <= 127
+    W1 = p[:H * F].reshape(H, F)
+    assert np.all(np.abs(V1 * r1[:, None] - W1 / 127.0) <= 0.5 * r1[:, None] * 1.0001 + 1e-12)
+    # the FP32 network with the blob's biases and head
+    o = np.cumsum([0, H * F, H, H, H * H, H, H, H, F, F])
+    pf = np.concatenate([W1.ravel(), qp[o[2]:o[3]], p[H * F + H:H * F + H + H * H], qp[o[5]:o[6]],
+                         qp[o[6]:o[7]], qp[o[7]:o[8]]]).astype(np.float32)
+    d8, d32 = [], []
+    for _ in range(200):
+        a = rng.uniform(-1, 1, F).astype(np.float32)
+        b = rng.uniform(-1, 1, F).astype(np.float32)
+        d8.append(O.sur_delta_q(F, qp, 1.0, a, b))
+        d32.append(O.sur_delta(F, pf, 1.0, a, b))
+    d8, d32 = np.array(d8), np.array(d32)
+    assert np.corrcoef(d8, d32)[0, 1] > 0.99
+    assert np.sqrt(np.mean((d8 - d32) ** 2)) < 0.05 * np.sqrt(np.mean(d32 ** 2))
