@@ -3841,15 +3841,16 @@ __global__ __launch_bounds__(TPB) void k_bmetro(const BChain* __restrict__ ch,
                                                 int total_groups, int stride) {
   constexpr int G = TPB / L;
   extern __shared__ double lds[];
-  const int gidx = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / L;
-  bool valid = gidx < total_groups;
-  int k = 0, i = 0;
-  if (valid) {
-    k = find_seg(seg, nseg, gidx);
-    i = gidx - seg[k].start;
-    valid = i < seg[k].n;
-  }
+  // the slices start at multiples of the wavefront's sites (d_seg_wcol), so
+  // the wavefront's first site names the chain of all of them: its tables
+  // and buffers are wave-uniform (scalar registers)
+  const int gw0 = __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * G +
+                                                 (int)(threadIdx.x / 64) * (64 / L));
+  const int gidx = gw0 + (int)(threadIdx.x % 64) / L;
+  const int k = __builtin_amdgcn_readfirstlane(find_seg(seg, nseg, gw0 < total_groups ? gw0 : total_groups - 1));
   const BSeg sg = seg[k];
+  const int i = gidx - sg.start;
+  const bool valid = gidx < total_groups && i < sg.n;
   const BChain* c = &ch[sg.chain];
   metro_group<L, LMAX, false>(c->T, c->config, c->zrows, c->prop, c->dE, c->acc,
                               valid ? c->st->beta : 0.0, sg.lo + i, valid, nullptr, lds, stride);
@@ -4308,6 +4309,11 @@ struct sce_batch {
   int nseg_prop = 0, total_prop = 0;
   std::vector<BSeg*> d_seg_col;
   std::vector<int> nseg_col, total_col;
+  // the same slices laid out for the walk at 16 / 32 / 64 lanes: every slice
+  // starts at a multiple of the wavefront's sites (4 / 2 / 1), so that a
+  // wavefront's sites belong to one chain
+  std::vector<BSeg*> d_seg_wcol[3];
+  std::vector<int> nseg_wcol[3], total_wcol[3];
   BSeg* d_seg_chunk[2] = {nullptr, nullptr};  // [no measurement, measurement]
   BSeg* d_seg_meas = nullptr;                 // measurement-only reduction
   int nseg_chunk[2] = {0, 0}, total_chunk[2] = {0, 0}, nseg_meas = 0, total_meas = 0;
@@ -4432,6 +4438,21 @@ extern "C" int32_t sce_batch_create(int32_t R, sce_tables* const* tables, const 
     b->d_seg_col.push_back(d);
     b->nseg_col.push_back((int)sc.size());
     b->total_col.push_back(t);
+    for (int li = 0; li < 3; ++li) {
+      const int gw = 4 >> li;
+      std::vector<BSeg> sw;
+      int tw = 0;
+      for (const BSeg& x : sc) {
+        tw = (tw + gw - 1) / gw * gw;
+        sw.push_back({x.chain, x.lo, x.n, tw});
+        tw += x.n;
+      }
+      tw = (tw + gw - 1) / gw * gw;
+      if ((r = upload_segs(b->mem, sw, &d)) != SCE_OK) return fail(r);
+      b->d_seg_wcol[li].push_back(d);
+      b->nseg_wcol[li].push_back((int)sw.size());
+      b->total_wcol[li].push_back(t == 0 ? 0 : tw);
+    }
   }
   for (int m = 0; m < 3; ++m) {
     std::vector<BSeg> sc;
@@ -4484,10 +4505,12 @@ static void launch_bpropose(sce_batch* b, hipStream_t s) {
 template <int L, int LMAX>
 static void launch_bmetro_l(sce_batch* b, int col, hipStream_t s) {
   constexpr int G = TPB / L;
-  const int total = b->total_col[col];
+  const int li = L == 16 ? 0 : (L == 32 ? 1 : 2);
+  const int total = b->total_wcol[li][col];
   const int nb = (total + G - 1) / G;
   const size_t shmem = (size_t)G * (size_t)b->stride * sizeof(double);
-  if (nb) k_bmetro<L, LMAX><<<nb, TPB, shmem, s>>>(b->d_ch, b->d_seg_col[col], b->nseg_col[col], total, b->stride);
+  if (nb)
+    k_bmetro<L, LMAX><<<nb, TPB, shmem, s>>>(b->d_ch, b->d_seg_wcol[li][col], b->nseg_wcol[li][col], total, b->stride);
 }
 
 template <int LMAX>
