@@ -339,6 +339,15 @@ struct RowDelta {
   }
 };
 
+// Orders one wavefront's LDS writes before its lanes read what other lanes
+// wrote (the compiler would otherwise move accesses of different addresses
+// across each other).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Fold of a group's lane partials into its first lane: the shuffle-down tree
 // a += a[lane + off] for off = L/2, ..., 1.  The steps of 8 and fewer stay
 // inside a 16-lane row and move data with DPP row shifts (row_shl: lane i
@@ -460,7 +469,7 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
 }
 
 // One group of L lanes updates one site of the colour (colour-order index q).
-// Called by every thread of the block (contains a block barrier).
+// Every lane of the group's wavefront calls it (wavefront-level LDS ordering).
 // TEST: ΔE only, for caller-chosen sites and rows (parity hook).
 template <int L, int LMAX, bool TEST>
 __device__ __forceinline__ void metro_group(const DevTables& T, double* __restrict__ config,
@@ -505,11 +514,12 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
     }
     rd.store(dz, lane);
   }
-  __syncthreads();
+  // a group's staged block is its own (L <= 64: one wavefront), so the walk
+  // waits on its own wavefront's writes only, not on the block's other sites
+  wave_lds_sync();
   double a = 0.0;
   if (valid) a = walk_delta<L, NLM>(wl, W, pre, lane, Z);
   a = fold_lanes<L>(a);
-  __syncthreads();  // the block's LDS is reused by the next site group
   if (!valid) return;
   if (TEST) {
     if (lane == 0) dE[q] = a;
@@ -5322,15 +5332,6 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
 // loads of the network are issued together before the first product.  qg: the
 // group's LDS (2 FPQ + 128 bytes of operands, 128 int32 results).  No block
 // barrier: the wavefront's own LDS only.
-// Orders one wavefront's LDS writes before its lanes read what other lanes
-// wrote (the compiler would otherwise move accesses of different addresses
-// across each other).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 // Lane l's share of a sublattice's int8 network: its operands of both layers
 // and the scales, biases and head weight of hidden unit l, every load issued
 // before the first use.  (Issued before the descriptor is formed, to overlap
