@@ -685,7 +685,12 @@ def main():
                      "duration_source": "HIP events around 20 back-to-back passes of the colour "
                                         "walks on the chain's stream, just before the timed region",
                      "fp64": {"flops_per_update": f_site, "achieved": fp64_tflops, "peak": 78.6,
-                              "unit": "TFLOP/s", "frac": fp64_tflops / 78.6}},
+                              "unit": "TFLOP/s", "frac": fp64_tflops / 78.6},
+                     # the whole sweep (16 walks, the reduction and the
+                     # launch gaps) over the timed region, beside the kernel
+                     "sweep_level": {"achieved": b_site * H.n_active / (wall / args.steps) / 1e9,
+                                     "frac": b_site * H.n_active / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
+                                     "unit": "GB/s"}},
     }
     # the all-site energy gradient (gpu_energy_gradient! twin) on the chain's rows
     grad_ms = S.gpu_time_gradient(gst, S.GRADIENT_LANES, 20)

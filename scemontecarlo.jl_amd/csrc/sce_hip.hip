@@ -617,7 +617,7 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
       });
     }
   }
-  __syncthreads();
+  wave_lds_sync();  // per-group LDS: the own wavefront only
   double c[NLM];
 #pragma unroll
   for (int k = 0; k < NLM; ++k) c[k] = 0.0;
@@ -5281,7 +5281,7 @@ __device__ __forceinline__ void hidden_pair(const floatx4* wq_generic, int nq, c
 // fma(+0, +0, z) steps, which return z except that a z of -0 becomes +0; tanh
 // maps both zeros to +0, so the hidden activations are the same bits.  The head
 // (w3 . h2 + v . phi) keeps separate multiplies and adds and folds over the 64
-// lanes.  Contains a block barrier: every thread of the block calls it.
+// lanes.  h1A/h1B belong to the wavefront: it orders its own LDS only.
 __device__ double sur_delta(const SurNet& net, const float* phiA, const float* phiB, float* h1A, float* h1B,
                             bool valid) {
   const int j = threadIdx.x % 64;
@@ -5294,7 +5294,7 @@ __device__ double sur_delta(const SurNet& net, const float* phiA, const float* p
     h1A[j] = sce::det_tanhf(zA);
     h1B[j] = sce::det_tanhf(zB);
   }
-  __syncthreads();
+  wave_lds_sync();  // per-group LDS: the own wavefront only
   if (valid) {
     float zA = ((gfloat*)net.b2)[j], zB = zA;
     hidden_pair(reinterpret_cast<const floatx4*>(net.W2Q) + j, SUR_H / 4, reinterpret_cast<const floatx4*>(h1A),
@@ -5499,13 +5499,13 @@ __device__ __forceinline__ void da_body(DevTables T, double* config, double* zro
     rd.store(dz, lane);
     NB = T.nb_ptr[atom + 1] - T.nb_ptr[atom];
   }
-  __syncthreads();
+  wave_lds_sync();  // per-group LDS: the own wavefront only
   if (valid) {
     const int f0 = T.feat_ptr[atom];
     sur_rows<L>(lane, net.F, sur_fpad(net.F), NB, T.feat + f0, T.feat_ptr[atom + 1] - f0, SpinsZ{Z, NLM}, e, e2,
                 phiA, phiB);
   }
-  __syncthreads();
+  wave_lds_sync();  // per-group LDS: the own wavefront only
   double dEs;
   if constexpr (Q)
     dEs = sur_delta_q(net, qnet_load(net), phiA, phiB,
@@ -5596,7 +5596,7 @@ __global__ __launch_bounds__(TPB) void k_features(DevTables T, const double* __r
     s = FROM_PROP ? T.color_sites[q_lo + qi] : sites[qi];
     stage_spins<L>(T, zrows, s, lane, E, &NB);
   }
-  __syncthreads();
+  wave_lds_sync();  // per-group LDS: the own wavefront only
   if (!valid) return;
   const SpinsE sp{E};
   const int atom = s % T.na;
