@@ -345,7 +345,9 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto"):
             "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()},
             "swap_acceptance_mean": float(np.nanmean(sw)), "swap_acceptance_min": float(np.nanmin(sw)),
             "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep; "
-                    "autotune keys: lanes per site, 'p' = one persistent launch per segment"}
+                    "autotune keys: lanes per site, 'p' = one persistent launch per segment; at 64 lanes the "
+                    "per-colour launches walk each site once for two replicas (shared walk, results those "
+                    "of 64 lanes per site)"}
 
 
 def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):

@@ -246,6 +246,17 @@ int32_t sce_batch_set_graph(sce_batch* b, int32_t on);
  * bit-identical results at the same lanes per site.  The reference runs rungs round-robin on one queue
  * (src/gpu/gpu_pt.jl:4-15). */
 int32_t sce_batch_set_persistent(sce_batch* b, int32_t on);
+/* At 64 lanes per site, one wavefront walks a site's sublattice walk once for
+ * `chains` (2 or 4) chains of a batch whose chains all share one tables
+ * handle (a temperature ladder), each chain's sums formed as at 64 lanes per
+ * site alone: the results are those of 64 lanes per site, each walk entry is
+ * loaded once per `chains` sites.  0 (the default): 2 for such batches of at
+ * least two chains whose neighbour stencil fits, else off; 1 turns it off;
+ * batches of different tables return SCE_EINVAL for 2 and 4.  Per-colour
+ * launches only.
+ * The reference has no counterpart: its rungs run one after another on one
+ * queue (src/gpu/gpu_pt.jl:4-15). */
+int32_t sce_batch_set_shared_walk(sce_batch* b, int32_t chains);
 /* Checkpoint restore of a "gpu_pt" run (resume, src/checkpoint.jl:555-635).
  * set_counters: the lane-owned state of chain k that batch creation does not
  * take — its keyed-RNG sweep index, acceptance counters and drift record

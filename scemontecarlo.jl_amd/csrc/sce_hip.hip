@@ -3866,6 +3866,141 @@ __global__ __launch_bounds__(TPB) void k_bmetro(const BChain* __restrict__ ch,
                               valid ? c->st->beta : 0.0, sg.lo + i, valid, nullptr, lds, stride);
 }
 
+// The walk of walk_delta<64> for K sites that share one sublattice walk, each
+// with its own staged block (LDS byte bases zb[k]): every entry is loaded once
+// and used K times, and site k's sum is the one walk_delta<64> forms for it
+// (the same operations in the same order on the same values).
+template <int K, int NLM>
+__device__ __forceinline__ void walk_delta_shared(const WalkEntry* __restrict__ wl, int Wp, int2 pre, int lane,
+                                                  const unsigned (&zb)[K], double (&a)[K]) {
+  constexpr int L = 64, B = WALK_UNROLL * L;
+  const WalkWords* __restrict__ ww = reinterpret_cast<const WalkWords*>(wl) + lane;
+  const int n1 = pre.x - pre.x % B, n2 = pre.y - pre.y % B;
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = 0.0;
+  int base = 0;
+  for (; base < n1; base += B) {
+    WalkWords en[WALK_UNROLL];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double p = lds_at(lds_add_lo(zb[k], en[u].f01));
+        a[k] = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb[k], en[u].f2t)), a[k]);
+      }
+  }
+  for (; base < n2; base += B) {
+    WalkWords en[WALK_UNROLL];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double p = lds_at(lds_add_lo(zb[k], en[u].f01)) * lds_at(lds_add_hi(zb[k], en[u].f01));
+        a[k] = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb[k], en[u].f2t)), a[k]);
+      }
+  }
+  for (; base < Wp; base += B) {
+    WalkWords en[WALK_UNROLL];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+#pragma unroll
+    for (int u = 0; u < WALK_UNROLL; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double p = (lds_at(lds_add_lo(zb[k], en[u].f01)) * lds_at(lds_add_hi(zb[k], en[u].f01))) *
+                         lds_at(lds_add_lo(zb[k], en[u].f2t));
+        a[k] = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb[k], en[u].f2t)), a[k]);
+      }
+  }
+}
+
+// One colour step of a ladder whose chains share one model (a homogeneous
+// batch): wavefront w takes position q = lo + w % n of the colour class for
+// the K chains K (w / n) ... K (w / n) + K - 1.  Their sites are the same site
+// of the same layout, so they share its sublattice walk: one wavefront walks
+// it once for all K, 64 lanes per site.  Each chain's update is the one
+// k_bmetro<64> makes (same staging, walk sums, fold and acceptance), so the
+// results are those of 64 lanes per site.
+template <int K, int LMAX>
+__global__ __launch_bounds__(TPB) void k_bmetro_shared(const BChain* __restrict__ ch, int R, int lo, int n,
+                                                       int total_waves, int stride) {
+  constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  extern __shared__ double lds[];
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+  const int gw = __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * (TPB / 64) + wv);
+  const int lane = (int)threadIdx.x % 64;
+  const bool wvalid = gw < total_waves;
+  const int q = lo + (wvalid ? gw % n : 0);
+  const int r0 = (wvalid ? gw / n : 0) * K;
+  const DevTables& T = ch[0].T;
+  double* Zw = lds + (size_t)wv * K * stride;
+  const int dzo = T.nb_max * NLM + 1;
+  unsigned zb[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) zb[k] = (unsigned)(size_t)(lds_cdouble*)(Zw + (size_t)k * stride);
+  int s = 0;
+  const WalkEntry* wl = nullptr;
+  int W = 0;
+  int2 pre = make_int2(0, 0);
+  double u_acc[K];
+  if (wvalid) {
+    s = __builtin_amdgcn_readfirstlane(T.color_sites[q]);
+    const int a0 = s % T.na;
+    const int w0 = T.walkp_ptr[a0];
+    W = T.walkp_ptr[a0 + 1] - w0;
+    wl = T.walkp + w0;
+    pre = T.walkp_pre[a0];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      u_acc[k] = 0.0;
+      if (r0 + k >= R) continue;
+      const BChain* c = &ch[r0 + k];
+      double* Z = Zw + (size_t)k * stride;
+      RowDelta<64, NLM> rd;
+      u_acc[k] = c->prop[(size_t)q * PS + 3];
+      stage_neighbours_q<64, NLM>(T, c->zrows, q, s, lane, Z, [&] {
+        rd.load(c->prop + (size_t)q * PS + 4, c->zrows + (size_t)s * NLM, lane);
+      });
+      rd.store(Z + dzo, lane);
+    }
+  }
+  // the K staged blocks are this wavefront's own
+  wave_lds_sync();
+  double a[K];
+  if (wvalid) walk_delta_shared<K, NLM>(wl, W, pre, lane, zb, a);
+  else
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = fold_lanes<64>(a[k]);
+  if (!wvalid) return;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (r0 + k >= R) break;
+    const BChain* c = &ch[r0 + k];
+    const double* pq = c->prop + (size_t)q * PS;
+    int accept = 0;
+    if (lane == 0) accept = (a[k] <= 0.0 || u_acc[k] < sce::det_exp(-c->st->beta * a[k])) ? 1 : 0;
+    accept = __shfl(accept, 0, 64);
+    if (accept) {
+      for (int m = lane; m < NLM; m += 64) c->zrows[(size_t)s * NLM + m] = pq[4 + m];
+      if (lane == 0) {
+        c->config[3 * (size_t)s] = pq[0];
+        c->config[3 * (size_t)s + 1] = pq[1];
+        c->config[3 * (size_t)s + 2] = pq[2];
+      }
+    }
+    if (lane == 0) {
+      c->dE[q] = accept ? a[k] : 0.0;
+      c->acc[q] = accept;
+    }
+  }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bchunks(const BChain* __restrict__ ch,
                                                  const BSeg* __restrict__ seg, int nseg,
@@ -4338,6 +4473,10 @@ struct sce_batch {
   long long sweeps_done = 0;
   bool use_graph = true;
   bool persistent = false;
+  // chains per wavefront that share one walk at 64 lanes per site
+  // (k_bmetro_shared; homogeneous batches only): 0 = two where the stencil
+  // fits (the default), 1 = off, 2 or 4
+  int share = 0;
   // buffers of the batched dependency-driven sweep (k_bflow)
   struct BFlow {
     int cap = 0, mcap = 0, grid = 0, G = 0, NS = 0;
@@ -4530,14 +4669,51 @@ static void launch_bmetro(int L, sce_batch* b, int col, hipStream_t s) {
   else launch_bmetro_l<64, LMAX>(b, col, s);
 }
 
+// A colour step of a homogeneous batch with K chains per wavefront sharing
+// the walk (k_bmetro_shared): position-major over the class, chain groups
+// outer.
+template <int K, int LMAX>
+static void launch_bmetro_shared_k(sce_batch* b, int col, hipStream_t s) {
+  const sce_tables* tb = b->tabs[0];
+  if (col >= tb->n_colors) return;
+  const int lo = tb->color_ptr[col], n = tb->color_ptr[col + 1] - lo;
+  if (n == 0) return;
+  constexpr int G = TPB / 64;
+  const long long waves = (long long)n * ((b->R + K - 1) / K);
+  const int nb = (int)((waves + G - 1) / G);
+  const size_t shmem = (size_t)G * K * (size_t)b->stride * sizeof(double);
+  k_bmetro_shared<K, LMAX><<<nb, TPB, shmem, s>>>(b->d_ch, b->R, lo, n, (int)waves, b->stride);
+}
+
+template <int LMAX>
+static void launch_bmetro_shared(int K, sce_batch* b, int col, hipStream_t s) {
+  if (K == 2) launch_bmetro_shared_k<2, LMAX>(b, col, s);
+  else launch_bmetro_shared_k<4, LMAX>(b, col, s);
+}
+
 static int32_t benqueue_sweep(sce_batch* b, int L, int meas, int exch) {
   if ((size_t)(TPB / L) * (size_t)b->stride * sizeof(double) > 65536)
     return sce_fail(SCE_EINVAL, "neighbour stencil too large for lanes = %d; use more lanes", L);
+  const auto shared_lds = [&](int k) { return (size_t)(TPB / 64) * k * (size_t)b->stride * sizeof(double); };
+  int share = 1;
+  if (L == 64 && b->homogeneous) {
+    if (b->share > 1) {
+      share = b->share;
+      if (shared_lds(share) > 65536)
+        return sce_fail(SCE_EINVAL, "neighbour stencil too large for %d chains per shared walk; share fewer", share);
+    } else if (b->share == 0 && b->R >= 2 && shared_lds(2) <= 65536) {
+      share = 2;  // the default: two chains per walk where the stencil fits
+    }
+  }
   hipStream_t s = b->stream;
   SCE_DISPATCH_LMAX(b->lmax, launch_bpropose, b, s);
   for (int col = 0; col < b->max_colors; ++col) {
     if (b->total_col[col] == 0) continue;
-    SCE_DISPATCH_LMAX(b->lmax, launch_bmetro, L, b, col, s);
+    if (share > 1) {
+      SCE_DISPATCH_LMAX(b->lmax, launch_bmetro_shared, share, b, col, s);
+    } else {
+      SCE_DISPATCH_LMAX(b->lmax, launch_bmetro, L, b, col, s);
+    }
   }
   k_bchunks<MODE_METRO><<<b->total_chunk[meas], 256, 0, s>>>(b->d_ch, b->d_seg_chunk[meas], b->nseg_chunk[meas], meas);
   k_bfinal<MODE_METRO><<<b->R, 256, 0, s>>>(b->d_ch, meas);
@@ -4819,6 +4995,19 @@ static int32_t benqueue_flow(sce_batch* b, int n, long long i0, int64_t measure_
 extern "C" int32_t sce_batch_set_persistent(sce_batch* b, int32_t on) {
   if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
   b->persistent = on != 0;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_set_shared_walk(sce_batch* b, int32_t chains) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  if (chains != 0 && chains != 1 && chains != 2 && chains != 4)
+    return sce_fail(SCE_EINVAL, "chains per shared walk must be 0 (default), 1, 2 or 4 (got %d)", chains);
+  if (chains > 1 && !b->homogeneous)
+    return sce_fail(SCE_EINVAL, "a shared walk needs every chain of the batch on the same tables");
+  if (chains != b->share) {
+    b->drop_graphs();
+    b->share = chains;
+  }
   return SCE_OK;
 }
 

@@ -103,6 +103,13 @@ class GPUBatch:
         lanes per site."""
         L.check(L.lib().sce_batch_set_persistent(self._h, 1 if on else 0))
 
+    def set_shared_walk(self, chains: int) -> None:
+        """At 64 lanes per site, walk each site's sublattice walk once for
+        `chains` (2 or 4) chains of a batch on one model; the results are
+        those of 64 lanes per site.  0 (the default) shares between two chains
+        where the neighbour stencil fits, 1 turns it off."""
+        L.check(L.lib().sce_batch_set_shared_walk(self._h, int(chains)))
+
     def set_counters(self, k: int, sweep_index: int, acc_metro: int, att_metro: int,
                      acc_or: int = 0, att_or: int = 0, max_drift: float = 0.0) -> None:
         """Restore chain k's keyed-RNG sweep index, counters and drift record."""
