@@ -151,6 +151,21 @@ constexpr int TREE_B = 512;   // elements per tree chunk
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
 // walks a contiguous range of the colour class (neighbour rows then stay in
 // that XCD's L2).
+// A load from global memory through a pointer the compiler cannot place: the
+// batch kernels read their chains' pointers from device memory, and loads
+// through those generic pointers would compile to flat instructions, which
+// also count against the LDS counter (a wait on an LDS gather would wait on
+// them too).
+template <class T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
+__device__ __forceinline__ int2 ldg_int2(const int2* p) {
+  const long long v = ldg(reinterpret_cast<const long long*>(p));
+  return make_int2((int)v, (int)(v >> 32));
+}
+
 __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
   const int x = b & 7, l = b >> 3, per = nb >> 3, rem = nb & 7;
   return x * per + (x < rem ? x : rem) + l;
@@ -304,17 +319,17 @@ __device__ __forceinline__ int stage_neighbours_q(const DevTables& T, const doub
     for (int u = 0; u < U; ++u) {
       const int i = b + u * L + lane;
       at[u] = i < total ? i : total - 1;
-      n[u] = nq[at[u] / NLM];
+      n[u] = ldg(nq + at[u] / NLM);
     }
     if (b == 0) between();
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = zrows[(size_t)n[u] * NLM + (at[u] - (at[u] / NLM) * NLM)];
+    for (int u = 0; u < U; ++u) v[u] = ldg(zrows + (size_t)n[u] * NLM + (at[u] - (at[u] / NLM) * NLM));
 #pragma unroll
     for (int u = 0; u < U; ++u) Z[at[u]] = v[u];
   }
   if (total == 0) between();  // no neighbours (single-site terms only)
   const int atom = s % T.na;
-  if (lane == 0) Z[(T.nb_ptr[atom + 1] - T.nb_ptr[atom]) * NLM] = 1.0;
+  if (lane == 0) Z[(ldg(T.nb_ptr + atom + 1) - ldg(T.nb_ptr + atom)) * NLM] = 1.0;
   return atom;
 }
 
@@ -329,8 +344,8 @@ struct RowDelta {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = lane + u * L < NLM ? lane + u * L : NLM - 1;
-      p[u] = pz[k];
-      c[u] = zs[k];
+      p[u] = ldg(pz + k);
+      c[u] = ldg(zs + k);
     }
   }
   __device__ __forceinline__ void store(double* dz, int lane) const {
@@ -409,6 +424,17 @@ __device__ __forceinline__ unsigned lds_add_hi(unsigned base, unsigned packed) {
   return r;
 }
 __device__ __forceinline__ double lds_at(unsigned addr) { return *(lds_cdouble*)(size_t)addr; }
+typedef int __attribute__((ext_vector_type(4))) walk_i32x4;
+typedef double __attribute__((ext_vector_type(2))) walk_f64x2;
+// One walk entry, as a 16-byte global load.
+__device__ __forceinline__ WalkWords ld_walk(const WalkWords* p) {
+  const walk_f64x2 v = ldg(reinterpret_cast<const walk_f64x2*>(p));
+  WalkWords e;
+  e.w = v.x;
+  e.f01 = (unsigned)__double2loint(v.y);
+  e.f2t = (unsigned)__double2hiint(v.y);
+  return e;
+}
 
 // The walk over a padded sublattice walk `wl` of Wp entries (Wp a multiple of
 // WALK_UNROLL * L).  Each lane's sum is the contract's: fma(w * p, dz[tgt], a)
@@ -437,7 +463,7 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
     WalkWords en[WALK_UNROLL];
     // every load of the batch is issued before the first use
 #pragma unroll
-    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
       const double p = lds_at(lds_add_lo(zb, en[u].f01));
@@ -447,7 +473,7 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
   for (; base < n2; base += B) {
     WalkWords en[WALK_UNROLL];
 #pragma unroll
-    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
       const double p = lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01));
@@ -457,7 +483,7 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
   for (; base < Wp; base += B) {
     WalkWords en[WALK_UNROLL];
 #pragma unroll
-    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) {
       const double p = (lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01))) *
@@ -494,20 +520,20 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   int2 pre = make_int2(0, 0);
   double u_acc = 0.0;
   if (valid) {
-    s = TEST ? test_sites[q] : T.color_sites[q];
+    s = TEST ? test_sites[q] : ldg(T.color_sites + q);
     if constexpr (L == 64) s = __builtin_amdgcn_readfirstlane(s);
     // independent of the staging: issued with it rather than after the barrier
     const int a0 = s % T.na;
-    const int w0 = T.walkp_ptr[a0];
-    W = T.walkp_ptr[a0 + 1] - w0;
+    const int w0 = ldg(T.walkp_ptr + a0);
+    W = ldg(T.walkp_ptr + a0 + 1) - w0;
     wl = T.walkp + w0;
-    pre = T.walkp_pre[a0];
+    pre = ldg_int2(T.walkp_pre + a0);
     RowDelta<L, NLM> rd;
     if constexpr (TEST) {
       rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
       stage_neighbours<L, NLM>(T, zrows, s, lane, Z, &atom);
     } else {
-      u_acc = prop[(size_t)q * PS + 3];
+      u_acc = ldg(prop + (size_t)q * PS + 3);
       atom = stage_neighbours_q<L, NLM>(T, zrows, q, s, lane, Z, [&] {
         rd.load(prop + (size_t)q * PS + 4, zrows + (size_t)s * NLM, lane);
       });
@@ -3882,7 +3908,7 @@ __device__ __forceinline__ void walk_delta_shared(const WalkEntry* __restrict__ 
   for (; base < n1; base += B) {
     WalkWords en[WALK_UNROLL];
 #pragma unroll
-    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u)
 #pragma unroll
@@ -3894,7 +3920,7 @@ __device__ __forceinline__ void walk_delta_shared(const WalkEntry* __restrict__ 
   for (; base < n2; base += B) {
     WalkWords en[WALK_UNROLL];
 #pragma unroll
-    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u)
 #pragma unroll
@@ -3906,7 +3932,7 @@ __device__ __forceinline__ void walk_delta_shared(const WalkEntry* __restrict__ 
   for (; base < Wp; base += B) {
     WalkWords en[WALK_UNROLL];
 #pragma unroll
-    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ww[base + u * L];
+    for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u)
 #pragma unroll
@@ -3948,22 +3974,24 @@ __global__ __launch_bounds__(TPB) void k_bmetro_shared(const BChain* __restrict_
   int2 pre = make_int2(0, 0);
   double u_acc[K];
   if (wvalid) {
-    s = __builtin_amdgcn_readfirstlane(T.color_sites[q]);
+    s = __builtin_amdgcn_readfirstlane(ldg(T.color_sites + q));
     const int a0 = s % T.na;
-    const int w0 = T.walkp_ptr[a0];
-    W = T.walkp_ptr[a0 + 1] - w0;
+    const int w0 = ldg(T.walkp_ptr + a0);
+    W = ldg(T.walkp_ptr + a0 + 1) - w0;
     wl = T.walkp + w0;
-    pre = T.walkp_pre[a0];
+    pre = ldg_int2(T.walkp_pre + a0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       u_acc[k] = 0.0;
       if (r0 + k >= R) continue;
       const BChain* c = &ch[r0 + k];
+      const double* prop = c->prop;
+      const double* zr = c->zrows;
       double* Z = Zw + (size_t)k * stride;
       RowDelta<64, NLM> rd;
-      u_acc[k] = c->prop[(size_t)q * PS + 3];
-      stage_neighbours_q<64, NLM>(T, c->zrows, q, s, lane, Z, [&] {
-        rd.load(c->prop + (size_t)q * PS + 4, c->zrows + (size_t)s * NLM, lane);
+      u_acc[k] = ldg(prop + (size_t)q * PS + 3);
+      stage_neighbours_q<64, NLM>(T, zr, q, s, lane, Z, [&] {
+        rd.load(prop + (size_t)q * PS + 4, zr + (size_t)s * NLM, lane);
       });
       rd.store(Z + dzo, lane);
     }
@@ -3987,11 +4015,13 @@ __global__ __launch_bounds__(TPB) void k_bmetro_shared(const BChain* __restrict_
     if (lane == 0) accept = (a[k] <= 0.0 || u_acc[k] < sce::det_exp(-c->st->beta * a[k])) ? 1 : 0;
     accept = __shfl(accept, 0, 64);
     if (accept) {
-      for (int m = lane; m < NLM; m += 64) c->zrows[(size_t)s * NLM + m] = pq[4 + m];
+      double* zr = c->zrows;
+      for (int m = lane; m < NLM; m += 64) zr[(size_t)s * NLM + m] = pq[4 + m];
       if (lane == 0) {
-        c->config[3 * (size_t)s] = pq[0];
-        c->config[3 * (size_t)s + 1] = pq[1];
-        c->config[3 * (size_t)s + 2] = pq[2];
+        double* cf = c->config;
+        cf[3 * (size_t)s] = pq[0];
+        cf[3 * (size_t)s + 1] = pq[1];
+        cf[3 * (size_t)s + 2] = pq[2];
       }
     }
     if (lane == 0) {
