@@ -14,7 +14,7 @@ from .gpu import (GPUTiledHamiltonian, GPUChainState, gpu_metropolis_sweep,  # n
                   gpu_chain_energy_gradient, gpu_time_gradient, GRADIENT_LANES, autotune_chain)
 from .observables import (LogBinner, BinStore, jackknife, standard_stats,  # noqa: F401
                           effective_samples, tau_int_windowed)
-from .batch import GPUBatch, gpu_run_pt, resume_pt, autotune_lanes, autotune_batch  # noqa: F401
+from .batch import GPUBatch, gpu_run_pt, resume_pt, autotune_lanes, autotune_batch, run_concurrent  # noqa: F401
 from .surrogate import Surrogate, gpu_da_run_sweeps  # noqa: F401
 
 __all__ = [
@@ -23,6 +23,7 @@ __all__ = [
     "gpu_metropolis_sweep", "gpu_overrelaxation_sweep", "gpu_run_sweeps", "gpu_renormalize",
     "to_host", "from_host", "swap_payload", "DEFAULT_LANES", "LogBinner", "BinStore",
     "jackknife", "standard_stats", "effective_samples", "tau_int_windowed", "GPUBatch", "gpu_run_pt", "resume_pt",
-    "autotune_lanes", "autotune_batch", "autotune_chain", "Surrogate", "gpu_da_run_sweeps", "gpu_energy_gradient",
+    "autotune_lanes", "autotune_batch", "run_concurrent", "autotune_chain", "Surrogate", "gpu_da_run_sweeps",
+    "gpu_energy_gradient",
     "gpu_chain_energy_gradient", "gpu_time_gradient", "GRADIENT_LANES",
 ]

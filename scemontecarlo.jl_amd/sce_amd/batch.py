@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
@@ -398,6 +399,35 @@ def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
     if best is None:
         raise RuntimeError("no candidate form could run this batch")
     return best
+
+
+def run_concurrent(runs):
+    """Run several batches at once: `runs` holds (batch, args, kwargs) for
+    GPUBatch.run, each batch at most once.  Every batch sweeps on its own
+    stream from its own host thread (the library calls release the GIL), so
+    the batches share the GPU; each batch's results are those of running it
+    alone.  Returns the run() results in order; re-raises the first error."""
+    batches = [r[0] for r in runs]
+    if len({id(b) for b in batches}) != len(batches):
+        raise ValueError("a batch can appear only once in a concurrent run")
+    out = [None] * len(runs)
+    err = [None] * len(runs)
+
+    def work(i, b, args, kwargs):
+        try:
+            out[i] = b.run(*args, **kwargs)
+        except BaseException as e:  # re-raised on the calling thread
+            err[i] = e
+    threads = [threading.Thread(target=work, args=(i, b, tuple(a), dict(k)))
+               for i, (b, a, k) in enumerate(runs)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for e in err:
+        if e is not None:
+            raise e
+    return out
 
 
 def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache: Optional[dict] = None,

@@ -244,6 +244,51 @@ def test_uneven_batch_with_zeeman_fields_bitwise():
             assert np.array_equal(cfg, c) and np.array_equal(zd, z) and Ed == E, k
 
 
+def test_concurrent_batches_equal_running_them_alone():
+    """run_concurrent: three uneven batches (per-chain models and Zeeman
+    fields, different lane counts) sweeping at once on their own streams end
+    bit-identical to the same batches run one after another, measurements
+    included; a second concurrent call continues them identically."""
+    from sce_amd.batch import run_concurrent
+    groups = [[("sc", 8), ("bcc", 6)], [("bcc", 7), ("sc", 9), ("sc", 6)], [("bcc", 5)]]
+    lanes = [16, 32, 64]
+
+    def make():
+        out = []
+        r = np.random.default_rng(5)
+        for g, specs in enumerate(groups):
+            gHs, sts, sds, kts = [], [], [], []
+            for k, (lat, N) in enumerate(specs):
+                na, terms = M.lattice_model(lat, pair_shells=2, pair_lmax=2, triangles=(lat == "bcc"),
+                                            seed=700 + 10 * g + k)
+                B = r.uniform(0.0, 10.0) * (lambda v: v / np.linalg.norm(v))(r.standard_normal(3))
+                H = S.TiledHamiltonian(na, terms, (N, N, N), magmoms=[2.0] * na, field=tuple(B))
+                gHs.append(S.GPUTiledHamiltonian(H))
+                sts.append(S.ChainState(H, rand_config(H.n_sites, 30 + 10 * g + k), 0.5))
+                sds.append(int(r.integers(1, 2**62)))
+                kts.append(float(r.uniform(0.005, 0.05)))
+            out.append(GPUBatch(gHs, sts, sds, kts))
+        return out
+    alone = make()
+    obs_a = [b.run(9, lanes[i], renorm_interval=4, measure_interval=2) for i, b in enumerate(alone)]
+    both = make()
+    runs = [(b, (9, lanes[i]), {"renorm_interval": 4, "measure_interval": 2}) for i, b in enumerate(both)]
+    obs_c = run_concurrent(runs)
+    for i in range(len(groups)):
+        assert np.array_equal(obs_a[i], obs_c[i]), i
+        for k in range(len(groups[i])):
+            ca, za, Ea = alone[i].download(k)
+            cc, zc, Ec = both[i].download(k)
+            assert np.array_equal(ca, cc) and np.array_equal(za, zc) and Ea == Ec, (i, k)
+            assert alone[i].info(k).acc_metro == both[i].info(k).acc_metro
+    for i, b in enumerate(alone):
+        b.run(5, lanes[i], renorm_interval=4, phase_done=9)
+    run_concurrent([(b, (5, lanes[i]), {"renorm_interval": 4, "phase_done": 9}) for i, b in enumerate(both)])
+    for i in range(len(groups)):
+        for k in range(len(groups[i])):
+            assert np.array_equal(alone[i].download(k)[1], both[i].download(k)[1]), (i, k)
+
+
 def test_swap_stats_writes_exactly_n_chains_minus_one():
     """sce_batch_swap_stats fills n_chains - 1 entries (pair k, k + 1), the length
     the header documents; nothing past them is written."""
