@@ -355,9 +355,10 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
     its own seeded body-2 l<=2 (+ body-3 l=1 triangles on bcc) model, kT and a
     Zeeman term (2 mu_B moments, field 0-10 T along a random axis).  Chains run in
     three size classes (N <= 10, <= 15, <= 20), one batch per class, with lanes
-    per site and launch form per class picked by the autotuner.  The timed run
-    sweeps the three batches at once, each on its own stream
-    (run_concurrent); each class is also timed alone."""
+    per site per class picked by the autotuner.  The timed run sweeps the
+    three batches at once, each on its own stream (run_concurrent, per-colour
+    launches: a persistent grid cannot share the GPU); each class is also
+    timed alone.  With form "persistent" the classes run one after another."""
     from sce_amd.batch import GPUBatch, autotune_batch, run_concurrent
 
     rng = np.random.default_rng(20261203)
@@ -385,7 +386,8 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
             b.set_graph(graph)
             return b
         tune = {}
-        lanes, pers = autotune_batch(make, _batch_candidates(form), sweeps=10, timings=tune)
+        cands = _batch_candidates("graph" if form == "auto" else form)
+        lanes, pers = autotune_batch(make, cands, sweeps=10, timings=tune)
         b = make()
         b.set_persistent(pers)
         b.run(5, lanes, 0)
@@ -400,11 +402,17 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
                                "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()}}
         runs.append((b, (sweeps, lanes, 1000), {"phase_done": sweeps}))
     # the timed run: every class at once, each batch on its own stream
+    together = not any(r[0]._persistent for r in runs)
     t0 = time.perf_counter()
-    run_concurrent(runs)
+    if together:
+        run_concurrent(runs)
+    else:
+        for b, a, k in runs:
+            b.run(*a, **k)
     t_all = time.perf_counter() - t0
     return {"workload": f"{n_chains} chains, sc/bcc N^3 with N in 6..20, {total_sites} spins, per-chain model, "
-                        "kT and Zeeman field; three size classes sweeping at once on their own streams",
+                        "kT and Zeeman field; three size classes "
+                        + ("sweeping at once on their own streams" if together else "one after another"),
             "spin_updates_per_s": total_sites * sweeps / t_all, "ms_per_sweep": 1e3 * t_all / sweeps,
             "spin_updates_per_s_classes_one_after_another": total_sites * sweeps / t_seq,
             "sweeps": sweeps, "classes": out}
@@ -417,12 +425,15 @@ def compound_leg(S, M, sweeps=10000, equil=1000, kts=(0.01, 0.02, 0.04, 0.08)):
     each of four fixed kT after `equil` sweeps with run_mc's step adaptation.
     Every compound sweep is one replayed hipGraph (measurement fused into the
     overrelaxation reduction, the next sweep's proposals made there too);
-    Metropolis and overrelaxation attempts per second are reported apart."""
-    from sce_amd.batch import _adapt
+    Metropolis and overrelaxation attempts per second are reported apart.
+    Each kT's chain is timed alone; then the four chains sweep `sweeps` more
+    at once, each on its own stream (run_concurrent)."""
+    from sce_amd.batch import _adapt, run_concurrent
     na, terms, dims = M.config_model("bcc16", seed=0)
     H = S.TiledHamiltonian(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
     out = []
+    chains = []
     for k, kT in enumerate(kts):
         st = S.ChainState(H, S.random_config(H.n_sites, np.random.default_rng(100 + k)), 0.6)
         g = S.GPUChainState(gH, st, seed=9000 + k)
@@ -457,10 +468,25 @@ def compound_leg(S, M, sweeps=10000, equil=1000, kts=(0.01, 0.02, 0.04, 0.08)):
             t0 = time.perf_counter()
             S.gpu_run_sweeps(g, gH, None, beta, 1000, renorm_interval=1000, or_per_metropolis=1, measure_interval=1)
             row["ms_per_compound_sweep_direct_launches"] = 1e3 * (time.perf_counter() - t0) / 1000
+            g.set_graph(True)
         out.append(row)
+        chains.append((g, beta))
+    # the four kT at once: each chain on its own stream
+    for g, _ in chains:
+        g.reset_counters()
+    runs = [(S.gpu_run_sweeps, (g, gH, None, beta, sweeps),
+             {"renorm_interval": 1000, "or_per_metropolis": 1, "measure_interval": 1}) for g, beta in chains]
+    t0 = time.perf_counter()
+    run_concurrent(runs)
+    wall = time.perf_counter() - t0
+    infs = [g.info() for g, _ in chains]
+    together = {"sweeps": sweeps, "ms_per_compound_sweep_of_all_chains": 1e3 * wall / sweeps,
+                "metropolis_attempts_per_s": sum(i.att_metro for i in infs) / wall,
+                "or_attempts_per_s": sum(i.att_or for i in infs) / wall}
     return {"workload": f"bcc16: bcc 16^3 x 2 atoms = {H.n_sites} spins, body-2 l<=2 to 3rd shell + l=1 "
                         "triangles; compound sweep = 1 Metropolis + 1 overrelaxation pass",
-            "sweeps_per_kT": sweeps, "equilibration_sweeps": equil, "colours": H.n_colors, "by_kT": out}
+            "sweeps_per_kT": sweeps, "equilibration_sweeps": equil, "colours": H.n_colors, "by_kT": out,
+            "all_kT_at_once": together}
 
 
 def setup_leg(S, sweeps=200, graph=True):

@@ -50,6 +50,7 @@ class GPUBatch:
                                          C.byref(h)))
         self._h = h
         self.phase_done = 0
+        self._graph, self._persistent = True, False
 
     def run(self, nsweeps: int, lanes: int = DEFAULT_LANES, renorm_interval: int = 1000,
             measure_interval: int = 0, exchange_interval: int = 0, phase_done: int = 0,
@@ -97,12 +98,14 @@ class GPUBatch:
 
     def set_graph(self, on: bool) -> None:
         L.check(L.lib().sce_batch_set_graph(self._h, 1 if on else 0))
+        self._graph = bool(on)
 
     def set_persistent(self, on: bool) -> None:
         """One persistent launch per renormalisation segment for every chain,
         exchange included; bit-identical to the per-colour launches at the same
         lanes per site."""
         L.check(L.lib().sce_batch_set_persistent(self._h, 1 if on else 0))
+        self._persistent = bool(on)
 
     def set_shared_walk(self, chains: int) -> None:
         """At 64 lanes per site, walk each site's sublattice walk once for
@@ -402,28 +405,46 @@ def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
 
 
 def run_concurrent(runs):
-    """Run several batches at once: `runs` holds (batch, args, kwargs) for
-    GPUBatch.run, each batch at most once.  Every batch sweeps on its own
-    stream from its own host thread (the library calls release the GIL), so
-    the batches share the GPU; each batch's results are those of running it
-    alone.  Returns the run() results in order; re-raises the first error."""
-    batches = [r[0] for r in runs]
-    if len({id(b) for b in batches}) != len(batches):
-        raise ValueError("a batch can appear only once in a concurrent run")
+    """Run several batches or chains at once: `runs` holds (target, args,
+    kwargs), where target is a GPUBatch (its run method is called) or a
+    function such as gpu_run_sweeps whose first argument is the chain it
+    sweeps; each batch or chain appears at most once.  Every run sweeps on its
+    batch's or chain's own stream from its own host thread (the library calls
+    release the GIL), so they share the GPU; each one's results are those of
+    running it alone.  For the duration the runs launch their kernels directly
+    instead of replaying captured graphs (a stream capture in one thread is
+    invalidated by a blocking HIP call in another; both forms give the same
+    bits), and persistent launches are refused: a cooperative grid needs all
+    its blocks resident, which work sharing the GPU can prevent.  Returns the
+    results in order; re-raises the first error."""
+    owners = [t if hasattr(t, "run") else (a[0] if a else t) for t, a, _ in runs]
+    if len({id(o) for o in owners}) != len(owners):
+        raise ValueError("a batch or chain can appear only once in a concurrent run")
+    if any(getattr(o, "_persistent", False) for o in owners):
+        raise ValueError("persistent launches cannot share the GPU; use per-colour launches")
+    graphs = [getattr(o, "_graph", None) for o in owners]
     out = [None] * len(runs)
     err = [None] * len(runs)
 
-    def work(i, b, args, kwargs):
+    def work(i, t, args, kwargs):
         try:
-            out[i] = b.run(*args, **kwargs)
+            out[i] = t.run(*args, **kwargs) if hasattr(t, "run") else t(*args, **kwargs)
         except BaseException as e:  # re-raised on the calling thread
             err[i] = e
-    threads = [threading.Thread(target=work, args=(i, b, tuple(a), dict(k)))
-               for i, (b, a, k) in enumerate(runs)]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
+    try:
+        for o, g in zip(owners, graphs):
+            if g:
+                o.set_graph(False)
+        threads = [threading.Thread(target=work, args=(i, t, tuple(a), dict(k)))
+                   for i, (t, a, k) in enumerate(runs)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    finally:
+        for o, g in zip(owners, graphs):
+            if g:
+                o.set_graph(True)
     for e in err:
         if e is not None:
             raise e

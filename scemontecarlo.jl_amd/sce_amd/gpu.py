@@ -73,6 +73,7 @@ class GPUChainState:
         L.check(L.lib().sce_chain_create(gH.handle, L.ptr(cfg), L.ptr(z), float(st.energy),
                                          C.c_uint64(self.seed), float(st.step), C.byref(h)))
         self._h = h
+        self._graph, self._persistent = True, False
 
     @property
     def handle(self):
@@ -138,6 +139,7 @@ class GPUChainState:
     def set_persistent(self, on: bool) -> None:
         """Persistent cooperative launches (opt-in) or per-colour launches (default)."""
         L.check(L.lib().sce_chain_set_persistent(self._h, 1 if on else 0))
+        self._persistent = bool(on)
 
     def binner_reset(self, bin_size: int = 0, nbins: int = 0) -> None:
         """Zero the device binner and feed it every later measurement row
@@ -181,6 +183,7 @@ class GPUChainState:
     def set_graph(self, on: bool) -> None:
         """Replay sweeps as hipGraphs (default) or launch their kernels directly."""
         L.check(L.lib().sce_chain_set_graph(self._h, 1 if on else 0))
+        self._graph = bool(on)
 
     def set_timing(self, on: bool) -> None:
         L.check(L.lib().sce_chain_set_timing(self._h, 1 if on else 0))

@@ -289,6 +289,34 @@ def test_concurrent_batches_equal_running_them_alone():
             assert np.array_equal(alone[i].download(k)[1], both[i].download(k)[1]), (i, k)
 
 
+def test_concurrent_chains_equal_running_them_alone():
+    """run_concurrent over single chains on one model (compound Metropolis +
+    overrelaxation sweeps and plain ones, at different kT): the same bits and
+    measurements as running each chain alone."""
+    from sce_amd.batch import run_concurrent
+    na, terms, dims = bcc_tet_small(3, dims=(4, 4, 4))
+    H = S.TiledHamiltonian(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    kts = [0.01, 0.02, 0.04]
+
+    def make():
+        return [S.GPUChainState(gH, S.ChainState(H, rand_config(H.n_sites, 60 + k), 0.7), seed=800 + k)
+                for k in range(len(kts))]
+    kw = [{"renorm_interval": 5, "or_per_metropolis": 1, "measure_interval": 1},
+          {"renorm_interval": 5, "measure_interval": 2}, {"renorm_interval": 0, "or_per_metropolis": 1}]
+    alone = make()
+    obs_a = [S.gpu_run_sweeps(g, gH, None, 1.0 / kts[k], 12, **kw[k]) for k, g in enumerate(alone)]
+    both = make()
+    obs_c = run_concurrent([(S.gpu_run_sweeps, (g, gH, None, 1.0 / kts[k], 12), kw[k]) for k, g in enumerate(both)])
+    for k in range(len(kts)):
+        assert np.array_equal(obs_a[k], obs_c[k]), k
+        ca, za, Ea = alone[k].download()
+        cc, zc, Ec = both[k].download()
+        assert np.array_equal(ca, cc) and np.array_equal(za, zc) and Ea == Ec, k
+        ia, ic = alone[k].info(), both[k].info()
+        assert (ia.acc_metro, ia.acc_or, ia.sweep_index) == (ic.acc_metro, ic.acc_or, ic.sweep_index), k
+
+
 def test_swap_stats_writes_exactly_n_chains_minus_one():
     """sce_batch_swap_stats fills n_chains - 1 entries (pair k, k + 1), the length
     the header documents; nothing past them is written."""
