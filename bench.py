@@ -315,9 +315,10 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto"):
     tune = {}
     # lanes per site and launch form (per-colour graphs or one persistent launch
     # per segment): scratch batches, before the run
-    lanes, pers = autotune_batch(make, _batch_candidates(form), sweeps=20, timings=tune)
+    lanes, pers, share = autotune_batch(make, _batch_candidates(form), sweeps=20, timings=tune)
     b = make()
     b.set_persistent(pers)
+    b.set_shared_walk(share)
     steps = [0.6] * R
     done = 0
     while done < sweeps_therm:            # untimed: exchange + step adaptation (gpu_pt.jl:115-140)
@@ -341,13 +342,12 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto"):
             "bytes_per_update": algorithmic_bytes_per_update(H),
             "algorithmic_GBs": R * H.n_active * sweeps_measure / wall * algorithmic_bytes_per_update(H) / 1e9,
             "ms_per_sweep_all_replicas": 1e3 * wall / sweeps_measure, "sweeps": sweeps_measure,
-            "lanes": lanes, "persistent": pers, "measurements": int(obs.shape[1]),
+            "lanes": lanes, "persistent": pers, "shared_walk": share, "measurements": int(obs.shape[1]),
             "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()},
             "swap_acceptance_mean": float(np.nanmean(sw)), "swap_acceptance_min": float(np.nanmin(sw)),
             "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep; "
-                    "autotune keys: lanes per site, 'p' = one persistent launch per segment; at 64 lanes the "
-                    "per-colour launches walk each site once for two replicas (shared walk, results those "
-                    "of 64 lanes per site)"}
+                    "autotune keys: lanes per site, 'p' = one persistent launch per segment, 's' = two "
+                    "replicas per wavefront on one shared walk (results those of 64 lanes per site)"}
 
 
 def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
@@ -387,7 +387,7 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
             return b
         tune = {}
         cands = _batch_candidates("graph" if form == "auto" else form)
-        lanes, pers = autotune_batch(make, cands, sweeps=10, timings=tune)
+        lanes, pers, _ = autotune_batch(make, cands, sweeps=10, timings=tune)
         b = make()
         b.set_persistent(pers)
         b.run(5, lanes, 0)

@@ -51,6 +51,8 @@ class GPUBatch:
         self._h = h
         self.phase_done = 0
         self._graph, self._persistent = True, False
+        # every chain on one tables handle: replica exchange and shared walks
+        self.homogeneous = all(g is self.gHs[0] for g in self.gHs)
 
     def run(self, nsweeps: int, lanes: int = DEFAULT_LANES, renorm_interval: int = 1000,
             measure_interval: int = 0, exchange_interval: int = 0, phase_done: int = 0,
@@ -261,11 +263,14 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
     if lanes == "auto":
         # tuned on a throwaway copy of the ladder, then pinned in the plan (and
         # so in every checkpoint and the result) like the reference's workgroupsize
-        cands = _BATCH_CANDIDATES if persistent is None else tuple((n, bool(persistent)) for n in (16, 32, 64))
-        plan.lanes, persistent = autotune_batch(lambda: _pt_batch(gH, plan, init), cands, sweeps=20)
+        cands = (_BATCH_CANDIDATES if persistent is None else
+                 tuple(c for c in _BATCH_CANDIDATES if c[1] == bool(persistent)))
+        plan.lanes, persistent, share = autotune_batch(lambda: _pt_batch(gH, plan, init), cands, sweeps=20)
     ck = _Checkpointer(checkpoint, checkpoint_interval) if checkpoint is not None else None
     b = _pt_batch(gH, plan, init)
     b.set_persistent(bool(persistent))
+    if lanes == "auto":
+        b.set_shared_walk(share)
     res = _pt_run(b, gH, plan, "therm", 0, None, ck)
     res.persistent = bool(persistent)
     return res
@@ -367,26 +372,34 @@ def resume_pt(path: str, gH: GPUTiledHamiltonian, checkpoint: Optional[str] = "s
     return _pt_run(st["batch"], gH, st["plan"], st["phase"], st["done"], st["rows"], ck)
 
 
-# (lanes per site, persistent launch) forms the batch autotuner tries
-_BATCH_CANDIDATES = ((16, False), (32, False), (64, False), (16, True), (32, True), (64, True))
+# (lanes per site, persistent launch, chains per shared walk) forms the batch
+# autotuner tries; the shared walk only where every chain is on one model
+_BATCH_CANDIDATES = ((16, False, 1), (32, False, 1), (64, False, 1), (64, False, 2),
+                     (16, True, 1), (32, True, 1), (64, True, 1))
 
 
 def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
                    timings: Optional[dict] = None):
     """The project autotuner for batches: times `sweeps` sweeps of a scratch
-    batch built by `make_batch()` in each (lanes per site, persistent) form —
-    per-colour launches replayed as hipGraphs, or one persistent launch per
-    segment, each at 16/32/64 lanes (sites per wavefront = 64 / lanes) — and
-    returns the fastest as (lanes, persistent).  `lanes` is part of
-    the determinism scope, so tuning runs on a throwaway batch before the
+    batch built by `make_batch()` in each (lanes per site, persistent, chains
+    per shared walk) form — per-colour launches replayed as hipGraphs, at 64
+    lanes also with two chains of a batch on one model walking each site once
+    (GPUBatch.set_shared_walk), or one persistent launch per segment, each at
+    16/32/64 lanes (sites per wavefront = 64 / lanes) — and returns the
+    fastest as (lanes, persistent, shared_walk).  `lanes` is part of the
+    determinism scope, so tuning runs on a throwaway batch before the
     production run and the caller records the choice with the run; the launch
-    form is not.  `timings` receives the seconds per sweep of every candidate
-    that ran, keyed "16", "32", "64" and "64p"."""
+    form and the shared walk are not.  `timings` receives the seconds per
+    sweep of every candidate that ran, keyed "16", "32", "64", "64s" (shared
+    walk) and "64p"."""
     best, best_t = None, math.inf
-    for lanes, pers in candidates:
+    for lanes, pers, share in candidates:
         b = make_batch()
+        if share > 1 and not b.homogeneous:
+            continue
         try:
             b.set_persistent(pers)
+            b.set_shared_walk(share)
             b.run(3, lanes, 0)          # capture + warm
             t0 = time.perf_counter()
             b.run(sweeps, lanes, 0)
@@ -396,9 +409,9 @@ def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
         finally:
             del b
         if timings is not None:
-            timings[f"{lanes}{'p' if pers else ''}"] = dt / sweeps
+            timings[f"{lanes}{'p' if pers else ''}{'s' if share > 1 else ''}"] = dt / sweeps
         if dt < best_t:
-            best, best_t = (int(lanes), bool(pers)), dt
+            best, best_t = (int(lanes), bool(pers), int(share)), dt
     if best is None:
         raise RuntimeError("no candidate form could run this batch")
     return best
@@ -460,7 +473,7 @@ def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache:
     if cache is not None and key is not None and key in cache:
         return cache[key]
     t = {} if timings is not None else None
-    lanes, _ = autotune_batch(make_batch, tuple((n, False) for n in candidates), sweeps, t)
+    lanes, _, _ = autotune_batch(make_batch, tuple((n, False, 1) for n in candidates), sweeps, t)
     if timings is not None:
         timings.update({int(k): v for k, v in t.items()})
     if cache is not None and key is not None:
