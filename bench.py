@@ -402,7 +402,7 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
                                "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()}}
         runs.append((b, (sweeps, lanes, 1000), {"phase_done": sweeps}))
     # the timed run: every class at once, each batch on its own stream
-    together = not any(r[0]._persistent for r in runs)
+    together = not any(c["persistent"] for c in out.values())
     t0 = time.perf_counter()
     if together:
         run_concurrent(runs)
