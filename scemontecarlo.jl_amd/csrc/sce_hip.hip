@@ -593,7 +593,13 @@ __global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__
 // l = 1 axis h = (c[3], c[1], c[2]), exact ΔE = Σ_k c_k (znew_k - zold_k)
 // TEST: writes c only.
 // ---------------------------------------------------------------------------
-template <int L, int LMAX, bool TEST>
+// SLOTS: the row sums accumulate in lane-private LDS slots (row-major, one
+// per lane and row after the group's staged block) with LDS adds, each entry
+// added to its row's slot, over the padded walk; otherwise in registers, every
+// entry's term selected into its row.  Each lane's slot of a row takes that
+// lane's terms in walk order either way (the LDS applies a wavefront's
+// operations in order), so both give the same sums.
+template <int L, int LMAX, bool TEST, bool SLOTS>
 __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ config,
                                             double* __restrict__ zrows, double* __restrict__ dE,
                                             int* __restrict__ acc, int* __restrict__ att,
@@ -603,12 +609,15 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), G = TPB / L;
   extern __shared__ double lds[];
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
-  const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
+  int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + g;
+  // one site per wavefront at 64 lanes: its index and site are wave-uniform
+  if constexpr (L == 64) qi = __builtin_amdgcn_readfirstlane(qi);
   const int q = q_lo + qi;
   int s = 0, atom = 0;
   bool valid = qi < q_n;
   if (valid) {
     s = TEST ? test_sites[q] : T.color_sites[q];
+    if constexpr (L == 64) s = __builtin_amdgcn_readfirstlane(s);
     if (!TEST && !T.has_l1[s]) {
       if (lane == 0) {
         dE[q] = 0.0;
@@ -618,8 +627,9 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
       valid = false;
     }
   }
-  const int stride = T.nb_max * NLM + 1 + NLM;
+  const int stride = T.nb_max * NLM + 1 + NLM + (SLOTS ? L * NLM : 0);
   double* Z = lds + g * stride;
+  double* slot = Z + T.nb_max * NLM + 1 + NLM;  // SLOTS: slot[k * L + lane]
   // the site's own spin and row and its keyed accept uniform do not depend on
   // the walk: read and drawn while the neighbour rows stage
   double e0[3] = {0.0, 0.0, 0.0}, zo0[NLM], u_or = 0.0;
@@ -643,11 +653,64 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
       });
     }
   }
-  wave_lds_sync();  // per-group LDS: the own wavefront only
   double c[NLM];
 #pragma unroll
   for (int k = 0; k < NLM; ++k) c[k] = 0.0;
-  if (valid) {
+  if constexpr (SLOTS) {
+#pragma unroll
+    for (int k = 0; k < NLM; ++k) slot[k * L + lane] = 0.0;
+  }
+  wave_lds_sync();  // per-group LDS: the own wavefront only
+  if (SLOTS && valid) {
+    // the padded walk (no-op padding, pair / two-factor prefixes as in
+    // walk_delta); an entry's target byte offset (dz0 + k) * 8 becomes its
+    // slot (k * L + lane) * 8: ((off - dz0 * 8) << log2 L) + lane * 8
+    constexpr int B = WALK_UNROLL * L, SH = L == 16 ? 4 : (L == 32 ? 5 : 6);
+    const int w0 = T.walkp_ptr[atom];
+    const int Wp = T.walkp_ptr[atom + 1] - w0;
+    const int2 pre = T.walkp_pre[atom];
+    const WalkWords* __restrict__ ww = reinterpret_cast<const WalkWords*>(T.walkp + w0) + lane;
+    const unsigned zb = (unsigned)(size_t)(lds_cdouble*)Z;
+    const unsigned sb = (unsigned)(size_t)(lds_cdouble*)slot + (unsigned)lane * 8u -
+                        ((unsigned)(T.nb_max * NLM + 1) * 8u << SH);
+    typedef __attribute__((address_space(3))) double lds_double;
+    auto add_to_row = [&](unsigned f2t, double v) {
+      lds_double* a = (lds_double*)(size_t)(sb + ((f2t >> 16) << SH));
+      __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    };
+    const int n1 = pre.x - pre.x % B, n2 = pre.y - pre.y % B;
+    int base = 0;
+    for (; base < n1; base += B) {
+      WalkWords en[WALK_UNROLL];
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL; ++u) add_to_row(en[u].f2t, en[u].w * lds_at(lds_add_lo(zb, en[u].f01)));
+    }
+    for (; base < n2; base += B) {
+      WalkWords en[WALK_UNROLL];
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL; ++u)
+        add_to_row(en[u].f2t, en[u].w * (lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01))));
+    }
+    for (; base < Wp; base += B) {
+      WalkWords en[WALK_UNROLL];
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL; ++u)
+        add_to_row(en[u].f2t, en[u].w * ((lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01))) *
+                                         lds_at(lds_add_lo(zb, en[u].f2t))));
+    }
+  }
+  if constexpr (SLOTS) {
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < NLM; ++k) c[k] = slot[k * L + lane];
+  }
+  if (!SLOTS && valid) {
     const int w0 = T.walk_ptr[atom];
     const int W = T.walk_ptr[atom + 1] - w0;
     const WalkEntry* __restrict__ wl = T.walk + w0;
@@ -2506,9 +2569,16 @@ template <int L, int LMAX, bool TEST>
 static void launch_or_l(const DevTables& D, double* config, double* zrows, double* dE, int* acc,
                         int* att, const ChainDev* st, int pass, int q_lo, int q_n, const int* ts,
                         double* cout, hipStream_t s) {
-  constexpr int G = TPB / L;
+  constexpr int G = TPB / L, NLM = (LMAX + 1) * (LMAX + 1);
   const int nb = (q_n + G - 1) / G;
-  if (nb) k_or<L, LMAX, TEST><<<nb, TPB, walk_lds_bytes(D, L), s>>>(D, config, zrows, dE, acc, att, st, pass, q_lo, q_n, ts, cout);
+  if (!nb) return;
+  // the LDS slots where the staged blocks leave room for them
+  const size_t slots = walk_lds_bytes(D, L) + (size_t)G * L * NLM * sizeof(double);
+  if (slots <= 65536 && D.walkp)
+    k_or<L, LMAX, TEST, true><<<nb, TPB, slots, s>>>(D, config, zrows, dE, acc, att, st, pass, q_lo, q_n, ts, cout);
+  else
+    k_or<L, LMAX, TEST, false><<<nb, TPB, walk_lds_bytes(D, L), s>>>(D, config, zrows, dE, acc, att, st, pass, q_lo,
+                                                                    q_n, ts, cout);
 }
 template <int LMAX, bool TEST>
 static void launch_or(int L, const DevTables& D, double* config, double* zrows, double* dE,

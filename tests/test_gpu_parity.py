@@ -13,7 +13,7 @@ import oracle as O
 import sce_amd as S
 from sce_amd import gpu as G
 from sce_amd._lib import SceError
-from fixtures import build_pair, fixture_cases, rand_config, bcc_small
+from fixtures import build_pair, fixture_cases, rand_config, bcc_small, bcc_tet_small
 
 pytestmark = pytest.mark.gpu
 
@@ -92,6 +92,29 @@ def test_keyed_proposal_bitwise():
         assert np.array_equal(r2.view(np.uint64), e2[i].view(np.uint64)) and ru == u[i]
         nflip += int(np.array_equal(r2, -e[i]))
     assert 0.1 * n < nflip < 0.3 * n
+
+
+@pytest.mark.parametrize("scale", [1e-300, 1e-310])
+def test_overrelaxation_row_sums_in_the_subnormal_range(scale):
+    """The overrelaxation walk's row sums (leave-one-out coefficients) where
+    the terms and partial sums are tiny (1e-300) or subnormal (1e-310): the
+    device's row sums, formed by LDS adds into lane-private slots, equal the
+    oracle's register sums bit for bit at every lane count."""
+    import dataclasses
+    na, terms, dims = bcc_tet_small(5)
+    terms = [dataclasses.replace(t, coef=t.coef * scale) for t in terms]
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    z = H.zrows(rand_config(H.n_sites, 41))
+    sites = np.array(T.color_sites[: min(64, T.n_active)], np.int32)
+    for lanes in LANES:
+        cdev = G.dev_walk_coeffs(gH, z, sites, lanes)
+        for i, s in enumerate(sites):
+            cref = T.walk_coeffs(int(s), z, lanes)
+            assert np.array_equal(cdev[i].view(np.uint64), cref.view(np.uint64)), (scale, lanes, s)
+        if scale < 2.3e-308:
+            nz = cdev[cdev != 0.0]
+            assert nz.size and np.abs(nz).max() < 2.3e-308  # subnormal sums were formed
 
 
 @pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
