@@ -418,7 +418,7 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
             "sweeps": sweeps, "classes": out}
 
 
-def compound_leg(S, M, sweeps=10000, equil=1000, kts=(0.01, 0.02, 0.04, 0.08)):
+def compound_leg(S, M, sweeps=10000, equil=1000, kts=(0.01, 0.02, 0.04, 0.08), graph=True):
     """Plain form on the compound sweep of run_mc (src/run.jl:143-150): bcc 16^3
     (8192 spins, body-2 l<=2 to the 3rd shell + l=1 triangles), each sweep one
     Metropolis sweep then one overrelaxation pass, `sweeps` measured sweeps at
@@ -437,6 +437,7 @@ def compound_leg(S, M, sweeps=10000, equil=1000, kts=(0.01, 0.02, 0.04, 0.08)):
     for k, kT in enumerate(kts):
         st = S.ChainState(H, S.random_config(H.n_sites, np.random.default_rng(100 + k)), 0.6)
         g = S.GPUChainState(gH, st, seed=9000 + k)
+        g.set_graph(graph)
         beta = 1.0 / kT
         done = 0
         while done < equil:
@@ -468,7 +469,7 @@ def compound_leg(S, M, sweeps=10000, equil=1000, kts=(0.01, 0.02, 0.04, 0.08)):
             t0 = time.perf_counter()
             S.gpu_run_sweeps(g, gH, None, beta, 1000, renorm_interval=1000, or_per_metropolis=1, measure_interval=1)
             row["ms_per_compound_sweep_direct_launches"] = 1e3 * (time.perf_counter() - t0) / 1000
-            g.set_graph(True)
+            g.set_graph(graph)
         out.append(row)
         chains.append((g, beta))
     # the four kT at once: each chain on its own stream
@@ -626,7 +627,7 @@ def main():
               flush=True)
         return
     if args.compound_only:
-        print(json.dumps({"compound": compound_leg(S, M)}), flush=True)
+        print(json.dumps({"compound": compound_leg(S, M, graph=not args.no_graph)}), flush=True)
         return
     if args.setup_model_only:
         print(json.dumps({"setup_model": setup_leg(S, graph=not args.no_graph)}), flush=True)
@@ -755,7 +756,7 @@ def main():
         line["batched"] = {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph, form=args.batch_form),
                            "uneven_batch": uneven_leg(S, M, graph=not args.no_graph, form=args.batch_form)}
     if not args.no_compound:
-        line["compound"] = compound_leg(S, M)
+        line["compound"] = compound_leg(S, M, graph=not args.no_graph)
     if not args.no_setup_model:
         line["setup_model"] = setup_leg(S, graph=not args.no_graph)
     if not args.no_cpu:
