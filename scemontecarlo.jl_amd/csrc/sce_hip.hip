@@ -386,6 +386,27 @@ __device__ __forceinline__ double fold_lanes(double a) {
   return a;
 }
 
+// fold_lanes<L> of the lane partials a row of LDS slots holds (slot lane of
+// `row`): the shuffle steps of 32 and 16 read the partials they add straight
+// from the slots, ((s[i] + s[i + 32]) + (s[i + 16] + s[i + 48])) at 64 lanes
+// (s[i] + s[i + 16] at 32), the same additions in the same order for the
+// lanes that feed the first; then the same DPP steps.
+template <int L>
+__device__ __forceinline__ double fold_slots(const double* row, int lane) {
+  double a;
+  if constexpr (L == 64)
+    a = (row[lane] + row[(lane + 32) & 63]) + (row[(lane + 16) & 63] + row[(lane + 48) & 63]);
+  else if constexpr (L == 32)
+    a = row[lane] + row[(lane + 16) & 31];
+  else
+    a = row[lane];
+  a = a + dpp_double<0x108>(a);
+  a = a + dpp_double<0x104>(a);
+  a = a + dpp_double<0x102>(a);
+  a = a + dpp_double<0x101>(a);
+  return a;
+}
+
 // ---------------------------------------------------------------------------
 // the colour walk: `L` lanes per site, 256/L sites per block.
 // Lane `lane` visits entries i = lane, lane + L, lane + 2L, ... in ascending
@@ -708,7 +729,7 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
   if constexpr (SLOTS) {
     wave_lds_sync();
 #pragma unroll
-    for (int k = 0; k < NLM; ++k) c[k] = slot[k * L + lane];
+    for (int k = 0; k < NLM; ++k) c[k] = fold_slots<L>(slot + k * L, lane);
   }
   if (!SLOTS && valid) {
     const int w0 = T.walk_ptr[atom];
@@ -733,8 +754,10 @@ __global__ __launch_bounds__(TPB) void k_or(DevTables T, double* __restrict__ co
     }
   }
   // per row, the shuffle-down tree into the first lane (DPP for its last steps)
+  if constexpr (!SLOTS) {
 #pragma unroll
-  for (int k = 0; k < NLM; ++k) c[k] = fold_lanes<L>(c[k]);
+    for (int k = 0; k < NLM; ++k) c[k] = fold_lanes<L>(c[k]);
+  }
   if (!valid || lane != 0) return;
   if (TEST) {
 #pragma unroll
