@@ -284,7 +284,7 @@ def _batch_candidates(form):
     return tuple(c for c in _BATCH_CANDIDATES if form == "auto" or c[1] == (form == "persistent"))
 
 
-def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto"):
+def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto", lanes=None, share=None):
     """Batched parallel tempering: 64 replicas of fcc 12^3 (6912 spins, body-2 l<=2 to
     the 4th shell) on a geometric kT ladder spanning 0.2-2.0 J1, replica exchange
     on the device after every sweep.  The autotuner picks lanes per site and the
@@ -313,9 +313,12 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto"):
         return b
 
     tune = {}
-    # lanes per site and launch form (per-colour graphs or one persistent launch
-    # per segment): scratch batches, before the run
-    lanes, pers, share = autotune_batch(make, _batch_candidates(form), sweeps=20, timings=tune)
+    if lanes is None:
+        # lanes per site and launch form (per-colour graphs or one persistent
+        # launch per segment): scratch batches, before the run
+        lanes, pers, share = autotune_batch(make, _batch_candidates(form), sweeps=20, timings=tune)
+    else:   # pinned by the caller: per-colour launches
+        pers, share = False, 0 if share is None else share
     b = make()
     b.set_persistent(pers)
     b.set_shared_walk(share)
@@ -578,6 +581,13 @@ def main():
                     help="skip the single-chain autotuner report (its persistent candidate is a cooperative launch)")
     ap.add_argument("--compound-only", action="store_true",
                     help="run only the bcc 16^3 Metropolis + overrelaxation leg and print its JSON")
+    ap.add_argument("--pt-only", action="store_true",
+                    help="run only the 64-replica fcc 12^3 ladder, at --pt-lanes (autotuned when not given) and "
+                         "--pt-shared-walk, for --pt-sweeps thermalisation and measured sweeps, and print its JSON")
+    ap.add_argument("--pt-lanes", type=int, default=None)
+    ap.add_argument("--pt-shared-walk", type=int, default=None,
+                    help="replicas per shared walk at 64 lanes (0: the library default, 1: off, 2, 4)")
+    ap.add_argument("--pt-sweeps", type=int, default=None)
     ap.add_argument("--no-setup-model", action="store_true",
                     help="skip the Nd2Fe14B setup-file leg (nbody 3 at 4^3 and 8^3 cells)")
     ap.add_argument("--setup-model-only", action="store_true",
@@ -624,6 +634,13 @@ def main():
         print(json.dumps({"batched": {"parallel_tempering": pt_leg(S, M, graph=not args.no_graph, form=args.batch_form),
                                       "uneven_batch": uneven_leg(S, M, graph=not args.no_graph,
                                                                  form=args.batch_form)}}),
+              flush=True)
+        return
+    if args.pt_only:
+        n = args.pt_sweeps
+        kw = {} if n is None else {"sweeps_therm": n, "sweeps_measure": n}
+        print(json.dumps({"parallel_tempering": pt_leg(S, M, graph=not args.no_graph, form=args.batch_form,
+                                                       lanes=args.pt_lanes, share=args.pt_shared_walk, **kw)}),
               flush=True)
         return
     if args.compound_only:
