@@ -148,9 +148,6 @@ constexpr float SUR_QIN_UNIT = 127.0f, SUR_QIN_PAIRS = 42.0f;
 __host__ __device__ inline int sur_fpq(int F) { return (F + 63) / 64 * 64; }
 constexpr int TREE_B = 512;   // elements per tree chunk
 
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
-// walks a contiguous range of the colour class (neighbour rows then stay in
-// that XCD's L2).
 // A load from global memory through a pointer the compiler cannot place: the
 // batch kernels read their chains' pointers from device memory, and loads
 // through those generic pointers would compile to flat instructions, which
@@ -166,6 +163,9 @@ __device__ __forceinline__ int2 ldg_int2(const int2* p) {
   return make_int2((int)v, (int)(v >> 32));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks a contiguous range of the colour class (neighbour rows then stay in
+// that XCD's L2).
 __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
   const int x = b & 7, l = b >> 3, per = nb >> 3, rem = nb & 7;
   return x * per + (x < rem ? x : rem) + l;
@@ -445,7 +445,6 @@ __device__ __forceinline__ unsigned lds_add_hi(unsigned base, unsigned packed) {
   return r;
 }
 __device__ __forceinline__ double lds_at(unsigned addr) { return *(lds_cdouble*)(size_t)addr; }
-typedef int __attribute__((ext_vector_type(4))) walk_i32x4;
 typedef double __attribute__((ext_vector_type(2))) walk_f64x2;
 // One walk entry, as a 16-byte global load.
 __device__ __forceinline__ WalkWords ld_walk(const WalkWords* p) {
