@@ -302,7 +302,7 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto", 
     kts = J1 * np.geomspace(0.2, 2.0, R)
     H = S.TiledHamiltonian(na, terms, dims)
     gH = S.GPUTiledHamiltonian(H)
-    from sce_amd.batch import GPUBatch, derive_seeds, _adapt, autotune_batch
+    from sce_amd.batch import GPUBatch, derive_seeds, _adapt, autotune_batch, set_walk_form
     init, exch, dev = derive_seeds(2026, R)
 
     def make():
@@ -316,12 +316,17 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto", 
     if lanes is None:
         # lanes per site and launch form (per-colour graphs or one persistent
         # launch per segment): scratch batches, before the run
-        lanes, pers, share = autotune_batch(make, _batch_candidates(form), sweeps=20, timings=tune)
+        lanes, pers, walk = autotune_batch(make, _batch_candidates(form), sweeps=20, timings=tune)
     else:   # pinned by the caller: per-colour launches
-        pers, share = False, 0 if share is None else share
+        pers, walk = False, None
     b = make()
     b.set_persistent(pers)
-    b.set_shared_walk(share)
+    if walk is None:
+        b.set_shared_walk(0 if share is None else share)
+    else:
+        set_walk_form(b, walk)
+    walk_name = ({"": "one site", "s": "two replicas", "q": "site pairs"}[walk] if walk is not None else
+                 {None: "library default", 0: "library default", 1: "one site"}.get(share, f"{share} replicas"))
     steps = [0.6] * R
     done = 0
     while done < sweeps_therm:            # untimed: exchange + step adaptation (gpu_pt.jl:115-140)
@@ -345,12 +350,13 @@ def pt_leg(S, M, sweeps_therm=200, sweeps_measure=500, graph=True, form="auto", 
             "bytes_per_update": algorithmic_bytes_per_update(H),
             "algorithmic_GBs": R * H.n_active * sweeps_measure / wall * algorithmic_bytes_per_update(H) / 1e9,
             "ms_per_sweep_all_replicas": 1e3 * wall / sweeps_measure, "sweeps": sweeps_measure,
-            "lanes": lanes, "persistent": pers, "shared_walk": share, "measurements": int(obs.shape[1]),
+            "lanes": lanes, "persistent": pers, "walk": walk_name, "measurements": int(obs.shape[1]),
             "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()},
             "swap_acceptance_mean": float(np.nanmean(sw)), "swap_acceptance_min": float(np.nanmin(sw)),
             "note": "measurement phase: sweeps + device exchange + (E, m) of every replica every sweep; "
                     "autotune keys: lanes per site, 'p' = one persistent launch per segment, 's' = two "
-                    "replicas per wavefront on one shared walk (results those of 64 lanes per site)"}
+                    "replicas per wavefront on one shared walk, 'q' = two same-sublattice sites of a replica "
+                    "per wavefront (both: results those of 64 lanes per site)"}
 
 
 def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
@@ -362,7 +368,7 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
     three batches at once, each on its own stream (run_concurrent, per-colour
     launches: a persistent grid cannot share the GPU); each class is also
     timed alone.  With form "persistent" the classes run one after another."""
-    from sce_amd.batch import GPUBatch, autotune_batch, run_concurrent
+    from sce_amd.batch import GPUBatch, autotune_batch, run_concurrent, set_walk_form
 
     rng = np.random.default_rng(20261203)
     classes = {0: [], 1: [], 2: []}
@@ -390,9 +396,10 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
             return b
         tune = {}
         cands = _batch_candidates("graph" if form == "auto" else form)
-        lanes, pers, _ = autotune_batch(make, cands, sweeps=10, timings=tune)
+        lanes, pers, walk = autotune_batch(make, cands, sweeps=10, timings=tune)
         b = make()
         b.set_persistent(pers)
+        set_walk_form(b, walk)
         b.run(5, lanes, 0)
         t0 = time.perf_counter()
         b.run(sweeps, lanes, 1000)
@@ -400,6 +407,7 @@ def uneven_leg(S, M, sweeps=100, n_chains=256, graph=True, form="auto"):
         t_seq += dt
         n_sites = sum(H.n_active for H, _, _, _ in chains)
         out[f"class_{cls}"] = {"chains": len(chains), "spins": n_sites, "lanes": lanes, "persistent": pers,
+                               "walk": {"": "one site", "q": "site pairs"}.get(walk, walk),
                                "ms_per_sweep_alone": 1e3 * dt / sweeps,
                                "spin_updates_per_s_alone": n_sites * sweeps / dt,
                                "autotune_ms_per_sweep": {k: 1e3 * v for k, v in tune.items()}}

@@ -257,6 +257,14 @@ int32_t sce_batch_set_persistent(sce_batch* b, int32_t on);
  * The reference has no counterpart: its rungs run one after another on one
  * queue (src/gpu/gpu_pt.jl:4-15). */
 int32_t sce_batch_set_shared_walk(sce_batch* b, int32_t chains);
+/* At 64 lanes per site, one wavefront walks two sites of one chain's colour
+ * class on the same sublattice (on != 0), for batches whose chains do not
+ * share walks across chains (sce_batch_set_shared_walk off or inapplicable):
+ * each entry of the sublattice walk is loaded once for both sites; the results
+ * are those of 64 lanes per site.  Per-colour launches only; off by default.
+ * The reference has no counterpart (its device kernel takes one site per
+ * work-item group, src/gpu/gpu_sweep.jl). */
+int32_t sce_batch_set_site_pairs(sce_batch* b, int32_t on);
 /* Checkpoint restore of a "gpu_pt" run (resume, src/checkpoint.jl:555-635).
  * set_counters: the lane-owned state of chain k that batch creation does not
  * take — its keyed-RNG sweep index, acceptance counters and drift record

@@ -1896,6 +1896,7 @@ struct sce_tables {
   DevBuf mem;
   DevTables dev{};
   std::vector<int32_t> color_ptr;  // 0-based offsets, n_colors + 1
+  std::vector<int32_t> color_sites;  // 0-based site ids, colour order (host copy)
   int32_t* d_color_ptr = nullptr;
   int max_class = 0;
   int fin_col = -1;  // the smallest non-empty colour class: its launch carries a deferred final
@@ -2286,6 +2287,7 @@ extern "C" int32_t sce_tables_create(const sce_tables_desc* d, sce_tables** out)
   int32_t* p32;
   if ((r = dupload(t->mem, &p32, cs.data(), cs.size())) != SCE_OK) return fail(r);
   D.color_sites = p32;
+  t->color_sites = cs;
   uint8_t* p8;
   if ((r = dupload(t->mem, &p8, d->site_has_l1, (size_t)d->n_sites)) != SCE_OK) return fail(r);
   D.has_l1 = p8;
@@ -4123,6 +4125,93 @@ __global__ __launch_bounds__(TPB) void k_bmetro_shared(const BChain* __restrict_
   }
 }
 
+// One colour step of any batch with two sites per wavefront: pair w of the
+// colour's table is {chain, q1, q2} with q1 and q2 positions of that chain's
+// class on one sublattice (q2 = -1: q1 alone), so both sites walk the same
+// sublattice walk, once (walk_delta_shared), 64 lanes per site each.  Each
+// site's update is the one k_bmetro<64> makes, so the results are those of
+// 64 lanes per site.
+template <int LMAX>
+__global__ __launch_bounds__(TPB) void k_bmetro_pair(const BChain* __restrict__ ch, const int4* __restrict__ pairs,
+                                                     int npairs, int stride) {
+  constexpr int K = 2, NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
+  extern __shared__ double lds[];
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+  const int gw = __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * (TPB / 64) + wv);
+  const int lane = (int)threadIdx.x % 64;
+  const bool wvalid = gw < npairs;
+  const int4 pr = pairs[wvalid ? gw : 0];
+  const int chain = __builtin_amdgcn_readfirstlane(pr.x);
+  int q[K] = {__builtin_amdgcn_readfirstlane(pr.y), __builtin_amdgcn_readfirstlane(pr.z)};
+  const bool valid[K] = {wvalid, wvalid && q[1] >= 0};
+  if (!valid[1]) q[1] = q[0];
+  const BChain* c = &ch[chain];
+  const DevTables& T = c->T;
+  const double* prop = c->prop;
+  const double* zr = c->zrows;
+  double* Zw = lds + (size_t)wv * K * stride;
+  const int dzo = T.nb_max * NLM + 1;
+  unsigned zb[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) zb[k] = (unsigned)(size_t)(lds_cdouble*)(Zw + (size_t)k * stride);
+  int s[K] = {0, 0};
+  const WalkEntry* wl = nullptr;
+  int W = 0;
+  int2 pre = make_int2(0, 0);
+  double u_acc[K] = {0.0, 0.0};
+  if (wvalid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) s[k] = __builtin_amdgcn_readfirstlane(ldg(T.color_sites + q[k]));
+    const int a0 = s[0] % T.na;
+    const int w0 = ldg(T.walkp_ptr + a0);
+    W = ldg(T.walkp_ptr + a0 + 1) - w0;
+    wl = T.walkp + w0;
+    pre = ldg_int2(T.walkp_pre + a0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!valid[k]) continue;
+      double* Z = Zw + (size_t)k * stride;
+      RowDelta<64, NLM> rd;
+      u_acc[k] = ldg(prop + (size_t)q[k] * PS + 3);
+      stage_neighbours_q<64, NLM>(T, zr, q[k], s[k], lane, Z, [&] {
+        rd.load(prop + (size_t)q[k] * PS + 4, zr + (size_t)s[k] * NLM, lane);
+      });
+      rd.store(Z + dzo, lane);
+    }
+  }
+  // the two staged blocks are this wavefront's own
+  wave_lds_sync();
+  double a[K];
+  if (wvalid) walk_delta_shared<K, NLM>(wl, W, pre, lane, zb, a);
+  else
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = fold_lanes<64>(a[k]);
+  if (!wvalid) return;
+  const double beta = c->st->beta;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!valid[k]) break;
+    const double* pq = prop + (size_t)q[k] * PS;
+    int accept = 0;
+    if (lane == 0) accept = (a[k] <= 0.0 || u_acc[k] < sce::det_exp(-beta * a[k])) ? 1 : 0;
+    accept = __shfl(accept, 0, 64);
+    if (accept) {
+      for (int m = lane; m < NLM; m += 64) c->zrows[(size_t)s[k] * NLM + m] = pq[4 + m];
+      if (lane == 0) {
+        c->config[3 * (size_t)s[k]] = pq[0];
+        c->config[3 * (size_t)s[k] + 1] = pq[1];
+        c->config[3 * (size_t)s[k] + 2] = pq[2];
+      }
+    }
+    if (lane == 0) {
+      c->dE[q[k]] = accept ? a[k] : 0.0;
+      c->acc[q[k]] = accept;
+    }
+  }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bchunks(const BChain* __restrict__ ch,
                                                  const BSeg* __restrict__ seg, int nseg,
@@ -4599,6 +4688,11 @@ struct sce_batch {
   // (k_bmetro_shared; homogeneous batches only): 0 = two where the stencil
   // fits (the default), 1 = off, 2 or 4
   int share = 0;
+  // site pairs per colour (k_bmetro_pair): two same-sublattice sites of one
+  // chain's colour class per wavefront at 64 lanes, {chain, q1, q2 (-1: none)}
+  bool pairs = false;
+  std::vector<int4*> d_pairs;
+  std::vector<int> npairs;
   // buffers of the batched dependency-driven sweep (k_bflow)
   struct BFlow {
     int cap = 0, mcap = 0, grid = 0, G = 0, NS = 0;
@@ -4724,6 +4818,23 @@ extern "C" int32_t sce_batch_create(int32_t R, sce_tables* const* tables, const 
       b->nseg_wcol[li].push_back((int)sw.size());
       b->total_wcol[li].push_back(t == 0 ? 0 : tw);
     }
+    // the colour's site pairs: per chain, the class's positions by
+    // sublattice, two at a time (a sublattice's odd last one alone)
+    std::vector<int4> pr;
+    for (int k = 0; k < R; ++k) {
+      const sce_tables* tb = tables[k];
+      if (col >= tb->n_colors) continue;
+      const int lo = tb->color_ptr[col], n = tb->color_ptr[col + 1] - lo, na = tb->dev.na;
+      std::vector<std::vector<int>> by((size_t)na);
+      for (int q = lo; q < lo + n; ++q) by[(size_t)(tb->color_sites[q] % na)].push_back(q);
+      for (const auto& v : by)
+        for (size_t i = 0; i < v.size(); i += 2) pr.push_back(make_int4(k, v[i], i + 1 < v.size() ? v[i + 1] : -1, 0));
+    }
+    b->npairs.push_back((int)pr.size());
+    if (pr.empty()) pr.push_back(make_int4(0, 0, -1, 0));
+    int4* dp;
+    if ((r = dupload(b->mem, &dp, pr.data(), pr.size())) != SCE_OK) return fail(r);
+    b->d_pairs.push_back(dp);
   }
   for (int m = 0; m < 3; ++m) {
     std::vector<BSeg> sc;
@@ -4808,6 +4919,15 @@ static void launch_bmetro_shared_k(sce_batch* b, int col, hipStream_t s) {
 }
 
 template <int LMAX>
+static void launch_bmetro_pair(sce_batch* b, int col, hipStream_t s) {
+  constexpr int G = TPB / 64;
+  const int n = b->npairs[col];
+  if (n == 0) return;
+  const size_t shmem = (size_t)G * 2 * (size_t)b->stride * sizeof(double);
+  k_bmetro_pair<LMAX><<<(n + G - 1) / G, TPB, shmem, s>>>(b->d_ch, b->d_pairs[col], n, b->stride);
+}
+
+template <int LMAX>
 static void launch_bmetro_shared(int K, sce_batch* b, int col, hipStream_t s) {
   if (K == 2) launch_bmetro_shared_k<2, LMAX>(b, col, s);
   else launch_bmetro_shared_k<4, LMAX>(b, col, s);
@@ -4827,12 +4947,17 @@ static int32_t benqueue_sweep(sce_batch* b, int L, int meas, int exch) {
       share = 2;  // the default: two chains per walk where the stencil fits
     }
   }
+  const bool pairs = L == 64 && share == 1 && b->pairs;
+  if (pairs && shared_lds(2) > 65536)
+    return sce_fail(SCE_EINVAL, "neighbour stencil too large for site pairs; turn them off");
   hipStream_t s = b->stream;
   SCE_DISPATCH_LMAX(b->lmax, launch_bpropose, b, s);
   for (int col = 0; col < b->max_colors; ++col) {
     if (b->total_col[col] == 0) continue;
     if (share > 1) {
       SCE_DISPATCH_LMAX(b->lmax, launch_bmetro_shared, share, b, col, s);
+    } else if (pairs) {
+      SCE_DISPATCH_LMAX(b->lmax, launch_bmetro_pair, b, col, s);
     } else {
       SCE_DISPATCH_LMAX(b->lmax, launch_bmetro, L, b, col, s);
     }
@@ -5117,6 +5242,15 @@ static int32_t benqueue_flow(sce_batch* b, int n, long long i0, int64_t measure_
 extern "C" int32_t sce_batch_set_persistent(sce_batch* b, int32_t on) {
   if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
   b->persistent = on != 0;
+  return SCE_OK;
+}
+
+extern "C" int32_t sce_batch_set_site_pairs(sce_batch* b, int32_t on) {
+  if (!b) return sce_fail(SCE_EINVAL, "NULL batch");
+  if ((on != 0) != b->pairs) {
+    b->drop_graphs();
+    b->pairs = on != 0;
+  }
   return SCE_OK;
 }
 

@@ -95,7 +95,7 @@ EXPORTS = [
     "sce_test_proposal", "sce_test_walk_delta", "sce_test_walk_coeffs", "sce_test_reduce",
     "sce_batch_create", "sce_batch_destroy", "sce_batch_run", "sce_batch_download",
     "sce_batch_info", "sce_batch_set_step", "sce_batch_reset_counters", "sce_batch_swap_stats",
-    "sce_batch_set_graph", "sce_batch_set_persistent", "sce_batch_set_shared_walk", "sce_batch_renormalize", "sce_batch_set_counters",
+    "sce_batch_set_graph", "sce_batch_set_persistent", "sce_batch_set_shared_walk", "sce_batch_set_site_pairs", "sce_batch_renormalize", "sce_batch_set_counters",
     "sce_batch_get_exchange", "sce_batch_set_exchange",
     "sce_sur_create", "sce_sur_destroy", "sce_sur_info", "sce_sur_get_params",
     "sce_sur_set_params", "sce_sur_set_int8", "sce_sur_get_qparams", "sce_sur_harvest", "sce_sur_train", "sce_sur_grad", "sce_sur_eval",
@@ -144,7 +144,7 @@ def lib():
             "sce_batch_run": [P, i64, i32, i64, i64, i64, i64, i64, P, i64, P],
             "sce_batch_download": [P, i32, P, P, P], "sce_batch_info": [P, i32, P],
             "sce_batch_set_step": [P, i32, d], "sce_batch_reset_counters": [P],
-            "sce_batch_swap_stats": [P, P, P], "sce_batch_set_graph": [P, i32], "sce_batch_set_persistent": [P, i32], "sce_batch_set_shared_walk": [P, i32],
+            "sce_batch_swap_stats": [P, P, P], "sce_batch_set_graph": [P, i32], "sce_batch_set_persistent": [P, i32], "sce_batch_set_shared_walk": [P, i32], "sce_batch_set_site_pairs": [P, i32],
             "sce_batch_renormalize": [P],
             "sce_batch_set_counters": [P, i32, i64, i64, i64, i64, i64, d],
             "sce_batch_get_exchange": [P, P, P], "sce_batch_set_exchange": [P, i64, i32, P, P],

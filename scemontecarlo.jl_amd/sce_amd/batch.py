@@ -109,6 +109,12 @@ class GPUBatch:
         L.check(L.lib().sce_batch_set_persistent(self._h, 1 if on else 0))
         self._persistent = bool(on)
 
+    def set_site_pairs(self, on: bool) -> None:
+        """At 64 lanes per site, walk two same-sublattice sites of one chain's
+        colour class per wavefront (any batch; used where chains do not share
+        walks); the results are those of 64 lanes per site."""
+        L.check(L.lib().sce_batch_set_site_pairs(self._h, 1 if on else 0))
+
     def set_shared_walk(self, chains: int) -> None:
         """At 64 lanes per site, walk each site's sublattice walk once for
         `chains` (2 or 4) chains of a batch on one model; the results are
@@ -265,12 +271,12 @@ def gpu_run_pt(gH: GPUTiledHamiltonian, temperature=None, kT=None, exchange_inte
         # so in every checkpoint and the result) like the reference's workgroupsize
         cands = (_BATCH_CANDIDATES if persistent is None else
                  tuple(c for c in _BATCH_CANDIDATES if c[1] == bool(persistent)))
-        plan.lanes, persistent, share = autotune_batch(lambda: _pt_batch(gH, plan, init), cands, sweeps=20)
+        plan.lanes, persistent, walk = autotune_batch(lambda: _pt_batch(gH, plan, init), cands, sweeps=20)
     ck = _Checkpointer(checkpoint, checkpoint_interval) if checkpoint is not None else None
     b = _pt_batch(gH, plan, init)
     b.set_persistent(bool(persistent))
     if lanes == "auto":
-        b.set_shared_walk(share)
+        set_walk_form(b, walk)
     res = _pt_run(b, gH, plan, "therm", 0, None, ck)
     res.persistent = bool(persistent)
     return res
@@ -372,34 +378,43 @@ def resume_pt(path: str, gH: GPUTiledHamiltonian, checkpoint: Optional[str] = "s
     return _pt_run(st["batch"], gH, st["plan"], st["phase"], st["done"], st["rows"], ck)
 
 
-# (lanes per site, persistent launch, chains per shared walk) forms the batch
-# autotuner tries; the shared walk only where every chain is on one model
-_BATCH_CANDIDATES = ((16, False, 1), (32, False, 1), (64, False, 1), (64, False, 2),
-                     (16, True, 1), (32, True, 1), (64, True, 1))
+# (lanes per site, persistent launch, walk form) forms the batch autotuner
+# tries.  Walk forms at 64 lanes: "" one site per walk, "s" two replicas of a
+# batch on one model per walk (set_shared_walk(2)), "q" two same-sublattice
+# sites of one chain per walk (set_site_pairs)
+_BATCH_CANDIDATES = ((16, False, ""), (32, False, ""), (64, False, ""), (64, False, "s"), (64, False, "q"),
+                     (16, True, ""), (32, True, ""), (64, True, ""))
+
+
+def set_walk_form(b: GPUBatch, walk: str) -> None:
+    """Apply an autotuned walk form to a batch (none of them changes results)."""
+    b.set_shared_walk(2 if walk == "s" else 1)
+    b.set_site_pairs(walk == "q")
 
 
 def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
                    timings: Optional[dict] = None):
     """The project autotuner for batches: times `sweeps` sweeps of a scratch
-    batch built by `make_batch()` in each (lanes per site, persistent, chains
-    per shared walk) form — per-colour launches replayed as hipGraphs, at 64
-    lanes also with two chains of a batch on one model walking each site once
-    (GPUBatch.set_shared_walk), or one persistent launch per segment, each at
-    16/32/64 lanes (sites per wavefront = 64 / lanes) — and returns the
-    fastest as (lanes, persistent, shared_walk).  `lanes` is part of the
-    determinism scope, so tuning runs on a throwaway batch before the
-    production run and the caller records the choice with the run; the launch
-    form and the shared walk are not.  `timings` receives the seconds per
-    sweep of every candidate that ran, keyed "16", "32", "64", "64s" (shared
-    walk) and "64p"."""
+    batch built by `make_batch()` in each (lanes per site, persistent, walk
+    form) form — per-colour launches replayed as hipGraphs, at 64 lanes also
+    with two sites per walk (two replicas of a batch on one model, "s", or two
+    same-sublattice sites of one chain, "q"), or one persistent launch per
+    segment, each at 16/32/64 lanes (sites per wavefront = 64 / lanes) — and
+    returns the fastest as (lanes, persistent, walk); set_walk_form applies
+    the walk form.  `lanes` is part of the determinism scope, so tuning runs on
+    a throwaway batch before the production run and the caller records the
+    choice with the run; the launch and walk forms are not.  `timings`
+    receives the seconds per sweep of every candidate that ran, keyed "16",
+    "32", "64", "64s", "64q" and "64p"."""
     best, best_t = None, math.inf
-    for lanes, pers, share in candidates:
+    for lanes, pers, walk in candidates:
         b = make_batch()
-        if share > 1 and not b.homogeneous:
+        if walk == "s" and not b.homogeneous:
+            del b
             continue
         try:
             b.set_persistent(pers)
-            b.set_shared_walk(share)
+            set_walk_form(b, walk)
             b.run(3, lanes, 0)          # capture + warm
             t0 = time.perf_counter()
             b.run(sweeps, lanes, 0)
@@ -409,9 +424,9 @@ def autotune_batch(make_batch, candidates=_BATCH_CANDIDATES, sweeps: int = 30,
         finally:
             del b
         if timings is not None:
-            timings[f"{lanes}{'p' if pers else ''}{'s' if share > 1 else ''}"] = dt / sweeps
+            timings[f"{lanes}{'p' if pers else ''}{walk}"] = dt / sweeps
         if dt < best_t:
-            best, best_t = (int(lanes), bool(pers), int(share)), dt
+            best, best_t = (int(lanes), bool(pers), walk), dt
     if best is None:
         raise RuntimeError("no candidate form could run this batch")
     return best
@@ -473,7 +488,7 @@ def autotune_lanes(make_batch, candidates=(16, 32, 64), sweeps: int = 30, cache:
     if cache is not None and key is not None and key in cache:
         return cache[key]
     t = {} if timings is not None else None
-    lanes, _, _ = autotune_batch(make_batch, tuple((n, False, 1) for n in candidates), sweeps, t)
+    lanes, _, _ = autotune_batch(make_batch, tuple((n, False, "") for n in candidates), sweeps, t)
     if timings is not None:
         timings.update({int(k): v for k, v in t.items()})
     if cache is not None and key is not None:

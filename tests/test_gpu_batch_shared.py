@@ -1,8 +1,11 @@
-"""Shared walks (GPUBatch.set_shared_walk): a ladder on one model with 2 or 4
-chains per wavefront walking each site's sublattice walk once is bit-identical
-to the oracle at 64 lanes per site and to the unshared launches — payloads,
-energies, counters, measurements and exchange statistics, with chain counts
-that leave a partial last group, in graph replay and direct launches."""
+"""Shared walks: a ladder on one model with 2 or 4 chains per wavefront
+walking each site's sublattice walk once (GPUBatch.set_shared_walk), and any
+batch with two same-sublattice sites of one chain per wavefront
+(GPUBatch.set_site_pairs), are bit-identical to the oracle at 64 lanes per
+site and to the unshared launches — payloads, energies, counters,
+measurements and exchange statistics, with chain counts and classes that
+leave a partial last group or a lone site, in graph replay and direct
+launches."""
 import numpy as np
 import pytest
 
@@ -135,3 +138,82 @@ def test_shared_walk_guards():
     u.run(3, 32, 0)
     for r in range(3):
         assert np.array_equal(h.download(r)[1], u.download(r)[1])
+
+
+def _uneven(seed, specs):
+    rng = np.random.default_rng(seed)
+    gHs, states, Ts, kts, seeds = [], [], [], [], []
+    for k, (lat, N) in enumerate(specs):
+        na, terms = M.lattice_model(lat, pair_shells=2, pair_lmax=2, triangles=(lat == "bcc"), seed=seed + k)
+        B = rng.uniform(0.0, 10.0) * (lambda v: v / np.linalg.norm(v))(rng.standard_normal(3))
+        H, T = build_pair(na, terms, (N, N, N), [2.0] * na, tuple(B))
+        gHs.append(S.GPUTiledHamiltonian(H))
+        states.append(S.ChainState(H, rand_config(H.n_sites, seed + 50 + k), 0.5))
+        Ts.append(T)
+        kts.append(float(rng.uniform(0.005, 0.05)))
+        seeds.append(int(rng.integers(1, 2**62)))
+    return gHs, states, Ts, kts, seeds
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_uneven_batch_site_pairs_bitwise(graph):
+    """Site pairs on an uneven batch (sc and bcc chains of different sizes,
+    per-chain models and Zeeman fields; odd class sizes leave lone sites):
+    every chain bit-identical to the oracle at 64 lanes per site."""
+    specs = [("sc", 5), ("bcc", 4), ("sc", 3), ("bcc", 5), ("bcc", 3)]
+    gHs, states, Ts, kts, seeds = _uneven(610, specs)
+    b = GPUBatch(gHs, states, seeds, kts)
+    b.set_site_pairs(True)
+    b.set_graph(graph)
+    obs = b.run(7, 64, renorm_interval=3, measure_interval=2)
+    for k in range(len(specs)):
+        c, z, E = states[k].config.copy(), states[k].zrows.copy(), states[k].energy
+        nacc = 0
+        for j in range(1, 8):
+            a, E, _, _ = Ts[k].keyed_sweep(c, z, E, 1.0 / kts[k], 0.5, seeds[k], j, 64)
+            nacc += a
+            if j % 3 == 0:
+                _, E = Ts[k].renormalize(c, z, E)
+        cfg, zd, Ed = b.download(k)
+        assert np.array_equal(cfg.view(np.uint64), c.view(np.uint64)), k
+        assert np.array_equal(zd.view(np.uint64), z.view(np.uint64)), k
+        assert Ed == E, k
+        assert b.info(k).acc_metro == nacc, k
+    assert obs.shape[1] == 3
+
+
+def test_site_pairs_equal_unpaired_at_full_size():
+    """Ten chains of the uneven benchmark's sizes (sc / bcc up to 20^3): site
+    pairs give the bits of 64 lanes per site unpaired, measurements included;
+    switching the pairs off and on between calls continues identically; on a
+    ladder they take the place of the replica sharing only when it is off."""
+    specs = [("sc", 20), ("bcc", 17), ("sc", 13), ("bcc", 11), ("sc", 9), ("bcc", 20), ("sc", 7), ("bcc", 8),
+             ("sc", 16), ("bcc", 6)]
+    gHs, states, _, kts, seeds = _uneven(720, specs)
+    runs = []
+    for mode in ("plain", "pairs", "switch"):
+        b = GPUBatch(gHs, states, seeds, kts)
+        b.set_site_pairs(mode != "plain")
+        obs = [b.run(4, 64, renorm_interval=3, measure_interval=1)]
+        if mode == "switch":
+            b.set_site_pairs(False)
+        obs.append(b.run(3, 64, renorm_interval=3, measure_interval=1, phase_done=4))
+        runs.append((obs, [b.download(k) for k in range(len(specs))]))
+    for obs, dl in runs[1:]:
+        for x, y in zip(obs, runs[0][0]):
+            assert np.array_equal(x, y)
+        for (c1, z1, e1), (c2, z2, e2) in zip(dl, runs[0][1]):
+            assert np.array_equal(c1, c2) and np.array_equal(z1, z2) and e1 == e2
+    na, terms, dims = bcc_tet_small(6)
+    H, T = build_pair(na, terms, dims)
+    gH = S.GPUTiledHamiltonian(H)
+    kts = np.geomspace(0.001, 0.02, 5)
+    b, seeds, sts = _batch(H, gH, 5, kts, 930)
+    b.set_shared_walk(1)
+    b.set_site_pairs(True)
+    b.run(4, 64, renorm_interval=0, exchange_interval=1, phase_done=0, phase_total=4)
+    u, _, _ = _batch(H, gH, 5, kts, 930)
+    u.set_shared_walk(1)
+    u.run(4, 64, renorm_interval=0, exchange_interval=1, phase_done=0, phase_total=4)
+    for r in range(5):
+        assert np.array_equal(b.download(r)[1], u.download(r)[1])
