@@ -3986,6 +3986,32 @@ __global__ __launch_bounds__(TPB) void k_bmetro(const BChain* __restrict__ ch,
                               valid ? c->st->beta : 0.0, sg.lo + i, valid, nullptr, lds, stride);
 }
 
+// The acceptance and write-back of one site of chain c at position q (site s)
+// by a whole wavefront, as metro_group<64> makes them: accept on
+// a <= 0 or u_acc < exp(-beta a), then the proposal's row and spin and the
+// position's dE and accept flag.
+template <int NLM>
+__device__ __forceinline__ void bsite_accept(const BChain* c, int q, int s, double a, double u_acc, double beta,
+                                             int lane) {
+  constexpr int PS = NLM + 4;
+  const double* pq = c->prop + (size_t)q * PS;
+  int accept = 0;
+  if (lane == 0) accept = (a <= 0.0 || u_acc < sce::det_exp(-beta * a)) ? 1 : 0;
+  accept = __shfl(accept, 0, 64);
+  if (accept) {
+    for (int m = lane; m < NLM; m += 64) c->zrows[(size_t)s * NLM + m] = pq[4 + m];
+    if (lane == 0) {
+      c->config[3 * (size_t)s] = pq[0];
+      c->config[3 * (size_t)s + 1] = pq[1];
+      c->config[3 * (size_t)s + 2] = pq[2];
+    }
+  }
+  if (lane == 0) {
+    c->dE[q] = accept ? a : 0.0;
+    c->acc[q] = accept;
+  }
+}
+
 // The walk of walk_delta<64> for K sites that share one sublattice walk, each
 // with its own staged block (LDS byte bases zb[k]): every entry is loaded once
 // and used K times, and site k's sum is the one walk_delta<64> forms for it
@@ -4104,24 +4130,7 @@ __global__ __launch_bounds__(TPB) void k_bmetro_shared(const BChain* __restrict_
   for (int k = 0; k < K; ++k) {
     if (r0 + k >= R) break;
     const BChain* c = &ch[r0 + k];
-    const double* pq = c->prop + (size_t)q * PS;
-    int accept = 0;
-    if (lane == 0) accept = (a[k] <= 0.0 || u_acc[k] < sce::det_exp(-c->st->beta * a[k])) ? 1 : 0;
-    accept = __shfl(accept, 0, 64);
-    if (accept) {
-      double* zr = c->zrows;
-      for (int m = lane; m < NLM; m += 64) zr[(size_t)s * NLM + m] = pq[4 + m];
-      if (lane == 0) {
-        double* cf = c->config;
-        cf[3 * (size_t)s] = pq[0];
-        cf[3 * (size_t)s + 1] = pq[1];
-        cf[3 * (size_t)s + 2] = pq[2];
-      }
-    }
-    if (lane == 0) {
-      c->dE[q] = accept ? a[k] : 0.0;
-      c->acc[q] = accept;
-    }
+    bsite_accept<NLM>(c, q, s, a[k], u_acc[k], c->st->beta, lane);
   }
 }
 
@@ -4193,22 +4202,7 @@ __global__ __launch_bounds__(TPB) void k_bmetro_pair(const BChain* __restrict__ 
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (!valid[k]) break;
-    const double* pq = prop + (size_t)q[k] * PS;
-    int accept = 0;
-    if (lane == 0) accept = (a[k] <= 0.0 || u_acc[k] < sce::det_exp(-beta * a[k])) ? 1 : 0;
-    accept = __shfl(accept, 0, 64);
-    if (accept) {
-      for (int m = lane; m < NLM; m += 64) c->zrows[(size_t)s[k] * NLM + m] = pq[4 + m];
-      if (lane == 0) {
-        c->config[3 * (size_t)s[k]] = pq[0];
-        c->config[3 * (size_t)s[k] + 1] = pq[1];
-        c->config[3 * (size_t)s[k] + 2] = pq[2];
-      }
-    }
-    if (lane == 0) {
-      c->dE[q[k]] = accept ? a[k] : 0.0;
-      c->acc[q[k]] = accept;
-    }
+    bsite_accept<NLM>(c, q[k], s[k], a[k], u_acc[k], beta, lane);
   }
 }
 
