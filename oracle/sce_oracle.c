@@ -855,6 +855,9 @@ static int* sorted_adjacency(const ora_tiled* t, int s, int* n) {
   return js;
 }
 
+/* lanes: a power of two up to ORA_MAX_LANES */
+#define ORA_MAX_LANES 128
+
 static void lane_fold(double* a, int lanes) {
   for (int off = lanes / 2; off >= 1; off >>= 1)
     for (int l = 0; l < off; ++l) a[l] = a[l] + a[l + off];
@@ -866,7 +869,7 @@ double ora_walk_delta(const ora_tiled* t, int s, const double* zrows, const doub
   double dz[64];
   const double* zold = zrows + (size_t)(s - 1) * nlm;
   for (int k = 0; k < nlm; ++k) dz[k] = znew[k] - zold[k];
-  double a[64] = {0};
+  double a[ORA_MAX_LANES] = {0};
   int n, i = 0;
   int* js = sorted_adjacency(t, s, &n);
   for (int q = 0; q < n; ++q) {
@@ -885,7 +888,7 @@ double ora_walk_delta(const ora_tiled* t, int s, const double* zrows, const doub
 
 void ora_walk_coeffs(const ora_tiled* t, int s, const double* zrows, int lanes, double* c) {
   int nlm = t->nlm;
-  double a[64][49];
+  double a[ORA_MAX_LANES][49];
   memset(a, 0, sizeof a);
   int n, i = 0;
   int* js = sorted_adjacency(t, s, &n);
@@ -900,7 +903,7 @@ void ora_walk_coeffs(const ora_tiled* t, int s, const double* zrows, int lanes, 
   }
   free(js);
   for (int k = 0; k < nlm; ++k) {
-    double v[64];
+    double v[ORA_MAX_LANES];
     for (int l = 0; l < lanes; ++l) v[l] = a[l][k];
     lane_fold(v, lanes);
     c[k] = v[0];

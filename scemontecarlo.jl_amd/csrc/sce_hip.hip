@@ -457,7 +457,8 @@ __device__ __forceinline__ WalkWords ld_walk(const WalkWords* p) {
 }
 
 // The walk over a padded sublattice walk `wl` of Wp entries (Wp a multiple of
-// WALK_UNROLL * L).  Each lane's sum is the contract's: fma(w * p, dz[tgt], a)
+// WALK_PAD = WALK_UNROLL * 64; at 128 lanes a last half batch may remain).
+// Each lane's sum is the contract's: fma(w * p, dz[tgt], a)
 // over its entries in ascending order, skipping the terms whose product p is
 // zero.  The skip needs no test here: a lane's sum starts at +0 and never
 // becomes -0 (a sum of values of opposite sign that cancels exactly rounds to
@@ -500,7 +501,7 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
       a = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb, en[u].f2t)), a);
     }
   }
-  for (; base < Wp; base += B) {
+  for (; base + B <= Wp; base += B) {
     WalkWords en[WALK_UNROLL];
 #pragma unroll
     for (int u = 0; u < WALK_UNROLL; ++u) en[u] = ld_walk(ww + base + u * L);
@@ -511,8 +512,29 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
       a = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb, en[u].f2t)), a);
     }
   }
+  if constexpr (L > 64) {
+    // walks are padded to WALK_PAD = WALK_UNROLL * 64 entries: at 128 lanes a
+    // last half batch may remain
+    static_assert(L == 2 * 64, "the tail is half a batch");
+    if (base < Wp) {
+      WalkWords en[WALK_UNROLL / 2];
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL / 2; ++u) en[u] = ld_walk(ww + base + u * L);
+#pragma unroll
+      for (int u = 0; u < WALK_UNROLL / 2; ++u) {
+        const double p = (lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01))) *
+                         lds_at(lds_add_lo(zb, en[u].f2t));
+        a = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb, en[u].f2t)), a);
+      }
+    }
+  }
   return a;
 }
+
+// LDS doubles a walk group needs beyond its staged block and dz row: at 128
+// lanes, the cross-wavefront step of the fold
+template <int L>
+__host__ __device__ constexpr int walk_extra() { return L > 64 ? 64 : 0; }
 
 // One group of L lanes updates one site of the colour (colour-order index q).
 // Every lane of the group's wavefront calls it (wavefront-level LDS ordering).
@@ -527,9 +549,9 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
                                             int stride, double* __restrict__ dE_raw = nullptr) {
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
-  if constexpr (L == 64) {
-    // one site per wavefront: its index, site and sublattice tables are
-    // wave-uniform (scalar registers)
+  if constexpr (L >= 64) {
+    // one site per wavefront (two at 128 lanes): its index, site and
+    // sublattice tables are wave-uniform (scalar registers)
     q = __builtin_amdgcn_readfirstlane(q);
     valid = __builtin_amdgcn_readfirstlane((int)valid) != 0;
   }
@@ -541,7 +563,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   double u_acc = 0.0;
   if (valid) {
     s = TEST ? test_sites[q] : ldg(T.color_sites + q);
-    if constexpr (L == 64) s = __builtin_amdgcn_readfirstlane(s);
+    if constexpr (L >= 64) s = __builtin_amdgcn_readfirstlane(s);
     // independent of the staging: issued with it rather than after the barrier
     const int a0 = s % T.na;
     const int w0 = ldg(T.walkp_ptr + a0);
@@ -561,11 +583,25 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
     rd.store(dz, lane);
   }
   // a group's staged block is its own (L <= 64: one wavefront), so the walk
-  // waits on its own wavefront's writes only, not on the block's other sites
-  wave_lds_sync();
+  // waits on its own wavefront's writes only, not on the block's other sites;
+  // at 128 lanes the group's two wavefronts meet at the block's barrier
+  if constexpr (L <= 64)
+    wave_lds_sync();
+  else
+    __syncthreads();
   double a = 0.0;
   if (valid) a = walk_delta<L, NLM>(wl, W, pre, lane, Z);
-  a = fold_lanes<L>(a);
+  if constexpr (L > 64) {
+    // the fold's first step (+64) crosses the group's two wavefronts through
+    // the 64 doubles after its dz row; then the first wavefront folds
+    double* xs = dz + NLM;
+    if (lane >= 64) xs[lane - 64] = a;
+    __syncthreads();
+    a = a + xs[lane & 63];
+    a = fold_lanes<64>(a);
+  } else {
+    a = fold_lanes<L>(a);
+  }
   if (!valid) return;
   if (TEST) {
     if (lane == 0) dE[q] = a;
@@ -574,7 +610,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   int accept = 0;
   const double* pq = prop + (size_t)q * PS;
   if (lane == 0) accept = (a <= 0.0 || u_acc < sce::det_exp(-beta * a)) ? 1 : 0;
-  accept = __shfl(accept, 0, L);
+  accept = __shfl(accept, 0, L > 64 ? 64 : L);  // at 128 lanes only the first wavefront writes
   if (accept) {
     for (int k = lane; k < NLM; k += L) zrows[(size_t)s * NLM + k] = pq[4 + k];
     if (lane == 0) {
@@ -603,7 +639,8 @@ __global__ __launch_bounds__(TPB) void k_metro(DevTables T, double* __restrict__
   const int qi = xcd_swizzle(blockIdx.x, gridDim.x) * G + (int)threadIdx.x / L;
   const double beta = TEST ? 0.0 : st->beta;
   metro_group<L, LMAX, TEST>(T, config, zrows, prop, dE, acc, beta, q_lo + qi, qi < q_n,
-                             test_sites, lds, T.nb_max * (LMAX + 1) * (LMAX + 1) + 1 + (LMAX + 1) * (LMAX + 1),
+                             test_sites, lds,
+                             T.nb_max * (LMAX + 1) * (LMAX + 1) + 1 + (LMAX + 1) * (LMAX + 1) + walk_extra<L>(),
                              dE_raw);
 }
 
@@ -1119,7 +1156,7 @@ __global__ __launch_bounds__(TPB) void k_metro_fin(DevTables T, double* __restri
   extern __shared__ double lds[];
   const int qi = xcd_swizzle(blockIdx.x - 1, gridDim.x - 1) * G + (int)threadIdx.x / L;
   metro_group<L, LMAX, false>(T, config, zrows, prop, dE, acc, st->beta, q_lo + qi, qi < q_n, nullptr, lds,
-                              T.nb_max * NLM + 1 + NLM);
+                              T.nb_max * NLM + 1 + NLM + walk_extra<L>());
 }
 
 // A colour walk whose extra blocks (after the nwalk walk blocks) make
@@ -1144,7 +1181,7 @@ __global__ __launch_bounds__(TPB) void k_metro_prop(DevTables T, double* __restr
   extern __shared__ double lds[];
   const int qi = xcd_swizzle(blockIdx.x, nwalk) * G + (int)threadIdx.x / L;
   metro_group<L, LMAX, false>(T, config, zrows, prop, dE, acc, st->beta, q_lo + qi, qi < q_n, nullptr, lds,
-                              T.nb_max * NLM + 1 + NLM);
+                              T.nb_max * NLM + 1 + NLM + walk_extra<L>());
 }
 
 // Block-level recursive tree over m partial sums (the levels above the first
@@ -2485,12 +2522,16 @@ extern "C" int32_t sce_tables_layout_info(const sce_tables* t, int64_t o[5]) {
 
 // ---- launch helpers --------------------------------------------------------
 static size_t walk_lds_bytes(const DevTables& D, int L) {
-  return (size_t)(TPB / L) * (size_t)(D.nb_max * D.nlm + 1 + D.nlm) * sizeof(double);
+  return (size_t)(TPB / L) * (size_t)(D.nb_max * D.nlm + 1 + D.nlm + (L > 64 ? 64 : 0)) * sizeof(double);
 }
 
-static int32_t check_lanes(int L) {
+// Lanes per site: 16, 32 or 64 everywhere; 128 (two wavefronts per site) for
+// plain Metropolis sweeps with per-colour launches and the walk test hook.
+static int32_t check_lanes(int L, bool allow128 = false) {
+  if (L == 128 && allow128) return SCE_OK;
   if (L != 16 && L != 32 && L != 64)
-    return sce_fail(SCE_EINVAL, "lanes must be 16, 32 or 64 (got %d)", L);
+    return sce_fail(SCE_EINVAL, "lanes must be 16, 32 or 64%s (got %d)",
+                    allow128 ? " or 128" : " here (128: plain Metropolis sweeps only)", L);
   return SCE_OK;
 }
 
@@ -2554,7 +2595,8 @@ static void launch_metro(int L, const DevTables& D, double* config, double* zrow
                          int q_n, const int* ts, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (L == 16) launch_metro_l<16, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
   else if (L == 32) launch_metro_l<32, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
-  else launch_metro_l<64, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
+  else if (L == 64) launch_metro_l<64, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
+  else launch_metro_l<128, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
 }
 template <int LMAX> static void launch_metro_run(int L, const DevTables& D, double* config, double* zrows, const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo, int q_n, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   launch_metro<LMAX, false>(L, D, config, zrows, prop, dE, acc, st, q_lo, q_n, nullptr, s, e0, e1);
@@ -2566,7 +2608,8 @@ static void launch_metro_fin(int L, const DevTables& D, double* config, double* 
   const size_t lb = walk_lds_bytes(D, L);
   if (L == 16) k_metro_fin<16, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
   else if (L == 32) k_metro_fin<32, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
-  else k_metro_fin<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+  else if (L == 64) k_metro_fin<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+  else k_metro_fin<128, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
 }
 template <int LMAX>
 static void launch_metro_prop(int L, const DevTables& D, double* config, double* zrows, double* prop, double* dE,
@@ -2577,7 +2620,8 @@ static void launch_metro_prop(int L, const DevTables& D, double* config, double*
   const size_t lb = walk_lds_bytes(D, L);
   if (L == 16) k_metro_prop<16, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
   else if (L == 32) k_metro_prop<32, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
-  else k_metro_prop<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
+  else if (L == 64) k_metro_prop<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
+  else k_metro_prop<128, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
 }
 template <int LMAX>
 static void launch_propose_range(const DevTables& D, const double* config, double* prop, const ChainDev* st,
@@ -3129,7 +3173,7 @@ extern "C" int32_t sce_swap_payload(sce_chain* a, sce_chain* b) {
 
 extern "C" int32_t sce_metropolis_sweep(sce_chain* c, double beta, int32_t lanes, int64_t* nacc) {
   CHK(chain_check(c));
-  CHK(check_lanes(lanes));
+  CHK(check_lanes(lanes, !c->persistent));
   CHK(set_beta(c, beta));
   c->timing = false;
   CHK(enqueue_metro(c, lanes));
@@ -3356,7 +3400,9 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
                                   int64_t renorm_interval, int64_t measure_interval, int32_t lanes,
                                   double* obs, int64_t obs_cap, int64_t* n_obs) {
   CHK(chain_check(c));
-  CHK(check_lanes(lanes));
+  // 128 lanes: plain sweeps with per-colour launches (no overrelaxation pass
+  // or persistent grid at that width)
+  CHK(check_lanes(lanes, or_per_metropolis == 0 && !c->persistent));
   if (nsweeps < 0) return sce_fail(SCE_EINVAL, "nsweeps must be >= 0");
   if (or_per_metropolis < 0) return sce_fail(SCE_EINVAL, "or_per_metropolis must be >= 0");
   const int64_t want_obs = measure_interval > 0 ? nsweeps / measure_interval : 0;
@@ -3700,7 +3746,7 @@ extern "C" int32_t sce_chain_time_gradient(sce_chain* c, int32_t lanes, int32_t 
 extern "C" int32_t sce_chain_time_walk(sce_chain* c, int32_t lanes, int32_t reps, double* ms_avg,
                                        int64_t* launches) {
   CHK(chain_check(c));
-  CHK(check_lanes(lanes));
+  CHK(check_lanes(lanes, true));
   CHK(check_lds(c->t->dev, lanes));
   if (reps < 1) return sce_fail(SCE_EINVAL, "reps must be >= 1");
   const DevTables& D = c->t->dev;
@@ -3848,7 +3894,7 @@ static int32_t check_sites(const sce_tables* t, int64_t n, const int32_t* sites,
 extern "C" int32_t sce_test_walk_delta(sce_tables* t, const double* zrows, int64_t n, const int32_t* sites,
                                        const double* znew, int32_t lanes, double* out) {
   if (!t) return sce_fail(SCE_EINVAL, "NULL tables");
-  CHK(check_lanes(lanes));
+  CHK(check_lanes(lanes, true));
   const DevTables& D = t->dev;
   CHK(check_lds(D, lanes));
   std::vector<int32_t> s0;
