@@ -522,7 +522,7 @@ def setup_leg(S, sweeps=200, graph=True):
         gH = S.GPUTiledHamiltonian(H)
         cfg = S.random_config(H.n_sites, rng)
         best = None
-        for lanes in (16, 32, 64, 128):
+        for lanes in (16, 32, 64, 128, 256):
             g = S.GPUChainState(gH, S.ChainState(H, cfg, 0.6), seed=5)
             g.set_graph(graph)
             S.gpu_run_sweeps(g, gH, None, 1 / 0.025, 20, renorm_interval=0, lanes=lanes)

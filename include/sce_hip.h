@@ -148,9 +148,10 @@ int32_t sce_swap_payload(sce_chain* a, sce_chain* b);
 
 /* gpu_metropolis_sweep!(gst, gH, β; workgroupsize): one Metropolis sweep;
  * `lanes` (16, 32 or 64) is the number of lanes that walk one site — part of
- * the determinism scope, as the reference's workgroupsize.  128 (two
- * wavefronts per site, the fold's first step through LDS) is accepted here
- * and by sce_run_sweeps for Metropolis-only sweeps with per-colour launches.
+ * the determinism scope, as the reference's workgroupsize.  128 and 256 (two
+ * and four wavefronts per site, the fold's cross-wavefront steps through LDS)
+ * are accepted here and by sce_run_sweeps for Metropolis-only sweeps with
+ * per-colour launches.
  * Writes the number of accepted moves. */
 int32_t sce_metropolis_sweep(sce_chain* c, double beta, int32_t lanes, int64_t* nacc);
 /* One overrelaxation sweep (updates.jl:66-87,231-290 on the device); `or_pass`

@@ -856,7 +856,7 @@ static int* sorted_adjacency(const ora_tiled* t, int s, int* n) {
 }
 
 /* lanes: a power of two up to ORA_MAX_LANES */
-#define ORA_MAX_LANES 128
+#define ORA_MAX_LANES 256
 
 static void lane_fold(double* a, int lanes) {
   for (int off = lanes / 2; off >= 1; off >>= 1)

@@ -305,6 +305,11 @@ __device__ __forceinline__ void stage_neighbours(const DevTables& T, const doubl
 // site's own row, which no walk entry reads), so the row loads wait on one
 // load, issued first, whose address is the position.  between() runs once,
 // after the first list loads are issued (loads that need s go there).
+// The 1.0 slot follows the sublattice's own rows, so for a sublattice with
+// fewer than nb_max neighbours it lies inside the padding.  One wavefront
+// stores it after its padding stores, in program order; a group of several
+// wavefronts (L > 64) has no such order across them, so there the padding
+// store of that slot writes 1.0 itself.
 // Returns the sublattice.
 template <int L, int NLM, class F>
 __device__ __forceinline__ int stage_neighbours_q(const DevTables& T, const double* __restrict__ zrows,
@@ -312,6 +317,9 @@ __device__ __forceinline__ int stage_neighbours_q(const DevTables& T, const doub
   constexpr int U = stage_round(L);
   const int32_t* nq = T.nbq + (size_t)q * T.nb_max;
   const int total = T.nb_max * NLM;
+  const int atom = s % T.na;
+  int one = -1;  // the 1.0 slot, where the padding stores must not reach it last
+  if constexpr (L > 64) one = (ldg(T.nb_ptr + atom + 1) - ldg(T.nb_ptr + atom)) * NLM;
   for (int b = 0; b < total; b += U * L) {
     int at[U], n[U];
     double v[U];
@@ -325,10 +333,9 @@ __device__ __forceinline__ int stage_neighbours_q(const DevTables& T, const doub
 #pragma unroll
     for (int u = 0; u < U; ++u) v[u] = ldg(zrows + (size_t)n[u] * NLM + (at[u] - (at[u] / NLM) * NLM));
 #pragma unroll
-    for (int u = 0; u < U; ++u) Z[at[u]] = v[u];
+    for (int u = 0; u < U; ++u) Z[at[u]] = (L > 64 && at[u] == one) ? 1.0 : v[u];
   }
   if (total == 0) between();  // no neighbours (single-site terms only)
-  const int atom = s % T.na;
   if (lane == 0) Z[(ldg(T.nb_ptr + atom + 1) - ldg(T.nb_ptr + atom)) * NLM] = 1.0;
   return atom;
 }
@@ -457,7 +464,8 @@ __device__ __forceinline__ WalkWords ld_walk(const WalkWords* p) {
 }
 
 // The walk over a padded sublattice walk `wl` of Wp entries (Wp a multiple of
-// WALK_PAD = WALK_UNROLL * 64; at 128 lanes a last half batch may remain).
+// WALK_PAD = WALK_UNROLL * 64; at 128 and 256 lanes up to L / 64 - 1 such
+// pads may remain after the whole batches).
 // Each lane's sum is the contract's: fma(w * p, dz[tgt], a)
 // over its entries in ascending order, skipping the terms whose product p is
 // zero.  The skip needs no test here: a lane's sum starts at +0 and never
@@ -513,15 +521,17 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
     }
   }
   if constexpr (L > 64) {
-    // walks are padded to WALK_PAD = WALK_UNROLL * 64 entries: at 128 lanes a
-    // last half batch may remain
-    static_assert(L == 2 * 64, "the tail is half a batch");
-    if (base < Wp) {
-      WalkWords en[WALK_UNROLL / 2];
+    // walks are padded to WALK_PAD = WALK_UNROLL * 64 entries: at L lanes up
+    // to L / 64 - 1 pads of WALK_PAD / L entries per lane remain, taken in
+    // ascending order
+    constexpr int UT = WALK_UNROLL * 64 / L;
+    static_assert(UT >= 1 && UT * L == WALK_UNROLL * 64, "a pad is whole entries per lane");
+    for (; base < Wp; base += UT * L) {
+      WalkWords en[UT];
 #pragma unroll
-      for (int u = 0; u < WALK_UNROLL / 2; ++u) en[u] = ld_walk(ww + base + u * L);
+      for (int u = 0; u < UT; ++u) en[u] = ld_walk(ww + base + u * L);
 #pragma unroll
-      for (int u = 0; u < WALK_UNROLL / 2; ++u) {
+      for (int u = 0; u < UT; ++u) {
         const double p = (lds_at(lds_add_lo(zb, en[u].f01)) * lds_at(lds_add_hi(zb, en[u].f01))) *
                          lds_at(lds_add_lo(zb, en[u].f2t));
         a = __builtin_fma(en[u].w * p, lds_at(lds_add_hi(zb, en[u].f2t)), a);
@@ -531,10 +541,11 @@ __device__ __forceinline__ double walk_delta(const WalkEntry* __restrict__ wl, i
   return a;
 }
 
-// LDS doubles a walk group needs beyond its staged block and dz row: at 128
-// lanes, the cross-wavefront step of the fold
+// LDS doubles a walk group needs beyond its staged block and dz row: above
+// 64 lanes, the partials of the group's second and later wavefronts for the
+// fold's cross-wavefront steps
 template <int L>
-__host__ __device__ constexpr int walk_extra() { return L > 64 ? 64 : 0; }
+__host__ __device__ constexpr int walk_extra() { return L > 64 ? L - 64 : 0; }
 
 // One group of L lanes updates one site of the colour (colour-order index q).
 // Every lane of the group's wavefront calls it (wavefront-level LDS ordering).
@@ -550,7 +561,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   constexpr int NLM = (LMAX + 1) * (LMAX + 1), PS = NLM + 4;
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
   if constexpr (L >= 64) {
-    // one site per wavefront (two at 128 lanes): its index, site and
+    // one site per wavefront (two or four at 128 or 256 lanes): its index, site and
     // sublattice tables are wave-uniform (scalar registers)
     q = __builtin_amdgcn_readfirstlane(q);
     valid = __builtin_amdgcn_readfirstlane((int)valid) != 0;
@@ -584,7 +595,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   }
   // a group's staged block is its own (L <= 64: one wavefront), so the walk
   // waits on its own wavefront's writes only, not on the block's other sites;
-  // at 128 lanes the group's two wavefronts meet at the block's barrier
+  // above 64 lanes the group's wavefronts meet at the block's barrier
   if constexpr (L <= 64)
     wave_lds_sync();
   else
@@ -592,12 +603,19 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   double a = 0.0;
   if (valid) a = walk_delta<L, NLM>(wl, W, pre, lane, Z);
   if constexpr (L > 64) {
-    // the fold's first step (+64) crosses the group's two wavefronts through
-    // the 64 doubles after its dz row; then the first wavefront folds
+    // the fold's steps of +128 and +64 cross the group's wavefronts through
+    // the L - 64 doubles after its dz row (xs[i] holds lane 64 + i's partial);
+    // then the first wavefront folds.  At 256 lanes lane l < 64 forms
+    // (a_l + a_l+128) + (a_l+64 + a_l+192), the tree's order.
+    static_assert(L == 128 || L == 256, "two or four wavefronts per site");
     double* xs = dz + NLM;
     if (lane >= 64) xs[lane - 64] = a;
     __syncthreads();
-    a = a + xs[lane & 63];
+    const int l0 = lane & 63;
+    if constexpr (L == 128)
+      a = a + xs[l0];
+    else
+      a = (a + xs[l0 + 64]) + (xs[l0] + xs[l0 + 128]);
     a = fold_lanes<64>(a);
   } else {
     a = fold_lanes<L>(a);
@@ -610,7 +628,7 @@ __device__ __forceinline__ void metro_group(const DevTables& T, double* __restri
   int accept = 0;
   const double* pq = prop + (size_t)q * PS;
   if (lane == 0) accept = (a <= 0.0 || u_acc < sce::det_exp(-beta * a)) ? 1 : 0;
-  accept = __shfl(accept, 0, L > 64 ? 64 : L);  // at 128 lanes only the first wavefront writes
+  accept = __shfl(accept, 0, L > 64 ? 64 : L);  // above 64 lanes only the first wavefront writes
   if (accept) {
     for (int k = lane; k < NLM; k += L) zrows[(size_t)s * NLM + k] = pq[4 + k];
     if (lane == 0) {
@@ -2522,16 +2540,17 @@ extern "C" int32_t sce_tables_layout_info(const sce_tables* t, int64_t o[5]) {
 
 // ---- launch helpers --------------------------------------------------------
 static size_t walk_lds_bytes(const DevTables& D, int L) {
-  return (size_t)(TPB / L) * (size_t)(D.nb_max * D.nlm + 1 + D.nlm + (L > 64 ? 64 : 0)) * sizeof(double);
+  return (size_t)(TPB / L) * (size_t)(D.nb_max * D.nlm + 1 + D.nlm + (L > 64 ? L - 64 : 0)) * sizeof(double);
 }
 
-// Lanes per site: 16, 32 or 64 everywhere; 128 (two wavefronts per site) for
-// plain Metropolis sweeps with per-colour launches and the walk test hook.
-static int32_t check_lanes(int L, bool allow128 = false) {
-  if (L == 128 && allow128) return SCE_OK;
+// Lanes per site: 16, 32 or 64 everywhere; 128 and 256 (two and four
+// wavefronts per site) for plain Metropolis sweeps with per-colour launches
+// and the walk test hook.
+static int32_t check_lanes(int L, bool allow_wide = false) {
+  if ((L == 128 || L == 256) && allow_wide) return SCE_OK;
   if (L != 16 && L != 32 && L != 64)
     return sce_fail(SCE_EINVAL, "lanes must be 16, 32 or 64%s (got %d)",
-                    allow128 ? " or 128" : " here (128: plain Metropolis sweeps only)", L);
+                    allow_wide ? ", 128 or 256" : " here (128, 256: plain Metropolis sweeps only)", L);
   return SCE_OK;
 }
 
@@ -2596,7 +2615,8 @@ static void launch_metro(int L, const DevTables& D, double* config, double* zrow
   if (L == 16) launch_metro_l<16, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
   else if (L == 32) launch_metro_l<32, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
   else if (L == 64) launch_metro_l<64, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
-  else launch_metro_l<128, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
+  else if (L == 128) launch_metro_l<128, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
+  else launch_metro_l<256, LMAX, TEST>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, ts, s, e0, e1);
 }
 template <int LMAX> static void launch_metro_run(int L, const DevTables& D, double* config, double* zrows, const double* prop, double* dE, int* acc, const ChainDev* st, int q_lo, int q_n, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   launch_metro<LMAX, false>(L, D, config, zrows, prop, dE, acc, st, q_lo, q_n, nullptr, s, e0, e1);
@@ -2609,7 +2629,8 @@ static void launch_metro_fin(int L, const DevTables& D, double* config, double* 
   if (L == 16) k_metro_fin<16, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
   else if (L == 32) k_metro_fin<32, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
   else if (L == 64) k_metro_fin<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
-  else k_metro_fin<128, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+  else if (L == 128) k_metro_fin<128, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
+  else k_metro_fin<256, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, f);
 }
 template <int LMAX>
 static void launch_metro_prop(int L, const DevTables& D, double* config, double* zrows, double* prop, double* dE,
@@ -2621,7 +2642,8 @@ static void launch_metro_prop(int L, const DevTables& D, double* config, double*
   if (L == 16) k_metro_prop<16, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
   else if (L == 32) k_metro_prop<32, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
   else if (L == 64) k_metro_prop<64, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
-  else k_metro_prop<128, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
+  else if (L == 128) k_metro_prop<128, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
+  else k_metro_prop<256, LMAX><<<nb, TPB, lb, s>>>(D, config, zrows, prop, dE, acc, st, q_lo, q_n, nwalk, r0, r1);
 }
 template <int LMAX>
 static void launch_propose_range(const DevTables& D, const double* config, double* prop, const ChainDev* st,
@@ -3400,8 +3422,8 @@ extern "C" int32_t sce_run_sweeps(sce_chain* c, double beta, int64_t nsweeps, in
                                   int64_t renorm_interval, int64_t measure_interval, int32_t lanes,
                                   double* obs, int64_t obs_cap, int64_t* n_obs) {
   CHK(chain_check(c));
-  // 128 lanes: plain sweeps with per-colour launches (no overrelaxation pass
-  // or persistent grid at that width)
+  // 128 and 256 lanes: plain sweeps with per-colour launches (no
+  // overrelaxation pass or persistent grid at those widths)
   CHK(check_lanes(lanes, or_per_metropolis == 0 && !c->persistent));
   if (nsweeps < 0) return sce_fail(SCE_EINVAL, "nsweeps must be >= 0");
   if (or_per_metropolis < 0) return sce_fail(SCE_EINVAL, "or_per_metropolis must be >= 0");

@@ -243,11 +243,11 @@ def gpu_run_sweeps(gst: GPUChainState, gH: GPUTiledHamiltonian, st: Optional[Cha
 
 
 def autotune_chain(gH: GPUTiledHamiltonian, st: ChainState, beta: float, sweeps: int = 200,
-                   candidates=((16, False), (32, False), (64, False), (128, False), (64, True))):
+                   candidates=((16, False), (32, False), (64, False), (128, False), (256, False), (64, True))):
     """The project autotuner for single chains: times `sweeps` sweeps (measured
     every sweep, as production runs are) of a throwaway copy of the chain for
     every (lanes per site, persistent) candidate — per-colour graph launches at
-    16/32/64/128 lanes (128: two wavefronts per site), or the dependency-driven
+    16/32/64/128/256 lanes (128, 256: two, four wavefronts per site), or the dependency-driven
     persistent sweep at 64 — and returns
     ``(lanes, persistent, timings)`` of the fastest.  Lanes are part of the
     determinism scope: a caller pins the choice for its run and records it."""

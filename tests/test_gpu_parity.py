@@ -118,10 +118,12 @@ def test_overrelaxation_row_sums_in_the_subnormal_range(scale):
 
 
 @pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
-def test_walk_delta_at_128_lanes(case):
-    """Two wavefronts per site: the walk's ΔE at 128 lanes (entries dealt
-    round-robin over 128 lanes, the fold's +64 step across the wavefronts)
-    equals the oracle's 128-lane contract bit for bit."""
+@pytest.mark.parametrize("lanes", [128, 256])
+def test_walk_delta_at_wide_lanes(case, lanes):
+    """Two and four wavefronts per site: the walk's ΔE at 128 and 256 lanes
+    (entries dealt round-robin over the lanes, the fold's +128 and +64 steps
+    across the wavefronts) equals the oracle's contract at that width bit for
+    bit."""
     name, na, terms, dims, mm, fl = case
     H, T = build_pair(na, terms, dims, mm, fl)
     gH = S.GPUTiledHamiltonian(H)
@@ -129,17 +131,18 @@ def test_walk_delta_at_128_lanes(case):
     rng = np.random.default_rng(9)
     sites = np.array(T.color_sites[: min(64, T.n_active)], np.int32)
     znew = np.array([O.zlm_row(v, H.lmax) for v in S.random_config(len(sites), rng)])
-    dev = G.dev_walk_delta(gH, z, sites, znew, 128)
+    dev = G.dev_walk_delta(gH, z, sites, znew, lanes)
     for i, s in enumerate(sites):
-        assert dev[i] == T.walk_delta(int(s), z, znew[i], 128), (name, s)
+        assert dev[i] == T.walk_delta(int(s), z, znew[i], lanes), (name, s)
 
 
 @pytest.mark.parametrize("case", fixture_cases()[:6], ids=lambda c: c[0])
-@pytest.mark.parametrize("graph", [True, False])
-def test_sweeps_at_128_lanes_bitwise(case, graph):
-    """Chained 128-lane sweeps with measurements and renormalisation, graph
-    replay and direct launches, against the oracle's keyed sweeps at 128
-    lanes; overrelaxation and persistent sweeps refuse the width."""
+@pytest.mark.parametrize("graph, lanes", [(True, 128), (False, 128), (True, 256), (False, 256)])
+def test_sweeps_at_wide_lanes_bitwise(case, graph, lanes):
+    """Chained 128- and 256-lane sweeps with measurements and
+    renormalisation, graph replay and direct launches, against the oracle's
+    keyed sweeps at that width; overrelaxation and persistent sweeps refuse
+    the width."""
     name, na, terms, dims, mm, fl = case
     H, T = build_pair(na, terms, dims, mm, fl)
     gH = S.GPUTiledHamiltonian(H)
@@ -147,11 +150,11 @@ def test_sweeps_at_128_lanes_bitwise(case, graph):
     g = S.GPUChainState(gH, st, seed=4242)
     g.set_graph(graph)
     beta = 1 / 0.02
-    obs = S.gpu_run_sweeps(g, gH, None, beta, 7, renorm_interval=3, lanes=128, measure_interval=2)
+    obs = S.gpu_run_sweeps(g, gH, None, beta, 7, renorm_interval=3, lanes=lanes, measure_interval=2)
     c, z, E = st.config.copy(), st.zrows.copy(), st.energy
     nacc = 0
     for j in range(1, 8):
-        a, E, _, _ = T.keyed_sweep(c, z, E, beta, 0.7, 4242, j, 128)
+        a, E, _, _ = T.keyed_sweep(c, z, E, beta, 0.7, 4242, j, lanes)
         nacc += a
         if j % 3 == 0:
             _, E = T.renormalize(c, z, E)
@@ -160,10 +163,10 @@ def test_sweeps_at_128_lanes_bitwise(case, graph):
     assert np.array_equal(zd.view(np.uint64), z.view(np.uint64)), name
     assert Ed == E and g.info().acc_metro == nacc and obs.shape[0] == 3
     with pytest.raises(SceError):
-        S.gpu_run_sweeps(g, gH, None, beta, 1, lanes=128, or_per_metropolis=1)
+        S.gpu_run_sweeps(g, gH, None, beta, 1, lanes=lanes, or_per_metropolis=1)
     g.set_persistent(True)
     with pytest.raises(SceError):
-        S.gpu_run_sweeps(g, gH, None, beta, 1, lanes=128)
+        S.gpu_run_sweeps(g, gH, None, beta, 1, lanes=lanes)
 
 
 @pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])

@@ -93,6 +93,36 @@ def test_chained_sweeps_equal_single_sweeps(case):
     assert abs(E - g.total_energy()) <= 1e-10 * max(1.0, abs(E))
 
 
+@pytest.mark.parametrize("case", CASES[1:], ids=lambda c: c[0])
+@pytest.mark.parametrize("lanes", [128, 256])
+def test_wide_lanes_bitwise(case, lanes):
+    """Two and four wavefronts per site on the heavy triplet walks (thousands
+    of entries per site, so every count of trailing pads occurs): walk rows
+    and chained Metropolis sweeps with renormalisation against the oracle at
+    that width."""
+    H, T, gH, _ = _case(case)
+    z = H.zrows(rand_config(H.n_sites, 41))
+    sites = np.array(T.color_sites[:64], np.int32)
+    znew = np.array([O.zlm_row(v, H.lmax) for v in S.random_config(len(sites), np.random.default_rng(8))])
+    dev = G.dev_walk_delta(gH, z, sites, znew, lanes)
+    for i, s in enumerate(sites):
+        assert dev[i] == T.walk_delta(int(s), z, znew[i], lanes), (lanes, s)
+    st = S.ChainState(H, rand_config(H.n_sites, 17), 0.6)
+    g = S.GPUChainState(gH, st, seed=515)
+    S.gpu_run_sweeps(g, gH, None, 25.0, 4, renorm_interval=2, measure_interval=1, lanes=lanes)
+    c2, z2, E = st.config.copy(), st.zrows.copy(), st.energy
+    acc = 0
+    for sw in range(1, 5):
+        a, E, _, _ = T.keyed_sweep(c2, z2, E, 25.0, 0.6, 515, sw, lanes)
+        acc += a
+        if sw % 2 == 0:
+            _, E = T.renormalize(c2, z2, E)
+    c, zd, Ed = g.download()
+    assert np.array_equal(c.view(np.uint64), c2.view(np.uint64))
+    assert np.array_equal(zd.view(np.uint64), z2.view(np.uint64)) and Ed == E
+    assert g.info().acc_metro == acc
+
+
 def test_wide_stencil_rejects_narrow_lanes():
     H, T, gH, _ = _case(CASES[0])
     gst = S.GPUChainState(gH, S.ChainState(H, rand_config(H.n_sites, 1), 0.6), seed=1)

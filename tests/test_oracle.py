@@ -331,7 +331,7 @@ def test_walk_contract_within_1e12_of_site_coeffs(case):
         c = T.site_coeffs(int(s), z)
         ref = O.delta_energy(c, z[s - 1], zn)
         scale = np.abs(c).sum() * (np.abs(zn).sum() + np.abs(z[s - 1]).sum())
-        for lanes in (16, 32, 64, 128):
+        for lanes in (16, 32, 64, 128, 256):
             assert abs(T.walk_delta(int(s), z, zn, lanes) - ref) <= 1e-12 * max(scale, 1e-30)
             assert np.all(np.abs(T.walk_coeffs(int(s), z, lanes) - c) <= 1e-12 * max(np.abs(c).sum(), 1e-30))
 
