@@ -13,7 +13,8 @@ _swap_payload!             swap_payload                 gpu_pt.jl:46-51
 =========================  ==============================================
 
 ``workgroupsize`` of the reference is ``lanes`` here: the number of lanes that
-walk one site (16, 32 or 64; default 64 = one wavefront), part of the
+walk one site (16, 32 or 64; default 64 = one wavefront; 128 and 256, two
+and four wavefronts, for plain Metropolis sweeps), part of the
 determinism scope exactly like the reference's pinned workgroup size.
 """
 from __future__ import annotations
