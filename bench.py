@@ -1,6 +1,6 @@
 """Headline benchmark: spin updates/s and effective samples/s of E of the device
 Metropolis sweep on bcc 32^3 (65,536 spins; body-2 l<=2 to the 3rd shell, l=1
-triangles and tetrahedra, isotropic SALC tensors).
+triangles and tetrahedra, seeded synthetic isotropic tensors).
 
 One "step" is one Metropolis sweep of the chain (every active site attempted
 once), with the energy recorded on the device every sweep.  Prints one JSON line.
